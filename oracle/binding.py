@@ -1,0 +1,201 @@
+"""ctypes binding of the oracle library (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker the HIP step is compared with. Imported by tests/, by
+``__graft_entry__.smoke()`` and by ``bench.py``'s CPU-baseline leg; the product package never imports it.
+
+The oracle restates the reference's ``AtmosphereModel.step`` (src/core/model.ts:873-877) in C++ with float32
+fields and float64 arithmetic; see stormlab_oracle.cpp for the file:line each part follows.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+# Field ids (oracle.h): the carried order of AtmosphereModel.carried() (model.ts:1319-1335), then extra arrays.
+CARRIED = ["rain", "snow", "graupel", "hail", "u", "v", "w", "theta", "q", "cloud", "ice", "iceNumber", "cold",
+           "chargeIce", "chargeGraupel", "chargeRain", "chargeAir"]
+EXTRA = ["fieldZ", "fallRain", "fallSnow", "fallGraupel", "fallHail", "fallChargeIce", "fallChargeGraupel"]
+FIELDS = CARRIED + EXTRA
+FIELD_ID = {name: k for k, name in enumerate(FIELDS)}
+COLUMNS = ["precipitation", "hailGround", "hailSize"]
+DIAG = ["time", "maxW", "maxThetaP", "rainTotal", "firstFlash", "flashesIC", "flashesCG", "clipped", "negativeFilled",
+        "chargeGround", "minW", "water"]
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("width", ctypes.c_double),
+                ("depth", ctypes.c_double), ("height", ctypes.c_double), ("bottom", ctypes.c_double)]
+
+
+class Profile(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in
+                ("theta0", "dtheta", "ztr", "ttr", "rh_drop", "rh_top", "qv_max", "shear", "shear_depth")]
+
+
+class Run(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_int), ("latitude", ctypes.c_double), ("turbulence", ctypes.c_double),
+                ("tropopause_km", ctypes.c_double), ("roughness", ctypes.c_double), ("frame_u", ctypes.c_double),
+                ("frame_v", ctypes.c_double), ("bubble_amp", ctypes.c_double), ("bubble_x", ctypes.c_double),
+                ("bubble_y", ctypes.c_double), ("bubble_z", ctypes.c_double), ("bubble_rh", ctypes.c_double),
+                ("bubble_rz", ctypes.c_double), ("noise_seed", ctypes.c_int), ("noise_amp", ctypes.c_double),
+                ("noise_top", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads liboracle.so (built by ``make -C oracle``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, d, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+        fp, dp = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(Profile), ctypes.POINTER(Run)]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_step.argtypes = [vp, d, i]
+        L.orc_set_lightning.argtypes = [vp, i]
+        for name in ("orc_get_field", "orc_get_column"):
+            getattr(L, name).argtypes = [vp, i, fp]
+            getattr(L, name).restype = i
+        for name in ("orc_set_field", "orc_set_column"):
+            getattr(L, name).argtypes = [vp, i, fp]
+            getattr(L, name).restype = i
+        L.orc_base_state.argtypes = [vp, dp]
+        L.orc_grid_levels.argtypes = [vp, dp, dp]
+        L.orc_diag.argtypes = [vp, dp]
+        L.orc_pressure.argtypes = [vp, dp]
+        L.orc_set_time.argtypes = [vp, d]
+        for name in ("orc_weno", "orc_upwind5"):
+            getattr(L, name).restype = d
+            getattr(L, name).argtypes = [d] * 5
+        L.orc_fft.argtypes = [i, dp, dp, i]
+        L.orc_hdft_forward.argtypes = [i, i, i, dp, dp, dp]
+        L.orc_hdft_inverse.argtypes = [i, i, i, dp, dp, dp]
+        for name, n in (("orc_qsat", 2), ("orc_qsat_ice", 2), ("orc_fall_speed", 3), ("orc_snow_fall_speed", 3),
+                        ("orc_graupel_fall_speed", 2), ("orc_hail_fall_speed", 2), ("orc_takahashi", 2),
+                        ("orc_takahashi_collision", 4)):
+            getattr(L, name).restype = d
+            getattr(L, name).argtypes = [d] * n
+        L.orc_rain_processes.restype = d
+        L.orc_rain_processes.argtypes = [fp, fp, fp, d, d, d, d]
+        L.orc_saturation_adjust.restype = d
+        L.orc_saturation_adjust.argtypes = [fp, fp, fp, d, d]
+        L.orc_flash_limit.restype = i
+        L.orc_flash_limit.argtypes = [d]
+        L.orc_stretch_ratio.restype = d
+        L.orc_stretch_ratio.argtypes = [d, d, i]
+        L.orc_project.restype = d
+        L.orc_project.argtypes = [ctypes.POINTER(Grid), dp, fp, fp, fp, d, dp]
+        L.orc_advect.argtypes = [ctypes.POINTER(Grid), dp, fp, fp, fp, fp, d, i, i]
+        L.orc_mulberry32.restype = ctypes.c_uint32
+        L.orc_mulberry32.argtypes = [ctypes.POINTER(ctypes.c_uint32)]
+        L.orc_electric_solve.argtypes = [ctypes.POINTER(Grid), dp, dp]
+        _lib = L
+    return _lib
+
+
+def fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def make_grid(g):
+    return Grid(g["nx"], g["ny"], g["nz"], g["width"], g["depth"], g["height"], g.get("bottom", 0.0))
+
+
+def make_profile(p):
+    return Profile(*(float(p[k]) for k, _ in Profile._fields_))
+
+
+def make_run(r):
+    return Run(*(r[k] for k, _ in Run._fields_))
+
+
+class OracleRun:
+    """One oracle model: AtmosphereModel restated (model.ts:183-1697) for a grid, sounding and run spec (dicts)."""
+
+    def __init__(self, grid, profile, run):
+        self.grid = grid
+        self.n = grid["nx"] * grid["ny"] * grid["nz"]
+        self.layer = grid["nx"] * grid["ny"]
+        self._g, self._p, self._r = make_grid(grid), make_profile(profile), make_run(run)
+        self.h = lib().orc_create(ctypes.byref(self._g), ctypes.byref(self._p), ctypes.byref(self._r))
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, n=1, dt=3.0):
+        lib().orc_step(self.h, dt, n)
+
+    def set_lightning(self, on):
+        lib().orc_set_lightning(self.h, 1 if on else 0)
+
+    def field(self, name):
+        out = np.empty(self.n, np.float32)
+        assert lib().orc_get_field(self.h, FIELD_ID[name], fptr(out)) == 0
+        return out
+
+    def set_field(self, name, values):
+        a = np.ascontiguousarray(values, dtype=np.float32)
+        assert a.size == self.n
+        assert lib().orc_set_field(self.h, FIELD_ID[name], fptr(a)) == 0
+
+    def column(self, name):
+        out = np.empty(self.layer, np.float32)
+        assert lib().orc_get_column(self.h, COLUMNS.index(name), fptr(out)) == 0
+        return out
+
+    def set_column(self, name, values):
+        a = np.ascontiguousarray(values, dtype=np.float32)
+        assert lib().orc_set_column(self.h, COLUMNS.index(name), fptr(a)) == 0
+
+    def base_state(self):
+        nz = self.grid["nz"]
+        out = np.empty(7 * nz, np.float64)
+        lib().orc_base_state(self.h, dptr(out))
+        return dict(zip(["p", "exner", "theta", "q", "u", "v", "rho"], out.reshape(7, nz)))
+
+    def levels(self):
+        nz = self.grid["nz"]
+        zs, hz = np.empty(nz), np.empty(nz)
+        lib().orc_grid_levels(self.h, dptr(zs), dptr(hz))
+        return zs, hz
+
+    def diag(self):
+        out = np.empty(len(DIAG), np.float64)
+        lib().orc_diag(self.h, dptr(out))
+        return dict(zip(DIAG, out))
+
+    def pressure(self):
+        out = np.empty(self.layer * (self.grid["nz"] - 1), np.float64)
+        lib().orc_pressure(self.h, dptr(out))
+        return out
+
+    def set_time(self, t):
+        lib().orc_set_time(self.h, t)
+
+    def state(self):
+        return {name: self.field(name) for name in CARRIED}
+
+    def load_state(self, state, columns=None, time=None):
+        for name, v in state.items():
+            self.set_field(name, v)
+        for name, v in (columns or {}).items():
+            self.set_column(name, v)
+        if time is not None:
+            self.set_time(time)

@@ -1,0 +1,107 @@
+/*
+ * StormLab step oracle: C ABI of the CPU restatement.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker the HIP step is compared with; it is loaded by tests/,
+ * by __graft_entry__.smoke() and by bench.py's CPU-baseline leg, never by the product path.
+ *
+ * Field storage is float32 (the reference's Float32Array fields), arithmetic is float64 (the reference's JS numbers),
+ * in the order of operations of the reference (src/core/model.ts AtmosphereModel.step and what it calls).
+ */
+#ifndef STORMLAB_ORACLE_H
+#define STORMLAB_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Grid (src/core/grid.ts GridOptions): periodic nx x ny, nz levels stretched from `bottom` m at the ground (0 = uniform). */
+typedef struct { int nx, ny, nz; double width, depth, height, bottom; } orc_grid;
+
+/*
+ * Analytic sounding (src/core/profiles.ts weismanKlemp): theta = theta0 + dtheta (z/ztr)^1.25 below ztr, isothermal ttr
+ * above; RH = 1 - rh_drop (z/ztr)^1.25 (rh_top above); vapour capped at qv_max; wind u = shear tanh(z / shear_depth), v = 0.
+ */
+typedef struct { double theta0, dtheta, ztr, ttr, rh_drop, rh_top, qv_max, shear, shear_depth; } orc_profile;
+
+/*
+ * One run: SimConfig values that matter at the baseline (src/core/fixtures.ts SUMMER_DAY with single: true) and the
+ * initial perturbation written after construction: a cos^2 thermal of amplitude bubble_amp centred at (bubble_x,
+ * bubble_y, bubble_z) with radii (bubble_rh, bubble_rz), plus uniform +-noise_amp K theta noise below noise_top from
+ * mulberry32(noise_seed) in node order.
+ */
+typedef struct {
+  int seed;                 /* SimConfig.seed (the lightning's random numbers use seed + 1) */
+  double latitude, turbulence, tropopause_km, roughness;
+  double frame_u, frame_v;  /* SimConfig.frameMotion */
+  double bubble_amp, bubble_x, bubble_y, bubble_z, bubble_rh, bubble_rz;
+  int noise_seed; double noise_amp, noise_top;
+} orc_run;
+
+/* Field ids: the carried order of AtmosphereModel.carried() (model.ts:1319-1335), then the extra per-node arrays. */
+enum {
+  ORC_RAIN = 0, ORC_SNOW, ORC_GRAUPEL, ORC_HAIL, ORC_U, ORC_V, ORC_W, ORC_THETA, ORC_Q, ORC_CLOUD, ORC_ICE, ORC_ICENUMBER,
+  ORC_COLD, ORC_CHARGE_ICE, ORC_CHARGE_GRAUPEL, ORC_CHARGE_RAIN, ORC_CHARGE_AIR,
+  ORC_CARRIED = 17,
+  ORC_FIELDZ = 17, ORC_FALL_RAIN, ORC_FALL_SNOW, ORC_FALL_GRAUPEL, ORC_FALL_HAIL, ORC_FALL_CHARGE_ICE, ORC_FALL_CHARGE_GRAUPEL,
+  ORC_FIELDS = 24
+};
+/* Column ids (STATE_COLUMNS, src/core/state.ts:10). */
+enum { ORC_PRECIPITATION = 0, ORC_HAIL_GROUND, ORC_HAIL_SIZE, ORC_COLUMNS };
+
+/* Diagnostics vector (orc_diag): */
+enum {
+  ORC_D_TIME = 0, ORC_D_MAXW, ORC_D_MAXTHETAP, ORC_D_RAINTOTAL, ORC_D_FIRSTFLASH, ORC_D_FLASHES_IC, ORC_D_FLASHES_CG,
+  ORC_D_CLIPPED, ORC_D_NEGFILLED, ORC_D_CHARGEGROUND, ORC_D_MINW, ORC_D_WATER, ORC_D_COUNT
+};
+
+void *orc_create(const orc_grid *grid, const orc_profile *profile, const orc_run *run);
+void orc_destroy(void *model);
+/* n steps of dt seconds (AtmosphereModel.step, then time += dt as the callers do). */
+void orc_step(void *model, double dt, int n);
+/* Switches lightning (discharge) on (1) or off (0); on by default (SimConfig.lightning). */
+void orc_set_lightning(void *model, int on);
+int orc_get_field(void *model, int field, float *out);
+int orc_set_field(void *model, int field, const float *in);
+int orc_get_column(void *model, int column, float *out);
+int orc_set_column(void *model, int column, const float *in);
+/* Base state per level: p, exner, theta, q, u, v, rho (7 x nz float64). */
+void orc_base_state(void *model, double *out);
+void orc_grid_levels(void *model, double *zs, double *hz);
+void orc_diag(void *model, double *out);
+/* The model's projection pressure (cells) after the last step, (nz-1) * layer float64. */
+void orc_pressure(void *model, double *out);
+void orc_set_time(void *model, double t);
+
+/* Known-answer surface of the pieces (for tests). */
+double orc_weno(double a, double b, double c, double d, double e);
+double orc_upwind5(double a, double b, double c, double d, double e);
+void orc_fft(int n, double *re, double *im, int sign);
+/* Forward 2-D real DFT of `layers` layers into the Hermitian half spectra re/im (nx*ny*layers each), and back. */
+void orc_hdft_forward(int nx, int ny, int layers, const double *src, double *re, double *im);
+void orc_hdft_inverse(int nx, int ny, int layers, double *re, double *im, double *dst);
+double orc_qsat(double tc, double p);
+double orc_qsat_ice(double tc, double p);
+double orc_fall_speed(double rain, double rho, double rho_ground);
+double orc_snow_fall_speed(double snow, double rho, double rho_ground);
+double orc_graupel_fall_speed(double g, double rho);
+double orc_hail_fall_speed(double h, double rho);
+/* rainProcesses on one node (q, cloud, rain as float32 in/out); returns the evaporated rain. */
+double orc_rain_processes(float *q, float *cloud, float *rain, double tk, double p, double rho, double dt);
+double orc_saturation_adjust(float *theta, float *q, float *cloud, double exner, double p);
+double orc_takahashi(double tc, double lwc);
+double orc_takahashi_collision(double tc, double lwc, double d, double dv);
+int orc_flash_limit(double dt);
+double orc_stretch_ratio(double bottom, double height, int intervals);
+/* Pressure projection on its own: random-test surface (u, v, w float32 in/out, rho per level); returns max |div| after. */
+double orc_project(const orc_grid *grid, const double *rho, float *u, float *v, float *w, double dt, double *div_before);
+/* Face projection + RK3 advection of one field (FluxTransport.setVelocity + advect), for the transport tests. */
+void orc_advect(const orc_grid *grid, const double *rho, const float *u, const float *v, const float *w,
+                float *q, double dt, int steps, int monotone);
+uint32_t orc_mulberry32(uint32_t *state);
+/* Electric potential of a node charge density (ElectricSolver.solve): rho, phi are n float64. */
+void orc_electric_solve(const orc_grid *grid, const double *rho, double *phi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
