@@ -1,0 +1,116 @@
+/*
+ * meteolab: the StormLab model step on MI355X (gfx950), C ABI.
+ *
+ * The drop-in boundary of the GPU step. The reference's step is AtmosphereModel.step(dt) (src/core/model.ts:873-877)
+ * on the model's Float32Array fields, and its GPU plugin is GpuDynamics (src/gpu/dynamics.ts:250-504: create, step,
+ * mix, matches, destroy, errors, adapter). Here a context owns one device and stream; runs of one grid size share a
+ * pool of slots (their state stays resident in HBM); a step advances a batch of runs of one pool by whole model steps.
+ *
+ * Conventions: every function returns 0 on success or a negative ML_E_* code; ml_last_error() gives the message.
+ * Plain pointers and sizes only; host buffers are the caller's. Field ids follow the carried order of
+ * AtmosphereModel.carried() (model.ts:1319-1335), then the per-node arrays a step recomputes. Node index
+ * i = x + nx (y + ny z), z = 0 at the ground. One context per host thread; calls on a context are stream-ordered and
+ * ml_sync() waits for them.
+ */
+#ifndef METEOLAB_H
+#define METEOLAB_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  ML_OK = 0, ML_E_ARG = -1, ML_E_HIP = -2, ML_E_FULL = -3, ML_E_STATE = -4, ML_E_UNSUPPORTED = -5
+};
+
+/* Field ids (carried order, model.ts:1319-1335, then fieldZ, the fall speeds and the condensate). */
+enum {
+  ML_RAIN = 0, ML_SNOW, ML_GRAUPEL, ML_HAIL, ML_U, ML_V, ML_W, ML_THETA, ML_Q, ML_CLOUD, ML_ICE, ML_ICENUMBER, ML_COLD,
+  ML_CHARGE_ICE, ML_CHARGE_GRAUPEL, ML_CHARGE_RAIN, ML_CHARGE_AIR,
+  ML_CARRIED = 17,
+  ML_FIELDZ = 17, ML_FALL_RAIN, ML_FALL_SNOW, ML_FALL_GRAUPEL, ML_FALL_HAIL, ML_FALL_CHARGE_ICE, ML_FALL_CHARGE_GRAUPEL,
+  ML_CONDENSATE, ML_FIELDS
+};
+/* Ground columns (STATE_COLUMNS, src/core/state.ts:10). */
+enum { ML_PRECIPITATION = 0, ML_HAIL_GROUND, ML_HAIL_SIZE, ML_COLUMNS };
+/* Per-level base state (environment.ts:124-130), float64, nz each, in this order. */
+enum { ML_LEVEL_P = 0, ML_LEVEL_EXNER, ML_LEVEL_THETA, ML_LEVEL_Q, ML_LEVEL_U, ML_LEVEL_V, ML_LEVEL_RHO, ML_LEVELS = 7 };
+/* Diagnostics of a run (model.ts:1672-1696 and the run metrics), float64. */
+enum {
+  ML_D_TIME = 0, ML_D_MAXW, ML_D_MINW, ML_D_MAXTHETAP, ML_D_RAINTOTAL, ML_D_WATER, ML_D_CHARGE, ML_D_CLIPPED,
+  ML_D_NEGFILLED, ML_D_CHARGEGROUND, ML_D_FLASHES_IC, ML_D_FLASHES_CG, ML_D_FIRSTFLASH, ML_D_ERROR, ML_D_COUNT
+};
+
+/* Grid (src/core/grid.ts GridOptions, createGrid :99-121): periodic nx x ny, nz levels stretched geometrically from
+ * `bottom` m at the ground to `height` (bottom = 0: uniform). */
+typedef struct { int nx, ny, nz; double width, depth, height, bottom; } ml_grid;
+
+/* Analytic sounding (src/core/profiles.ts:61-73 weismanKlemp): theta = theta0 + dtheta (z/ztr)^1.25 below ztr,
+ * isothermal ttr above; RH = 1 - rh_drop (z/ztr)^1.25 (rh_top above), vapour capped at qv_max; wind
+ * u = shear tanh(z / shear_depth), v = 0. */
+typedef struct { double theta0, dtheta, ztr, ttr, rh_drop, rh_top, qv_max, shear, shear_depth; } ml_profile;
+
+/* A run's settings (SimConfig values the step reads, config.ts:15-188) and its initial thermal and noise: a cos^2
+ * bubble of bubble_amp K (model.ts:94 ELLIPSOID form) and uniform +-noise_amp K theta noise below noise_top from
+ * mulberry32(noise_seed) in node order. */
+typedef struct {
+  int seed;
+  double latitude, turbulence, tropopause_km, roughness, frame_u, frame_v;
+  double bubble_amp, bubble_x, bubble_y, bubble_z, bubble_rh, bubble_rz;
+  int noise_seed;
+  double noise_amp, noise_top;
+} ml_run_config;
+
+typedef struct ml_ctx ml_ctx;
+
+/* A context on device `device`. precision 32: float32 arithmetic (the production kernels); 64: the same kernels in
+ * float64 (the precision study). Replaces GpuDynamics.create's device setup (gpu/dynamics.ts:273-278). */
+int ml_ctx_create(int device, int precision, ml_ctx **out);
+void ml_ctx_destroy(ml_ctx *ctx);
+/* Message of the last error on this context (GpuDynamics.errors, gpu/dynamics.ts:252). */
+const char *ml_last_error(ml_ctx *ctx);
+/* The device's name (GpuDynamics.adapter, gpu/dynamics.ts:251). */
+int ml_device_name(ml_ctx *ctx, char *out, int size);
+
+/* A pool of `capacity` run slots of one grid (a size class); electrified 1 carries the charges and runs the lightning
+ * (SimConfig.lightning, model.ts:1316). *pool receives its id. */
+int ml_pool_create(ml_ctx *ctx, const ml_grid *grid, int capacity, int electrified, int *pool);
+/* Device bytes one slot of a pool of this grid takes (the serving loop's admission by HBM footprint). */
+int64_t ml_slot_bytes(ml_ctx *ctx, const ml_grid *grid);
+/* Free slots left in a pool. */
+int ml_pool_free(ml_ctx *ctx, int pool);
+
+/* A run in a free slot of `pool`: its base state from the sounding (environment.ts:45-73, in float64 on the device) and
+ * its initial state (model.ts:399-451 single-storm start with the config's thermal and noise). *run receives a handle.
+ * Replaces new AtmosphereModel(config, grid, profile) for the step's state. */
+int ml_run_create(ml_ctx *ctx, int pool, const ml_profile *profile, const ml_run_config *config, int *run);
+/* Frees the run's slot. */
+int ml_run_release(ml_ctx *ctx, int run);
+
+/* Copies one field (n floats) / ground column (nx ny floats) between the host and a run (the model-hook round trip:
+ * dynamicsInputs / setCarried, model.ts:994-1026; state layout of src/core/state.ts). */
+int ml_run_upload(ml_ctx *ctx, int run, int field, const float *host);
+int ml_run_download(ml_ctx *ctx, int run, int field, float *host);
+int ml_run_upload_column(ml_ctx *ctx, int run, int column, const float *host);
+int ml_run_download_column(ml_ctx *ctx, int run, int column, float *host);
+/* The run's base state, ML_LEVELS x nz float64. */
+int ml_run_levels(ml_ctx *ctx, int run, double *out);
+/* The run's diagnostics, ML_D_COUNT float64 (stream-synchronising). */
+int ml_run_diag(ml_ctx *ctx, int run, double *out);
+/* Model time of the run, s (AtmosphereModel.time). */
+int ml_run_set_time(ml_ctx *ctx, int run, double time);
+
+/* Advances `count` runs of one pool by `steps` model steps of dt s each: AtmosphereModel.step(dt) then time += dt
+ * (model.ts:873-877 and its callers, e.g. tests/bench/threads.ts:38-42). One launch per kernel covers all the runs;
+ * the launch sequence of a batch is captured once into a hipGraph and replayed. Asynchronous: ml_sync waits. */
+int ml_step(ml_ctx *ctx, const int *runs, int count, int steps, double dt);
+int ml_sync(ml_ctx *ctx);
+
+/* Kernel timing of the last ml_step batch: stage by stage device time in ms from HIP events over `steps` eager steps
+ * (no graph); names[i] are written as a '\n'-separated list into `names` (size bytes). Returns the number of stages. */
+int ml_profile_step(ml_ctx *ctx, const int *runs, int count, int steps, double dt, double *ms, int max, char *names, int size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

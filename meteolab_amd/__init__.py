@@ -1,0 +1,258 @@
+"""StormLab model step on MI355X: the host side of the C ABI in include/meteolab.h (libmeteolab.so).
+
+The reference's step is ``AtmosphereModel.step(dt)`` (src/core/model.ts:873-877) and its GPU plugin ``GpuDynamics``
+(src/gpu/dynamics.ts:250-504). Here :class:`Context` owns a device, :class:`Pool` the resident runs of one grid size,
+and :class:`Run` one model run with the reference's field names; :func:`step` advances a batch of runs of one pool.
+The library must be built (``python -m meteolab_amd.build``); nothing here has a CPU path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmeteolab.so")
+
+# Field ids of include/meteolab.h: the carried order of AtmosphereModel.carried() (model.ts:1319-1335), then fieldZ,
+# the fall speeds and the condensate.
+CARRIED = ["rain", "snow", "graupel", "hail", "u", "v", "w", "theta", "q", "cloud", "ice", "iceNumber", "cold",
+           "chargeIce", "chargeGraupel", "chargeRain", "chargeAir"]
+FIELDS = CARRIED + ["fieldZ", "fallRain", "fallSnow", "fallGraupel", "fallHail", "fallChargeIce",
+                    "fallChargeGraupel", "condensate"]
+FIELD_ID = {name: k for k, name in enumerate(FIELDS)}
+COLUMNS = ["precipitation", "hailGround", "hailSize"]
+LEVELS = ["p", "exner", "theta", "q", "u", "v", "rho"]
+DIAG = ["time", "maxW", "minW", "maxThetaP", "rainTotal", "water", "charge", "clipped", "negativeFilled",
+        "chargeGround", "flashesIC", "flashesCG", "firstFlash", "error"]
+# STATE_FIELDS of src/core/state.ts:9 that this step carries (two-moment numbers are not part of the baseline step).
+STATE_FIELDS = ["u", "v", "w", "theta", "q", "cloud", "rain", "cold", "ice", "snow", "graupel", "hail", "chargeIce",
+                "chargeGraupel", "chargeRain", "chargeAir", "iceNumber"]
+
+
+class MeteolabError(RuntimeError):
+    """An error code of the C ABI, with ml_last_error()'s message."""
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("width", ctypes.c_double),
+                ("depth", ctypes.c_double), ("height", ctypes.c_double), ("bottom", ctypes.c_double)]
+
+
+class Profile(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in
+                ("theta0", "dtheta", "ztr", "ttr", "rh_drop", "rh_top", "qv_max", "shear", "shear_depth")]
+
+
+class RunConfig(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_int), ("latitude", ctypes.c_double), ("turbulence", ctypes.c_double),
+                ("tropopause_km", ctypes.c_double), ("roughness", ctypes.c_double), ("frame_u", ctypes.c_double),
+                ("frame_v", ctypes.c_double), ("bubble_amp", ctypes.c_double), ("bubble_x", ctypes.c_double),
+                ("bubble_y", ctypes.c_double), ("bubble_z", ctypes.c_double), ("bubble_rh", ctypes.c_double),
+                ("bubble_rz", ctypes.c_double), ("noise_seed", ctypes.c_int), ("noise_amp", ctypes.c_double),
+                ("noise_top", ctypes.c_double)]
+
+
+# Every symbol include/meteolab.h declares, with (restype, argtypes).
+_vp, _i, _d, _i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
+_ip, _fp, _dp, _cp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double), ctypes.c_char_p
+SIGNATURES = {
+    "ml_ctx_create": (_i, [_i, _i, ctypes.POINTER(_vp)]),
+    "ml_ctx_destroy": (None, [_vp]),
+    "ml_last_error": (_cp, [_vp]),
+    "ml_device_name": (_i, [_vp, _cp, _i]),
+    "ml_pool_create": (_i, [_vp, ctypes.POINTER(Grid), _i, _i, _ip]),
+    "ml_slot_bytes": (_i64, [_vp, ctypes.POINTER(Grid)]),
+    "ml_pool_free": (_i, [_vp, _i]),
+    "ml_run_create": (_i, [_vp, _i, ctypes.POINTER(Profile), ctypes.POINTER(RunConfig), _ip]),
+    "ml_run_release": (_i, [_vp, _i]),
+    "ml_run_upload": (_i, [_vp, _i, _i, _fp]),
+    "ml_run_download": (_i, [_vp, _i, _i, _fp]),
+    "ml_run_upload_column": (_i, [_vp, _i, _i, _fp]),
+    "ml_run_download_column": (_i, [_vp, _i, _i, _fp]),
+    "ml_run_levels": (_i, [_vp, _i, _dp]),
+    "ml_run_diag": (_i, [_vp, _i, _dp]),
+    "ml_run_set_time": (_i, [_vp, _i, _d]),
+    "ml_step": (_i, [_vp, _ip, _i, _i, _d]),
+    "ml_sync": (_i, [_vp]),
+    "ml_profile_step": (_i, [_vp, _ip, _i, _i, _d, _dp, _i, _cp, _i]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libmeteolab.so. There is no fallback: a missing library is an error."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libmeteolab.so is not built ({LIB_PATH}); run python -m meteolab_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _fptr(a):
+    return a.ctypes.data_as(_fp)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """A device and its stream (ml_ctx_create). precision 32: the production float32 kernels; 64: the same in float64."""
+
+    def __init__(self, device=0, precision=32):
+        self.precision = precision
+        h = ctypes.c_void_p()
+        rc = lib().ml_ctx_create(device, precision, ctypes.byref(h))
+        if rc != 0:
+            raise MeteolabError(f"ml_ctx_create failed ({rc}): no usable HIP device {device}")
+        self.h = h
+
+    def check(self, rc):
+        if rc < 0:
+            raise MeteolabError(f"meteolab error {rc}: {lib().ml_last_error(self.h).decode()}")
+        return rc
+
+    @property
+    def adapter(self):
+        buf = ctypes.create_string_buffer(256)
+        self.check(lib().ml_device_name(self.h, buf, 256))
+        return buf.value.decode()
+
+    def pool(self, grid, capacity, electrified=True):
+        return Pool(self, grid, capacity, electrified)
+
+    def slot_bytes(self, grid):
+        return self.check(lib().ml_slot_bytes(self.h, ctypes.byref(make_grid(grid))))
+
+    def sync(self):
+        self.check(lib().ml_sync(self.h))
+
+    def close(self):
+        if self.h:
+            lib().ml_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def make_grid(g):
+    return Grid(int(g["nx"]), int(g["ny"]), int(g["nz"]), float(g["width"]), float(g["depth"]), float(g["height"]),
+                float(g.get("bottom", 0.0)))
+
+
+class Pool:
+    """Slots for `capacity` runs of one grid (ml_pool_create); their state stays resident on the device."""
+
+    def __init__(self, ctx, grid, capacity, electrified=True):
+        self.ctx, self.grid, self.capacity = ctx, dict(grid), capacity
+        self.n = grid["nx"] * grid["ny"] * grid["nz"]
+        self.layer = grid["nx"] * grid["ny"]
+        pid = ctypes.c_int()
+        self._g = make_grid(grid)
+        ctx.check(lib().ml_pool_create(ctx.h, ctypes.byref(self._g), capacity, 1 if electrified else 0, ctypes.byref(pid)))
+        self.id = pid.value
+
+    def free(self):
+        return self.ctx.check(lib().ml_pool_free(self.ctx.h, self.id))
+
+    def run(self, profile, run):
+        return Run(self, profile, run)
+
+
+class Run:
+    """One model run in a pool slot: its base state and initial state are built on the device (ml_run_create)."""
+
+    def __init__(self, pool, profile, run):
+        self.pool, self.ctx = pool, pool.ctx
+        self.n, self.layer, self.grid = pool.n, pool.layer, pool.grid
+        p = Profile(*(float(profile[k]) for k, _ in Profile._fields_))
+        r = RunConfig(*(run[k] for k, _ in RunConfig._fields_))
+        h = ctypes.c_int()
+        self.ctx.check(lib().ml_run_create(self.ctx.h, pool.id, ctypes.byref(p), ctypes.byref(r), ctypes.byref(h)))
+        self.h = h.value
+
+    def release(self):
+        if self.h is not None:
+            self.ctx.check(lib().ml_run_release(self.ctx.h, self.h))
+            self.h = None
+
+    def field(self, name):
+        out = np.empty(self.n, np.float32)
+        self.ctx.check(lib().ml_run_download(self.ctx.h, self.h, FIELD_ID[name], _fptr(out)))
+        return out
+
+    def set_field(self, name, values):
+        a = np.ascontiguousarray(values, dtype=np.float32)
+        if a.size != self.n:
+            raise ValueError(f"{name}: {a.size} values for {self.n} nodes")
+        self.ctx.check(lib().ml_run_upload(self.ctx.h, self.h, FIELD_ID[name], _fptr(a)))
+
+    def column(self, name):
+        out = np.empty(self.layer, np.float32)
+        self.ctx.check(lib().ml_run_download_column(self.ctx.h, self.h, COLUMNS.index(name), _fptr(out)))
+        return out
+
+    def set_column(self, name, values):
+        a = np.ascontiguousarray(values, dtype=np.float32)
+        if a.size != self.layer:
+            raise ValueError(f"{name}: {a.size} values for {self.layer} columns")
+        self.ctx.check(lib().ml_run_upload_column(self.ctx.h, self.h, COLUMNS.index(name), _fptr(a)))
+
+    def levels(self):
+        nz = self.grid["nz"]
+        out = np.empty(len(LEVELS) * nz, np.float64)
+        self.ctx.check(lib().ml_run_levels(self.ctx.h, self.h, _dptr(out)))
+        return dict(zip(LEVELS, out.reshape(len(LEVELS), nz)))
+
+    def diagnostics(self):
+        out = np.empty(len(DIAG), np.float64)
+        self.ctx.check(lib().ml_run_diag(self.ctx.h, self.h, _dptr(out)))
+        return dict(zip(DIAG, out))
+
+    def set_time(self, t):
+        self.ctx.check(lib().ml_run_set_time(self.ctx.h, self.h, float(t)))
+
+    def state(self):
+        return {name: self.field(name) for name in CARRIED}
+
+    def load_state(self, state, columns=None, time=None):
+        for name, v in state.items():
+            self.set_field(name, v)
+        for name, v in (columns or {}).items():
+            self.set_column(name, v)
+        if time is not None:
+            self.set_time(time)
+
+    def step(self, n=1, dt=3.0):
+        step([self], n, dt)
+        self.ctx.sync()
+
+
+def step(runs, n=1, dt=3.0):
+    """Advances the runs (one pool) by n steps of dt (ml_step); asynchronous, Context.sync() waits."""
+    if not runs:
+        return
+    ctx = runs[0].ctx
+    ids = (ctypes.c_int * len(runs))(*[r.h for r in runs])
+    ctx.check(lib().ml_step(ctx.h, ids, len(runs), int(n), float(dt)))
+
+
+def profile_step(runs, n=3, dt=3.0, max_stages=64):
+    """Device time per launch stage of one step (HIP events, eager launches), ms averaged over n steps."""
+    ctx = runs[0].ctx
+    ids = (ctypes.c_int * len(runs))(*[r.h for r in runs])
+    ms = np.zeros(max_stages)
+    names = ctypes.create_string_buffer(4096)
+    k = ctx.check(lib().ml_profile_step(ctx.h, ids, len(runs), int(n), float(dt), _dptr(ms), max_stages, names, 4096))
+    return dict(zip(names.value.decode().split("\n")[:k], ms[:k]))

@@ -1,0 +1,131 @@
+// Layout of the StormLab step in HBM and the parameters every kernel receives.
+//
+// Runs of one grid size live in a pool of slots. Each array kind is one allocation [slot][...], so a batch of runs is
+// one launch over the list of active slots (blockIdx.y or .z indexes the list) and a run keeps its slot, and its
+// state stays resident, from admission to release. Fields are float32 (the reference's Float32Array fields); the
+// intermediate arrays (face mass fluxes, spectra, eddy viscosities, potentials) have the arithmetic type R of the
+// kernels: float in production, double in the float64 variant of the same kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace ml {
+
+// Field ids: the carried order of AtmosphereModel.carried() (src/core/model.ts:1319-1335), then the per-node arrays a
+// step recomputes: the vertical electric field (model.ts:1550-1551) and the fall speeds (model.ts:1220-1235).
+enum : int {
+  F_RAIN = 0, F_SNOW, F_GRAUPEL, F_HAIL, F_U, F_V, F_W, F_THETA, F_Q, F_CLOUD, F_ICE, F_ICENUMBER, F_COLD,
+  F_CHARGE_ICE, F_CHARGE_GRAUPEL, F_CHARGE_RAIN, F_CHARGE_AIR, N_CARRIED,
+  F_FIELDZ = N_CARRIED, F_FALL_RAIN, F_FALL_SNOW, F_FALL_GRAUPEL, F_FALL_HAIL, F_FALL_CI, F_FALL_CG, F_CONDENSATE,
+  N_FIELDS
+};
+// Ground columns (STATE_COLUMNS, src/core/state.ts:10).
+enum : int { C_PRECIPITATION = 0, C_HAIL_GROUND, C_HAIL_SIZE, N_COLUMNS };
+// Per-level tables of a run (double, nz each): the base state (src/core/environment.ts:124-130) and what the step
+// derives from it.
+enum : int {
+  L_P = 0, L_EXNER, L_THETA, L_Q, L_U, L_V, L_RHO, L_UENV, L_VENV, L_LW, L_EINIT, L_CAPACITANCE, N_LEVELS
+};
+// Per-run parameters (double).
+enum : int {
+  P_FRAME_U = 0, P_FRAME_V, P_CORIOLIS, P_CS, P_DRAG, P_SPONGE_START, P_SEED, P_BUBBLE_AMP, P_BUBBLE_X, P_BUBBLE_Y,
+  P_BUBBLE_Z, P_BUBBLE_RH, P_BUBBLE_RZ, P_NOISE_SEED, P_NOISE_AMP, P_NOISE_TOP, P_QVMAX, P_SHEAR, P_SHEAR_DEPTH,
+  P_THETA0, P_DTHETA, P_ZTR, P_TTR, P_RH_DROP, P_RH_TOP, P_ELECTRIFIED, P_TIME, N_PARAMS = 32
+};
+// Per-run scalars (8-byte words). Per step: cleared by k_begin; the rest accumulate over the run.
+enum : int {
+  S_COURANT = 0,         // 7 words: the largest Courant number of each fall (float bits, atomicMax on uint32)
+  S_FILL = 8,            // 7 x 3 doubles: removed, kept, negative of the positivity fix of each conserved field
+  S_STRONGEST = 29,      // uint64 key of the largest (|E|/E_init)^2 (float bits << 32 | ~node) of the step's solve
+  S_STEP_END = 30,       // [0, S_STEP_END) is cleared at the start of each step
+  S_CLIPPED = 32,        // int64: velocity components clipped since the start (model.ts:1586-1595)
+  S_NEGATIVE = 33,       // double: negative mass removed by the positivity fix since the start (model.ts:1119)
+  S_CHARGE_GROUND = 34,  // double: charge carried to the ground by falling carriers (model.ts:584-587)
+  S_FLASH_IC = 35, S_FLASH_CG = 36, S_FIRST_FLASH = 37,
+  S_ERROR = 38,          // int64: nonzero when a kernel of the run met a condition it does not handle
+  N_SCALARS = 48
+};
+// The falls of a step in order (model.ts:928-932, sedimentStep): water into the gauge, then charge with its carrier's
+// speed.
+constexpr int N_FALLS = 7;
+// The conserved (positivity-fixed) carried fields (model.ts:1319-1327): rain, snow, graupel, hail, q, cloud, ice.
+constexpr int N_CONSERVED = 7;
+
+// Geometry of a size class (src/core/grid.ts:99-121).
+struct Geo {
+  int nx, ny, nz, layer, n;
+  int mw, nm;            // kept Hermitian columns mw = nx/2 + 1 and modes nm = mw * ny
+  double dx, dy, dz, height;
+};
+
+// Device arrays of a pool (all [slot][...]) and the class tables. R: arithmetic type of the intermediate arrays.
+template <class R> struct Bufs {
+  float *fields;   // [slot][N_FIELDS][n]
+  float *stx;      // [slot][N_CARRIED][n] RK3 stage buffer X (stage 0 output) / mixed fields
+  float *sty;      // [slot][N_CARRIED][n] RK3 stage buffer Y (stage 1 output)
+  R *mass;         // [slot][4][n]  face mass fluxes mx, my, mz and their divergence
+  R *phi;          // [slot][n]     potential of the face projection
+  R *cells;        // [slot][n]     right-hand side of a spectral solve (cells or interior layers)
+  R *press;        // [slot][n]     pressure of the projection (model.ts:358), cells
+  R *spec;         // [slot][nz][nm][2] Hermitian half spectra, re/im interleaved
+  double *mean0;   // [slot][nz]    float64 layer sums of the right-hand side (the m = n = 0 mode)
+  R *km, *kh;      // [slot][n]     eddy viscosities
+  R *ephi;         // [slot][n]     electric potential (electric.ts:36-60)
+  float *cols;     // [slot][N_COLUMNS][layer]
+  double *lev;     // [slot][N_LEVELS][nz]
+  double *par;     // [slot][N_PARAMS]
+  double *scal;    // [slot][N_SCALARS]
+  const double *zs, *dzs, *hz;   // class levels (nz, nz - 1, nz)
+  const R *sym;    // [4][nm] per kept mode: face-projection lambda, pressure alpha, beta, electric lambda
+  const R *twx;    // [nx][2] exp(-2 pi i k / nx)
+  const R *twy;    // [ny][2] exp(-2 pi i k / ny)
+  const int *runs; // active slots of this launch
+
+  __device__ float *f(int slot, int k, const Geo &g) const { return fields + (size_t(slot) * N_FIELDS + k) * g.n; }
+  __device__ float *x(int slot, int k, const Geo &g) const { return stx + (size_t(slot) * N_CARRIED + k) * g.n; }
+  __device__ float *y(int slot, int k, const Geo &g) const { return sty + (size_t(slot) * N_CARRIED + k) * g.n; }
+  // Buffer set: 0 the state, 1 stage buffer X, 2 stage buffer Y.
+  __device__ float *set(int which, int slot, int k, const Geo &g) const { return which == 0 ? f(slot, k, g) : which == 1 ? x(slot, k, g) : y(slot, k, g); }
+  __device__ R *m(int slot, int k, const Geo &g) const { return mass + (size_t(slot) * 4 + k) * g.n; }
+  __device__ R *arr(R *base, int slot, const Geo &g) const { return base + size_t(slot) * g.n; }
+  __device__ R *sp(int slot, const Geo &g) const { return spec + size_t(slot) * g.nz * g.nm * 2; }
+  __device__ double *m0(int slot, const Geo &g) const { return mean0 + size_t(slot) * g.nz; }
+  __device__ float *col(int slot, int k, const Geo &g) const { return cols + (size_t(slot) * N_COLUMNS + k) * g.layer; }
+  __device__ double *lv(int slot, int k, const Geo &g) const { return lev + (size_t(slot) * N_LEVELS + k) * g.nz; }
+  __device__ double *pr(int slot) const { return par + size_t(slot) * N_PARAMS; }
+  __device__ double *sc(int slot) const { return scal + size_t(slot) * N_SCALARS; }
+};
+
+// ---- wave64 / block reductions --------------------------------------------------------------------------------
+template <class T> __device__ inline T waveSum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <class T> __device__ inline T waveMax(T v) {
+  for (int o = 32; o > 0; o >>= 1) { T w = __shfl_xor(v, o, 64); v = w > v ? w : v; }
+  return v;
+}
+// Sum over a block of up to 1024 threads (blockDim.x a multiple of 64); every thread gets the total.
+template <class T> __device__ inline T blockSum(T v, T *shared) {
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+  v = waveSum(v);
+  __syncthreads();
+  if (lane == 0) shared[wave] = v;
+  __syncthreads();
+  T t = 0;
+  for (int w = 0; w < waves; w++) t += shared[w];
+  return t;
+}
+template <class T> __device__ inline T blockMax(T v, T *shared) {
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+  v = waveMax(v);
+  __syncthreads();
+  if (lane == 0) shared[wave] = v;
+  __syncthreads();
+  T t = shared[0];
+  for (int w = 1; w < waves; w++) t = shared[w] > t ? shared[w] : t;
+  return t;
+}
+
+}  // namespace ml

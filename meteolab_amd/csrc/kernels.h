@@ -1,0 +1,595 @@
+// Data-parallel kernels of the StormLab step, batched over runs: blockIdx.y (or .z) indexes the active-slot list,
+// blockIdx.x tiles a run. R is the arithmetic type (float: production; double: the float64 variant of the same
+// kernels). Fields are float32 in both. Each kernel names the reference function it computes.
+#pragma once
+#include "common.h"
+#include "physics.h"
+
+namespace ml {
+
+constexpr int TPB = 256;  // threads per block of the per-node kernels (four wave64)
+
+__device__ inline int idiv(int a, int b) { return a / b; }
+
+// ---- step start: clear the per-step scalars of each run ----------------------------------------------------------
+template <class R> __global__ void k_begin(Geo g, Bufs<R> b) {
+  int slot = b.runs[blockIdx.x];
+  double *s = b.sc(slot);
+  for (int t = threadIdx.x; t < S_STEP_END; t += blockDim.x) s[t] = 0;
+}
+
+// ---- base state (src/core/environment.ts:45-73, profiles.ts:61-73) in float64, one block per run -----------------
+__device__ inline double profTheta(const double *P, double z) {
+  return z <= P[P_ZTR] ? P[P_THETA0] + P[P_DTHETA] * pow(z / P[P_ZTR], 1.25) : (P[P_THETA0] + P[P_DTHETA]) * exp(G * (z - P[P_ZTR]) / (CP * P[P_TTR]));
+}
+__device__ inline double profRh(const double *P, double z) { return z <= P[P_ZTR] ? 1 - P[P_RH_DROP] * pow(z / P[P_ZTR], 1.25) : P[P_RH_TOP]; }
+__device__ inline double tempAt(const double *P, double z, double p) { return profTheta(P, z) * pow(p / 100000, KAPPA) - 273.15; }
+__device__ inline double mixRatio(const double *P, double z, double t, double p) { double r = profRh(P, z) * qsatP(t, p); return r < P[P_QVMAX] ? r : P[P_QVMAX]; }
+__device__ inline double virtualT(const double *P, double z, double p) { double t = tempAt(P, z, p); return (t + 273.15) * (1 + .61 * mixRatio(P, z, t, p)); }
+// The hydrostatic integration every 10 m (environment.ts:64-73) is a serial recurrence: thread 0 of the block runs it
+// into shared memory, then the levels are read off in parallel.
+template <class R> __global__ void k_base_state(Geo g, Bufs<R> b) {
+  int slot = b.runs[blockIdx.x];
+  const double *P = b.pr(slot);
+  extern __shared__ double lnP[];
+  double top = g.height + 2000; int count = int(ceil(top / 10)) + 1;
+  if (threadIdx.x == 0) {
+    lnP[0] = log(101325.0);
+    for (int k = 0; k + 1 < count; k++) {
+      double z = k * 10.0, p = exp(lnP[k]);
+      lnP[k + 1] = lnP[k] - G * 10 / (RD * .5 * (virtualT(P, z, p) + virtualT(P, z + 10, p)));
+    }
+  }
+  __syncthreads();
+  for (int z = threadIdx.x; z < g.nz; z += blockDim.x) {
+    double alt = b.zs[z], x = (alt > 0 ? alt : 0) / 10; int k = int(floor(x)); if (k > count - 2) k = count - 2;
+    double p = exp(lnP[k] + (lnP[k + 1] - lnP[k]) * (x - k)), exner = pow(p / 100000, KAPPA);
+    double tEnv = tempAt(P, alt, p), theta = (tEnv + 273.15) * pow(100000 / p, KAPPA), q = mixRatio(P, alt, tEnv, p);
+    double u = P[P_SHEAR] * tanh(alt / P[P_SHEAR_DEPTH]), v = 0, rho = p / (RD * theta * exner * (1 + .61 * q));
+    b.lv(slot, L_P, g)[z] = p; b.lv(slot, L_EXNER, g)[z] = exner; b.lv(slot, L_THETA, g)[z] = theta; b.lv(slot, L_Q, g)[z] = q;
+    b.lv(slot, L_U, g)[z] = u; b.lv(slot, L_V, g)[z] = v; b.lv(slot, L_RHO, g)[z] = rho;
+    b.lv(slot, L_UENV, g)[z] = u - P[P_FRAME_U]; b.lv(slot, L_VENV, g)[z] = v - P[P_FRAME_V];
+    b.lv(slot, L_LW, g)[z] = b.hz[z] / g.dz * rho;                       // levelWeight (model.ts:362)
+    b.lv(slot, L_EINIT, g)[z] = 2.84e5 * rho / 1.225;                     // lightning.ts:172
+    double L = cbrt(g.dx * g.dy * b.hz[z]);
+    b.lv(slot, L_CAPACITANCE, g)[z] = .25 * 2 * PI * EPSILON0 * L / log(L / 1.0);  // lightning.ts:175
+  }
+}
+
+// ---- initial state (model.ts:399-451 in single-storm mode), the run's thermal and its theta noise -----------------
+template <class R> __global__ void k_init(Geo g, Bufs<R> b) {
+  int slot = b.runs[blockIdx.y];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  int z = i / g.layer, y = (i - z * g.layer) / g.nx, x = i - z * g.layer - y * g.nx;
+  const double *P = b.pr(slot);
+  for (int f = 0; f < N_FIELDS; f++) b.f(slot, f, g)[i] = 0;
+  b.f(slot, F_U, g)[i] = float(b.lv(slot, L_U, g)[z] - P[P_FRAME_U]);
+  b.f(slot, F_V, g)[i] = float(b.lv(slot, L_V, g)[z] - P[P_FRAME_V]);
+  float th = float(b.lv(slot, L_THETA, g)[z]);
+  double ax = (x * g.dx - P[P_BUBBLE_X]) / P[P_BUBBLE_RH], ay = (y * g.dy - P[P_BUBBLE_Y]) / P[P_BUBBLE_RH], az = (b.zs[z] - P[P_BUBBLE_Z]) / P[P_BUBBLE_RZ];
+  double beta = sqrt(ax * ax + ay * ay + az * az);
+  if (beta < 1) { double c = cos(PI * beta / 2); th = float(th + P[P_BUBBLE_AMP] * (c * c)); }
+  // Draw k of mulberry32(noise seed) goes to the k-th node below the noise top in node order: the nodes below it are
+  // the first levels, so that is node i (draw i + 1).
+  if (b.zs[z] < P[P_NOISE_TOP]) th = float(th + (2 * mulberryAt(uint32_t(int(P[P_NOISE_SEED])), uint32_t(i + 1)) - 1) * P[P_NOISE_AMP]);
+  b.f(slot, F_THETA, g)[i] = th;
+  b.f(slot, F_Q, g)[i] = float(b.lv(slot, L_Q, g)[z]);
+  if (i < g.layer) for (int k = 0; k < N_COLUMNS; k++) b.col(slot, k, g)[i] = 0;
+}
+
+// ---- fall speeds, Courant numbers and the hail swath (model.ts:1220-1235 'fall', 575-580 fallParts, 697-706) --------
+template <class R> __global__ __launch_bounds__(TPB) void k_fall(Geo g, Bufs<R> b, double dt, int electrified) {
+  int slot = b.runs[blockIdx.y];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double cour[N_FALLS] = {0, 0, 0, 0, 0, 0, 0};
+  if (i < g.n) {
+    int z = i / g.layer;
+    const double *rho = b.lv(slot, L_RHO, g);
+    R rz = R(rho[z]), r0 = R(rho[0]);
+    float rain = b.f(slot, F_RAIN, g)[i], snow = b.f(slot, F_SNOW, g)[i], graupel = b.f(slot, F_GRAUPEL, g)[i], hail = b.f(slot, F_HAIL, g)[i];
+    float vr = float(fallSpeedRain(R(rain), rz, r0)), vs = float(snowFallSpeed(R(snow), rz, r0));
+    float vg = float(graupelFallSpeed(R(graupel), rz)), vh = float(hailFallSpeed(R(hail), rz));
+    b.f(slot, F_FALL_RAIN, g)[i] = vr; b.f(slot, F_FALL_SNOW, g)[i] = vs; b.f(slot, F_FALL_GRAUPEL, g)[i] = vg; b.f(slot, F_FALL_HAIL, g)[i] = vh;
+    // fallParts: the Courant number c vt of each fall where its field is non-zero (float64, as it decides the sub-steps).
+    double c = dt * rho[z] / (b.lv(slot, L_LW, g)[z] * g.dz);
+    cour[0] = rain != 0 ? c * vr : 0; cour[1] = snow != 0 ? c * vs : 0; cour[2] = graupel != 0 ? c * vg : 0; cour[3] = hail != 0 ? c * vh : 0;
+    if (electrified) {
+      float ice = b.f(slot, F_ICE, g)[i];
+      R is = R(ice) + R(snow), gh = R(graupel) + R(hail);
+      float vci = float(is > 0 ? R(snow) * R(vs) / is : R(0));
+      float vcg = float(gh > 0 ? (R(graupel) * R(vg) + R(hail) * R(vh)) / gh : R(0));
+      b.f(slot, F_FALL_CI, g)[i] = vci; b.f(slot, F_FALL_CG, g)[i] = vcg;
+      cour[4] = b.f(slot, F_CHARGE_ICE, g)[i] != 0 ? c * vci : 0;
+      cour[5] = b.f(slot, F_CHARGE_GRAUPEL, g)[i] != 0 ? c * vcg : 0;
+      cour[6] = b.f(slot, F_CHARGE_RAIN, g)[i] != 0 ? c * vr : 0;
+    }
+    // collectHail: the swath at the ground node before the sedimentation moves the hail.
+    if (z == 0 && hail > 1e-9f) {
+      float *hg = b.col(slot, C_HAIL_GROUND, g), *hs = b.col(slot, C_HAIL_SIZE, g);
+      hg[i] = float(hg[i] + R(r0 * R(hail) * R(vh) * R(dt)));
+      R size = hailMaxDiameter(R(hail), r0) * 1000;
+      hs[i] = float(mmax(R(hs[i]), size));
+    }
+  }
+  __shared__ double part[N_FALLS][TPB / 64];
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int k = 0; k < N_FALLS; k++) { double m = waveMax(cour[k]); if (lane == 0) part[k][wave] = m; }
+  __syncthreads();
+  if (threadIdx.x < N_FALLS) {
+    double m = 0;
+    for (int w = 0; w < TPB / 64; w++) m = part[threadIdx.x][w] > m ? part[threadIdx.x][w] : m;
+    // Non-negative doubles order as their bit patterns.
+    if (m > 0) atomicMax(reinterpret_cast<unsigned long long *>(b.sc(slot) + S_COURANT + threadIdx.x), (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// ---- sedimentation: flux-form upwind fall per column with sub-steps (model.ts:565-598, sedimentStep 928-932) -------
+// One thread per column; the column of the falling field and its speed are staged in LDS, so the bottom-up sweep of
+// each sub-step reads no memory.
+constexpr int SED_COLS = 64;
+template <class R> __global__ __launch_bounds__(SED_COLS) void k_sediment(Geo g, Bufs<R> b, double dt, int electrified) {
+  int slot = b.runs[blockIdx.y];
+  int col = blockIdx.x * SED_COLS + threadIdx.x;
+  extern __shared__ float sed[];
+  float *qa = sed, *va = sed + SED_COLS * g.nz;  // [z][thread]
+  const double *rho = b.lv(slot, L_RHO, g), *lw = b.lv(slot, L_LW, g);
+  const int fields[N_FALLS] = {F_RAIN, F_SNOW, F_GRAUPEL, F_HAIL, F_CHARGE_ICE, F_CHARGE_GRAUPEL, F_CHARGE_RAIN};
+  const int falls[N_FALLS] = {F_FALL_RAIN, F_FALL_SNOW, F_FALL_GRAUPEL, F_FALL_HAIL, F_FALL_CI, F_FALL_CG, F_FALL_RAIN};
+  double charge = 0;
+  bool live = col < g.layer;
+  for (int k = 0; k < (electrified ? N_FALLS : 4); k++) {
+    double courant = __longlong_as_double((long long)reinterpret_cast<unsigned long long *>(b.sc(slot))[S_COURANT + k]);
+    // No node moves anything (every non-zero value has zero speed): the fall changes nothing.
+    if (!(courant > 0)) continue;
+    int parts = int(ceil(courant / .8)); if (parts < 1) parts = 1;
+    R pdt = R(dt / parts);
+    float *a = b.f(slot, fields[k], g) + col; const float *vt = b.f(slot, falls[k], g) + col;
+    if (live) for (int z = 0; z < g.nz; z++) { qa[z * SED_COLS + threadIdx.x] = a[size_t(z) * g.layer]; va[z * SED_COLS + threadIdx.x] = vt[size_t(z) * g.layer]; }
+    if (live) {
+      bool deposit = k < 4;
+      float *prec = b.col(slot, C_PRECIPITATION, g);
+      for (int p = 0; p < parts; p++) {
+        float a0 = qa[threadIdx.x], v0 = va[threadIdx.x];
+        if (deposit) prec[col] = float(prec[col] + R(rho[0]) * R(a0) * R(v0) * pdt);
+        else charge += double(a0) * v0 * rho[0] * double(pdt) * g.dx * g.dy;
+        float here = a0, hv = v0;
+        for (int z = 0; z < g.nz; z++) {
+          R inflow = 0; float above = 0, av = 0;
+          if (z < g.nz - 1) { above = qa[(z + 1) * SED_COLS + threadIdx.x]; av = va[(z + 1) * SED_COLS + threadIdx.x]; inflow = R(rho[z + 1]) * (R(above) * R(av)); }
+          qa[z * SED_COLS + threadIdx.x] = float(R(here) + pdt * (inflow - R(rho[z]) * R(here) * R(hv)) / R(lw[z] * g.dz));
+          here = above; hv = av;
+        }
+      }
+      for (int z = 0; z < g.nz; z++) a[size_t(z) * g.layer] = qa[z * SED_COLS + threadIdx.x];
+    }
+  }
+  if (electrified) {
+    __shared__ double red[SED_COLS / 64];
+    double t = blockSum(charge, red);
+    if (threadIdx.x == 0 && t != 0) atomicAdd(b.sc(slot) + S_CHARGE_GROUND, t);
+  }
+}
+
+// ---- face mass fluxes and their divergence (advection.ts:190-211 faces + divergence) --------------------------------
+template <class R> __device__ inline void faceFluxes(const Geo &g, const double *rho, const float *u, const float *v, const float *w, int x, int y, int z, R &mx, R &my, R &mz) {
+  int i = x + g.nx * (y + g.ny * z), l = z * g.layer;
+  int xp = x + 1 == g.nx ? 0 : x + 1, yp = y + 1 == g.ny ? 0 : y + 1;
+  R r = R(rho[z]);
+  mx = r * (R(u[i]) + R(u[xp + y * g.nx + l])) / 2;
+  my = r * (R(v[i]) + R(v[x + yp * g.nx + l])) / 2;
+  mz = z < g.nz - 1 ? R((rho[z] + rho[z + 1]) / 2) * (R(w[i]) + R(w[i + g.layer])) / 2 : R(0);
+}
+// Face potential correction (advection.ts:244-255 correctFaces) of the faces of node (x, y, z).
+template <class R> __device__ inline void correctedFaces(const Geo &g, const Bufs<R> &b, const double *rho, const float *u, const float *v, const float *w, const R *phi, int x, int y, int z, R &mx, R &my, R &mz) {
+  faceFluxes(g, rho, u, v, w, x, y, z, mx, my, mz);
+  int i = x + g.nx * (y + g.ny * z), l = z * g.layer;
+  int xp = x + 1 == g.nx ? 0 : x + 1, yp = y + 1 == g.ny ? 0 : y + 1;
+  R r = R(rho[z]);
+  mx -= r * (phi[xp + y * g.nx + l] - phi[i]) / R(g.dx);
+  my -= r * (phi[x + yp * g.nx + l] - phi[i]) / R(g.dy);
+  if (z < g.nz - 1) mz -= R((rho[z] + rho[z + 1]) / 2) * (phi[i + g.layer] - phi[i]) / R(b.dzs[z]);
+}
+// corrected = 0: faces and divergence of the node velocities (the right-hand side of the face projection);
+// corrected = 1: the projected faces (faces minus the potential's gradient) and their divergence.
+template <class R> __global__ __launch_bounds__(TPB) void k_faces(Geo g, Bufs<R> b, int corrected) {
+  int slot = b.runs[blockIdx.y];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  int z = i / g.layer, y = (i - z * g.layer) / g.nx, x = i - z * g.layer - y * g.nx;
+  const double *rho = b.lv(slot, L_RHO, g);
+  const float *u = b.f(slot, F_U, g), *v = b.f(slot, F_V, g), *w = b.f(slot, F_W, g);
+  const R *phi = b.arr(b.phi, slot, g);
+  int xm = x == 0 ? g.nx - 1 : x - 1, ym = y == 0 ? g.ny - 1 : y - 1;
+  R mx, my, mz, mxw, myd, mzb = 0, t0, t1;
+  if (corrected) {
+    correctedFaces(g, b, rho, u, v, w, phi, x, y, z, mx, my, mz);
+    correctedFaces(g, b, rho, u, v, w, phi, xm, y, z, mxw, t0, t1);
+    correctedFaces(g, b, rho, u, v, w, phi, x, ym, z, t0, myd, t1);
+    if (z > 0) correctedFaces(g, b, rho, u, v, w, phi, x, y, z - 1, t0, t1, mzb);
+  } else {
+    faceFluxes(g, rho, u, v, w, x, y, z, mx, my, mz);
+    faceFluxes(g, rho, u, v, w, xm, y, z, mxw, t0, t1);
+    faceFluxes(g, rho, u, v, w, x, ym, z, t0, myd, t1);
+    if (z > 0) faceFluxes(g, rho, u, v, w, x, y, z - 1, t0, t1, mzb);
+  }
+  b.m(slot, 0, g)[i] = mx; b.m(slot, 1, g)[i] = my; b.m(slot, 2, g)[i] = mz;
+  R div = (mx - mxw) / R(g.dx) + (my - myd) / R(g.dy) + (mz - mzb) / R(b.hz[z]);
+  // The right-hand side of the face projection is the divergence itself (modes multiplies by h).
+  if (corrected) b.m(slot, 3, g)[i] = div; else b.arr(b.cells, slot, g)[i] = div;
+}
+
+// ---- transport: flux-form WENO5 / linear upwind tendency and the RK3 stage update --------------------------------
+// (advection.ts:294-399 tendencyWeno/tendencyLinear/weno/upwind5, model.ts:1237-1251 the stage update, 1204-1214 the
+// buoyancy in the w tendency, 1045 the rigid walls of the stages)
+//
+// WENO5 weights without underflow in float: the stencil is divided by its largest magnitude s (the weights are
+// scale-free but for the 1e-40 in the reference's epsilon, which becomes 1e-40 / s^2 and is kept), and the weights are
+// taken relative to epsilon, w_k = gamma_k (eps / b_k)^2, which are bounded by gamma_k.
+template <class R> __device__ inline R wenoFace(R a, R b_, R c, R d, R e) {
+  R s = mmax(mmax(mmax(mabs(a), mabs(b_)), mmax(mabs(c), mabs(d))), mabs(e));
+  if (s == 0) return 0;
+  R is = 1 / s;
+  a *= is; b_ *= is; c *= is; d *= is; e *= is;
+  R p0 = (2 * a - 7 * b_ + 11 * c) / 6, p1 = (-b_ + 5 * c + 2 * d) / 6, p2 = (2 * c + 5 * d - e) / 6;
+  R s0 = a - 2 * b_ + c, t0 = a - 4 * b_ + 3 * c, s1 = b_ - 2 * c + d, t1 = b_ - d, s2 = c - 2 * d + e, t2 = 3 * c - 4 * d + e;
+  R tiny = mmin(R(1e-20) * is, R(1e15));
+  R eps = R(2e-7) * (a * a + b_ * b_ + c * c + d * d + e * e) + tiny * tiny;
+  R b0 = eps + R(13.0 / 12) * s0 * s0 + R(.25) * t0 * t0, b1 = eps + R(13.0 / 12) * s1 * s1 + R(.25) * t1 * t1, b2 = eps + R(13.0 / 12) * s2 * s2 + R(.25) * t2 * t2;
+  R r0 = eps / b0, r1 = eps / b1, r2 = eps / b2;
+  R w0 = R(.1) * r0 * r0, w1 = R(.6) * r1 * r1, w2 = R(.3) * r2 * r2;
+  return s * ((w0 * p0 + w1 * p1 + w2 * p2) / (w0 + w1 + w2));
+}
+template <class R> __device__ inline R upwindFace(R a, R b_, R c, R d, R e) { return (2 * a - 13 * b_ + 47 * c + 27 * d - 3 * e) / 60; }
+template <class R> __device__ inline R faceValue(bool monotone, R a, R b_, R c, R d, R e) { return monotone ? wenoFace(a, b_, c, d, e) : upwindFace(a, b_, c, d, e); }
+// Flux through the face between node values q0 (this node) and q1 (next), with upstream neighbours.
+// Flow towards +: stencil (m2, m1, q0, p1, p2); towards -: (p3, p2, p1, q0, m1).
+template <class R> __device__ inline R faceFlux(bool monotone, R m, R qm2, R qm1, R q0, R qp1, R qp2, R qp3) {
+  return m >= 0 ? m * faceValue(monotone, qm2, qm1, q0, qp1, qp2) : m * faceValue(monotone, qp3, qp2, qp1, q0, qm1);
+}
+__device__ inline bool monotoneField(int f) { return !(f == F_U || f == F_V || f == F_W || f == F_THETA); }
+
+// One block per (level, field, run): the level of the field is staged in LDS (periodic in x and y, so the whole level
+// is its own halo), the x and y face fluxes of the level go to LDS once, the vertical stencil comes from memory.
+// stage 0: state -> X, stage 1: X -> Y, stage 2: Y -> state (each node reads its own start value first).
+template <class R> __global__ __launch_bounds__(TPB) void k_stage(Geo g, Bufs<R> b, int stage, double dt) {
+  int slot = b.runs[blockIdx.z], z = blockIdx.x, f = blockIdx.y;
+  int inSet = stage, outSet = stage == 2 ? 0 : stage + 1;
+  const float *q = b.set(inSet, slot, f, g);
+  float *out = b.set(outSet, slot, f, g);
+  const float *start = b.f(slot, f, g);
+  const int layer = g.layer, nx = g.nx, ny = g.ny, nz = g.nz;
+  extern __shared__ char smem[];
+  float *ql = reinterpret_cast<float *>(smem);
+  R *fxl = reinterpret_cast<R *>(smem + ((layer * sizeof(float) + 15) / 16) * 16), *fyl = fxl + layer;
+  const float *qz = q + size_t(z) * layer;
+  for (int c = threadIdx.x; c < layer; c += TPB) ql[c] = qz[c];
+  __syncthreads();
+  const bool mono = monotoneField(f);
+  const R *mx = b.m(slot, 0, g) + size_t(z) * layer, *my = b.m(slot, 1, g) + size_t(z) * layer;
+  for (int c = threadIdx.x; c < layer; c += TPB) {
+    int y = c / nx, x = c - y * nx, row = y * nx;
+    auto X = [&](int o) { int xx = x + o; xx += xx < 0 ? nx : 0; xx -= xx >= nx ? nx : 0; return R(ql[row + xx]); };
+    auto Y = [&](int o) { int yy = y + o; yy += yy < 0 ? ny : 0; yy -= yy >= ny ? ny : 0; return R(ql[yy * nx + x]); };
+    fxl[c] = faceFlux(mono, mx[c], X(-2), X(-1), X(0), X(1), X(2), X(3));
+    fyl[c] = faceFlux(mono, my[c], Y(-2), Y(-1), Y(0), Y(1), Y(2), Y(3));
+  }
+  __syncthreads();
+  const double *rho = b.lv(slot, L_RHO, g);
+  const R rz = R(rho[z]), hz = R(b.hz[z]), frac = R((stage == 0 ? 1.0 / 3 : stage == 1 ? 1.0 / 2 : 1.0) * dt);
+  const R *mz = b.m(slot, 2, g), *div = b.m(slot, 3, g);
+  const bool isW = f == F_W;
+  double thvEnv = 0;
+  if (isW) { const double te = b.lv(slot, L_THETA, g)[z], qe = b.lv(slot, L_Q, g)[z]; thvEnv = te * (1 + .61 * qe); }
+  int zc[7];
+  for (int k = 0; k < 7; k++) { int zz = z - 3 + k; zc[k] = (zz < 0 ? 0 : zz > nz - 1 ? nz - 1 : zz) * layer; }
+  for (int c = threadIdx.x; c < layer; c += TPB) {
+    int y = c / nx, x = c - y * nx;
+    int i = z * layer + c;
+    R col[7];
+    for (int k = 0; k < 7; k++) col[k] = k == 3 ? R(ql[c]) : R(q[zc[k] + c]);
+    // Top face of this node (none through the model top) and of the node below (none into the ground).
+    R fzt = z == nz - 1 ? R(0) : faceFlux(mono, mz[i], col[1], col[2], col[3], col[4], col[5], col[6]);
+    R fzb = z == 0 ? R(0) : faceFlux(mono, mz[i - layer], col[0], col[1], col[2], col[3], col[4], col[5]);
+    int xm = x == 0 ? nx - 1 : x - 1, ym = y == 0 ? ny - 1 : y - 1;
+    R flux = (fxl[c] - fxl[y * nx + xm]) / R(g.dx) + (fyl[c] - fyl[ym * nx + x]) / R(g.dy) + (fzt - fzb) / hz;
+    R tend = -(flux - R(ql[c]) * div[i]) / rz;
+    if (isW) {
+      // g B with the fields of this stage; the departure of theta_v from the environment in float64 (it is a small
+      // difference of numbers near 300 K).
+      const float th = b.set(inSet, slot, F_THETA, g)[i], qv = b.set(inSet, slot, F_Q, g)[i];
+      R loaded = R(b.set(inSet, slot, F_CLOUD, g)[i]) + R(b.set(inSet, slot, F_ICE, g)[i])
+        + (R(b.set(inSet, slot, F_RAIN, g)[i]) + R(b.set(inSet, slot, F_SNOW, g)[i]) + R(b.set(inSet, slot, F_GRAUPEL, g)[i]) + R(b.set(inSet, slot, F_HAIL, g)[i]));
+      double dthv = (double(th) * (1 + .61 * double(qv)) - thvEnv) / thvEnv;
+      tend += R(G) * (R(dthv) - loaded);
+    }
+    float val = float(R(start[i]) + frac * tend);
+    if (isW && stage < 2 && (z == 0 || z == nz - 1)) val = 0;
+    out[i] = val;
+  }
+}
+
+// ---- pressure projection: dual-cell divergence and the node-velocity correction (pressure.ts:50-92) --------------------
+// set: which buffer set holds u, v, w (0 state, 1 X, 2 Y).
+template <class R> __global__ __launch_bounds__(TPB) void k_pdiv(Geo g, Bufs<R> b, int set, double dt) {
+  int slot = b.runs[blockIdx.y];
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int cells = g.layer * (g.nz - 1);
+  if (c >= cells) return;
+  int k = c / g.layer, j = (c - k * g.layer) / g.nx, i = c - k * g.layer - j * g.nx;
+  const float *u = b.set(set, slot, F_U, g), *v = b.set(set, slot, F_V, g), *w = b.set(set, slot, F_W, g);
+  const double *rho = b.lv(slot, L_RHO, g);
+  int l0 = k * g.layer, l1 = l0 + g.layer, y0 = j * g.nx, y1 = (j + 1 == g.ny ? 0 : j + 1) * g.nx, i1 = i + 1 == g.nx ? 0 : i + 1;
+  int a = i + y0 + l0, bb = i1 + y0 + l0, e = i + y1 + l0, f = i1 + y1 + l0, gg = i + y0 + l1, h = i1 + y0 + l1, s = i + y1 + l1, t = i1 + y1 + l1;
+  R r0 = R(rho[k] / 4), r1 = R(rho[k + 1] / 4);
+  R ddx = (r0 * (R(u[bb]) + u[f] - u[a] - u[e]) + r1 * (R(u[h]) + u[t] - u[gg] - u[s])) / R(g.dx);
+  R ddy = (r0 * (R(v[e]) + v[f] - v[a] - v[bb]) + r1 * (R(v[s]) + v[t] - v[gg] - v[h])) / R(g.dy);
+  R ddz = (r1 * (R(w[gg]) + w[h] + w[s] + w[t]) - r0 * (R(w[a]) + w[bb] + w[e] + w[f])) / R(b.dzs[k]);
+  b.arr(b.cells, slot, g)[c] = (ddx + ddy + ddz) / R(dt);
+}
+// final = 0: the stage projection, in place on set `set`. final = 1: the end-of-step projection of the mixed wind in X
+// into the state, with the safety limits (model.ts:1586-1595 clipWind), and the other mixed fields copied from X.
+template <class R> __global__ __launch_bounds__(TPB) void k_pcorrect(Geo g, Bufs<R> b, int set, double dt, int final) {
+  int slot = b.runs[blockIdx.y];
+  int node = blockIdx.x * blockDim.x + threadIdx.x;
+  bool live = node < g.n;
+  unsigned long long clipped = 0;
+  if (live) {
+    int k = node / g.layer, j = (node - k * g.layer) / g.nx, i = node - k * g.layer - j * g.nx;
+    const R *p = b.arr(b.press, slot, g);
+    int cl = g.layer, nz = g.nz;
+    int lo = (k > 0 ? k - 1 : 0) * cl, hi = (k < nz - 2 ? k : nz - 2) * cl;
+    R wLo = k > 0 ? R(b.dzs[k - 1] / b.hz[k]) : R(0), wHi = k < nz - 1 ? R(b.dzs[k] / b.hz[k]) : R(0);
+    int jm = (j == 0 ? g.ny - 1 : j - 1) * g.nx, j0 = j * g.nx, im = i == 0 ? g.nx - 1 : i - 1;
+    auto P = [&](int ii, int jj, int base) { return p[base + ii + jj]; };
+    R gx = (wLo * (P(i, jm, lo) + P(i, j0, lo) - P(im, jm, lo) - P(im, j0, lo)) + wHi * (P(i, jm, hi) + P(i, j0, hi) - P(im, jm, hi) - P(im, j0, hi))) / (4 * R(g.dx));
+    R gy = (wLo * (P(im, j0, lo) + P(i, j0, lo) - P(im, jm, lo) - P(i, jm, lo)) + wHi * (P(im, j0, hi) + P(i, j0, hi) - P(im, jm, hi) - P(i, jm, hi))) / (4 * R(g.dy));
+    float *u = b.set(set, slot, F_U, g), *v = b.set(set, slot, F_V, g), *w = b.set(set, slot, F_W, g);
+    float un = float(R(u[node]) - R(dt) * gx), vn = float(R(v[node]) - R(dt) * gy), wn = w[node];
+    if (k > 0 && k < nz - 1) {
+      int below = (k - 1) * cl, above = k * cl;
+      R gz = (P(im, jm, above) + P(i, jm, above) + P(im, j0, above) + P(i, j0, above) - P(im, jm, below) - P(i, jm, below) - P(im, j0, below) - P(i, j0, below)) / (4 * R(b.hz[k]));
+      wn = float(R(wn) - R(dt) * gz);
+    }
+    if (final) {
+      if (un < -120.f || un > 120.f) { un = un < -120.f ? -120.f : 120.f; clipped++; }
+      if (vn < -120.f || vn > 120.f) { vn = vn < -120.f ? -120.f : 120.f; clipped++; }
+      if (wn < -100.f || wn > 100.f) { wn = wn < -100.f ? -100.f : 100.f; clipped++; }
+      b.f(slot, F_U, g)[node] = un; b.f(slot, F_V, g)[node] = vn; b.f(slot, F_W, g)[node] = wn;
+      const int others[5] = {F_THETA, F_Q, F_CLOUD, F_ICE, F_ICENUMBER};
+      for (int o = 0; o < 5; o++) b.f(slot, others[o], g)[node] = b.x(slot, others[o], g)[node];
+    } else {
+      u[node] = un; v[node] = vn; w[node] = wn;
+    }
+  }
+  if (final) {
+    unsigned long long any = __ballot(clipped != 0);
+    if (any) {
+      __shared__ unsigned long long red[TPB / 64];
+      unsigned long long t = blockSum(clipped, red);
+      if (threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long *>(b.sc(slot) + S_CLIPPED), t);
+    }
+  }
+}
+
+// ---- positivity fix sums (model.ts:606-617 fillNegative): removed, kept, negative of each conserved field ------------
+__device__ inline int conservedField(int k) { const int c[N_CONSERVED] = {F_RAIN, F_SNOW, F_GRAUPEL, F_HAIL, F_Q, F_CLOUD, F_ICE}; return c[k]; }
+constexpr double TINY_WATER = 1e-10;
+template <class R> __global__ __launch_bounds__(TPB) void k_fill_sums(Geo g, Bufs<R> b) {
+  int slot = b.runs[blockIdx.y];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[N_CONSERVED][3];
+  for (int k = 0; k < N_CONSERVED; k++) acc[k][0] = acc[k][1] = acc[k][2] = 0;
+  if (i < g.n) {
+    double wz = b.lv(slot, L_LW, g)[i / g.layer];
+    for (int k = 0; k < N_CONSERVED; k++) {
+      double x = b.f(slot, conservedField(k), g)[i];
+      if (x < TINY_WATER) { if (x != 0) { acc[k][0] = wz * x; if (x < 0) acc[k][2] = -wz * x; } }
+      else acc[k][1] = wz * x;
+    }
+  }
+  __shared__ double red[TPB / 64];
+  for (int k = 0; k < N_CONSERVED; k++) for (int j = 0; j < 3; j++) {
+    double t = blockSum(acc[k][j], red);
+    if (threadIdx.x == 0 && t != 0) atomicAdd(b.sc(slot) + S_FILL + 3 * k + j, t);
+  }
+}
+
+// ---- column physics of every node (model.ts:1116-1127 'finish' + physics(z) 1347-1531) ------------------------------
+// The positivity fix of the conserved fields and the decay of the cold-pool indicator come first (the 'finish'
+// phase), then the node's microphysics, charging, Coriolis, wind relaxation and the sponge.
+template <class R> __global__ __launch_bounds__(TPB) void k_physics(Geo g, Bufs<R> b, double dt, int electrified) {
+  int slot = b.runs[blockIdx.y];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  int z = i / g.layer;
+  double *S = b.sc(slot);
+  float F[N_CARRIED];
+  int nc = electrified ? N_CARRIED : F_COLD + 1;
+  for (int k = 0; k < nc; k++) F[k] = b.f(slot, k, g)[i];
+  for (int k = nc; k < N_CARRIED; k++) F[k] = 0;
+  // 'finish': fillNegative with the domain sums, then cold *= .9992^dt.
+  for (int k = 0; k < N_CONSERVED; k++) {
+    double removed = S[S_FILL + 3 * k], kept = S[S_FILL + 3 * k + 1], negative = S[S_FILL + 3 * k + 2];
+    if (removed == 0 && negative == 0) continue;
+    double scale = kept > 0 ? fmax(0.0, 1 + removed / kept) : 0;
+    float &a = F[conservedField(k)];
+    a = a < float(TINY_WATER) ? 0.f : float(double(a) * scale);
+  }
+  F[F_COLD] = float(double(F[F_COLD]) * pow(.9992, dt));
+  const double *P = b.pr(slot);
+  const double alt = b.zs[z], H = g.height;
+  const R p = R(b.lv(slot, L_P, g)[z]), rhoZ = R(b.lv(slot, L_RHO, g)[z]), exner = R(b.lv(slot, L_EXNER, g)[z]), rhoGround = R(b.lv(slot, L_RHO, g)[0]);
+  const double ue = b.lv(slot, L_UENV, g)[z], ve = b.lv(slot, L_VENV, g)[z], thEnv = b.lv(slot, L_THETA, g)[z];
+  const double spongeStart = P[P_SPONGE_START], keep = exp(-1e-4 * dt), fc = P[P_CORIOLIS];
+  const R Rdt = R(dt);
+  float &u = F[F_U], &v = F[F_V], &w = F[F_W], &theta = F[F_THETA], &q = F[F_Q], &cloud = F[F_CLOUD], &rain = F[F_RAIN], &cold = F[F_COLD];
+  float &ice = F[F_ICE], &iceNumber = F[F_ICENUMBER], &snow = F[F_SNOW], &graupel = F[F_GRAUPEL], &hail = F[F_HAIL];
+  float &cI = F[F_CHARGE_ICE], &cG = F[F_CHARGE_GRAUPEL], &cR = F[F_CHARGE_RAIN], &cA = F[F_CHARGE_AIR];
+  R before[4], after[4], cNow[4];
+  if (electrified) { before[0] = R(ice) + R(snow); before[1] = R(graupel) + R(hail); before[2] = rain; before[3] = cloud; }
+  R evap = rainProcesses(q, cloud, rain, R(theta) * exner, p, rhoZ, Rdt);
+  if (evap > 0) { R cool = R(LV / CP) / exner * evap; theta = float(R(theta) - cool); cold = float(R(cold) + cool); }
+  freezeRain(theta, rain, graupel, exner);
+  settleCrystals<R>(ice, iceNumber);
+  cold = float(R(cold) + iceProcesses(theta, q, ice, snow, iceNumber, exner, p, rhoZ, Rdt));
+  R iceBefore = ice, cloudBefore = cloud;
+  cold = float(R(cold) + snowProcesses(theta, q, cloud, ice, snow, rain, exner, p, rhoZ, rhoGround, Rdt));
+  R rainBefore = rain;
+  cold = float(R(cold) + graupelProcesses(theta, q, cloud, ice, snow, rain, graupel, exner, p, rhoZ, Rdt));
+  R rainFrozen = R(theta) * exner < R(273.15) ? mmax(R(0), rainBefore - R(rain)) : R(0);
+  cold = float(R(cold) + hailProcesses(theta, q, cloud, ice, snow, rain, graupel, hail, exner, p, rhoZ, Rdt));
+  if (iceBefore > 0) iceNumber = float(R(iceNumber) * (R(ice) / iceBefore));
+  R splinters = splintering(R(theta) * exner - R(273.15)) * R(SPLINTERS) * mmax(R(0), cloudBefore - R(cloud))
+    + (rainFrozen > 0 ? fragmentsPerKg(mpow(R(PI * RHO_W * N0R) / (rhoZ * rainBefore), R(.25))) * rainFrozen : R(0));
+  if (splinters > 0) {
+    R fromG = mmin(R(graupel), splinters * R(CRYSTAL_MASS)), fromS = mmin(R(snow), splinters * R(CRYSTAL_MASS) - fromG);
+    graupel = float(R(graupel) - fromG); snow = float(R(snow) - fromS); ice = float(R(ice) + (fromG + fromS)); iceNumber = float(R(iceNumber) + (fromG + fromS) / R(CRYSTAL_MASS));
+  }
+  R cond = saturationAdjustMixed(theta, q, cloud, ice, exner, p, iceNumber);
+  settleCrystals<R>(ice, iceNumber);
+  if (electrified) {
+    after[0] = R(ice) + R(snow); after[1] = R(graupel) + R(hail); after[2] = rain; after[3] = cloud;
+    cNow[0] = cI; cNow[1] = cG; cNow[2] = cR; cNow[3] = cA;
+    if (cNow[0] != 0 || cNow[1] != 0 || cNow[2] != 0 || cNow[3] != 0) transferCharge(cNow, before, after);
+    R tc = R(theta) * exner - R(273.15);
+    bool wetG = wetGrowing(R(graupel), R(N0G), R(RHO_G), tc, R(q), p, rhoZ, R(cloud), R(rain), R(ice), R(snow));
+    bool wetH = wetGrowing(R(hail), R(N0H), R(RHO_H), tc, R(q), p, rhoZ, R(cloud), R(rain), R(ice), R(snow));
+    R separated = chargingRate(tc, rhoZ, R(cloud), R(ice), R(snow), R(graupel), R(hail), R(b.f(slot, F_FALL_SNOW, g)[i]), wetG, wetH, R(iceNumber) * rhoZ) * Rdt / rhoZ;
+    R induced = inductiveRate(tc, rhoZ, R(cloud), R(graupel), R(hail), R(b.f(slot, F_FIELDZ, g)[i]), (cNow[1] + separated) * rhoZ, wetG, wetH) * Rdt / rhoZ;
+    cI = float(cNow[0] - separated); cG = float(cNow[1] + separated + induced); cR = float(cNow[2]); cA = float(cNow[3] - induced);
+  }
+  if (cond < 0) cold = float(R(cold) - R(LV / CP) / exner * cond * R(.14));
+  // Coriolis and the relaxation of the wind's departure from the environment (float64: small departures of ~25 m/s).
+  double du = (double(u) - ue) * keep, dv = (double(v) - ve) * keep;
+  u = float(ue + du + fc * dv * dt); v = float(ve + dv - fc * du * dt);
+  cold = cold < 1e-3f ? 0.f : (cold < 15.f ? cold : 15.f);
+  if (alt > spongeStart) {
+    double s = fmin(1.0, fmax(0.0, (alt - spongeStart) / (H - spongeStart))) * .06 * dt, sc = fmin(1.0, fmax(0.0, s));
+    w = float(double(w) * (1 - s));
+    u = float(double(u) + (ue - double(u)) * sc); v = float(double(v) + (ve - double(v)) * sc); theta = float(double(theta) + (thEnv - double(theta)) * sc);
+  }
+  for (int k = 0; k < nc; k++) b.f(slot, k, g)[i] = F[k];
+  b.f(slot, F_CONDENSATE, g)[i] = float(R(cloud) + R(ice));
+}
+
+// ---- Smagorinsky K (turbulence.ts:87-137 viscosity: anisotropic widths, cube-root vertical length, theta_e stability)
+template <class R> __global__ __launch_bounds__(TPB) void k_viscosity(Geo g, Bufs<R> b, double dt) {
+  int slot = b.runs[blockIdx.y];
+  const double cs = b.pr(slot)[P_CS];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, layer = g.layer;
+  int z = i / layer, y = (i - z * layer) / nx, x = i - z * layer - y * nx;
+  const float *u = b.f(slot, F_U, g), *v = b.f(slot, F_V, g), *w = b.f(slot, F_W, g), *theta = b.f(slot, F_THETA, g), *q = b.f(slot, F_Q, g);
+  const double *exner = b.lv(slot, L_EXNER, g);
+  double lenH2 = cs * sqrt(g.dx * g.dy); lenH2 *= lenH2;
+  double capH = .05 * fmin(g.dx, g.dy) * fmin(g.dx, g.dy) / dt;
+  double local = z == 0 ? b.dzs[0] : z == nz - 1 ? b.dzs[nz - 2] : b.hz[z], len2 = cs * cbrt(g.dx * g.dy * local); len2 *= len2;
+  double mn = fmin(fmin(g.dx, g.dy), fmin(z > 0 ? b.dzs[z - 1] : 1e300, z < nz - 1 ? b.dzs[z] : 1e300)), cap = .05 * mn * mn / dt;
+  int up = z < nz - 1 ? layer : 0, down = z > 0 ? -layer : 0;
+  R span = R(b.zs[z + (up ? 1 : 0)] - b.zs[z - (down ? 1 : 0)]);
+  int row = y * nx + z * layer;
+  int e = (x + 1 == nx ? 0 : x + 1) + row, wst = (x == 0 ? nx - 1 : x - 1) + row;
+  int n = x + (y + 1 == ny ? 0 : y + 1) * nx + z * layer, s = x + (y == 0 ? ny - 1 : y - 1) * nx + z * layer;
+  R hx = R(2 * g.dx), hy = R(2 * g.dy);
+  R dudx = (R(u[e]) - R(u[wst])) / hx, dvdy = (R(v[n]) - R(v[s])) / hy, dwdz = (R(w[i + up]) - R(w[i + down])) / span;
+  R dudy = (R(u[n]) - R(u[s])) / hy, dvdx = (R(v[e]) - R(v[wst])) / hx;
+  R dudz = (R(u[i + up]) - R(u[i + down])) / span, dwdx = (R(w[e]) - R(w[wst])) / hx;
+  R dvdz = (R(v[i + up]) - R(v[i + down])) / span, dwdy = (R(w[n]) - R(w[s])) / hy;
+  R s2 = 2 * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + (dudy + dvdx) * (dudy + dvdx) + (dudz + dwdx) * (dudz + dwdx) + (dvdz + dwdy) * (dvdz + dwdy);
+  R n2 = R(G / b.lv(slot, L_THETA, g)[z]) * (R(theta[i + up]) - R(theta[i + down])) / span;
+  if (b.f(slot, F_CONDENSATE, g)[i] > 1e-5f) {
+    auto te = [&](int k, int level) { return R(theta[k]) * mexp(R(LV) * R(q[k]) / (R(CP) * R(theta[k]) * R(exner[level]))); };
+    n2 = R(G) / te(i, z) * (te(i + up, z + (up ? 1 : 0)) - te(i + down, z - (down ? 1 : 0))) / span;
+  }
+  R rate = msqrt(mmax(R(0), s2 - n2 * 3));
+  b.arr(b.km, slot, g)[i] = mmin(R(cap), R(len2) * rate);
+  b.arr(b.kh, slot, g)[i] = mmin(R(capH), R(lenH2) * rate);
+}
+
+// ---- subgrid mixing of the 8 mixed fields, surface drag and rigid walls (turbulence.ts:143-185 mix, model.ts:1161-1195)
+// Gather form of the reference's face loop: each node sums the fluxes through its six faces. The new values go to X;
+// the end-of-step projection moves them into the state.
+template <class R> __global__ __launch_bounds__(TPB) void k_mix(Geo g, Bufs<R> b, double dt) {
+  int slot = b.runs[blockIdx.y];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, layer = g.layer;
+  int z = i / layer, y = (i - z * layer) / nx, x = i - z * layer - y * nx;
+  int row = y * nx + z * layer;
+  int e = (x + 1 == nx ? 0 : x + 1) + row, wst = (x == 0 ? nx - 1 : x - 1) + row;
+  int n = x + (y + 1 == ny ? 0 : y + 1) * nx + z * layer, s = x + (y == 0 ? ny - 1 : y - 1) * nx + z * layer;
+  const R *km = b.arr(b.km, slot, g), *kh = b.arr(b.kh, slot, g);
+  const double *rho = b.lv(slot, L_RHO, g);
+  const R dx = R(g.dx), dy = R(g.dy);
+  R khi = kh[i], kmi = km[i];
+  R kx0 = (khi + kh[e]) / 2, kx1 = (kh[wst] + khi) / 2, ky0 = (khi + kh[n]) / 2, ky1 = (kh[s] + khi) / 2;
+  R kzu = 0, kzd = 0, cu = 0, cd = 0;
+  if (z < nz - 1) { kzu = (kmi + km[i + layer]) / 2 * R((rho[z] + rho[z + 1]) / 2) / R(b.dzs[z]); cu = R(1 / (rho[z] * b.hz[z])); }
+  if (z > 0) { kzd = (km[i - layer] + kmi) / 2 * R((rho[z - 1] + rho[z]) / 2) / R(b.dzs[z - 1]); cd = R(1 / (rho[z] * b.hz[z])); }
+  const int mixed[8] = {F_U, F_V, F_W, F_THETA, F_Q, F_CLOUD, F_ICE, F_ICENUMBER};
+  const R factors[8] = {1, 1, 1, 3, 3, 3, 3, 3};
+  float outv[8];
+  for (int k = 0; k < 8; k++) {
+    const float *a = b.f(slot, mixed[k], g);
+    // Departures from the base state of the level (u, v: the environment's wind in the frame; theta, q: the base state).
+    double base = 0, bu = 0, bd = 0;
+    int lvl = mixed[k] == F_U ? L_UENV : mixed[k] == F_V ? L_VENV : mixed[k] == F_THETA ? L_THETA : mixed[k] == F_Q ? L_Q : -1;
+    if (lvl >= 0) { const double *B = b.lv(slot, lvl, g); base = B[z]; bu = z < nz - 1 ? B[z + 1] : 0; bd = z > 0 ? B[z - 1] : 0; }
+    R here = R(double(a[i]) - base);
+    R t = (kx0 * (R(double(a[e]) - base) - here) - kx1 * (here - R(double(a[wst]) - base))) / (dx * dx)
+        + (ky0 * (R(double(a[n]) - base) - here) - ky1 * (here - R(double(a[s]) - base))) / (dy * dy);
+    if (z < nz - 1) t += kzu * (R(double(a[i + layer]) - bu) - here) * cu;
+    if (z > 0) t -= kzd * (here - R(double(a[i - layer]) - bd)) * cd;
+    outv[k] = float(R(a[i]) + R(dt) * factors[k] * t);
+  }
+  if (z == 0) {
+    // Surface drag on the ground node: dV/dt = -C_D (|V| V - |V_env| V_env) / h0, on the wind over the ground.
+    const double *P = b.pr(slot);
+    double ue0 = b.lv(slot, L_U, g)[0], ve0 = b.lv(slot, L_V, g)[0], envSpeed = hypot(ue0, ve0), drag = P[P_DRAG] * dt / b.hz[0];
+    double ug = double(outv[0]) + P[P_FRAME_U], vg = double(outv[1]) + P[P_FRAME_V], speed = hypot(ug, vg);
+    outv[0] = float(outv[0] - drag * (speed * ug - envSpeed * ue0)); outv[1] = float(outv[1] - drag * (speed * vg - envSpeed * ve0));
+  }
+  if (z == 0 || z == nz - 1) outv[2] = 0;
+  for (int k = 0; k < 8; k++) b.x(slot, mixed[k], g)[i] = outv[k];
+}
+
+// ---- end of step: advance the run's clock -----------------------------------------------------------------------
+template <class R> __global__ void k_end(Geo g, Bufs<R> b, double dt) {
+  int slot = b.runs[blockIdx.x];
+  if (threadIdx.x == 0) {
+    double *S = b.sc(slot), negative = 0;
+    for (int k = 0; k < N_CONSERVED; k++) negative += S[S_FILL + 3 * k + 2];
+    S[S_NEGATIVE] += negative;
+    b.pr(slot)[P_TIME] += dt;
+  }
+}
+
+// ---- diagnostics of a run (model.ts:1672-1696 and the harness's metrics): one block per run --------------------------
+enum : int { D_MAXW = 0, D_MINW, D_MAXTHETAP, D_RAINTOTAL, D_WATER, D_CHARGE, N_DIAG_DEV };
+template <class R> __global__ void k_diag(Geo g, Bufs<R> b, double *out) {
+  int slot = b.runs[blockIdx.x];
+  double mx = -1e300, mn = 1e300, tp = -1e300, rt = 0, water = 0, charge = 0;
+  const float *w = b.f(slot, F_W, g), *th = b.f(slot, F_THETA, g);
+  const double *te = b.lv(slot, L_THETA, g), *rho = b.lv(slot, L_RHO, g);
+  for (int i = threadIdx.x; i < g.n; i += blockDim.x) {
+    int z = i / g.layer;
+    double wv = w[i]; mx = fmax(mx, wv); mn = fmin(mn, wv); tp = fmax(tp, double(th[i]) - te[z]);
+    double lvl = double(b.f(slot, F_Q, g)[i]) + b.f(slot, F_CLOUD, g)[i] + b.f(slot, F_ICE, g)[i] + b.f(slot, F_RAIN, g)[i] + b.f(slot, F_SNOW, g)[i] + b.f(slot, F_GRAUPEL, g)[i] + b.f(slot, F_HAIL, g)[i];
+    water += b.hz[z] * rho[z] * lvl / g.layer;
+    double cq = double(b.f(slot, F_CHARGE_ICE, g)[i]) + b.f(slot, F_CHARGE_GRAUPEL, g)[i] + b.f(slot, F_CHARGE_RAIN, g)[i] + b.f(slot, F_CHARGE_AIR, g)[i];
+    charge += cq * rho[z] * g.dx * g.dy * b.hz[z];
+  }
+  const float *prec = b.col(slot, C_PRECIPITATION, g);
+  for (int c = threadIdx.x; c < g.layer; c += blockDim.x) { rt = fmax(rt, double(prec[c])); water += double(prec[c]) / g.layer; }
+  __shared__ double red[16];
+  mx = blockMax(mx, red); mn = -blockMax(-mn, red); tp = blockMax(tp, red); rt = blockMax(rt, red);
+  water = blockSum(water, red); charge = blockSum(charge, red);
+  if (threadIdx.x == 0) {
+    double *o = out + size_t(blockIdx.x) * N_DIAG_DEV;
+    o[D_MAXW] = mx; o[D_MINW] = mn; o[D_MAXTHETAP] = tp; o[D_RAINTOTAL] = rt; o[D_WATER] = water; o[D_CHARGE] = charge;
+  }
+}
+
+}  // namespace ml

@@ -1,0 +1,529 @@
+// Host runtime of the StormLab step on MI355X and its C ABI (include/meteolab.h).
+//
+// A context owns one device and one stream. A pool holds the runs of one grid size in slots; each array kind is one
+// allocation over the slots, so the kernels of a step cover a whole batch of runs in one launch each. The launch
+// sequence of a batch (one model step) is captured into a hipGraph once and replayed: all data-dependent control of a
+// step (sedimentation sub-steps, positivity-fix scales, lightning) is decided on the device, so a replay needs no
+// host round trip.
+#include "../../include/meteolab.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "lightning.h"
+#include "spectral.h"
+
+using namespace ml;
+
+namespace {
+
+constexpr int MAX_POOLS = 64, SLOT_BITS = 16;
+
+struct Batch {
+  std::vector<int> runs;
+  int *dev = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double dt = 0;
+};
+
+struct Pool {
+  Geo g{};
+  Plan plan{};
+  int cap = 0, electrified = 1;
+  size_t rb = 4;  // bytes of the arithmetic type
+  void *fields = nullptr, *stx = nullptr, *sty = nullptr, *mass = nullptr, *phi = nullptr, *cells = nullptr, *press = nullptr,
+       *spec = nullptr, *mean0 = nullptr, *km = nullptr, *kh = nullptr, *ephi = nullptr, *cols = nullptr, *lev = nullptr,
+       *par = nullptr, *scal = nullptr, *zs = nullptr, *dzs = nullptr, *hz = nullptr, *sym = nullptr, *twx = nullptr, *twy = nullptr;
+  int *single = nullptr;  // single[s] = s: the run list of one slot
+  double *diag = nullptr; // [cap][N_DIAG_DEV]
+  std::vector<double> zsH, dzsH, hzH;
+  std::vector<char> used;
+  std::map<std::vector<int>, Batch *> batches;
+};
+
+}  // namespace
+
+struct ml_ctx {
+  int device = 0, precision = 32;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t prop{};
+  std::string err;
+  std::vector<Pool *> pools;
+};
+
+namespace {
+
+int fail(ml_ctx *c, int code, const std::string &msg) { if (c) c->err = msg; return code; }
+#define HIPCHK(c, x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return fail(c, ML_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); } while (0)
+
+// Geometric stretching ratio (grid.ts:91-97) and the levels of createGrid (grid.ts:99-121).
+double stretchRatio(double bottom, double height, int intervals) {
+  if (bottom * intervals >= height) return 1;
+  double lo = 1, hi = 2;
+  while (bottom * (std::pow(hi, intervals) - 1) / (hi - 1) < height) hi *= 2;
+  for (int k = 0; k < 100; k++) { double mid = (lo + hi) / 2; if (bottom * (std::pow(mid, intervals) - 1) / (mid - 1) < height) lo = mid; else hi = mid; }
+  return (lo + hi) / 2;
+}
+void levels(const ml_grid &o, std::vector<double> &zs, std::vector<double> &dzs, std::vector<double> &hz) {
+  int nz = o.nz; double dz = o.height / (nz - 1);
+  zs.assign(nz, 0.0);
+  bool uniform = !(o.bottom > 0);
+  if (uniform) for (int z = 0; z < nz; z++) zs[z] = z * dz;
+  else {
+    double r = stretchRatio(o.bottom, o.height, nz - 1);
+    for (int z = 1; z < nz; z++) zs[z] = zs[z - 1] + o.bottom * std::pow(r, z - 1);
+    for (int z = 1; z < nz; z++) zs[z] *= o.height / zs[nz - 1];
+  }
+  dzs.assign(nz - 1, 0.0);
+  for (int k = 0; k < nz - 1; k++) dzs[k] = uniform ? dz : zs[k + 1] - zs[k];
+  hz.assign(nz, 0.0);
+  for (int k = 0; k < nz; k++) hz[k] = ((k > 0 ? dzs[k - 1] : 0) + (k < nz - 1 ? dzs[k] : 0)) / 2;
+}
+// Radices of an FFT length: fours first, then twos, threes, fives (spectral.ts:20-24 tries 4, 2, 3, 5, ...).
+bool radices(int n, int *out, int &count) {
+  count = 0;
+  const int order[4] = {4, 2, 3, 5};
+  for (int p : order) while (n % p == 0 && n > 1) { if (count == 8) return false; out[count++] = p; n /= p; }
+  return n == 1;
+}
+
+template <class R> Bufs<R> bufs(const Pool &p, const int *runs) {
+  Bufs<R> b;
+  b.fields = (float *)p.fields; b.stx = (float *)p.stx; b.sty = (float *)p.sty; b.mass = (R *)p.mass; b.phi = (R *)p.phi;
+  b.cells = (R *)p.cells; b.press = (R *)p.press; b.spec = (R *)p.spec; b.mean0 = (double *)p.mean0; b.km = (R *)p.km; b.kh = (R *)p.kh;
+  b.ephi = (R *)p.ephi; b.cols = (float *)p.cols; b.lev = (double *)p.lev; b.par = (double *)p.par; b.scal = (double *)p.scal;
+  b.zs = (const double *)p.zs; b.dzs = (const double *)p.dzs; b.hz = (const double *)p.hz; b.sym = (const R *)p.sym;
+  b.twx = (const R *)p.twx; b.twy = (const R *)p.twy; b.runs = runs;
+  return b;
+}
+
+inline unsigned blocks(size_t n, unsigned t) { return unsigned((n + t - 1) / t); }
+
+// Named launch stages, for the per-stage timing.
+struct Timer {
+  hipStream_t s;
+  std::vector<hipEvent_t> *events = nullptr;
+  std::vector<std::string> *names = nullptr;
+  void mark(const char *name) {
+    if (!events) return;
+    hipEvent_t e; hipEventCreate(&e); hipEventRecord(e, s); events->push_back(e); names->push_back(name);
+  }
+};
+
+template <class R> void spectralSolve(const Pool &p, const Bufs<R> &b, int nr, hipStream_t s, int kind, int L, int invKind) {
+  const Geo &g = p.g;
+  size_t lds = specLdsBytes(g, sizeof(R));
+  hipLaunchKernelGGL(k_spec_fwd<R>, dim3(L, nr), dim3(SPEC_THREADS), lds, s, g, b, p.plan, L);
+  hipLaunchKernelGGL(k_modes<R>, dim3(blocks(g.nm, MODE_THREADS), nr), dim3(MODE_THREADS), size_t(MODE_THREADS) * L * sizeof(R), s, g, b, kind, L);
+  hipLaunchKernelGGL(k_spec_inv<R>, dim3(L, nr), dim3(SPEC_THREADS), lds, s, g, b, p.plan, L, invKind);
+}
+
+// One model step of the runs in `list` (device array of nr slots): AtmosphereModel.step (model.ts:873-877).
+template <class R> void launchStep(const Pool &p, const int *list, int nr, double dt, hipStream_t s, Timer *t) {
+  const Geo &g = p.g;
+  Bufs<R> b = bufs<R>(p, list);
+  const int el = p.electrified, nc = el ? N_CARRIED : F_COLD + 1;
+  const unsigned nb = blocks(g.n, TPB), cb = blocks(size_t(g.layer) * (g.nz - 1), TPB);
+  auto mark = [&](const char *n) { if (t) t->mark(n); };
+  mark("start");
+  // beginStep: fall speeds, hail swath, sedimentation (model.ts:893-902).
+  hipLaunchKernelGGL(k_begin<R>, dim3(nr), dim3(64), 0, s, g, b);
+  hipLaunchKernelGGL(k_fall<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt, el);
+  mark("fall");
+  hipLaunchKernelGGL(k_sediment<R>, dim3(blocks(g.layer, SED_COLS), nr), dim3(SED_COLS), size_t(2) * SED_COLS * g.nz * sizeof(float), s, g, b, dt, el);
+  mark("sediment");
+  // dynamics: face projection (FluxTransport.setVelocity) and the three RK3 stages with the stage projections.
+  hipLaunchKernelGGL(k_faces<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 0);
+  spectralSolve<R>(p, b, nr, s, 0, g.nz, 0);
+  hipLaunchKernelGGL(k_faces<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 1);
+  mark("faces");
+  const size_t stageLds = ((g.layer * sizeof(float) + 15) / 16) * 16 + 2 * g.layer * sizeof(R);
+  for (int stage = 0; stage < 3; stage++) {
+    hipLaunchKernelGGL(k_stage<R>, dim3(g.nz, nc, nr), dim3(TPB), stageLds, s, g, b, stage, dt);
+    mark(stage == 0 ? "transport0" : stage == 1 ? "transport1" : "transport2");
+    if (stage < 2) {
+      hipLaunchKernelGGL(k_pdiv<R>, dim3(cb, nr), dim3(TPB), 0, s, g, b, stage + 1, dt);
+      spectralSolve<R>(p, b, nr, s, 1, g.nz - 1, 1);
+      hipLaunchKernelGGL(k_pcorrect<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, stage + 1, dt, 0);
+      mark(stage == 0 ? "pressure0" : "pressure1");
+    }
+  }
+  // endStep: positivity fix + physics (physicsStep), turbulence + drag (mixStep), projection (model.ts:1103-1113).
+  hipLaunchKernelGGL(k_fill_sums<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b);
+  hipLaunchKernelGGL(k_physics<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt, el);
+  mark("physics");
+  hipLaunchKernelGGL(k_viscosity<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt);
+  mark("viscosity");
+  hipLaunchKernelGGL(k_mix<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt);
+  mark("mix");
+  hipLaunchKernelGGL(k_pdiv<R>, dim3(cb, nr), dim3(TPB), 0, s, g, b, 1, dt);
+  spectralSolve<R>(p, b, nr, s, 1, g.nz - 1, 1);
+  hipLaunchKernelGGL(k_pcorrect<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 1, dt, 1);
+  mark("pressure2");
+  // afterStep: lightning (model.ts:1537-1552).
+  if (el) {
+    unsigned ib = blocks(size_t(g.layer) * (g.nz - 2), 256);
+    hipLaunchKernelGGL(k_charge_rhs<R>, dim3(ib, nr), dim3(256), 0, s, g, b);
+    spectralSolve<R>(p, b, nr, s, 2, g.nz - 2, 2);
+    hipLaunchKernelGGL(k_strongest<R>, dim3(ib, nr), dim3(256), 0, s, g, b);
+    hipLaunchKernelGGL(k_conduct<R>, dim3(ib, nr), dim3(256), 0, s, g, b, dt);
+    mark("lightning");
+  }
+  hipLaunchKernelGGL(k_end<R>, dim3(nr), dim3(64), 0, s, g, b, dt);
+}
+
+template <class R> void fillTables(Pool &p, std::vector<char> &sym, std::vector<char> &twx, std::vector<char> &twy) {
+  const Geo &g = p.g;
+  std::vector<R> s(4 * size_t(g.nm)), tx(2 * g.nx), ty(2 * g.ny);
+  auto exact = [](double x) { return x < 1e-12 ? 0 : x > 1 - 1e-12 ? 1 : x; };
+  for (int n = 0; n < g.ny; n++) for (int m = 0; m < g.mw; m++) {
+    int mode = m + g.mw * n;
+    double sxr = std::sin(M_PI * m / g.nx), syr = std::sin(M_PI * n / g.ny);
+    double lam = 4 * std::pow(sxr, 2) / (g.dx * g.dx) + 4 * std::pow(syr, 2) / (g.dy * g.dy);
+    double sx = exact(std::pow(std::sin(M_PI * m / g.nx), 2)), cx = exact(std::pow(std::cos(M_PI * m / g.nx), 2));
+    double sy = exact(std::pow(std::sin(M_PI * n / g.ny), 2)), cy = exact(std::pow(std::cos(M_PI * n / g.ny), 2));
+    s[mode] = R(m == 0 && n == 0 ? 0 : lam);
+    s[g.nm + mode] = R(4 * (sx * cy / (g.dx * g.dx) + cx * sy / (g.dy * g.dy)));
+    s[2 * g.nm + mode] = R(cx * cy);
+    s[3 * g.nm + mode] = R(lam);
+  }
+  for (int k = 0; k < g.nx; k++) { tx[2 * k] = R(std::cos(2 * M_PI * k / g.nx)); tx[2 * k + 1] = R(-std::sin(2 * M_PI * k / g.nx)); }
+  for (int k = 0; k < g.ny; k++) { ty[2 * k] = R(std::cos(2 * M_PI * k / g.ny)); ty[2 * k + 1] = R(-std::sin(2 * M_PI * k / g.ny)); }
+  sym.assign((char *)s.data(), (char *)(s.data() + s.size())); twx.assign((char *)tx.data(), (char *)(tx.data() + tx.size()));
+  twy.assign((char *)ty.data(), (char *)(ty.data() + ty.size()));
+}
+
+struct Sizes { size_t fields, stage, mass, node, spec, mean0, cols, lev, par, scal; size_t slot() const { return fields + 2 * stage + mass + 6 * node + spec + mean0 + cols + lev + par + scal; } };
+Sizes sizesOf(const Geo &g, size_t rb) {
+  Sizes s;
+  s.fields = size_t(N_FIELDS) * g.n * 4; s.stage = size_t(N_CARRIED) * g.n * 4; s.mass = 4 * size_t(g.n) * rb; s.node = size_t(g.n) * rb;
+  s.spec = size_t(g.nz) * g.nm * 2 * rb; s.mean0 = size_t(g.nz) * 8; s.cols = size_t(N_COLUMNS) * g.layer * 4;
+  s.lev = size_t(N_LEVELS) * g.nz * 8; s.par = N_PARAMS * 8; s.scal = N_SCALARS * 8;
+  return s;
+}
+int geoOf(ml_ctx *c, const ml_grid *o, Geo &g, Plan &pl) {
+  if (!o || o->nx < 2 || o->ny < 2 || o->nz < 4 || o->nz > 64 || o->nx % 2 || o->ny % 2 || !(o->width > 0) || !(o->depth > 0) || !(o->height > 0))
+    return fail(c, ML_E_ARG, "grid: nx, ny even and >= 2, 4 <= nz <= 64, positive extents");
+  if (!radices(o->nx, pl.rx, pl.nrx) || !radices(o->ny, pl.ry, pl.nry)) return fail(c, ML_E_UNSUPPORTED, "grid: nx and ny must factor into 2, 3 and 5");
+  g.nx = o->nx; g.ny = o->ny; g.nz = o->nz; g.layer = g.nx * g.ny; g.n = g.layer * g.nz;
+  g.mw = g.nx / 2 + 1; g.nm = g.mw * g.ny;
+  g.dx = o->width / g.nx; g.dy = o->depth / g.ny; g.dz = o->height / (g.nz - 1); g.height = o->height;
+  return 0;
+}
+
+Pool *poolOf(ml_ctx *c, int id) { return id >= 0 && id < int(c->pools.size()) ? c->pools[id] : nullptr; }
+bool decode(ml_ctx *c, int run, Pool *&p, int &slot) {
+  p = poolOf(c, run >> SLOT_BITS); slot = run & ((1 << SLOT_BITS) - 1);
+  return p && slot < p->cap && p->used[slot];
+}
+
+void freeBatch(Batch *bt) {
+  if (bt->exec) hipGraphExecDestroy(bt->exec);
+  if (bt->graph) hipGraphDestroy(bt->graph);
+  if (bt->dev) hipFree(bt->dev);
+  delete bt;
+}
+void dropBatches(Pool *p, int slot) {
+  for (auto it = p->batches.begin(); it != p->batches.end();) {
+    if (slot < 0 || std::find(it->first.begin(), it->first.end(), slot) != it->first.end()) { freeBatch(it->second); it = p->batches.erase(it); }
+    else ++it;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ml_ctx_create(int device, int precision, ml_ctx **out) {
+  if (!out || (precision != 32 && precision != 64)) return ML_E_ARG;
+  ml_ctx *c = new ml_ctx();
+  c->device = device; c->precision = precision;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipGetDeviceProperties(&c->prop, device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete c; return ML_E_HIP; }
+  *out = c;
+  return 0;
+}
+
+void ml_ctx_destroy(ml_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  for (Pool *p : c->pools) {
+    if (!p) continue;
+    dropBatches(p, -1);
+    void *all[] = {p->fields, p->stx, p->sty, p->mass, p->phi, p->cells, p->press, p->spec, p->mean0, p->km, p->kh, p->ephi, p->cols,
+                   p->lev, p->par, p->scal, p->zs, p->dzs, p->hz, p->sym, p->twx, p->twy, p->single, p->diag};
+    for (void *a : all) if (a) hipFree(a);
+    delete p;
+  }
+  hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *ml_last_error(ml_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+int ml_device_name(ml_ctx *c, char *out, int size) {
+  if (!c || !out || size <= 0) return ML_E_ARG;
+  std::snprintf(out, size, "%s (%s)", c->prop.name, c->prop.gcnArchName);
+  return 0;
+}
+
+int64_t ml_slot_bytes(ml_ctx *c, const ml_grid *grid) {
+  Geo g; Plan pl;
+  if (!c || geoOf(c, grid, g, pl)) return ML_E_ARG;
+  return int64_t(sizesOf(g, c->precision == 64 ? 8 : 4).slot());
+}
+
+int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified, int *pool) {
+  if (!c || !pool || capacity < 1 || capacity >= (1 << SLOT_BITS)) return fail(c, ML_E_ARG, "pool: capacity 1 .. 65535");
+  if (int(c->pools.size()) >= MAX_POOLS) return fail(c, ML_E_FULL, "too many pools");
+  Pool *p = new Pool();
+  if (int r = geoOf(c, grid, p->g, p->plan)) { delete p; return r; }
+  HIPCHK(c, hipSetDevice(c->device));
+  const Geo &g = p->g;
+  p->cap = capacity; p->electrified = electrified ? 1 : 0; p->rb = c->precision == 64 ? 8 : 4;
+  // LDS the kernels need must fit the device's block limit.
+  size_t stageLds = ((g.layer * sizeof(float) + 15) / 16) * 16 + 2 * g.layer * p->rb;
+  if (stageLds > c->prop.sharedMemPerBlock || specLdsBytes(g, p->rb) > c->prop.sharedMemPerBlock) { delete p; return fail(c, ML_E_UNSUPPORTED, "grid too large for a level in LDS"); }
+  Sizes s = sizesOf(g, p->rb);
+  size_t cap = capacity;
+  struct { void **ptr; size_t bytes; } allocs[] = {
+    {&p->fields, s.fields * cap}, {&p->stx, s.stage * cap}, {&p->sty, s.stage * cap}, {&p->mass, s.mass * cap}, {&p->phi, s.node * cap},
+    {&p->cells, s.node * cap}, {&p->press, s.node * cap}, {&p->spec, s.spec * cap}, {&p->mean0, s.mean0 * cap}, {&p->km, s.node * cap},
+    {&p->kh, s.node * cap}, {&p->ephi, s.node * cap}, {&p->cols, s.cols * cap}, {&p->lev, s.lev * cap}, {&p->par, s.par * cap},
+    {&p->scal, s.scal * cap}, {(void **)&p->diag, cap * N_DIAG_DEV * 8}};
+  for (auto &a : allocs) {
+    hipError_t e = hipMalloc(a.ptr, a.bytes);
+    if (e == hipSuccess) e = hipMemset(*a.ptr, 0, a.bytes);
+    if (e != hipSuccess) {
+      for (auto &f : allocs) if (*f.ptr) { hipFree(*f.ptr); *f.ptr = nullptr; }
+      delete p;
+      return fail(c, ML_E_HIP, std::string("pool allocation: ") + hipGetErrorString(e));
+    }
+  }
+  levels(*grid, p->zsH, p->dzsH, p->hzH);
+  std::vector<char> sym, twx, twy;
+  if (p->rb == 8) fillTables<double>(*p, sym, twx, twy); else fillTables<float>(*p, sym, twx, twy);
+  std::vector<int> single(capacity); for (int k = 0; k < capacity; k++) single[k] = k;
+  struct { void **ptr; const void *src; size_t bytes; } tabs[] = {
+    {&p->zs, p->zsH.data(), p->zsH.size() * 8}, {&p->dzs, p->dzsH.data(), p->dzsH.size() * 8}, {&p->hz, p->hzH.data(), p->hzH.size() * 8},
+    {&p->sym, sym.data(), sym.size()}, {&p->twx, twx.data(), twx.size()}, {&p->twy, twy.data(), twy.size()},
+    {(void **)&p->single, single.data(), single.size() * sizeof(int)}};
+  for (auto &t : tabs) {
+    HIPCHK(c, hipMalloc(t.ptr, t.bytes));
+    HIPCHK(c, hipMemcpy(*t.ptr, t.src, t.bytes, hipMemcpyHostToDevice));
+  }
+  p->used.assign(capacity, 0);
+  c->pools.push_back(p);
+  *pool = int(c->pools.size()) - 1;
+  return 0;
+}
+
+int ml_pool_free(ml_ctx *c, int pool) {
+  Pool *p = c ? poolOf(c, pool) : nullptr;
+  if (!p) return ML_E_ARG;
+  return int(std::count(p->used.begin(), p->used.end(), 0));
+}
+
+int ml_run_create(ml_ctx *c, int pool, const ml_profile *pr, const ml_run_config *rc, int *run) {
+  Pool *p = c ? poolOf(c, pool) : nullptr;
+  if (!p || !pr || !rc || !run) return fail(c, ML_E_ARG, "run: bad pool or arguments");
+  int slot = -1;
+  for (int k = 0; k < p->cap; k++) if (!p->used[k]) { slot = k; break; }
+  if (slot < 0) return fail(c, ML_E_FULL, "pool full");
+  HIPCHK(c, hipSetDevice(c->device));
+  const Geo &g = p->g;
+  double P[N_PARAMS] = {0};
+  P[P_FRAME_U] = rc->frame_u; P[P_FRAME_V] = rc->frame_v;
+  P[P_CORIOLIS] = 2 * 7.2921e-5 * std::sin(rc->latitude * M_PI / 180);
+  P[P_CS] = .18 * rc->turbulence / .55;
+  P[P_DRAG] = std::pow(.4 / std::log(2 * p->hzH[0] / 4 / rc->roughness), 2);
+  P[P_SPONGE_START] = std::max(rc->tropopause_km * 1000 + 2500, 14000.0);
+  P[P_SEED] = rc->seed; P[P_BUBBLE_AMP] = rc->bubble_amp; P[P_BUBBLE_X] = rc->bubble_x; P[P_BUBBLE_Y] = rc->bubble_y; P[P_BUBBLE_Z] = rc->bubble_z;
+  P[P_BUBBLE_RH] = rc->bubble_rh; P[P_BUBBLE_RZ] = rc->bubble_rz; P[P_NOISE_SEED] = rc->noise_seed; P[P_NOISE_AMP] = rc->noise_amp; P[P_NOISE_TOP] = rc->noise_top;
+  P[P_QVMAX] = pr->qv_max; P[P_SHEAR] = pr->shear; P[P_SHEAR_DEPTH] = pr->shear_depth; P[P_THETA0] = pr->theta0; P[P_DTHETA] = pr->dtheta;
+  P[P_ZTR] = pr->ztr; P[P_TTR] = pr->ttr; P[P_RH_DROP] = pr->rh_drop; P[P_RH_TOP] = pr->rh_top; P[P_ELECTRIFIED] = p->electrified; P[P_TIME] = 0;
+  HIPCHK(c, hipMemcpyAsync((double *)p->par + size_t(slot) * N_PARAMS, P, sizeof(P), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync((double *)p->scal + size_t(slot) * N_SCALARS, 0, N_SCALARS * 8, c->stream));
+  Sizes s = sizesOf(g, p->rb);
+  HIPCHK(c, hipMemsetAsync((char *)p->stx + s.stage * slot, 0, s.stage, c->stream));
+  HIPCHK(c, hipMemsetAsync((char *)p->sty + s.stage * slot, 0, s.stage, c->stream));
+  HIPCHK(c, hipMemsetAsync((char *)p->ephi + s.node * slot, 0, s.node, c->stream));
+  HIPCHK(c, hipMemsetAsync((char *)p->press + s.node * slot, 0, s.node, c->stream));
+  const int *one = p->single + slot;
+  size_t lnp = (size_t(std::ceil((g.height + 2000) / 10)) + 2) * sizeof(double);
+  if (p->rb == 8) {
+    Bufs<double> b = bufs<double>(*p, one);
+    hipLaunchKernelGGL(k_base_state<double>, dim3(1), dim3(64), lnp, c->stream, g, b);
+    hipLaunchKernelGGL(k_init<double>, dim3(blocks(g.n, 256), 1), dim3(256), 0, c->stream, g, b);
+  } else {
+    Bufs<float> b = bufs<float>(*p, one);
+    hipLaunchKernelGGL(k_base_state<float>, dim3(1), dim3(64), lnp, c->stream, g, b);
+    hipLaunchKernelGGL(k_init<float>, dim3(blocks(g.n, 256), 1), dim3(256), 0, c->stream, g, b);
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  p->used[slot] = 1;
+  *run = (pool << SLOT_BITS) | slot;
+  return 0;
+}
+
+int ml_run_release(ml_ctx *c, int run) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot)) return fail(c, ML_E_ARG, "bad run");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  dropBatches(p, slot);
+  p->used[slot] = 0;
+  return 0;
+}
+
+int ml_run_upload(ml_ctx *c, int run, int field, const float *host) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot) || field < 0 || field >= N_FIELDS || !host) return fail(c, ML_E_ARG, "upload: bad run or field");
+  size_t n = p->g.n;
+  HIPCHK(c, hipMemcpyAsync((float *)p->fields + (size_t(slot) * N_FIELDS + field) * n, host, n * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+int ml_run_download(ml_ctx *c, int run, int field, float *host) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot) || field < 0 || field >= N_FIELDS || !host) return fail(c, ML_E_ARG, "download: bad run or field");
+  size_t n = p->g.n;
+  HIPCHK(c, hipMemcpyAsync(host, (float *)p->fields + (size_t(slot) * N_FIELDS + field) * n, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+int ml_run_upload_column(ml_ctx *c, int run, int column, const float *host) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot) || column < 0 || column >= N_COLUMNS || !host) return fail(c, ML_E_ARG, "upload column: bad run or column");
+  size_t l = p->g.layer;
+  HIPCHK(c, hipMemcpyAsync((float *)p->cols + (size_t(slot) * N_COLUMNS + column) * l, host, l * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+int ml_run_download_column(ml_ctx *c, int run, int column, float *host) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot) || column < 0 || column >= N_COLUMNS || !host) return fail(c, ML_E_ARG, "download column: bad run or column");
+  size_t l = p->g.layer;
+  HIPCHK(c, hipMemcpyAsync(host, (float *)p->cols + (size_t(slot) * N_COLUMNS + column) * l, l * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+int ml_run_levels(ml_ctx *c, int run, double *out) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot) || !out) return fail(c, ML_E_ARG, "levels: bad run");
+  HIPCHK(c, hipMemcpyAsync(out, (double *)p->lev + size_t(slot) * N_LEVELS * p->g.nz, size_t(ML_LEVELS) * p->g.nz * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+int ml_run_set_time(ml_ctx *c, int run, double time) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot)) return fail(c, ML_E_ARG, "set time: bad run");
+  HIPCHK(c, hipMemcpyAsync((double *)p->par + size_t(slot) * N_PARAMS + P_TIME, &time, 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ml_run_diag(ml_ctx *c, int run, double *out) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot) || !out) return fail(c, ML_E_ARG, "diag: bad run");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int *one = p->single + slot;
+  if (p->rb == 8) hipLaunchKernelGGL(k_diag<double>, dim3(1), dim3(1024), 0, c->stream, p->g, bufs<double>(*p, one), p->diag);
+  else hipLaunchKernelGGL(k_diag<float>, dim3(1), dim3(1024), 0, c->stream, p->g, bufs<float>(*p, one), p->diag);
+  HIPCHK(c, hipGetLastError());
+  double d[N_DIAG_DEV], sc[N_SCALARS], par[N_PARAMS];
+  HIPCHK(c, hipMemcpyAsync(d, p->diag, sizeof(d), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(sc, (double *)p->scal + size_t(slot) * N_SCALARS, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(par, (double *)p->par + size_t(slot) * N_PARAMS, sizeof(par), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  auto i64 = [&](int k) { int64_t v; std::memcpy(&v, sc + k, 8); return double(v); };
+  out[ML_D_TIME] = par[P_TIME]; out[ML_D_MAXW] = d[D_MAXW]; out[ML_D_MINW] = d[D_MINW]; out[ML_D_MAXTHETAP] = d[D_MAXTHETAP];
+  out[ML_D_RAINTOTAL] = d[D_RAINTOTAL]; out[ML_D_WATER] = d[D_WATER]; out[ML_D_CHARGE] = d[D_CHARGE]; out[ML_D_CLIPPED] = i64(S_CLIPPED);
+  out[ML_D_NEGFILLED] = sc[S_NEGATIVE]; out[ML_D_CHARGEGROUND] = sc[S_CHARGE_GROUND]; out[ML_D_FLASHES_IC] = i64(S_FLASH_IC);
+  out[ML_D_FLASHES_CG] = i64(S_FLASH_CG); out[ML_D_FIRSTFLASH] = sc[S_FIRST_FLASH]; out[ML_D_ERROR] = i64(S_ERROR);
+  return 0;
+}
+
+static int batchOf(ml_ctx *c, const int *runs, int count, Pool *&p, Batch *&bt) {
+  if (!c || !runs || count < 1) return fail(c, ML_E_ARG, "step: no runs");
+  std::vector<int> slots(count);
+  p = nullptr;
+  for (int k = 0; k < count; k++) {
+    Pool *q; int slot;
+    if (!decode(c, runs[k], q, slot)) return fail(c, ML_E_ARG, "step: bad run");
+    if (p && q != p) return fail(c, ML_E_ARG, "step: runs of one pool per call");
+    p = q; slots[k] = slot;
+  }
+  auto it = p->batches.find(slots);
+  if (it != p->batches.end()) { bt = it->second; return 0; }
+  bt = new Batch();
+  bt->runs = slots;
+  HIPCHK(c, hipMalloc((void **)&bt->dev, count * sizeof(int)));
+  HIPCHK(c, hipMemcpy(bt->dev, slots.data(), count * sizeof(int), hipMemcpyHostToDevice));
+  p->batches[slots] = bt;
+  return 0;
+}
+
+int ml_step(ml_ctx *c, const int *runs, int count, int steps, double dt) {
+  Pool *p; Batch *bt;
+  if (int r = batchOf(c, runs, count, p, bt)) return r;
+  if (steps < 0 || !(dt > 0)) return fail(c, ML_E_ARG, "step: steps >= 0, dt > 0");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!bt->exec || bt->dt != dt) {
+    if (bt->exec) { hipGraphExecDestroy(bt->exec); bt->exec = nullptr; }
+    if (bt->graph) { hipGraphDestroy(bt->graph); bt->graph = nullptr; }
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    if (p->rb == 8) launchStep<double>(*p, bt->dev, count, dt, c->stream, nullptr);
+    else launchStep<float>(*p, bt->dev, count, dt, c->stream, nullptr);
+    HIPCHK(c, hipStreamEndCapture(c->stream, &bt->graph));
+    HIPCHK(c, hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0));
+    bt->dt = dt;
+  }
+  for (int k = 0; k < steps; k++) HIPCHK(c, hipGraphLaunch(bt->exec, c->stream));
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int ml_sync(ml_ctx *c) {
+  if (!c) return ML_E_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ml_profile_step(ml_ctx *c, const int *runs, int count, int steps, double dt, double *ms, int max, char *names, int size) {
+  Pool *p; Batch *bt;
+  if (int r = batchOf(c, runs, count, p, bt)) return r;
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<hipEvent_t> events; std::vector<std::string> labels;
+  Timer t{c->stream, &events, &labels};
+  std::vector<double> acc;
+  std::vector<std::string> first;
+  for (int k = 0; k < steps; k++) {
+    events.clear(); labels.clear();
+    if (p->rb == 8) launchStep<double>(*p, bt->dev, count, dt, c->stream, &t); else launchStep<float>(*p, bt->dev, count, dt, c->stream, &t);
+    t.mark("end");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (acc.empty()) { acc.assign(events.size(), 0.0); first = labels; }
+    for (size_t e = 1; e < events.size(); e++) { float v = 0; hipEventElapsedTime(&v, events[e - 1], events[e]); acc[e] += v; }
+    for (hipEvent_t e : events) hipEventDestroy(e);
+  }
+  int nstages = int(acc.size()) - 1;
+  std::string list;
+  for (int e = 1; e <= nstages && e - 1 < max; e++) { if (ms) ms[e - 1] = acc[e] / std::max(1, steps); list += first[e]; list += '\n'; }
+  if (names && size > 0) { std::strncpy(names, list.c_str(), size - 1); names[size - 1] = 0; }
+  return nstages;
+}
+
+}  // extern "C"

@@ -1,0 +1,230 @@
+// Spectral solves of the StormLab step: the horizontal 2-D real DFT of a stack of levels (spectral.ts:81-151
+// HorizontalDFT, Hermitian half m <= nx/2, two real rows per complex transform), the vertical tridiagonal solve of
+// every horizontal mode (advection.ts:222-241 face projection, pressure.ts:104-140 pressure, electric.ts:36-60
+// electric potential), and the inverse transform.
+//
+// One block per (level, run) transforms a level in LDS: mixed-radix Stockham FFTs (radices 4, 2, 3, 5 from the
+// factorisation of nx and ny, so 40 = 4·2·5 and 80 = 4·4·5), rows first, then the kept columns. One thread per
+// (mode, run) solves the mode's column by the Thomas algorithm; the m = n = 0 mode (the level means) is solved in
+// float64 from float64 level sums.
+#pragma once
+#include "common.h"
+
+namespace ml {
+
+template <class R> struct cx { R re, im; };
+template <class R> __device__ inline cx<R> cmul(cx<R> a, cx<R> b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+
+// Radix plans of a size class: radices of nx then of ny.
+struct Plan { int rx[8], nrx, ry[8], nry; };
+
+constexpr int SPEC_THREADS = 256;
+
+// `count` sequences of length N in LDS, element k of sequence s at s * sStride + k * kStride; tw[k] = exp(-2 pi i k/N)
+// (sign -1, forward) or its conjugate (sign +1). Stockham stages ping-pong between a and b; returns the buffer that
+// holds the result. Ends with a barrier.
+template <class R> __device__ cx<R> *fftStages(cx<R> *a, cx<R> *b, int N, int count, int sStride, int kStride, const cx<R> *tw, const int *rad, int nrad, int sign) {
+  cx<R> *src = a, *dst = b;
+  int Ns = 1;
+  for (int st = 0; st < nrad; st++) {
+    const int p = rad[st], span = N / p, total = count * span, tstep = N / (Ns * p), wp = N / p;
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+      int s = t / span, j = t - s * span, k = j % Ns;
+      cx<R> v[5], o[5];
+      for (int r = 0; r < p; r++) {
+        cx<R> x = src[s * sStride + (j + r * span) * kStride];
+        cx<R> w = tw[k * r * tstep];
+        if (sign > 0) w.im = -w.im;
+        v[r] = r == 0 ? x : cmul(x, w);
+      }
+      if (p == 2) {
+        o[0] = {v[0].re + v[1].re, v[0].im + v[1].im}; o[1] = {v[0].re - v[1].re, v[0].im - v[1].im};
+      } else if (p == 4) {
+        // W_4 = -i forward, +i inverse.
+        cx<R> s02 = {v[0].re + v[2].re, v[0].im + v[2].im}, d02 = {v[0].re - v[2].re, v[0].im - v[2].im};
+        cx<R> s13 = {v[1].re + v[3].re, v[1].im + v[3].im}, d13 = {v[1].re - v[3].re, v[1].im - v[3].im};
+        cx<R> jd13 = sign < 0 ? cx<R>{d13.im, -d13.re} : cx<R>{-d13.im, d13.re};  // d13 times W_4
+        o[0] = {s02.re + s13.re, s02.im + s13.im}; o[2] = {s02.re - s13.re, s02.im - s13.im};
+        o[1] = {d02.re + jd13.re, d02.im + jd13.im}; o[3] = {d02.re - jd13.re, d02.im - jd13.im};
+      } else {
+        for (int q = 0; q < p; q++) {
+          cx<R> acc = v[0];
+          for (int r = 1; r < p; r++) {
+            cx<R> w = tw[((r * q) % p) * wp];
+            if (sign > 0) w.im = -w.im;
+            cx<R> m = cmul(v[r], w);
+            acc.re += m.re; acc.im += m.im;
+          }
+          o[q] = acc;
+        }
+      }
+      int d = (j / Ns) * Ns * p + k;
+      for (int q = 0; q < p; q++) dst[s * sStride + (d + q * Ns) * kStride] = o[q];
+    }
+    __syncthreads();
+    cx<R> *t = src; src = dst; dst = t;
+    Ns *= p;
+  }
+  return src;
+}
+
+// LDS of a level transform: two complex buffers of max(nx ny / 2, mw ny) and the twiddles of x and y.
+__host__ __device__ inline size_t specLdsBytes(const Geo &g, size_t realBytes) {
+  size_t buf = size_t(g.layer / 2 > g.nm ? g.layer / 2 : g.nm);
+  return (2 * buf + g.nx + g.ny) * 2 * realBytes;
+}
+
+// Forward transform of L levels of the right-hand side in `cells` into the spectra, and the float64 level sums.
+template <class R> __global__ __launch_bounds__(SPEC_THREADS) void k_spec_fwd(Geo g, Bufs<R> b, Plan pl, int L) {
+  int slot = b.runs[blockIdx.y], lev = blockIdx.x;
+  const int nx = g.nx, ny = g.ny, mw = g.mw, layer = g.layer, half = ny / 2;
+  extern __shared__ char smem[];
+  const int buf = layer / 2 > g.nm ? layer / 2 : g.nm;
+  cx<R> *A = reinterpret_cast<cx<R> *>(smem), *B = A + buf, *tx = B + buf, *ty = tx + nx;
+  for (int k = threadIdx.x; k < nx; k += blockDim.x) { tx[k] = {b.twx[2 * k], b.twx[2 * k + 1]}; }
+  for (int k = threadIdx.x; k < ny; k += blockDim.x) { ty[k] = {b.twy[2 * k], b.twy[2 * k + 1]}; }
+  const R *src = b.arr(b.cells, slot, g) + size_t(lev) * layer;
+  double sum = 0;
+  for (int t = threadIdx.x; t < layer / 2; t += blockDim.x) {
+    int p = t / nx, x = t - p * nx;
+    R v0 = src[2 * p * nx + x], v1 = src[(2 * p + 1) * nx + x];
+    A[t] = {v0, v1};
+    sum += double(v0) + double(v1);
+  }
+  __shared__ double red[SPEC_THREADS / 64];
+  sum = blockSum(sum, red);  // contains a barrier
+  if (threadIdx.x == 0) b.m0(slot, g)[lev] = sum;
+  // Row pairs: z = row_2p + i row_2p+1, complex FFT of length nx.
+  cx<R> *Z = fftStages(A, B, nx, half, nx, 1, tx, pl.rx, pl.nrx, -1);
+  cx<R> *C = Z == A ? B : A;
+  // Split into the two rows' Hermitian halves (m = 0 .. nx/2): X_2p = (Z_m + conj Z_-m)/2, X_2p+1 = (Z_m - conj Z_-m)/2i.
+  for (int t = threadIdx.x; t < half * mw; t += blockDim.x) {
+    int p = t / mw, m = t - p * mw;
+    cx<R> z = Z[p * nx + m], c = Z[p * nx + (m ? nx - m : 0)];
+    C[(2 * p) * mw + m] = {(z.re + c.re) / 2, (z.im - c.im) / 2};
+    C[(2 * p + 1) * mw + m] = {(z.im + c.im) / 2, (c.re - z.re) / 2};
+  }
+  __syncthreads();
+  // Columns: mw sequences of length ny, element n of column m at n * mw + m.
+  cx<R> *Y = fftStages(C, Z, ny, mw, 1, mw, ty, pl.ry, pl.nry, -1);
+  R *out = b.sp(slot, g) + size_t(lev) * g.nm * 2;
+  for (int t = threadIdx.x; t < g.nm; t += blockDim.x) { out[2 * t] = Y[t].re; out[2 * t + 1] = Y[t].im; }
+}
+
+// Inverse transform of L levels of the spectra into dst (kind 0: the face potential phi, 1: the pressure, 2: the
+// electric potential's interior levels 1 .. nz-2), normalised by nx ny.
+template <class R> __global__ __launch_bounds__(SPEC_THREADS) void k_spec_inv(Geo g, Bufs<R> b, Plan pl, int L, int kind) {
+  int slot = b.runs[blockIdx.y], lev = blockIdx.x;
+  const int nx = g.nx, ny = g.ny, mw = g.mw, layer = g.layer, half = ny / 2, mh = nx / 2;
+  extern __shared__ char smem[];
+  const int buf = layer / 2 > g.nm ? layer / 2 : g.nm;
+  cx<R> *A = reinterpret_cast<cx<R> *>(smem), *B = A + buf, *tx = B + buf, *ty = tx + nx;
+  for (int k = threadIdx.x; k < nx; k += blockDim.x) { tx[k] = {b.twx[2 * k], b.twx[2 * k + 1]}; }
+  for (int k = threadIdx.x; k < ny; k += blockDim.x) { ty[k] = {b.twy[2 * k], b.twy[2 * k + 1]}; }
+  const R *in = b.sp(slot, g) + size_t(lev) * g.nm * 2;
+  for (int t = threadIdx.x; t < g.nm; t += blockDim.x) A[t] = {in[2 * t], in[2 * t + 1]};
+  __syncthreads();
+  cx<R> *Y = fftStages(A, B, ny, mw, 1, mw, ty, pl.ry, pl.nry, +1);
+  cx<R> *Z = Y == A ? B : A;
+  // Rows 2p, 2p+1 from their Hermitian halves: z_m = A_m + i B_m with A_-m = conj A_m; the m = 0 and m = nx/2 terms
+  // are taken real (spectral.ts inverseLayers).
+  for (int t = threadIdx.x; t < half * nx; t += blockDim.x) {
+    int p = t / nx, m = t - p * nx;
+    int c = m <= mh ? m : nx - m; R s = m <= mh ? R(1) : R(-1); bool real = c == 0 || 2 * c == nx;
+    cx<R> a = Y[(2 * p) * mw + c], bb = Y[(2 * p + 1) * mw + c];
+    R ar = a.re, ai = real ? R(0) : s * a.im, br = bb.re, bi = real ? R(0) : s * bb.im;
+    Z[p * nx + m] = {ar - bi, ai + br};
+  }
+  __syncthreads();
+  cx<R> *X = fftStages(Z, Y, nx, half, nx, 1, tx, pl.rx, pl.nrx, +1);
+  R *dst = kind == 0 ? b.arr(b.phi, slot, g) : kind == 1 ? b.arr(b.press, slot, g) : b.arr(b.ephi, slot, g) + layer;
+  dst += size_t(lev) * layer;
+  const R norm = R(1) / R(layer);
+  for (int t = threadIdx.x; t < layer / 2; t += blockDim.x) {
+    int p = t / nx, x = t - p * nx;
+    dst[2 * p * nx + x] = X[t].re * norm;
+    dst[(2 * p + 1) * nx + x] = X[t].im * norm;
+  }
+}
+
+// Row k of the tridiagonal system of mode (symbol a, b) of kind 0 (face projection, advection.ts:222-241, L = nz),
+// 1 (pressure, pressure.ts:119-140, L = nz - 1) or 2 (electric, electric.ts:42-52, L = nz - 2): lo, di, up and the
+// factor on the right-hand side.
+template <class T> __device__ inline void modeRow(int kind, int k, int L, T a, T bsym, const double *rho, const double *dzs, const double *hz, T &lo, T &di, T &up, T &rf) {
+  rf = 1;
+  if (kind == 0) {
+    auto face = [&](int j) { return j < 0 || j >= L - 1 ? T(0) : T((rho[j] + rho[j + 1]) / 2); };
+    lo = k > 0 ? face(k - 1) / T(dzs[k - 1]) : T(0);
+    up = k < L - 1 ? face(k) / T(dzs[k]) : T(0);
+    di = -T(rho[k]) * T(hz[k]) * a - lo - up;
+    rf = T(hz[k]);
+  } else if (kind == 1) {
+    di = -a / 2 * (T(rho[k]) * T(dzs[k]) / (2 * T(hz[k])) + T(rho[k + 1]) * T(dzs[k]) / (2 * T(hz[k + 1]))); lo = 0; up = 0;
+    if (k > 0) { lo = -a / 2 * T(rho[k]) * T(dzs[k - 1]) / (2 * T(hz[k])); T v = bsym * T(rho[k]) / (T(hz[k]) * T(dzs[k])); lo += v; di -= v; }
+    if (k < L - 1) { up = -a / 2 * T(rho[k + 1]) * T(dzs[k + 1]) / (2 * T(hz[k + 1])); T v = bsym * T(rho[k + 1]) / (T(hz[k + 1]) * T(dzs[k])); up += v; di -= v; }
+  } else {
+    int kk = k + 1;
+    T l0 = T(1 / (hz[kk] * dzs[kk - 1])), u0 = T(1 / (hz[kk] * dzs[kk]));
+    di = -l0 - u0 - a;
+    lo = k > 0 ? l0 : T(0); up = k < L - 1 ? u0 : T(0);
+  }
+}
+
+constexpr int MODE_THREADS = 64;
+// One thread per (mode, run); the elimination's factors stay in LDS, its right-hand sides overwrite the spectra.
+template <class R> __global__ __launch_bounds__(MODE_THREADS) void k_modes(Geo g, Bufs<R> b, int kind, int L) {
+  int slot = b.runs[blockIdx.y];
+  int mode = blockIdx.x * MODE_THREADS + threadIdx.x;
+  if (mode >= g.nm) return;
+  extern __shared__ char smem[];
+  R *cpv = reinterpret_cast<R *>(smem) + threadIdx.x;  // [k][thread]
+  const double *rho = b.lv(slot, L_RHO, g), *dzs = b.dzs, *hz = b.hz;
+  R *spec = b.sp(slot, g);
+  const size_t ls = size_t(g.nm) * 2;
+  const R a = b.sym[(kind == 0 ? 0 : kind == 1 ? 1 : 3) * g.nm + mode], bsym = kind == 1 ? b.sym[2 * g.nm + mode] : R(0);
+  if (kind == 1 && ((a == 0 && bsym == 0) || (L == 1 && a == 0))) {
+    for (int k = 0; k < L; k++) { spec[k * ls + 2 * mode] = 0; spec[k * ls + 2 * mode + 1] = 0; }
+    return;
+  }
+  const bool pinned = kind != 2 && a == 0;
+  if (mode == 0) {
+    // The level means, in float64 from the float64 sums (their imaginary part is zero).
+    const double *m0 = b.m0(slot, g);
+    double cp[64], d[64];
+    for (int k = 0; k < L && k < 64; k++) {
+      double lo, di, up, rf;
+      modeRow<double>(kind, k, L, double(a), double(bsym), rho, dzs, hz, lo, di, up, rf);
+      double r = rf * m0[k];
+      if (pinned && k == 0) { lo = 0; di = 1; up = 0; r = 0; }
+      if (pinned && k == 1) lo = 0;
+      double piv = k == 0 ? di : di - lo * cp[k - 1];
+      cp[k] = up / piv;
+      d[k] = (r - (k == 0 ? 0 : lo * d[k - 1])) / piv;
+    }
+    for (int k = L - 1; k >= 0; k--) {
+      if (k < L - 1) d[k] -= cp[k] * d[k + 1];
+      spec[k * ls] = R(d[k]); spec[k * ls + 1] = 0;
+    }
+    return;
+  }
+  R pr = 0, pi = 0;
+  for (int k = 0; k < L; k++) {
+    R lo, di, up, rf;
+    modeRow<R>(kind, k, L, a, bsym, rho, dzs, hz, lo, di, up, rf);
+    R r = rf * spec[k * ls + 2 * mode], s = rf * spec[k * ls + 2 * mode + 1];
+    if (pinned && k == 0) { lo = 0; di = 1; up = 0; r = 0; s = 0; }
+    if (pinned && k == 1) lo = 0;
+    R piv = k == 0 ? di : di - lo * cpv[(k - 1) * MODE_THREADS];
+    cpv[k * MODE_THREADS] = up / piv;
+    pr = (r - (k == 0 ? R(0) : lo * pr)) / piv;
+    pi = (s - (k == 0 ? R(0) : lo * pi)) / piv;
+    spec[k * ls + 2 * mode] = pr; spec[k * ls + 2 * mode + 1] = pi;
+  }
+  for (int k = L - 2; k >= 0; k--) {
+    R c = cpv[k * MODE_THREADS];
+    pr = spec[k * ls + 2 * mode] - c * pr; pi = spec[k * ls + 2 * mode + 1] - c * pi;
+    spec[k * ls + 2 * mode] = pr; spec[k * ls + 2 * mode + 1] = pi;
+  }
+}
+
+}  // namespace ml
