@@ -97,6 +97,10 @@ int ml_run_download_column(ml_ctx *ctx, int run, int column, float *host);
 int ml_run_levels(ml_ctx *ctx, int run, double *out);
 /* The run's diagnostics, ML_D_COUNT float64 (stream-synchronising). */
 int ml_run_diag(ml_ctx *ctx, int run, double *out);
+/* The run's most recent flashes (at most 64 are kept), oldest first: up to `max` records of 8 float64 each: time (s),
+ * CG (1) or IC (0), start node, channel nodes, charge (C: neutralized by an IC, lowered to the ground by a CG),
+ * potential (V), charge moment change (C m) (lightning.ts Flash, :57-72). Returns the number written. */
+int ml_run_flashes(ml_ctx *ctx, int run, double *out, int max);
 /* Model time of the run, s (AtmosphereModel.time). */
 int ml_run_set_time(ml_ctx *ctx, int run, double time);
 

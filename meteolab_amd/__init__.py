@@ -72,6 +72,7 @@ SIGNATURES = {
     "ml_run_levels": (_i, [_vp, _i, _dp]),
     "ml_run_diag": (_i, [_vp, _i, _dp]),
     "ml_run_set_time": (_i, [_vp, _i, _d]),
+    "ml_run_flashes": (_i, [_vp, _i, _dp, _i]),
     "ml_step": (_i, [_vp, _ip, _i, _i, _d]),
     "ml_sync": (_i, [_vp]),
     "ml_profile_step": (_i, [_vp, _ip, _i, _i, _d, _dp, _i, _cp, _i]),
@@ -219,6 +220,12 @@ class Run:
         out = np.empty(len(DIAG), np.float64)
         self.ctx.check(lib().ml_run_diag(self.ctx.h, self.h, _dptr(out)))
         return dict(zip(DIAG, out))
+
+    def flashes(self, max_count=64):
+        out = np.zeros(max_count * 8, np.float64)
+        k = self.ctx.check(lib().ml_run_flashes(self.ctx.h, self.h, _dptr(out), max_count))
+        keys = ["time", "cg", "start", "nodes", "charge", "potential", "moment"]
+        return [dict(zip(keys, out[8 * j:8 * j + 7])) for j in range(k)]
 
     def set_time(self, t):
         self.ctx.check(lib().ml_run_set_time(self.ctx.h, self.h, float(t)))

@@ -42,10 +42,18 @@ enum : int {
   S_CLIPPED = 32,        // int64: velocity components clipped since the start (model.ts:1586-1595)
   S_NEGATIVE = 33,       // double: negative mass removed by the positivity fix since the start (model.ts:1119)
   S_CHARGE_GROUND = 34,  // double: charge carried to the ground by falling carriers (model.ts:584-587)
-  S_FLASH_IC = 35, S_FLASH_CG = 36, S_FIRST_FLASH = 37,
+  S_FLASH_IC = 35, S_FLASH_CG = 36,  // int64 flash counts (model.ts:1541 flashCount)
+  S_FIRST_FLASH = 37,    // double: time of the first flash, s (-1: none yet)
   S_ERROR = 38,          // int64: nonzero when a kernel of the run met a condition it does not handle
+  S_FLASH_ID = 39,       // int64: the lightning's stamp counter (lightning.ts flashId)
+  S_FROM_GROUND = 40,    // double: charge the CG flashes took from the ground (lightning.ts fromGround)
+  S_FLASH_LOG = 41,      // int64: flashes recorded in the run's flash log (a ring of FLASH_LOG entries)
   N_SCALARS = 48
 };
+// Lightning channel limits (lightning.ts:32-37) and the per-run flash log.
+constexpr int MAX_CHANNEL = 3000, STROKES = 4, FLASH_LOG = 64, FLASH_FIELDS = 8;
+// A flash log entry: time, CG (1) or IC (0), start node, channel nodes, charge (C), potential (V), charge moment (C m).
+enum : int { FL_TIME = 0, FL_CG, FL_START, FL_NODES, FL_CHARGE, FL_POTENTIAL, FL_MOMENT };
 // The falls of a step in order (model.ts:928-932, sedimentStep): water into the gauge, then charge with its carrier's
 // speed.
 constexpr int N_FALLS = 7;
@@ -65,22 +73,29 @@ template <class R> struct Bufs {
   float *stx;      // [slot][N_CARRIED][n] RK3 stage buffer X (stage 0 output) / mixed fields
   float *sty;      // [slot][N_CARRIED][n] RK3 stage buffer Y (stage 1 output)
   R *mass;         // [slot][4][n]  face mass fluxes mx, my, mz and their divergence
-  R *phi;          // [slot][n]     potential of the face projection
-  R *cells;        // [slot][n]     right-hand side of a spectral solve (cells or interior layers)
-  R *press;        // [slot][n]     pressure of the projection (model.ts:358), cells
-  R *spec;         // [slot][nz][nm][2] Hermitian half spectra, re/im interleaved
+  double *phi;     // [slot][n]     potential of the face projection (float64, the spectral solves' output)
+  double *cells;   // [slot][n]     right-hand side of a spectral solve (cells or interior layers) (float64)
+  double *press;   // [slot][n]     pressure of the projection (model.ts:358), cells (float64)
+  double *spec;    // [slot][nz][nm][2] Hermitian half spectra, re/im interleaved (float64: see spectral.h)
   double *mean0;   // [slot][nz]    float64 layer sums of the right-hand side (the m = n = 0 mode)
   R *km, *kh;      // [slot][n]     eddy viscosities
-  R *ephi;         // [slot][n]     electric potential (electric.ts:36-60)
+  double *ephi;    // [slot][n]     electric potential (electric.ts:36-60) (float64)
   float *cols;     // [slot][N_COLUMNS][layer]
   double *lev;     // [slot][N_LEVELS][nz]
   double *par;     // [slot][N_PARAMS]
   double *scal;    // [slot][N_SCALARS]
   const double *zs, *dzs, *hz;   // class levels (nz, nz - 1, nz)
-  const R *sym;    // [4][nm] per kept mode: face-projection lambda, pressure alpha, beta, electric lambda
-  const R *twx;    // [nx][2] exp(-2 pi i k / nx)
-  const R *twy;    // [ny][2] exp(-2 pi i k / ny)
+  const double *sym;  // [4][nm] per kept mode: face-projection lambda, pressure alpha, beta, electric lambda
+  const double *twx;  // [nx][2] exp(-2 pi i k / nx)
+  const double *twy;  // [ny][2] exp(-2 pi i k / ny)
   const int *runs; // active slots of this launch
+  // Lightning state of each slot (lightning.ts:135-170): node charge density, the channel's per-node marks and front.
+  R *lrho;         // [slot][n]     charge density of the last solve, C/m3
+  int *lmark;      // [slot][5][n]  mark, failed, seen, link, frontier
+  double *lnode;   // [slot][2][n]  scale (link length times threshold) and the front's weights
+  int *chan;       // [slot][2][MAX_CHANNEL] channel nodes and their parents
+  double *cdep;    // [slot][MAX_CHANNEL] charge deposited on each channel node
+  double *flog;    // [slot][FLASH_LOG][FLASH_FIELDS]
 
   __device__ float *f(int slot, int k, const Geo &g) const { return fields + (size_t(slot) * N_FIELDS + k) * g.n; }
   __device__ float *x(int slot, int k, const Geo &g) const { return stx + (size_t(slot) * N_CARRIED + k) * g.n; }
@@ -88,13 +103,15 @@ template <class R> struct Bufs {
   // Buffer set: 0 the state, 1 stage buffer X, 2 stage buffer Y.
   __device__ float *set(int which, int slot, int k, const Geo &g) const { return which == 0 ? f(slot, k, g) : which == 1 ? x(slot, k, g) : y(slot, k, g); }
   __device__ R *m(int slot, int k, const Geo &g) const { return mass + (size_t(slot) * 4 + k) * g.n; }
-  __device__ R *arr(R *base, int slot, const Geo &g) const { return base + size_t(slot) * g.n; }
-  __device__ R *sp(int slot, const Geo &g) const { return spec + size_t(slot) * g.nz * g.nm * 2; }
+  template <class T> __device__ T *arr(T *base, int slot, const Geo &g) const { return base + size_t(slot) * g.n; }
+  __device__ double *sp(int slot, const Geo &g) const { return spec + size_t(slot) * g.nz * g.nm * 2; }
   __device__ double *m0(int slot, const Geo &g) const { return mean0 + size_t(slot) * g.nz; }
   __device__ float *col(int slot, int k, const Geo &g) const { return cols + (size_t(slot) * N_COLUMNS + k) * g.layer; }
   __device__ double *lv(int slot, int k, const Geo &g) const { return lev + (size_t(slot) * N_LEVELS + k) * g.nz; }
   __device__ double *pr(int slot) const { return par + size_t(slot) * N_PARAMS; }
   __device__ double *sc(int slot) const { return scal + size_t(slot) * N_SCALARS; }
+  __device__ int *lm(int slot, int k, const Geo &g) const { return lmark + (size_t(slot) * 5 + k) * g.n; }
+  __device__ double *ln(int slot, int k, const Geo &g) const { return lnode + (size_t(slot) * 2 + k) * g.n; }
 };
 
 // ---- wave64 / block reductions --------------------------------------------------------------------------------

@@ -181,14 +181,14 @@ template <class R> __device__ inline void faceFluxes(const Geo &g, const double 
   mz = z < g.nz - 1 ? R((rho[z] + rho[z + 1]) / 2) * (R(w[i]) + R(w[i + g.layer])) / 2 : R(0);
 }
 // Face potential correction (advection.ts:244-255 correctFaces) of the faces of node (x, y, z).
-template <class R> __device__ inline void correctedFaces(const Geo &g, const Bufs<R> &b, const double *rho, const float *u, const float *v, const float *w, const R *phi, int x, int y, int z, R &mx, R &my, R &mz) {
+template <class R> __device__ inline void correctedFaces(const Geo &g, const Bufs<R> &b, const double *rho, const float *u, const float *v, const float *w, const double *phi, int x, int y, int z, R &mx, R &my, R &mz) {
   faceFluxes(g, rho, u, v, w, x, y, z, mx, my, mz);
   int i = x + g.nx * (y + g.ny * z), l = z * g.layer;
   int xp = x + 1 == g.nx ? 0 : x + 1, yp = y + 1 == g.ny ? 0 : y + 1;
   R r = R(rho[z]);
-  mx -= r * (phi[xp + y * g.nx + l] - phi[i]) / R(g.dx);
-  my -= r * (phi[x + yp * g.nx + l] - phi[i]) / R(g.dy);
-  if (z < g.nz - 1) mz -= R((rho[z] + rho[z + 1]) / 2) * (phi[i + g.layer] - phi[i]) / R(b.dzs[z]);
+  mx -= r * R(phi[xp + y * g.nx + l] - phi[i]) / R(g.dx);
+  my -= r * R(phi[x + yp * g.nx + l] - phi[i]) / R(g.dy);
+  if (z < g.nz - 1) mz -= R((rho[z] + rho[z + 1]) / 2) * R(phi[i + g.layer] - phi[i]) / R(b.dzs[z]);
 }
 // corrected = 0: faces and divergence of the node velocities (the right-hand side of the face projection);
 // corrected = 1: the projected faces (faces minus the potential's gradient) and their divergence.
@@ -199,24 +199,34 @@ template <class R> __global__ __launch_bounds__(TPB) void k_faces(Geo g, Bufs<R>
   int z = i / g.layer, y = (i - z * g.layer) / g.nx, x = i - z * g.layer - y * g.nx;
   const double *rho = b.lv(slot, L_RHO, g);
   const float *u = b.f(slot, F_U, g), *v = b.f(slot, F_V, g), *w = b.f(slot, F_W, g);
-  const R *phi = b.arr(b.phi, slot, g);
-  int xm = x == 0 ? g.nx - 1 : x - 1, ym = y == 0 ? g.ny - 1 : y - 1;
-  R mx, my, mz, mxw, myd, mzb = 0, t0, t1;
-  if (corrected) {
-    correctedFaces(g, b, rho, u, v, w, phi, x, y, z, mx, my, mz);
-    correctedFaces(g, b, rho, u, v, w, phi, xm, y, z, mxw, t0, t1);
-    correctedFaces(g, b, rho, u, v, w, phi, x, ym, z, t0, myd, t1);
-    if (z > 0) correctedFaces(g, b, rho, u, v, w, phi, x, y, z - 1, t0, t1, mzb);
-  } else {
-    faceFluxes(g, rho, u, v, w, x, y, z, mx, my, mz);
-    faceFluxes(g, rho, u, v, w, xm, y, z, mxw, t0, t1);
-    faceFluxes(g, rho, u, v, w, x, ym, z, t0, myd, t1);
-    if (z > 0) faceFluxes(g, rho, u, v, w, x, y, z - 1, t0, t1, mzb);
-  }
+  const double *phi = b.arr(b.phi, slot, g);
+  int xm = x == 0 ? g.nx - 1 : x - 1, ym = y == 0 ? g.ny - 1 : y - 1, xp = x + 1 == g.nx ? 0 : x + 1, yp = y + 1 == g.ny ? 0 : y + 1;
+  int l = z * g.layer;
+  R mx, my, mz;
+  if (corrected) correctedFaces(g, b, rho, u, v, w, phi, x, y, z, mx, my, mz);
+  else faceFluxes(g, rho, u, v, w, x, y, z, mx, my, mz);
   b.m(slot, 0, g)[i] = mx; b.m(slot, 1, g)[i] = my; b.m(slot, 2, g)[i] = mz;
-  R div = (mx - mxw) / R(g.dx) + (my - myd) / R(g.dy) + (mz - mzb) / R(b.hz[z]);
-  // The right-hand side of the face projection is the divergence itself (modes multiplies by h).
-  if (corrected) b.m(slot, 3, g)[i] = div; else b.arr(b.cells, slot, g)[i] = div;
+  // The divergence of the faces in difference form: mx_i - mx_{i-1} = rho (u_{i+1} - u_{i-1}) / 2, so no float sum of
+  // two ~25 m/s winds is ever differenced; float64, as the right-hand side of the projection (whose low modes amplify
+  // its rounding).
+  double rzd = rho[z];
+  double dv = rzd * (double(u[xp + y * g.nx + l]) - double(u[xm + y * g.nx + l])) / (2 * g.dx) + rzd * (double(v[x + yp * g.nx + l]) - double(v[x + ym * g.nx + l])) / (2 * g.dy);
+  double dup = z < g.nz - 1 ? (rho[z] + rho[z + 1]) / 2 * (double(w[i]) + double(w[i + g.layer])) : 0.0;
+  double ddn = z > 0 ? (rho[z - 1] + rho[z]) / 2 * (double(w[i - g.layer]) + double(w[i])) : 0.0;
+  dv += (dup - ddn) / (2 * b.hz[z]);
+  R rz = R(rzd), div = R(dv);
+  if (corrected) {
+    // Minus the divergence of rho grad(phi) through the same faces (differences of phi first).
+    double pc = phi[i];
+    R lx = R((phi[xp + y * g.nx + l] - pc) - (pc - phi[xm + y * g.nx + l])), ly = R((phi[x + yp * g.nx + l] - pc) - (pc - phi[x + ym * g.nx + l]));
+    R gu = z < g.nz - 1 ? R((rho[z] + rho[z + 1]) / 2) * R(phi[i + g.layer] - pc) / R(b.dzs[z]) : R(0);
+    R gd = z > 0 ? R((rho[z - 1] + rho[z]) / 2) * R(pc - phi[i - g.layer]) / R(b.dzs[z - 1]) : R(0);
+    div -= rz * lx / R(g.dx * g.dx) + rz * ly / R(g.dy * g.dy) + (gu - gd) / R(b.hz[z]);
+    b.m(slot, 3, g)[i] = div;
+  } else {
+    // The right-hand side of the face projection is the divergence itself (modes multiplies by h).
+    b.arr(b.cells, slot, g)[i] = dv;
+  }
 }
 
 // ---- transport: flux-form WENO5 / linear upwind tendency and the RK3 stage update --------------------------------
@@ -241,6 +251,7 @@ template <class R> __device__ inline R wenoFace(R a, R b_, R c, R d, R e) {
   return s * ((w0 * p0 + w1 * p1 + w2 * p2) / (w0 + w1 + w2));
 }
 template <class R> __device__ inline R upwindFace(R a, R b_, R c, R d, R e) { return (2 * a - 13 * b_ + 47 * c + 27 * d - 3 * e) / 60; }
+__device__ inline double upwind5d(double a, double b_, double c, double d, double e) { return (2 * a - 13 * b_ + 47 * c + 27 * d - 3 * e) / 60; }
 template <class R> __device__ inline R faceValue(bool monotone, R a, R b_, R c, R d, R e) { return monotone ? wenoFace(a, b_, c, d, e) : upwindFace(a, b_, c, d, e); }
 // Flux through the face between node values q0 (this node) and q1 (next), with upstream neighbours.
 // Flow towards +: stencil (m2, m1, q0, p1, p2); towards -: (p3, p2, p1, q0, m1).
@@ -262,8 +273,14 @@ template <class R> __global__ __launch_bounds__(TPB) void k_stage(Geo g, Bufs<R>
   extern __shared__ char smem[];
   float *ql = reinterpret_cast<float *>(smem);
   R *fxl = reinterpret_cast<R *>(smem + ((layer * sizeof(float) + 15) / 16) * 16), *fyl = fxl + layer;
+  // u, v and theta (linear upwind5) are transported as departures from their level's base profile B(z), plus the
+  // profile's own tendency: the scheme is linear, so this is the same tendency, without float32 sums of ~25 m/s or
+  // ~300 K values (the reference's WebGPU step carries theta the same way, gpu/dynamics.ts:78-97).
+  const int lvl = f == F_U ? L_UENV : f == F_V ? L_VENV : f == F_THETA ? L_THETA : -1;
+  const double *B = lvl >= 0 ? b.lv(slot, lvl, g) : nullptr;
+  const double Bz = B ? B[z] : 0;
   const float *qz = q + size_t(z) * layer;
-  for (int c = threadIdx.x; c < layer; c += TPB) ql[c] = qz[c];
+  for (int c = threadIdx.x; c < layer; c += TPB) ql[c] = B ? float(double(qz[c]) - Bz) : qz[c];
   __syncthreads();
   const bool mono = monotoneField(f);
   const R *mx = b.m(slot, 0, g) + size_t(z) * layer, *my = b.m(slot, 1, g) + size_t(z) * layer;
@@ -282,18 +299,28 @@ template <class R> __global__ __launch_bounds__(TPB) void k_stage(Geo g, Bufs<R>
   double thvEnv = 0;
   if (isW) { const double te = b.lv(slot, L_THETA, g)[z], qe = b.lv(slot, L_Q, g)[z]; thvEnv = te * (1 + .61 * qe); }
   int zc[7];
-  for (int k = 0; k < 7; k++) { int zz = z - 3 + k; zc[k] = (zz < 0 ? 0 : zz > nz - 1 ? nz - 1 : zz) * layer; }
+  double Bc[7];
+  for (int k = 0; k < 7; k++) { int zz = z - 3 + k; zz = zz < 0 ? 0 : zz > nz - 1 ? nz - 1 : zz; zc[k] = zz * layer; Bc[k] = B ? B[zz] : 0; }
+  // The base profile's face values at the top and bottom faces of the level for either flow direction (float64),
+  // minus B(z): its tendency is the vertical flux of these, the horizontal fluxes cancelling against -B div.
+  R pt[2] = {0, 0}, pb[2] = {0, 0};
+  if (B) {
+    pt[0] = R(upwind5d(Bc[1], Bc[2], Bc[3], Bc[4], Bc[5]) - Bz); pt[1] = R(upwind5d(Bc[6], Bc[5], Bc[4], Bc[3], Bc[2]) - Bz);
+    pb[0] = R(upwind5d(Bc[0], Bc[1], Bc[2], Bc[3], Bc[4]) - Bz); pb[1] = R(upwind5d(Bc[5], Bc[4], Bc[3], Bc[2], Bc[1]) - Bz);
+  }
   for (int c = threadIdx.x; c < layer; c += TPB) {
     int y = c / nx, x = c - y * nx;
     int i = z * layer + c;
     R col[7];
-    for (int k = 0; k < 7; k++) col[k] = k == 3 ? R(ql[c]) : R(q[zc[k] + c]);
+    for (int k = 0; k < 7; k++) col[k] = k == 3 ? R(ql[c]) : B ? R(double(q[zc[k] + c]) - Bc[k]) : R(q[zc[k] + c]);
     // Top face of this node (none through the model top) and of the node below (none into the ground).
-    R fzt = z == nz - 1 ? R(0) : faceFlux(mono, mz[i], col[1], col[2], col[3], col[4], col[5], col[6]);
-    R fzb = z == 0 ? R(0) : faceFlux(mono, mz[i - layer], col[0], col[1], col[2], col[3], col[4], col[5]);
+    R mzt = z == nz - 1 ? R(0) : mz[i], mzb = z == 0 ? R(0) : mz[i - layer];
+    R fzt = z == nz - 1 ? R(0) : faceFlux(mono, mzt, col[1], col[2], col[3], col[4], col[5], col[6]);
+    R fzb = z == 0 ? R(0) : faceFlux(mono, mzb, col[0], col[1], col[2], col[3], col[4], col[5]);
     int xm = x == 0 ? nx - 1 : x - 1, ym = y == 0 ? ny - 1 : y - 1;
     R flux = (fxl[c] - fxl[y * nx + xm]) / R(g.dx) + (fyl[c] - fyl[ym * nx + x]) / R(g.dy) + (fzt - fzb) / hz;
     R tend = -(flux - R(ql[c]) * div[i]) / rz;
+    if (B) tend -= (mzt * pt[mzt >= 0 ? 0 : 1] - mzb * pb[mzb >= 0 ? 0 : 1]) / (hz * rz);
     if (isW) {
       // g B with the fields of this stage; the departure of theta_v from the environment in float64 (it is a small
       // difference of numbers near 300 K).
@@ -321,11 +348,12 @@ template <class R> __global__ __launch_bounds__(TPB) void k_pdiv(Geo g, Bufs<R> 
   const double *rho = b.lv(slot, L_RHO, g);
   int l0 = k * g.layer, l1 = l0 + g.layer, y0 = j * g.nx, y1 = (j + 1 == g.ny ? 0 : j + 1) * g.nx, i1 = i + 1 == g.nx ? 0 : i + 1;
   int a = i + y0 + l0, bb = i1 + y0 + l0, e = i + y1 + l0, f = i1 + y1 + l0, gg = i + y0 + l1, h = i1 + y0 + l1, s = i + y1 + l1, t = i1 + y1 + l1;
-  R r0 = R(rho[k] / 4), r1 = R(rho[k + 1] / 4);
-  R ddx = (r0 * (R(u[bb]) + u[f] - u[a] - u[e]) + r1 * (R(u[h]) + u[t] - u[gg] - u[s])) / R(g.dx);
-  R ddy = (r0 * (R(v[e]) + v[f] - v[a] - v[bb]) + r1 * (R(v[s]) + v[t] - v[gg] - v[h])) / R(g.dy);
-  R ddz = (r1 * (R(w[gg]) + w[h] + w[s] + w[t]) - r0 * (R(w[a]) + w[bb] + w[e] + w[f])) / R(b.dzs[k]);
-  b.arr(b.cells, slot, g)[c] = (ddx + ddy + ddz) / R(dt);
+  // float64 from the float32 winds (the sums of four winds are exact in float64), as the reference computes it.
+  double r0 = rho[k] / 4, r1 = rho[k + 1] / 4;
+  double ddx = (r0 * (double(u[bb]) + u[f] - u[a] - u[e]) + r1 * (double(u[h]) + u[t] - u[gg] - u[s])) / g.dx;
+  double ddy = (r0 * (double(v[e]) + v[f] - v[a] - v[bb]) + r1 * (double(v[s]) + v[t] - v[gg] - v[h])) / g.dy;
+  double ddz = (r1 * (double(w[gg]) + w[h] + w[s] + w[t]) - r0 * (double(w[a]) + w[bb] + w[e] + w[f])) / b.dzs[k];
+  b.arr(b.cells, slot, g)[c] = (ddx + ddy + ddz) / dt;
 }
 // final = 0: the stage projection, in place on set `set`. final = 1: the end-of-step projection of the mixed wind in X
 // into the state, with the safety limits (model.ts:1586-1595 clipWind), and the other mixed fields copied from X.
@@ -336,19 +364,20 @@ template <class R> __global__ __launch_bounds__(TPB) void k_pcorrect(Geo g, Bufs
   unsigned long long clipped = 0;
   if (live) {
     int k = node / g.layer, j = (node - k * g.layer) / g.nx, i = node - k * g.layer - j * g.nx;
-    const R *p = b.arr(b.press, slot, g);
+    const double *p = b.arr(b.press, slot, g);
     int cl = g.layer, nz = g.nz;
     int lo = (k > 0 ? k - 1 : 0) * cl, hi = (k < nz - 2 ? k : nz - 2) * cl;
     R wLo = k > 0 ? R(b.dzs[k - 1] / b.hz[k]) : R(0), wHi = k < nz - 1 ? R(b.dzs[k] / b.hz[k]) : R(0);
     int jm = (j == 0 ? g.ny - 1 : j - 1) * g.nx, j0 = j * g.nx, im = i == 0 ? g.nx - 1 : i - 1;
-    auto P = [&](int ii, int jj, int base) { return p[base + ii + jj]; };
-    R gx = (wLo * (P(i, jm, lo) + P(i, j0, lo) - P(im, jm, lo) - P(im, j0, lo)) + wHi * (P(i, jm, hi) + P(i, j0, hi) - P(im, jm, hi) - P(im, j0, hi))) / (4 * R(g.dx));
-    R gy = (wLo * (P(im, j0, lo) + P(i, j0, lo) - P(im, jm, lo) - P(i, jm, lo)) + wHi * (P(im, j0, hi) + P(i, j0, hi) - P(im, jm, hi) - P(i, jm, hi))) / (4 * R(g.dy));
+    auto P = [&](int ii, int jj, int base) { return p[base + ii + jj]; };  // float64: differences before rounding
+    // Differences of neighbouring pressures first (as for the divergence).
+    R gx = (wLo * R((P(i, jm, lo) - P(im, jm, lo)) + (P(i, j0, lo) - P(im, j0, lo))) + wHi * R((P(i, jm, hi) - P(im, jm, hi)) + (P(i, j0, hi) - P(im, j0, hi)))) / (4 * R(g.dx));
+    R gy = (wLo * R((P(im, j0, lo) - P(im, jm, lo)) + (P(i, j0, lo) - P(i, jm, lo))) + wHi * R((P(im, j0, hi) - P(im, jm, hi)) + (P(i, j0, hi) - P(i, jm, hi)))) / (4 * R(g.dy));
     float *u = b.set(set, slot, F_U, g), *v = b.set(set, slot, F_V, g), *w = b.set(set, slot, F_W, g);
     float un = float(R(u[node]) - R(dt) * gx), vn = float(R(v[node]) - R(dt) * gy), wn = w[node];
     if (k > 0 && k < nz - 1) {
       int below = (k - 1) * cl, above = k * cl;
-      R gz = (P(im, jm, above) + P(i, jm, above) + P(im, j0, above) + P(i, j0, above) - P(im, jm, below) - P(i, jm, below) - P(im, j0, below) - P(i, j0, below)) / (4 * R(b.hz[k]));
+      R gz = R(((P(im, jm, above) - P(im, jm, below)) + (P(i, jm, above) - P(i, jm, below))) + ((P(im, j0, above) - P(im, j0, below)) + (P(i, j0, above) - P(i, j0, below)))) / (4 * R(b.hz[k]));
       wn = float(R(wn) - R(dt) * gz);
     }
     if (final) {

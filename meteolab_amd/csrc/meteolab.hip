@@ -42,7 +42,8 @@ struct Pool {
   size_t rb = 4;  // bytes of the arithmetic type
   void *fields = nullptr, *stx = nullptr, *sty = nullptr, *mass = nullptr, *phi = nullptr, *cells = nullptr, *press = nullptr,
        *spec = nullptr, *mean0 = nullptr, *km = nullptr, *kh = nullptr, *ephi = nullptr, *cols = nullptr, *lev = nullptr,
-       *par = nullptr, *scal = nullptr, *zs = nullptr, *dzs = nullptr, *hz = nullptr, *sym = nullptr, *twx = nullptr, *twy = nullptr;
+       *par = nullptr, *scal = nullptr, *zs = nullptr, *dzs = nullptr, *hz = nullptr, *sym = nullptr, *twx = nullptr, *twy = nullptr,
+       *lrho = nullptr, *lmark = nullptr, *lnode = nullptr, *chan = nullptr, *cdep = nullptr, *flog = nullptr;
   int *single = nullptr;  // single[s] = s: the run list of one slot
   double *diag = nullptr; // [cap][N_DIAG_DEV]
   std::vector<double> zsH, dzsH, hzH;
@@ -98,11 +99,12 @@ bool radices(int n, int *out, int &count) {
 
 template <class R> Bufs<R> bufs(const Pool &p, const int *runs) {
   Bufs<R> b;
-  b.fields = (float *)p.fields; b.stx = (float *)p.stx; b.sty = (float *)p.sty; b.mass = (R *)p.mass; b.phi = (R *)p.phi;
-  b.cells = (R *)p.cells; b.press = (R *)p.press; b.spec = (R *)p.spec; b.mean0 = (double *)p.mean0; b.km = (R *)p.km; b.kh = (R *)p.kh;
-  b.ephi = (R *)p.ephi; b.cols = (float *)p.cols; b.lev = (double *)p.lev; b.par = (double *)p.par; b.scal = (double *)p.scal;
-  b.zs = (const double *)p.zs; b.dzs = (const double *)p.dzs; b.hz = (const double *)p.hz; b.sym = (const R *)p.sym;
-  b.twx = (const R *)p.twx; b.twy = (const R *)p.twy; b.runs = runs;
+  b.fields = (float *)p.fields; b.stx = (float *)p.stx; b.sty = (float *)p.sty; b.mass = (R *)p.mass; b.phi = (double *)p.phi;
+  b.cells = (double *)p.cells; b.press = (double *)p.press; b.spec = (double *)p.spec; b.mean0 = (double *)p.mean0; b.km = (R *)p.km; b.kh = (R *)p.kh;
+  b.ephi = (double *)p.ephi; b.cols = (float *)p.cols; b.lev = (double *)p.lev; b.par = (double *)p.par; b.scal = (double *)p.scal;
+  b.zs = (const double *)p.zs; b.dzs = (const double *)p.dzs; b.hz = (const double *)p.hz; b.sym = (const double *)p.sym;
+  b.twx = (const double *)p.twx; b.twy = (const double *)p.twy; b.runs = runs;
+  b.lrho = (R *)p.lrho; b.lmark = (int *)p.lmark; b.lnode = (double *)p.lnode; b.chan = (int *)p.chan; b.cdep = (double *)p.cdep; b.flog = (double *)p.flog;
   return b;
 }
 
@@ -121,9 +123,9 @@ struct Timer {
 
 template <class R> void spectralSolve(const Pool &p, const Bufs<R> &b, int nr, hipStream_t s, int kind, int L, int invKind) {
   const Geo &g = p.g;
-  size_t lds = specLdsBytes(g, sizeof(R));
+  size_t lds = specLdsBytes(g, sizeof(double));
   hipLaunchKernelGGL(k_spec_fwd<R>, dim3(L, nr), dim3(SPEC_THREADS), lds, s, g, b, p.plan, L);
-  hipLaunchKernelGGL(k_modes<R>, dim3(blocks(g.nm, MODE_THREADS), nr), dim3(MODE_THREADS), size_t(MODE_THREADS) * L * sizeof(R), s, g, b, kind, L);
+  hipLaunchKernelGGL(k_modes<R>, dim3(blocks(g.nm, MODE_THREADS), nr), dim3(MODE_THREADS), size_t(MODE_THREADS) * L * sizeof(double), s, g, b, kind, L);
   hipLaunchKernelGGL(k_spec_inv<R>, dim3(L, nr), dim3(SPEC_THREADS), lds, s, g, b, p.plan, L, invKind);
 }
 
@@ -172,16 +174,20 @@ template <class R> void launchStep(const Pool &p, const int *list, int nr, doubl
   // afterStep: lightning (model.ts:1537-1552).
   if (el) {
     unsigned ib = blocks(size_t(g.layer) * (g.nz - 2), 256);
-    hipLaunchKernelGGL(k_charge_rhs<R>, dim3(ib, nr), dim3(256), 0, s, g, b);
+    hipLaunchKernelGGL(k_charge_rhs<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b);
     spectralSolve<R>(p, b, nr, s, 2, g.nz - 2, 2);
     hipLaunchKernelGGL(k_strongest<R>, dim3(ib, nr), dim3(256), 0, s, g, b);
+    // flashLimit (lightning.ts:46): 10 flashes per 3 s of model time.
+    int limit = std::max(1, int(std::floor(10 * dt / 3 + .5)));
+    hipLaunchKernelGGL(k_discharge<R>, dim3(nr), dim3(DISCHARGE_THREADS), specLdsBytes(g, sizeof(double)), s, g, b, p.plan, dt, limit);
     hipLaunchKernelGGL(k_conduct<R>, dim3(ib, nr), dim3(256), 0, s, g, b, dt);
     mark("lightning");
   }
   hipLaunchKernelGGL(k_end<R>, dim3(nr), dim3(64), 0, s, g, b, dt);
 }
 
-template <class R> void fillTables(Pool &p, std::vector<char> &sym, std::vector<char> &twx, std::vector<char> &twy) {
+void fillTables(Pool &p, std::vector<char> &sym, std::vector<char> &twx, std::vector<char> &twy) {
+  typedef double R;
   const Geo &g = p.g;
   std::vector<R> s(4 * size_t(g.nm)), tx(2 * g.nx), ty(2 * g.ny);
   auto exact = [](double x) { return x < 1e-12 ? 0 : x > 1 - 1e-12 ? 1 : x; };
@@ -202,12 +208,17 @@ template <class R> void fillTables(Pool &p, std::vector<char> &sym, std::vector<
   twy.assign((char *)ty.data(), (char *)(ty.data() + ty.size()));
 }
 
-struct Sizes { size_t fields, stage, mass, node, spec, mean0, cols, lev, par, scal; size_t slot() const { return fields + 2 * stage + mass + 6 * node + spec + mean0 + cols + lev + par + scal; } };
+struct Sizes {
+  size_t fields, stage, mass, node, node8, spec, mean0, cols, lev, par, scal, lmark, lnode, chan, cdep, flog;
+  size_t slot() const { return fields + 2 * stage + mass + 3 * node + 4 * node8 + spec + mean0 + cols + lev + par + scal + lmark + lnode + chan + cdep + flog; }
+};
 Sizes sizesOf(const Geo &g, size_t rb) {
   Sizes s;
-  s.fields = size_t(N_FIELDS) * g.n * 4; s.stage = size_t(N_CARRIED) * g.n * 4; s.mass = 4 * size_t(g.n) * rb; s.node = size_t(g.n) * rb;
-  s.spec = size_t(g.nz) * g.nm * 2 * rb; s.mean0 = size_t(g.nz) * 8; s.cols = size_t(N_COLUMNS) * g.layer * 4;
+  s.fields = size_t(N_FIELDS) * g.n * 4; s.stage = size_t(N_CARRIED) * g.n * 4; s.mass = 4 * size_t(g.n) * rb; s.node = size_t(g.n) * rb; s.node8 = size_t(g.n) * 8;
+  s.spec = size_t(g.nz) * g.nm * 2 * 8; s.mean0 = size_t(g.nz) * 8; s.cols = size_t(N_COLUMNS) * g.layer * 4;
   s.lev = size_t(N_LEVELS) * g.nz * 8; s.par = N_PARAMS * 8; s.scal = N_SCALARS * 8;
+  s.lmark = 5 * size_t(g.n) * 4; s.lnode = 2 * size_t(g.n) * 8; s.chan = 2 * size_t(MAX_CHANNEL) * 4; s.cdep = size_t(MAX_CHANNEL) * 8;
+  s.flog = size_t(FLASH_LOG) * FLASH_FIELDS * 8;
   return s;
 }
 int geoOf(ml_ctx *c, const ml_grid *o, Geo &g, Plan &pl) {
@@ -263,7 +274,8 @@ void ml_ctx_destroy(ml_ctx *c) {
     if (!p) continue;
     dropBatches(p, -1);
     void *all[] = {p->fields, p->stx, p->sty, p->mass, p->phi, p->cells, p->press, p->spec, p->mean0, p->km, p->kh, p->ephi, p->cols,
-                   p->lev, p->par, p->scal, p->zs, p->dzs, p->hz, p->sym, p->twx, p->twy, p->single, p->diag};
+                   p->lev, p->par, p->scal, p->zs, p->dzs, p->hz, p->sym, p->twx, p->twy, p->single, p->diag, p->lrho, p->lmark,
+                   p->lnode, p->chan, p->cdep, p->flog};
     for (void *a : all) if (a) hipFree(a);
     delete p;
   }
@@ -295,14 +307,15 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
   p->cap = capacity; p->electrified = electrified ? 1 : 0; p->rb = c->precision == 64 ? 8 : 4;
   // LDS the kernels need must fit the device's block limit.
   size_t stageLds = ((g.layer * sizeof(float) + 15) / 16) * 16 + 2 * g.layer * p->rb;
-  if (stageLds > c->prop.sharedMemPerBlock || specLdsBytes(g, p->rb) > c->prop.sharedMemPerBlock) { delete p; return fail(c, ML_E_UNSUPPORTED, "grid too large for a level in LDS"); }
+  if (stageLds > c->prop.sharedMemPerBlock || specLdsBytes(g, 8) + 4096 > c->prop.sharedMemPerBlock) { delete p; return fail(c, ML_E_UNSUPPORTED, "grid too large for a level in LDS"); }
   Sizes s = sizesOf(g, p->rb);
   size_t cap = capacity;
   struct { void **ptr; size_t bytes; } allocs[] = {
-    {&p->fields, s.fields * cap}, {&p->stx, s.stage * cap}, {&p->sty, s.stage * cap}, {&p->mass, s.mass * cap}, {&p->phi, s.node * cap},
-    {&p->cells, s.node * cap}, {&p->press, s.node * cap}, {&p->spec, s.spec * cap}, {&p->mean0, s.mean0 * cap}, {&p->km, s.node * cap},
-    {&p->kh, s.node * cap}, {&p->ephi, s.node * cap}, {&p->cols, s.cols * cap}, {&p->lev, s.lev * cap}, {&p->par, s.par * cap},
-    {&p->scal, s.scal * cap}, {(void **)&p->diag, cap * N_DIAG_DEV * 8}};
+    {&p->fields, s.fields * cap}, {&p->stx, s.stage * cap}, {&p->sty, s.stage * cap}, {&p->mass, s.mass * cap}, {&p->phi, s.node8 * cap},
+    {&p->cells, s.node8 * cap}, {&p->press, s.node8 * cap}, {&p->spec, s.spec * cap}, {&p->mean0, s.mean0 * cap}, {&p->km, s.node * cap},
+    {&p->kh, s.node * cap}, {&p->ephi, s.node8 * cap}, {&p->cols, s.cols * cap}, {&p->lev, s.lev * cap}, {&p->par, s.par * cap},
+    {&p->scal, s.scal * cap}, {(void **)&p->diag, cap * N_DIAG_DEV * 8}, {&p->lrho, s.node * cap}, {&p->lmark, s.lmark * cap},
+    {&p->lnode, s.lnode * cap}, {&p->chan, s.chan * cap}, {&p->cdep, s.cdep * cap}, {&p->flog, s.flog * cap}};
   for (auto &a : allocs) {
     hipError_t e = hipMalloc(a.ptr, a.bytes);
     if (e == hipSuccess) e = hipMemset(*a.ptr, 0, a.bytes);
@@ -314,7 +327,7 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
   }
   levels(*grid, p->zsH, p->dzsH, p->hzH);
   std::vector<char> sym, twx, twy;
-  if (p->rb == 8) fillTables<double>(*p, sym, twx, twy); else fillTables<float>(*p, sym, twx, twy);
+  fillTables(*p, sym, twx, twy);
   std::vector<int> single(capacity); for (int k = 0; k < capacity; k++) single[k] = k;
   struct { void **ptr; const void *src; size_t bytes; } tabs[] = {
     {&p->zs, p->zsH.data(), p->zsH.size() * 8}, {&p->dzs, p->dzsH.data(), p->dzsH.size() * 8}, {&p->hz, p->hzH.data(), p->hzH.size() * 8},
@@ -359,8 +372,12 @@ int ml_run_create(ml_ctx *c, int pool, const ml_profile *pr, const ml_run_config
   Sizes s = sizesOf(g, p->rb);
   HIPCHK(c, hipMemsetAsync((char *)p->stx + s.stage * slot, 0, s.stage, c->stream));
   HIPCHK(c, hipMemsetAsync((char *)p->sty + s.stage * slot, 0, s.stage, c->stream));
-  HIPCHK(c, hipMemsetAsync((char *)p->ephi + s.node * slot, 0, s.node, c->stream));
-  HIPCHK(c, hipMemsetAsync((char *)p->press + s.node * slot, 0, s.node, c->stream));
+  HIPCHK(c, hipMemsetAsync((char *)p->ephi + s.node8 * slot, 0, s.node8, c->stream));
+  HIPCHK(c, hipMemsetAsync((char *)p->press + s.node8 * slot, 0, s.node8, c->stream));
+  HIPCHK(c, hipMemsetAsync((char *)p->lmark + s.lmark * slot, 0, s.lmark, c->stream));
+  HIPCHK(c, hipMemsetAsync((char *)p->flog + s.flog * slot, 0, s.flog, c->stream));
+  double firstFlash = -1;
+  HIPCHK(c, hipMemcpyAsync((double *)p->scal + size_t(slot) * N_SCALARS + S_FIRST_FLASH, &firstFlash, 8, hipMemcpyHostToDevice, c->stream));
   const int *one = p->single + slot;
   size_t lnp = (size_t(std::ceil((g.height + 2000) / 10)) + 2) * sizeof(double);
   if (p->rb == 8) {
@@ -454,6 +471,23 @@ int ml_run_diag(ml_ctx *c, int run, double *out) {
   out[ML_D_NEGFILLED] = sc[S_NEGATIVE]; out[ML_D_CHARGEGROUND] = sc[S_CHARGE_GROUND]; out[ML_D_FLASHES_IC] = i64(S_FLASH_IC);
   out[ML_D_FLASHES_CG] = i64(S_FLASH_CG); out[ML_D_FIRSTFLASH] = sc[S_FIRST_FLASH]; out[ML_D_ERROR] = i64(S_ERROR);
   return 0;
+}
+
+int ml_run_flashes(ml_ctx *c, int run, double *out, int max) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot) || (!out && max > 0)) return fail(c, ML_E_ARG, "flashes: bad run");
+  double sc[N_SCALARS], log[FLASH_LOG * FLASH_FIELDS];
+  HIPCHK(c, hipMemcpyAsync(sc, (double *)p->scal + size_t(slot) * N_SCALARS, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(log, (double *)p->flog + size_t(slot) * FLASH_LOG * FLASH_FIELDS, sizeof(log), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int64_t total; std::memcpy(&total, sc + S_FLASH_LOG, 8);
+  int kept = int(std::min<int64_t>(total, FLASH_LOG)), n = std::min(kept, max);
+  // Oldest first among the last `kept` flashes; the last n of them.
+  for (int k = 0; k < n; k++) {
+    int64_t idx = total - n + k;
+    std::memcpy(out + size_t(k) * FLASH_FIELDS, log + size_t(idx % FLASH_LOG) * FLASH_FIELDS, FLASH_FIELDS * 8);
+  }
+  return n;
 }
 
 static int batchOf(ml_ctx *c, const int *runs, int count, Pool *&p, Batch *&bt) {

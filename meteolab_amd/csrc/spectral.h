@@ -5,8 +5,10 @@
 //
 // One block per (level, run) transforms a level in LDS: mixed-radix Stockham FFTs (radices 4, 2, 3, 5 from the
 // factorisation of nx and ny, so 40 = 4·2·5 and 80 = 4·4·5), rows first, then the kept columns. One thread per
-// (mode, run) solves the mode's column by the Thomas algorithm; the m = n = 0 mode (the level means) is solved in
-// float64 from float64 level sums.
+// (mode, run) solves the mode's column by the Thomas algorithm. The solves run in float64 whatever the kernels'
+// arithmetic type: the low horizontal modes are ill-conditioned vertical systems that amplify float32 rounding of the
+// transforms and the elimination (measured: float32 spectra left v and w 1.2e-5 from the float64 step after one step);
+// the m = n = 0 mode (the level means) takes float64 level sums as its right-hand side.
 #pragma once
 #include "common.h"
 
@@ -74,77 +76,87 @@ __host__ __device__ inline size_t specLdsBytes(const Geo &g, size_t realBytes) {
   return (2 * buf + g.nx + g.ny) * 2 * realBytes;
 }
 
-// Forward transform of L levels of the right-hand side in `cells` into the spectra, and the float64 level sums.
-template <class R> __global__ __launch_bounds__(SPEC_THREADS) void k_spec_fwd(Geo g, Bufs<R> b, Plan pl, int L) {
-  int slot = b.runs[blockIdx.y], lev = blockIdx.x;
+// Forward transform of level `lev` of the right-hand side in `cells` into the spectra, and its float64 sum (the block
+// works together; `smem` holds the LDS of specLdsBytes, `red` 16 doubles).
+template <class R> __device__ void forwardLevel(const Geo &g, const Bufs<R> &b, const Plan &pl, int slot, int lev, char *smem, double *red) {
+  typedef double S;
   const int nx = g.nx, ny = g.ny, mw = g.mw, layer = g.layer, half = ny / 2;
-  extern __shared__ char smem[];
   const int buf = layer / 2 > g.nm ? layer / 2 : g.nm;
-  cx<R> *A = reinterpret_cast<cx<R> *>(smem), *B = A + buf, *tx = B + buf, *ty = tx + nx;
+  cx<S> *A = reinterpret_cast<cx<S> *>(smem), *B = A + buf, *tx = B + buf, *ty = tx + nx;
   for (int k = threadIdx.x; k < nx; k += blockDim.x) { tx[k] = {b.twx[2 * k], b.twx[2 * k + 1]}; }
   for (int k = threadIdx.x; k < ny; k += blockDim.x) { ty[k] = {b.twy[2 * k], b.twy[2 * k + 1]}; }
-  const R *src = b.arr(b.cells, slot, g) + size_t(lev) * layer;
+  const double *src = b.arr(b.cells, slot, g) + size_t(lev) * layer;
   double sum = 0;
   for (int t = threadIdx.x; t < layer / 2; t += blockDim.x) {
     int p = t / nx, x = t - p * nx;
-    R v0 = src[2 * p * nx + x], v1 = src[(2 * p + 1) * nx + x];
+    S v0 = src[2 * p * nx + x], v1 = src[(2 * p + 1) * nx + x];
     A[t] = {v0, v1};
-    sum += double(v0) + double(v1);
+    sum += v0 + v1;
   }
-  __shared__ double red[SPEC_THREADS / 64];
-  sum = blockSum(sum, red);  // contains a barrier
+  sum = blockSum(sum, red);  // contains barriers
   if (threadIdx.x == 0) b.m0(slot, g)[lev] = sum;
   // Row pairs: z = row_2p + i row_2p+1, complex FFT of length nx.
-  cx<R> *Z = fftStages(A, B, nx, half, nx, 1, tx, pl.rx, pl.nrx, -1);
-  cx<R> *C = Z == A ? B : A;
+  cx<S> *Z = fftStages(A, B, nx, half, nx, 1, tx, pl.rx, pl.nrx, -1);
+  cx<S> *C = Z == A ? B : A;
   // Split into the two rows' Hermitian halves (m = 0 .. nx/2): X_2p = (Z_m + conj Z_-m)/2, X_2p+1 = (Z_m - conj Z_-m)/2i.
   for (int t = threadIdx.x; t < half * mw; t += blockDim.x) {
     int p = t / mw, m = t - p * mw;
-    cx<R> z = Z[p * nx + m], c = Z[p * nx + (m ? nx - m : 0)];
+    cx<S> z = Z[p * nx + m], c = Z[p * nx + (m ? nx - m : 0)];
     C[(2 * p) * mw + m] = {(z.re + c.re) / 2, (z.im - c.im) / 2};
     C[(2 * p + 1) * mw + m] = {(z.im + c.im) / 2, (c.re - z.re) / 2};
   }
   __syncthreads();
   // Columns: mw sequences of length ny, element n of column m at n * mw + m.
-  cx<R> *Y = fftStages(C, Z, ny, mw, 1, mw, ty, pl.ry, pl.nry, -1);
-  R *out = b.sp(slot, g) + size_t(lev) * g.nm * 2;
+  cx<S> *Y = fftStages(C, Z, ny, mw, 1, mw, ty, pl.ry, pl.nry, -1);
+  S *out = b.sp(slot, g) + size_t(lev) * g.nm * 2;
   for (int t = threadIdx.x; t < g.nm; t += blockDim.x) { out[2 * t] = Y[t].re; out[2 * t + 1] = Y[t].im; }
+  __syncthreads();
 }
 
-// Inverse transform of L levels of the spectra into dst (kind 0: the face potential phi, 1: the pressure, 2: the
+// Inverse transform of level `lev` of the spectra into dst (kind 0: the face potential phi, 1: the pressure, 2: the
 // electric potential's interior levels 1 .. nz-2), normalised by nx ny.
-template <class R> __global__ __launch_bounds__(SPEC_THREADS) void k_spec_inv(Geo g, Bufs<R> b, Plan pl, int L, int kind) {
-  int slot = b.runs[blockIdx.y], lev = blockIdx.x;
+template <class R> __device__ void inverseLevel(const Geo &g, const Bufs<R> &b, const Plan &pl, int slot, int lev, int kind, char *smem) {
+  typedef double S;
   const int nx = g.nx, ny = g.ny, mw = g.mw, layer = g.layer, half = ny / 2, mh = nx / 2;
-  extern __shared__ char smem[];
   const int buf = layer / 2 > g.nm ? layer / 2 : g.nm;
-  cx<R> *A = reinterpret_cast<cx<R> *>(smem), *B = A + buf, *tx = B + buf, *ty = tx + nx;
+  cx<S> *A = reinterpret_cast<cx<S> *>(smem), *B = A + buf, *tx = B + buf, *ty = tx + nx;
   for (int k = threadIdx.x; k < nx; k += blockDim.x) { tx[k] = {b.twx[2 * k], b.twx[2 * k + 1]}; }
   for (int k = threadIdx.x; k < ny; k += blockDim.x) { ty[k] = {b.twy[2 * k], b.twy[2 * k + 1]}; }
-  const R *in = b.sp(slot, g) + size_t(lev) * g.nm * 2;
+  const S *in = b.sp(slot, g) + size_t(lev) * g.nm * 2;
   for (int t = threadIdx.x; t < g.nm; t += blockDim.x) A[t] = {in[2 * t], in[2 * t + 1]};
   __syncthreads();
-  cx<R> *Y = fftStages(A, B, ny, mw, 1, mw, ty, pl.ry, pl.nry, +1);
-  cx<R> *Z = Y == A ? B : A;
+  cx<S> *Y = fftStages(A, B, ny, mw, 1, mw, ty, pl.ry, pl.nry, +1);
+  cx<S> *Z = Y == A ? B : A;
   // Rows 2p, 2p+1 from their Hermitian halves: z_m = A_m + i B_m with A_-m = conj A_m; the m = 0 and m = nx/2 terms
   // are taken real (spectral.ts inverseLayers).
   for (int t = threadIdx.x; t < half * nx; t += blockDim.x) {
     int p = t / nx, m = t - p * nx;
-    int c = m <= mh ? m : nx - m; R s = m <= mh ? R(1) : R(-1); bool real = c == 0 || 2 * c == nx;
-    cx<R> a = Y[(2 * p) * mw + c], bb = Y[(2 * p + 1) * mw + c];
-    R ar = a.re, ai = real ? R(0) : s * a.im, br = bb.re, bi = real ? R(0) : s * bb.im;
+    int c = m <= mh ? m : nx - m; S s = m <= mh ? S(1) : S(-1); bool real = c == 0 || 2 * c == nx;
+    cx<S> a = Y[(2 * p) * mw + c], bb = Y[(2 * p + 1) * mw + c];
+    S ar = a.re, ai = real ? S(0) : s * a.im, br = bb.re, bi = real ? S(0) : s * bb.im;
     Z[p * nx + m] = {ar - bi, ai + br};
   }
   __syncthreads();
-  cx<R> *X = fftStages(Z, Y, nx, half, nx, 1, tx, pl.rx, pl.nrx, +1);
-  R *dst = kind == 0 ? b.arr(b.phi, slot, g) : kind == 1 ? b.arr(b.press, slot, g) : b.arr(b.ephi, slot, g) + layer;
+  cx<S> *X = fftStages(Z, Y, nx, half, nx, 1, tx, pl.rx, pl.nrx, +1);
+  double *dst = kind == 0 ? b.arr(b.phi, slot, g) : kind == 1 ? b.arr(b.press, slot, g) : b.arr(b.ephi, slot, g) + layer;
   dst += size_t(lev) * layer;
-  const R norm = R(1) / R(layer);
+  const S norm = S(1) / S(layer);
   for (int t = threadIdx.x; t < layer / 2; t += blockDim.x) {
     int p = t / nx, x = t - p * nx;
     dst[2 * p * nx + x] = X[t].re * norm;
     dst[(2 * p + 1) * nx + x] = X[t].im * norm;
   }
+  __syncthreads();
+}
+
+template <class R> __global__ __launch_bounds__(SPEC_THREADS) void k_spec_fwd(Geo g, Bufs<R> b, Plan pl, int L) {
+  extern __shared__ char smem[];
+  __shared__ double red[SPEC_THREADS / 64];
+  forwardLevel(g, b, pl, b.runs[blockIdx.y], blockIdx.x, smem, red);
+}
+template <class R> __global__ __launch_bounds__(SPEC_THREADS) void k_spec_inv(Geo g, Bufs<R> b, Plan pl, int L, int kind) {
+  extern __shared__ char smem[];
+  inverseLevel(g, b, pl, b.runs[blockIdx.y], blockIdx.x, kind, smem);
 }
 
 // Row k of the tridiagonal system of mode (symbol a, b) of kind 0 (face projection, advection.ts:222-241, L = nz),
@@ -171,60 +183,45 @@ template <class T> __device__ inline void modeRow(int kind, int k, int L, T a, T
 }
 
 constexpr int MODE_THREADS = 64;
+// Solves the column of `mode` in place in the spectra (float64); cp[k * cpStride] holds the elimination's factors.
+template <class R> __device__ void modeSolve(const Geo &g, const Bufs<R> &b, int slot, int mode, int kind, int L, double *cpv, int cpStride) {
+  const double *rho = b.lv(slot, L_RHO, g), *dzs = b.dzs, *hz = b.hz;
+  double *spec = b.sp(slot, g);
+  const size_t ls = size_t(g.nm) * 2;
+  const double a = b.sym[(kind == 0 ? 0 : kind == 1 ? 1 : 3) * g.nm + mode], bsym = kind == 1 ? b.sym[2 * g.nm + mode] : 0.0;
+  if (kind == 1 && ((a == 0 && bsym == 0) || (L == 1 && a == 0))) {
+    for (int k = 0; k < L; k++) { spec[k * ls + 2 * mode] = 0; spec[k * ls + 2 * mode + 1] = 0; }
+    return;
+  }
+  const bool pinned = kind != 2 && a == 0;
+  // The m = n = 0 mode (the level means) takes its right-hand side from the float64 level sums (its imaginary part is 0).
+  const double *m0 = mode == 0 ? b.m0(slot, g) : nullptr;
+  double pr = 0, pi = 0;
+  for (int k = 0; k < L; k++) {
+    double lo, di, up, rf;
+    modeRow<double>(kind, k, L, a, bsym, rho, dzs, hz, lo, di, up, rf);
+    double r = rf * (m0 ? m0[k] : spec[k * ls + 2 * mode]), s = m0 ? 0.0 : rf * spec[k * ls + 2 * mode + 1];
+    if (pinned && k == 0) { lo = 0; di = 1; up = 0; r = 0; s = 0; }
+    if (pinned && k == 1) lo = 0;
+    double piv = k == 0 ? di : di - lo * cpv[(k - 1) * cpStride];
+    cpv[k * cpStride] = up / piv;
+    pr = (r - (k == 0 ? 0.0 : lo * pr)) / piv;
+    pi = (s - (k == 0 ? 0.0 : lo * pi)) / piv;
+    spec[k * ls + 2 * mode] = pr; spec[k * ls + 2 * mode + 1] = pi;
+  }
+  for (int k = L - 2; k >= 0; k--) {
+    double c = cpv[k * cpStride];
+    pr = spec[k * ls + 2 * mode] - c * pr; pi = spec[k * ls + 2 * mode + 1] - c * pi;
+    spec[k * ls + 2 * mode] = pr; spec[k * ls + 2 * mode + 1] = pi;
+  }
+}
 // One thread per (mode, run); the elimination's factors stay in LDS, its right-hand sides overwrite the spectra.
 template <class R> __global__ __launch_bounds__(MODE_THREADS) void k_modes(Geo g, Bufs<R> b, int kind, int L) {
   int slot = b.runs[blockIdx.y];
   int mode = blockIdx.x * MODE_THREADS + threadIdx.x;
   if (mode >= g.nm) return;
   extern __shared__ char smem[];
-  R *cpv = reinterpret_cast<R *>(smem) + threadIdx.x;  // [k][thread]
-  const double *rho = b.lv(slot, L_RHO, g), *dzs = b.dzs, *hz = b.hz;
-  R *spec = b.sp(slot, g);
-  const size_t ls = size_t(g.nm) * 2;
-  const R a = b.sym[(kind == 0 ? 0 : kind == 1 ? 1 : 3) * g.nm + mode], bsym = kind == 1 ? b.sym[2 * g.nm + mode] : R(0);
-  if (kind == 1 && ((a == 0 && bsym == 0) || (L == 1 && a == 0))) {
-    for (int k = 0; k < L; k++) { spec[k * ls + 2 * mode] = 0; spec[k * ls + 2 * mode + 1] = 0; }
-    return;
-  }
-  const bool pinned = kind != 2 && a == 0;
-  if (mode == 0) {
-    // The level means, in float64 from the float64 sums (their imaginary part is zero).
-    const double *m0 = b.m0(slot, g);
-    double cp[64], d[64];
-    for (int k = 0; k < L && k < 64; k++) {
-      double lo, di, up, rf;
-      modeRow<double>(kind, k, L, double(a), double(bsym), rho, dzs, hz, lo, di, up, rf);
-      double r = rf * m0[k];
-      if (pinned && k == 0) { lo = 0; di = 1; up = 0; r = 0; }
-      if (pinned && k == 1) lo = 0;
-      double piv = k == 0 ? di : di - lo * cp[k - 1];
-      cp[k] = up / piv;
-      d[k] = (r - (k == 0 ? 0 : lo * d[k - 1])) / piv;
-    }
-    for (int k = L - 1; k >= 0; k--) {
-      if (k < L - 1) d[k] -= cp[k] * d[k + 1];
-      spec[k * ls] = R(d[k]); spec[k * ls + 1] = 0;
-    }
-    return;
-  }
-  R pr = 0, pi = 0;
-  for (int k = 0; k < L; k++) {
-    R lo, di, up, rf;
-    modeRow<R>(kind, k, L, a, bsym, rho, dzs, hz, lo, di, up, rf);
-    R r = rf * spec[k * ls + 2 * mode], s = rf * spec[k * ls + 2 * mode + 1];
-    if (pinned && k == 0) { lo = 0; di = 1; up = 0; r = 0; s = 0; }
-    if (pinned && k == 1) lo = 0;
-    R piv = k == 0 ? di : di - lo * cpv[(k - 1) * MODE_THREADS];
-    cpv[k * MODE_THREADS] = up / piv;
-    pr = (r - (k == 0 ? R(0) : lo * pr)) / piv;
-    pi = (s - (k == 0 ? R(0) : lo * pi)) / piv;
-    spec[k * ls + 2 * mode] = pr; spec[k * ls + 2 * mode + 1] = pi;
-  }
-  for (int k = L - 2; k >= 0; k--) {
-    R c = cpv[k * MODE_THREADS];
-    pr = spec[k * ls + 2 * mode] - c * pr; pi = spec[k * ls + 2 * mode + 1] - c * pi;
-    spec[k * ls + 2 * mode] = pr; spec[k * ls + 2 * mode + 1] = pi;
-  }
+  modeSolve(g, b, slot, mode, kind, L, reinterpret_cast<double *>(smem) + threadIdx.x, MODE_THREADS);
 }
 
 }  // namespace ml

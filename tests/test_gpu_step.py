@@ -1,0 +1,141 @@
+"""Parity of the HIP step (through the C ABI) with the oracle, the CPU restatement of the reference's step
+(AtmosphereModel.step, src/core/model.ts:873-877). Per-field relative L2 ||gpu - oracle|| / ||oracle|| <= 1e-5 after one
+and after ten steps; fields the oracle holds at exactly zero must stay within 1e-12 of zero."""
+import numpy as np
+import pytest
+
+import meteolab_amd as M
+from meteolab_amd import config as C
+from oracle.binding import OracleRun
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = M.Context(0, 32)
+    print("device:", c.adapter, flush=True)
+    yield c
+    c.close()
+
+
+def rel_l2(a, b):
+    a, b = a.astype(np.float64), b.astype(np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / nb if nb > 0 else np.linalg.norm(a)
+
+
+def compare(run, oracle, tol=TOL, fields=M.CARRIED, zero_tol=1e-12):
+    worst = {}
+    for name in fields:
+        a, b = run.field(name), oracle.field(name)
+        assert np.isfinite(a).all(), name
+        if np.linalg.norm(b.astype(np.float64)) == 0:
+            assert np.linalg.norm(a.astype(np.float64)) <= zero_tol, (name, np.linalg.norm(a))
+            continue
+        worst[name] = rel_l2(a, b)
+    bad = {k: v for k, v in worst.items() if v > tol}
+    assert not bad, f"fields over {tol}: {bad} (all: {worst})"
+    return worst
+
+
+def test_base_state_and_initial_state_match(ctx):
+    g = C.STANDARD
+    prof, run = C.profile(), C.run(g)
+    o = OracleRun(g, prof, run)
+    r = ctx.pool(g, 1).run(prof, run)
+    lv, bs = r.levels(), o.base_state()
+    for k in ("p", "exner", "theta", "q", "u", "v", "rho"):
+        scale = max(np.abs(bs[k]).max(), 1e-300)
+        assert np.abs(lv[k] - bs[k]).max() / scale < 1e-13, k
+    for name in M.CARRIED:
+        assert np.array_equal(r.field(name), o.field(name)), name
+
+
+@pytest.mark.parametrize("grid", ["small", "standard"])
+def test_parity_after_one_and_ten_steps(ctx, grid):
+    g = C.small_grid() if grid == "small" else C.STANDARD
+    prof, run = C.profile(), C.run(g)
+    o = OracleRun(g, prof, run)
+    r = ctx.pool(g, 1).run(prof, run)
+    o.step(1); r.step(1)
+    w1 = compare(r, o)
+    o.step(9); r.step(9)
+    w10 = compare(r, o)
+    print(grid, "1 step", max(w1.values()), "10 steps", max(w10.values()))
+    d, od = r.diagnostics(), o.diag()
+    assert d["time"] == od["time"] == 30.0
+    assert d["maxW"] == pytest.approx(od["maxW"], rel=1e-5)
+
+
+def mature(g, steps, seed=1):
+    """The oracle run `steps` steps into the storm (cloud, rain and ice present), for a restart on the GPU."""
+    o = OracleRun(g, C.profile(), C.run(g, seed=seed))
+    o.step(steps)
+    return o
+
+
+def restart(ctx, o, g, seed=1):
+    r = ctx.pool(g, 1).run(C.profile(), C.run(g, seed=seed))
+    for name in M.CARRIED + ["fieldZ"]:
+        r.set_field(name, o.field(name))
+    for name in M.COLUMNS:
+        r.set_column(name, o.column(name))
+    r.set_time(o.diag()["time"])
+    return r
+
+
+def test_parity_in_a_mature_storm(ctx):
+    # 32.5 min into the storm on the small grid: cloud, rain, snow, graupel, hail, ice and charge are all present.
+    g = C.small_grid()
+    o = mature(g, 650)
+    st = o.state()
+    for name in ("cloud", "rain", "ice", "snow", "graupel", "hail", "chargeIce"):
+        assert np.abs(st[name]).max() > 0, name
+    r = restart(ctx, o, g)
+    o.step(1); r.step(1)
+    w1 = compare(r, o)
+    o.step(9); r.step(9)
+    w10 = compare(r, o, tol=1e-4)
+    print("mature 1 step", w1, "\n10 steps", w10)
+    assert r.column("precipitation").max() == pytest.approx(o.column("precipitation").max(), rel=1e-4)
+
+
+def test_batch_alone_equals_in_the_mix(ctx):
+    # Three different runs stepped together give what each gives alone (the serving loop's batching invariant).
+    g = C.small_grid()
+    specs = [C.run(g, seed=s, amp=a, offset=off) for s, a, off in ((1, 2.0, (0, 0)), (2, 2.6, (3000, -2000)), (3, 1.6, (-4000, 1000)))]
+    pool = ctx.pool(g, 4)
+    mixed = [pool.run(C.profile(), s) for s in specs]
+    M.step(mixed, 20)
+    ctx.sync()
+    for s, rm in zip(specs, mixed):
+        alone = ctx.pool(g, 1).run(C.profile(), s)
+        alone.step(20)
+        for name in M.CARRIED:
+            a, b = rm.field(name), alone.field(name)
+            assert rel_l2(a, b) <= 1e-6, name
+
+
+def test_float64_kernels_match_the_oracle_closely():
+    g = C.small_grid()
+    prof, run = C.profile(), C.run(g)
+    o = OracleRun(g, prof, run)
+    with M.Context(0, 64) as c64:
+        r = c64.pool(g, 1).run(prof, run)
+        o.step(10); r.step(10)
+        w = compare(r, o, tol=1e-6)
+        print("float64 kernels, 10 steps", max(w.values()))
+
+
+def test_graph_replay_equals_eager_launches(ctx):
+    g = C.small_grid()
+    pool = ctx.pool(g, 2)
+    a = pool.run(C.profile(), C.run(g))
+    b = pool.run(C.profile(), C.run(g))
+    a.step(4)                      # captured graph, replayed
+    M.profile_step([b], 4)         # the same kernels launched one by one
+    for name in M.CARRIED:
+        assert rel_l2(a.field(name), b.field(name)) <= 1e-6, name
