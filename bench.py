@@ -1,0 +1,149 @@
+"""Benchmark of the StormLab step on one MI355X: cell-steps/s of batched runs against the plain one-run step, p50/p99
+latency per model-minute, the dominant kernel's roofline and the oracle's CPU rate on a bounded sample.
+
+Default workload (the ensemble configuration): 64 standard-domain runs (40x32x50, dt 3 s), each with its own seeded
+thermal (amplitude 1.5-3 K, centre offset +-5 km) and theta noise, advanced together for 600 steps (30 model minutes),
+one launch per kernel for all runs, one captured hipGraph per step. The plain rate is one run of the same domain on its
+own. A cell-step is one grid node advanced by one model step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--runs R] [--grid standard|large]
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import meteolab_amd as M  # noqa: E402
+from meteolab_amd import config as C  # noqa: E402
+
+PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
+DT = 3.0
+METRIC = "cell-steps/s on 1 MI355X, batched vs plain; p50/p99 latency per model-minute"
+
+
+def stage_bytes(n, stage, carried=17):
+    """Algorithmic HBM bytes of one k_stage launch (one RK3 stage of every carried field) per run of n nodes: each
+    field's stage input read, its start read (stage 0: the same array), its output written (float32), the face mass
+    fluxes and their divergence read once (4 x float32) and the 8 fields of the buoyancy read once."""
+    per_field = 8 if stage == 0 else 12
+    return n * (carried * per_field + 16 + 8 * 4)
+
+
+# Compulsory float32 traffic of a whole step per cell (the step's fields, each phase fused, no reuse across the step's
+# global syncs): fall speeds 44, sedimentation 84, face projection 96, RK3 transport 592, stage projections 160,
+# positivity fix 92, column physics 148, turbulence and mixing 104, final projection 80, lightning 88.
+STEP_BYTES_PER_CELL = 1488
+
+
+def percentile(a, q):
+    return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
+
+
+def cpu_baseline(grid, steps):
+    """The oracle (single-threaded C++ restatement of the reference's step) on a bounded sample of the workload."""
+    from oracle.binding import OracleRun
+    o = OracleRun(grid, C.profile(), C.ensemble_member(grid, 0))
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    n = C.nodes(grid)
+    return {"value": n * steps / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"1 run of {grid['nx']}x{grid['ny']}x{grid['nz']} (ensemble member 0), steps 1-{steps} from the "
+                      f"initial state, 1 thread, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--runs", type=int, default=64)
+    ap.add_argument("--grid", default="standard", choices=["standard", "large"])
+    ap.add_argument("--plain-steps", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (--gpus {a.gpus} was asked): the batched runs are independent and a "
+              "multi-GPU run would shard them by run with no exchange; no multi-GPU figure is measured.", file=sys.stderr)
+        sys.exit(2)
+
+    grid = C.GRIDS[a.grid]
+    n = C.nodes(grid)
+    prof = C.profile()
+    ctx = M.Context(0, 32)
+
+    # Batched: R runs in one pool, one launch per kernel for all of them.
+    pool = ctx.pool(grid, a.runs)
+    runs = [pool.run(prof, C.ensemble_member(grid, k)) for k in range(a.runs)]
+    if a.warmup:
+        M.step(runs, a.warmup, DT)
+        ctx.sync()
+    ms = M.step_timed(runs, a.steps, DT)
+    total_s = ms.sum() / 1e3
+    batched = a.runs * n * a.steps / total_s
+    # Latency per model-minute: the device time of each consecutive 20 steps (60 s of model time) of the batch.
+    per_minute = int(round(60 / DT))
+    minutes = [ms[k:k + per_minute].sum() for k in range(0, len(ms) - per_minute + 1, per_minute)]
+
+    # Kernel times (HIP events between the kernels of eager launches on the library's stream) on the same runs.
+    launches = M.profile_step(runs, 3, DT)
+    kt = M.kernel_times(launches)
+    stage_ms = kt["k_stage"][1]
+    stage_b = sum(stage_bytes(n, s) for s in range(3)) * a.runs
+    achieved = stage_b / (stage_ms / 1e3)
+    dominant = max(kt.items(), key=lambda kv: kv[1][1])[0]
+
+    # Plain: one run of the same domain on its own.
+    ppool = ctx.pool(grid, 1)
+    plain_run = ppool.run(prof, C.ensemble_member(grid, 0))
+    M.step([plain_run], 3, DT)
+    pms = M.step_timed([plain_run], a.plain_steps, DT)
+    plain = n * a.plain_steps / (pms.sum() / 1e3)
+
+    out = {
+        "metric": METRIC,
+        "value": batched,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": float(ms.mean()),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"ensemble of {a.runs} {a.grid}-domain runs ({grid['nx']}x{grid['ny']}x{grid['nz']}, "
+                               f"dt 3 s) batched in one launch per kernel, {a.steps} steps after {a.warmup} warm-up steps",
+                   "runs": a.runs, "grid": f"{grid['nx']}x{grid['ny']}x{grid['nz']}", "dt": DT, "parallelism": "1 GPU"},
+        "plain": {"value": plain, "unit": "cell-steps/s", "ms_per_step": float(pms.mean()), "steps": a.plain_steps},
+        "batched_over_plain": batched / plain,
+        "latency_per_model_minute_ms": {"p50": percentile(minutes, 50), "p99": percentile(minutes, 99),
+                                        "runs_per_batch": a.runs, "minutes": len(minutes)},
+        "roofline": {"bound": "hbm", "kernel": "k_stage", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None,
+                     "bytes_per_launch": stage_b / 3, "ms_per_launch": stage_ms / 3},
+        "step_roofline": {"bytes_per_cell_step": STEP_BYTES_PER_CELL, "achieved_GBps": batched * STEP_BYTES_PER_CELL / 1e9,
+                          "frac": batched * STEP_BYTES_PER_CELL / PEAK_HBM},
+        "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][1])},
+        "dominant_kernel": dominant,
+        "copy_GBps": ctx.copy_bandwidth(),
+        "device": ctx.adapter,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(grid, a.cpu_steps)
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

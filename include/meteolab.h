@@ -109,9 +109,16 @@ int ml_run_set_time(ml_ctx *ctx, int run, double time);
  * the launch sequence of a batch is captured once into a hipGraph and replayed. Asynchronous: ml_sync waits. */
 int ml_step(ml_ctx *ctx, const int *runs, int count, int steps, double dt);
 int ml_sync(ml_ctx *ctx);
+/* ml_step of `steps` steps with a HIP event on the context's stream after each: ms[k] receives the device time of
+ * step k (the graph replay), for throughput and latency measurement. Synchronous. */
+int ml_step_timed(ml_ctx *ctx, const int *runs, int count, int steps, double dt, double *ms);
+/* A plain device copy of `bytes` (read + write), `iters` times: the achievable HBM bandwidth in GB/s into *gbps, and a
+ * kernel of known traffic for calibrating the HBM byte counters. No counterpart in the reference. */
+int ml_copy_probe(ml_ctx *ctx, int64_t bytes, int iters, double *gbps);
 
-/* Kernel timing of the last ml_step batch: stage by stage device time in ms from HIP events over `steps` eager steps
- * (no graph); names[i] are written as a '\n'-separated list into `names` (size bytes). Returns the number of stages. */
+/* Kernel timing: the batch's step launched kernel by kernel (no graph) `steps` times with a HIP event after every
+ * kernel; ms[i] receives launch i's device time averaged over the steps and `names` (size bytes) the kernels' names,
+ * '\n'-separated, in launch order. Returns the number of launches per step (at most `max` are written). */
 int ml_profile_step(ml_ctx *ctx, const int *runs, int count, int steps, double dt, double *ms, int max, char *names, int size);
 
 #ifdef __cplusplus

@@ -75,6 +75,8 @@ SIGNATURES = {
     "ml_run_flashes": (_i, [_vp, _i, _dp, _i]),
     "ml_step": (_i, [_vp, _ip, _i, _i, _d]),
     "ml_sync": (_i, [_vp]),
+    "ml_step_timed": (_i, [_vp, _ip, _i, _i, _d, _dp]),
+    "ml_copy_probe": (_i, [_vp, _i64, _i, _dp]),
     "ml_profile_step": (_i, [_vp, _ip, _i, _i, _d, _dp, _i, _cp, _i]),
 }
 
@@ -134,6 +136,12 @@ class Context:
 
     def sync(self):
         self.check(lib().ml_sync(self.h))
+
+    def copy_bandwidth(self, nbytes=1 << 30, iters=20):
+        """Achievable HBM bandwidth of a plain copy (read + write), GB/s (ml_copy_probe)."""
+        out = ctypes.c_double()
+        self.check(lib().ml_copy_probe(self.h, int(nbytes), int(iters), ctypes.byref(out)))
+        return out.value
 
     def close(self):
         if self.h:
@@ -255,11 +263,30 @@ def step(runs, n=1, dt=3.0):
     ctx.check(lib().ml_step(ctx.h, ids, len(runs), int(n), float(dt)))
 
 
-def profile_step(runs, n=3, dt=3.0, max_stages=64):
-    """Device time per launch stage of one step (HIP events, eager launches), ms averaged over n steps."""
+def step_timed(runs, n, dt=3.0):
+    """ml_step_timed: n graph-replayed steps with a HIP event after each; the device time of each step, ms."""
     ctx = runs[0].ctx
     ids = (ctypes.c_int * len(runs))(*[r.h for r in runs])
-    ms = np.zeros(max_stages)
-    names = ctypes.create_string_buffer(4096)
-    k = ctx.check(lib().ml_profile_step(ctx.h, ids, len(runs), int(n), float(dt), _dptr(ms), max_stages, names, 4096))
-    return dict(zip(names.value.decode().split("\n")[:k], ms[:k]))
+    ms = np.zeros(int(n))
+    ctx.check(lib().ml_step_timed(ctx.h, ids, len(runs), int(n), float(dt), _dptr(ms)))
+    return ms
+
+
+def profile_step(runs, n=3, dt=3.0, max_launches=128):
+    """Device time of each kernel launch of a step (HIP events between eager launches), ms averaged over n steps:
+    a list of (kernel name, ms) in launch order."""
+    ctx = runs[0].ctx
+    ids = (ctypes.c_int * len(runs))(*[r.h for r in runs])
+    ms = np.zeros(max_launches)
+    names = ctypes.create_string_buffer(8192)
+    k = ctx.check(lib().ml_profile_step(ctx.h, ids, len(runs), int(n), float(dt), _dptr(ms), max_launches, names, 8192))
+    return list(zip(names.value.decode().split("\n")[:k], ms[:k]))
+
+
+def kernel_times(launches):
+    """Per kernel name: (launches per step, total ms per step) from profile_step's list."""
+    out = {}
+    for name, ms in launches:
+        c, t = out.get(name, (0, 0.0))
+        out[name] = (c + 1, t + float(ms))
+    return out

@@ -239,11 +239,16 @@ template <class R> __global__ __launch_bounds__(TPB) void k_faces(Geo g, Bufs<R>
 template <class R> __device__ inline R wenoFace(R a, R b_, R c, R d, R e) {
   R s = mmax(mmax(mmax(mabs(a), mabs(b_)), mmax(mabs(c), mabs(d))), mabs(e));
   if (s == 0) return 0;
+  if (s < R(1e-25)) {
+    // The smoothness indicators (~s^2) vanish beside the reference's 1e-40 in epsilon: the linear weights, exactly the
+    // limit of its weights (and 1/s would overflow a float for a denormal s).
+    return (R(.1) * (2 * a - 7 * b_ + 11 * c) + R(.6) * (-b_ + 5 * c + 2 * d) + R(.3) * (2 * c + 5 * d - e)) / 6;
+  }
   R is = 1 / s;
   a *= is; b_ *= is; c *= is; d *= is; e *= is;
   R p0 = (2 * a - 7 * b_ + 11 * c) / 6, p1 = (-b_ + 5 * c + 2 * d) / 6, p2 = (2 * c + 5 * d - e) / 6;
   R s0 = a - 2 * b_ + c, t0 = a - 4 * b_ + 3 * c, s1 = b_ - 2 * c + d, t1 = b_ - d, s2 = c - 2 * d + e, t2 = 3 * c - 4 * d + e;
-  R tiny = mmin(R(1e-20) * is, R(1e15));
+  R tiny = R(1e-20) * is;  // <= 1e5
   R eps = R(2e-7) * (a * a + b_ * b_ + c * c + d * d + e * e) + tiny * tiny;
   R b0 = eps + R(13.0 / 12) * s0 * s0 + R(.25) * t0 * t0, b1 = eps + R(13.0 / 12) * s1 * s1 + R(.25) * t1 * t1, b2 = eps + R(13.0 / 12) * s2 * s2 + R(.25) * t2 * t2;
   R r0 = eps / b0, r1 = eps / b1, r2 = eps / b2;
@@ -427,7 +432,10 @@ template <class R> __global__ __launch_bounds__(TPB) void k_fill_sums(Geo g, Buf
 // ---- column physics of every node (model.ts:1116-1127 'finish' + physics(z) 1347-1531) ------------------------------
 // The positivity fix of the conserved fields and the decay of the cold-pool indicator come first (the 'finish'
 // phase), then the node's microphysics, charging, Coriolis, wind relaxation and the sponge.
-template <class R> __global__ __launch_bounds__(TPB) void k_physics(Geo g, Bufs<R> b, double dt, int electrified) {
+// R here is the arithmetic of the node's physics, BR the pools' type: the physics runs in float64 in both variants
+// (measured: float32 microphysics left the rain's charge 1.4e-5 from the float64 step after one step of a mature
+// storm, as the charge follows water increments far below a float32 mass's rounding); fields stay float32.
+template <class BR, class R> __global__ __launch_bounds__(TPB) void k_physics(Geo g, Bufs<BR> b, double dt, int electrified) {
   int slot = b.runs[blockIdx.y];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.n) return;
@@ -455,8 +463,10 @@ template <class R> __global__ __launch_bounds__(TPB) void k_physics(Geo g, Bufs<
   float &u = F[F_U], &v = F[F_V], &w = F[F_W], &theta = F[F_THETA], &q = F[F_Q], &cloud = F[F_CLOUD], &rain = F[F_RAIN], &cold = F[F_COLD];
   float &ice = F[F_ICE], &iceNumber = F[F_ICENUMBER], &snow = F[F_SNOW], &graupel = F[F_GRAUPEL], &hail = F[F_HAIL];
   float &cI = F[F_CHARGE_ICE], &cG = F[F_CHARGE_GRAUPEL], &cR = F[F_CHARGE_RAIN], &cA = F[F_CHARGE_AIR];
-  R before[4], after[4], cNow[4];
-  if (electrified) { before[0] = R(ice) + R(snow); before[1] = R(graupel) + R(hail); before[2] = rain; before[3] = cloud; }
+  // The water of the four charge carriers before and after the microphysics, and the charges, in float64: the charge
+  // follows mass changes that can be far below the float32 rounding of a sum such as ice + snow.
+  double before[4], after[4], cNow[4];
+  if (electrified) { before[0] = double(ice) + snow; before[1] = double(graupel) + hail; before[2] = rain; before[3] = cloud; }
   R evap = rainProcesses(q, cloud, rain, R(theta) * exner, p, rhoZ, Rdt);
   if (evap > 0) { R cool = R(LV / CP) / exner * evap; theta = float(R(theta) - cool); cold = float(R(cold) + cool); }
   freezeRain(theta, rain, graupel, exner);
@@ -478,15 +488,15 @@ template <class R> __global__ __launch_bounds__(TPB) void k_physics(Geo g, Bufs<
   R cond = saturationAdjustMixed(theta, q, cloud, ice, exner, p, iceNumber);
   settleCrystals<R>(ice, iceNumber);
   if (electrified) {
-    after[0] = R(ice) + R(snow); after[1] = R(graupel) + R(hail); after[2] = rain; after[3] = cloud;
+    after[0] = double(ice) + snow; after[1] = double(graupel) + hail; after[2] = rain; after[3] = cloud;
     cNow[0] = cI; cNow[1] = cG; cNow[2] = cR; cNow[3] = cA;
-    if (cNow[0] != 0 || cNow[1] != 0 || cNow[2] != 0 || cNow[3] != 0) transferCharge(cNow, before, after);
+    if (cNow[0] != 0 || cNow[1] != 0 || cNow[2] != 0 || cNow[3] != 0) transferCharge<double>(cNow, before, after);
     R tc = R(theta) * exner - R(273.15);
     bool wetG = wetGrowing(R(graupel), R(N0G), R(RHO_G), tc, R(q), p, rhoZ, R(cloud), R(rain), R(ice), R(snow));
     bool wetH = wetGrowing(R(hail), R(N0H), R(RHO_H), tc, R(q), p, rhoZ, R(cloud), R(rain), R(ice), R(snow));
     R separated = chargingRate(tc, rhoZ, R(cloud), R(ice), R(snow), R(graupel), R(hail), R(b.f(slot, F_FALL_SNOW, g)[i]), wetG, wetH, R(iceNumber) * rhoZ) * Rdt / rhoZ;
-    R induced = inductiveRate(tc, rhoZ, R(cloud), R(graupel), R(hail), R(b.f(slot, F_FIELDZ, g)[i]), (cNow[1] + separated) * rhoZ, wetG, wetH) * Rdt / rhoZ;
-    cI = float(cNow[0] - separated); cG = float(cNow[1] + separated + induced); cR = float(cNow[2]); cA = float(cNow[3] - induced);
+    R induced = inductiveRate(tc, rhoZ, R(cloud), R(graupel), R(hail), R(b.f(slot, F_FIELDZ, g)[i]), R(cNow[1] + double(separated)) * rhoZ, wetG, wetH) * Rdt / rhoZ;
+    cI = float(cNow[0] - double(separated)); cG = float(cNow[1] + double(separated) + double(induced)); cR = float(cNow[2]); cA = float(cNow[3] - double(induced));
   }
   if (cond < 0) cold = float(R(cold) - R(LV / CP) / exner * cond * R(.14));
   // Coriolis and the relaxation of the wind's departure from the environment (float64: small departures of ~25 m/s).

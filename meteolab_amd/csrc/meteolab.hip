@@ -121,15 +121,19 @@ struct Timer {
   }
 };
 
-template <class R> void spectralSolve(const Pool &p, const Bufs<R> &b, int nr, hipStream_t s, int kind, int L, int invKind) {
+template <class R> void spectralSolve(const Pool &p, const Bufs<R> &b, int nr, hipStream_t s, int kind, int L, int invKind, Timer *t) {
   const Geo &g = p.g;
   size_t lds = specLdsBytes(g, sizeof(double));
   hipLaunchKernelGGL(k_spec_fwd<R>, dim3(L, nr), dim3(SPEC_THREADS), lds, s, g, b, p.plan, L);
+  if (t) t->mark("k_spec_fwd");
   hipLaunchKernelGGL(k_modes<R>, dim3(blocks(g.nm, MODE_THREADS), nr), dim3(MODE_THREADS), size_t(MODE_THREADS) * L * sizeof(double), s, g, b, kind, L);
+  if (t) t->mark("k_modes");
   hipLaunchKernelGGL(k_spec_inv<R>, dim3(L, nr), dim3(SPEC_THREADS), lds, s, g, b, p.plan, L, invKind);
+  if (t) t->mark("k_spec_inv");
 }
 
-// One model step of the runs in `list` (device array of nr slots): AtmosphereModel.step (model.ts:873-877).
+// One model step of the runs in `list` (device array of nr slots): AtmosphereModel.step (model.ts:873-877). With a
+// timer, an event follows every kernel (named by the kernel).
 template <class R> void launchStep(const Pool &p, const int *list, int nr, double dt, hipStream_t s, Timer *t) {
   const Geo &g = p.g;
   Bufs<R> b = bufs<R>(p, list);
@@ -139,51 +143,60 @@ template <class R> void launchStep(const Pool &p, const int *list, int nr, doubl
   mark("start");
   // beginStep: fall speeds, hail swath, sedimentation (model.ts:893-902).
   hipLaunchKernelGGL(k_begin<R>, dim3(nr), dim3(64), 0, s, g, b);
+  mark("k_begin");
   hipLaunchKernelGGL(k_fall<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt, el);
-  mark("fall");
+  mark("k_fall");
   hipLaunchKernelGGL(k_sediment<R>, dim3(blocks(g.layer, SED_COLS), nr), dim3(SED_COLS), size_t(2) * SED_COLS * g.nz * sizeof(float), s, g, b, dt, el);
-  mark("sediment");
+  mark("k_sediment");
   // dynamics: face projection (FluxTransport.setVelocity) and the three RK3 stages with the stage projections.
   hipLaunchKernelGGL(k_faces<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 0);
-  spectralSolve<R>(p, b, nr, s, 0, g.nz, 0);
+  mark("k_faces");
+  spectralSolve<R>(p, b, nr, s, 0, g.nz, 0, t);
   hipLaunchKernelGGL(k_faces<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 1);
-  mark("faces");
+  mark("k_faces");
   const size_t stageLds = ((g.layer * sizeof(float) + 15) / 16) * 16 + 2 * g.layer * sizeof(R);
   for (int stage = 0; stage < 3; stage++) {
     hipLaunchKernelGGL(k_stage<R>, dim3(g.nz, nc, nr), dim3(TPB), stageLds, s, g, b, stage, dt);
-    mark(stage == 0 ? "transport0" : stage == 1 ? "transport1" : "transport2");
+    mark("k_stage");
     if (stage < 2) {
       hipLaunchKernelGGL(k_pdiv<R>, dim3(cb, nr), dim3(TPB), 0, s, g, b, stage + 1, dt);
-      spectralSolve<R>(p, b, nr, s, 1, g.nz - 1, 1);
+      mark("k_pdiv");
+      spectralSolve<R>(p, b, nr, s, 1, g.nz - 1, 1, t);
       hipLaunchKernelGGL(k_pcorrect<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, stage + 1, dt, 0);
-      mark(stage == 0 ? "pressure0" : "pressure1");
+      mark("k_pcorrect");
     }
   }
   // endStep: positivity fix + physics (physicsStep), turbulence + drag (mixStep), projection (model.ts:1103-1113).
   hipLaunchKernelGGL(k_fill_sums<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b);
-  hipLaunchKernelGGL(k_physics<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt, el);
-  mark("physics");
+  mark("k_fill_sums");
+  hipLaunchKernelGGL((k_physics<R, double>), dim3(nb, nr), dim3(TPB), 0, s, g, b, dt, el);
+  mark("k_physics");
   hipLaunchKernelGGL(k_viscosity<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt);
-  mark("viscosity");
+  mark("k_viscosity");
   hipLaunchKernelGGL(k_mix<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt);
-  mark("mix");
+  mark("k_mix");
   hipLaunchKernelGGL(k_pdiv<R>, dim3(cb, nr), dim3(TPB), 0, s, g, b, 1, dt);
-  spectralSolve<R>(p, b, nr, s, 1, g.nz - 1, 1);
+  mark("k_pdiv");
+  spectralSolve<R>(p, b, nr, s, 1, g.nz - 1, 1, t);
   hipLaunchKernelGGL(k_pcorrect<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 1, dt, 1);
-  mark("pressure2");
+  mark("k_pcorrect");
   // afterStep: lightning (model.ts:1537-1552).
   if (el) {
     unsigned ib = blocks(size_t(g.layer) * (g.nz - 2), 256);
     hipLaunchKernelGGL(k_charge_rhs<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b);
-    spectralSolve<R>(p, b, nr, s, 2, g.nz - 2, 2);
+    mark("k_charge_rhs");
+    spectralSolve<R>(p, b, nr, s, 2, g.nz - 2, 2, t);
     hipLaunchKernelGGL(k_strongest<R>, dim3(ib, nr), dim3(256), 0, s, g, b);
+    mark("k_strongest");
     // flashLimit (lightning.ts:46): 10 flashes per 3 s of model time.
     int limit = std::max(1, int(std::floor(10 * dt / 3 + .5)));
     hipLaunchKernelGGL(k_discharge<R>, dim3(nr), dim3(DISCHARGE_THREADS), specLdsBytes(g, sizeof(double)), s, g, b, p.plan, dt, limit);
+    mark("k_discharge");
     hipLaunchKernelGGL(k_conduct<R>, dim3(ib, nr), dim3(256), 0, s, g, b, dt);
-    mark("lightning");
+    mark("k_conduct");
   }
   hipLaunchKernelGGL(k_end<R>, dim3(nr), dim3(64), 0, s, g, b, dt);
+  mark("k_end");
 }
 
 void fillTables(Pool &p, std::vector<char> &sym, std::vector<char> &twx, std::vector<char> &twy) {
@@ -530,6 +543,61 @@ int ml_step(ml_ctx *c, const int *runs, int count, int steps, double dt) {
   return 0;
 }
 
+int ml_step_timed(ml_ctx *c, const int *runs, int count, int steps, double dt, double *ms) {
+  if (!ms || steps < 1) return fail(c, ML_E_ARG, "step timed: steps >= 1 and an output array");
+  // Capture (or reuse) the batch's graph with one step, then time each replay between events on the stream.
+  Pool *p; Batch *bt;
+  if (int r = batchOf(c, runs, count, p, bt)) return r;
+  if (!bt->exec || bt->dt != dt) { if (int r = ml_step(c, runs, count, 0, dt)) return r; }
+  std::vector<hipEvent_t> ev(steps + 1);
+  for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+  HIPCHK(c, hipEventRecord(ev[0], c->stream));
+  for (int k = 0; k < steps; k++) {
+    HIPCHK(c, hipGraphLaunch(bt->exec, c->stream));
+    HIPCHK(c, hipEventRecord(ev[k + 1], c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < steps; k++) { float v = 0; HIPCHK(c, hipEventElapsedTime(&v, ev[k], ev[k + 1])); ms[k] = v; }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+// A plain copy of float4s (the reference kernel of known traffic for the HBM counters and the achievable bandwidth).
+__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" {
+
+int ml_copy_probe(ml_ctx *c, int64_t bytes, int iters, double *gbps) {
+  if (!c || bytes < 1024 || iters < 1 || !gbps) return fail(c, ML_E_ARG, "copy probe: bytes >= 1024, iters >= 1");
+  HIPCHK(c, hipSetDevice(c->device));
+  size_t n = size_t(bytes) / 16;
+  float4 *a = nullptr, *b = nullptr;
+  HIPCHK(c, hipMalloc(&a, n * 16));
+  hipError_t e = hipMalloc(&b, n * 16);
+  if (e != hipSuccess) { (void)hipFree(a); return fail(c, ML_E_HIP, "copy probe allocation"); }
+  (void)hipMemsetAsync(a, 0, n * 16, c->stream);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0); (void)hipEventCreate(&e1);
+  unsigned grid = unsigned(std::min<size_t>((n + 255) / 256, size_t(c->prop.multiProcessorCount) * 16));
+  hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, c->stream, a, b, n);
+  (void)hipEventRecord(e0, c->stream);
+  for (int k = 0; k < iters; k++) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, c->stream, a, b, n);
+  (void)hipEventRecord(e1, c->stream);
+  e = hipStreamSynchronize(c->stream);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipFree(a); (void)hipFree(b);
+  if (e != hipSuccess) return fail(c, ML_E_HIP, hipGetErrorString(e));
+  *gbps = 2.0 * double(n) * 16 * iters / (ms / 1e3) / 1e9;
+  return 0;
+}
+
 int ml_sync(ml_ctx *c) {
   if (!c) return ML_E_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -547,7 +615,6 @@ int ml_profile_step(ml_ctx *c, const int *runs, int count, int steps, double dt,
   for (int k = 0; k < steps; k++) {
     events.clear(); labels.clear();
     if (p->rb == 8) launchStep<double>(*p, bt->dev, count, dt, c->stream, &t); else launchStep<float>(*p, bt->dev, count, dt, c->stream, &t);
-    t.mark("end");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (acc.empty()) { acc.assign(events.size(), 0.0); first = labels; }
     for (size_t e = 1; e < events.size(); e++) { float v = 0; hipEventElapsedTime(&v, events[e - 1], events[e]); acc[e] += v; }
