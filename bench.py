@@ -136,6 +136,7 @@ def main():
                           "frac": batched * STEP_BYTES_PER_CELL / PEAK_HBM},
         "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][1])},
         "dominant_kernel": dominant,
+        "tuning": {"batched": M.batch_tuning(runs), "plain": M.batch_tuning([plain_run])},
         "copy_GBps": ctx.copy_bandwidth(),
         "device": ctx.adapter,
     }

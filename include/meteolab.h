@@ -109,6 +109,14 @@ int ml_run_set_time(ml_ctx *ctx, int run, double time);
  * the launch sequence of a batch is captured once into a hipGraph and replayed. Asynchronous: ml_sync waits. */
 int ml_step(ml_ctx *ctx, const int *runs, int count, int steps, double dt);
 int ml_sync(ml_ctx *ctx);
+/* The autotuned launch shape of a batch's transport kernel (chosen by timing on the device when the batch is first
+ * stepped): shape[0] the layout (0 strips of a level with shape[1] field groups per block, 1 one level of one field
+ * per block), shape[2] 1 if each run's blocks are kept on one XCD; *ms the stage-0 transport time it measured.
+ * No counterpart in the reference. */
+int ml_batch_tuning(ml_ctx *ctx, const int *runs, int count, int *shape, float *ms);
+/* Sets a batch's transport launch shape (as ml_batch_tuning reports it) in place of the autotuned one; the next
+ * ml_step recaptures the step. No counterpart in the reference. */
+int ml_batch_set_tuning(ml_ctx *ctx, const int *runs, int count, const int *shape);
 /* ml_step of `steps` steps with a HIP event on the context's stream after each: ms[k] receives the device time of
  * step k (the graph replay), for throughput and latency measurement. Synchronous. */
 int ml_step_timed(ml_ctx *ctx, const int *runs, int count, int steps, double dt, double *ms);

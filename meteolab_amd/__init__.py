@@ -77,6 +77,8 @@ SIGNATURES = {
     "ml_sync": (_i, [_vp]),
     "ml_step_timed": (_i, [_vp, _ip, _i, _i, _d, _dp]),
     "ml_copy_probe": (_i, [_vp, _i64, _i, _dp]),
+    "ml_batch_tuning": (_i, [_vp, _ip, _i, _ip, _fp]),
+    "ml_batch_set_tuning": (_i, [_vp, _ip, _i, _ip]),
     "ml_profile_step": (_i, [_vp, _ip, _i, _i, _d, _dp, _i, _cp, _i]),
 }
 
@@ -281,6 +283,27 @@ def profile_step(runs, n=3, dt=3.0, max_launches=128):
     names = ctypes.create_string_buffer(8192)
     k = ctx.check(lib().ml_profile_step(ctx.h, ids, len(runs), int(n), float(dt), _dptr(ms), max_launches, names, 8192))
     return list(zip(names.value.decode().split("\n")[:k], ms[:k]))
+
+
+def batch_tuning(runs):
+    """The autotuned launch shape of this batch's transport kernel (ml_batch_tuning)."""
+    ctx = runs[0].ctx
+    ids = (ctypes.c_int * len(runs))(*[r.h for r in runs])
+    shape = (ctypes.c_int * 3)()
+    ms = ctypes.c_float()
+    ctx.check(lib().ml_batch_tuning(ctx.h, ids, len(runs), shape, ctypes.byref(ms)))
+    layout = "level" if shape[0] == 1 else "strip"
+    return {"transport_layout": layout, "field_groups": shape[1] if shape[0] == 0 else 1,
+            "xcd_per_run": bool(shape[2]), "stage0_ms": round(ms.value, 4)}
+
+
+def set_batch_tuning(runs, layout, groups=1, xcd=False):
+    """Forces this batch's transport launch shape (ml_batch_set_tuning): layout "strip" (k_stage, `groups` field
+    groups per block) or "level" (k_stage_level)."""
+    ctx = runs[0].ctx
+    ids = (ctypes.c_int * len(runs))(*[r.h for r in runs])
+    shape = (ctypes.c_int * 3)(1 if layout == "level" else 0, int(groups), 1 if xcd else 0)
+    ctx.check(lib().ml_batch_set_tuning(ctx.h, ids, len(runs), shape))
 
 
 def kernel_times(launches):

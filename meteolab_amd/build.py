@@ -29,6 +29,7 @@ def build(verbose=False):
     res = subprocess.run(cmd, capture_output=True, text=True)
     out = "\n".join(line for line in (res.stdout + res.stderr).splitlines() if "packed-fp32-ops" not in line)
     if res.returncode != 0:
+        print(out, file=sys.stderr)
         raise subprocess.CalledProcessError(res.returncode, cmd, output=out)
     if verbose and out.strip():
         print(out, file=sys.stderr)

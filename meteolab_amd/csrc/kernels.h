@@ -234,8 +234,9 @@ template <class R> __global__ __launch_bounds__(TPB) void k_faces(Geo g, Bufs<R>
 // buoyancy in the w tendency, 1045 the rigid walls of the stages)
 //
 // WENO5 weights without underflow in float: the stencil is divided by its largest magnitude s (the weights are
-// scale-free but for the 1e-40 in the reference's epsilon, which becomes 1e-40 / s^2 and is kept), and the weights are
-// taken relative to epsilon, w_k = gamma_k (eps / b_k)^2, which are bounded by gamma_k.
+// scale-free but for the 1e-40 in the reference's epsilon, which becomes 1e-40 / s^2 and is kept). The normalised
+// indicators then lie in [2e-7, ~100], so the reference's one-division form (both sums times (b0 b1 b2)^2) stays
+// within float range; a stencil below 1e-20, where 1e-40 / s^2 exceeds 1, takes the weights one by one relative to eps.
 template <class R> __device__ inline R wenoFace(R a, R b_, R c, R d, R e) {
   R s = mmax(mmax(mmax(mabs(a), mabs(b_)), mmax(mabs(c), mabs(d))), mabs(e));
   if (s == 0) return 0;
@@ -251,8 +252,14 @@ template <class R> __device__ inline R wenoFace(R a, R b_, R c, R d, R e) {
   R tiny = R(1e-20) * is;  // <= 1e5
   R eps = R(2e-7) * (a * a + b_ * b_ + c * c + d * d + e * e) + tiny * tiny;
   R b0 = eps + R(13.0 / 12) * s0 * s0 + R(.25) * t0 * t0, b1 = eps + R(13.0 / 12) * s1 * s1 + R(.25) * t1 * t1, b2 = eps + R(13.0 / 12) * s2 * s2 + R(.25) * t2 * t2;
-  R r0 = eps / b0, r1 = eps / b1, r2 = eps / b2;
-  R w0 = R(.1) * r0 * r0, w1 = R(.6) * r1 * r1, w2 = R(.3) * r2 * r2;
+  R w0, w1, w2;
+  if (tiny > 1) {
+    R r0 = eps / b0, r1 = eps / b1, r2 = eps / b2;
+    w0 = R(.1) * r0 * r0; w1 = R(.6) * r1 * r1; w2 = R(.3) * r2 * r2;
+  } else {
+    R c0 = b0 * b0, c1 = b1 * b1, c2 = b2 * b2;
+    w0 = R(.1) * c1 * c2; w1 = R(.6) * c0 * c2; w2 = R(.3) * c0 * c1;
+  }
   return s * ((w0 * p0 + w1 * p1 + w2 * p2) / (w0 + w1 + w2));
 }
 template <class R> __device__ inline R upwindFace(R a, R b_, R c, R d, R e) { return (2 * a - 13 * b_ + 47 * c + 27 * d - 3 * e) / 60; }
@@ -264,13 +271,157 @@ template <class R> __device__ inline R faceFlux(bool monotone, R m, R qm2, R qm1
   return m >= 0 ? m * faceValue(monotone, qm2, qm1, q0, qp1, qp2) : m * faceValue(monotone, qp3, qp2, qp1, q0, qm1);
 }
 __device__ inline bool monotoneField(int f) { return !(f == F_U || f == F_V || f == F_W || f == F_THETA); }
+// Index of a carried field among the conserved ones (rain, snow, graupel, hail, q, cloud, ice), -1 if it is not one.
+__device__ inline int conservedIndex(int f) { return f <= F_HAIL ? f : f == F_Q ? 4 : f == F_CLOUD ? 5 : f == F_ICE ? 6 : -1; }
+constexpr double TINY_WATER = 1e-10;
 
-// One block per (level, field, run): the level of the field is staged in LDS (periodic in x and y, so the whole level
-// is its own halo), the x and y face fluxes of the level go to LDS once, the vertical stencil comes from memory.
-// stage 0: state -> X, stage 1: X -> Y, stage 2: Y -> state (each node reads its own start value first).
-template <class R> __global__ __launch_bounds__(TPB) void k_stage(Geo g, Bufs<R> b, int stage, double dt) {
-  int slot = b.runs[blockIdx.z], z = blockIdx.x, f = blockIdx.y;
-  int inSet = stage, outSet = stage == 2 ? 0 : stage + 1;
+// Transport blocks: a strip of TY rows of one level of one run, for all carried fields in turn, so the face mass fluxes
+// are read once per node. Each field's rows of the strip with three halo rows on either side (periodic in y; the rows
+// are whole, so periodic in x) are staged in LDS; the x and y face fluxes of the strip go to LDS; the vertical stencil
+// comes from memory (the blocks of a run are kept on one XCD, see blockRun, so its levels share that XCD's L2).
+// stage 0: state -> X, stage 1: X -> Y, stage 2: Y -> state (each node reads its own start value first); stage 2 also
+// sums the positivity fix of the conserved fields (model.ts:606-617 fillNegative) for the physics kernel.
+struct StageTile { int ty, tiles, threads; };
+__host__ __device__ inline StageTile stageTile(const Geo &g) {
+  // Strips of about 256-512 nodes that divide the level.
+  int ty = 1;
+  for (int t = 1; t <= g.ny; t++) if (g.ny % t == 0 && g.nx * t <= 512) ty = t;
+  StageTile s; s.ty = ty; s.tiles = g.ny / ty; s.threads = ((g.nx * ty + 63) / 64) * 64;
+  return s;
+}
+__host__ __device__ inline size_t stageLdsBytes(const Geo &g, size_t rb) {
+  StageTile t = stageTile(g);
+  size_t q = size_t(t.ty + 6) * g.nx * 4, fx = size_t(t.ty) * g.nx * rb, fy = size_t(t.ty + 1) * g.nx * rb;
+  return ((q + 15) / 16) * 16 + fx + fy;
+}
+// Block -> (run index in the launch list, block within the run). Workgroups go to the eight XCDs in turn; with `xcd`
+// set and a batch that divides into eight, each run's blocks are all given to one XCD, in order.
+__device__ inline void blockRun(int nr, int perRun, int xcd, int &runIdx, int &b) {
+  int p = blockIdx.x;
+  if (xcd && nr % 8 == 0) { int x = p & 7, q = p >> 3; runIdx = x + 8 * (q / perRun); b = q % perRun; }
+  else { runIdx = p / perRun; b = p % perRun; }
+}
+
+// `groups` splits the fields among blocks (group k takes fields [k nc / groups, (k + 1) nc / groups)): more blocks in
+// flight against the mass fluxes read once per group; the autotuner picks it per batch.
+template <class R> __global__ __launch_bounds__(512) void k_stage(Geo g, Bufs<R> b, int stage, double dt, int nr, int nc, int groups, int xcd) {
+  const StageTile T = stageTile(g);
+  int runIdx, blk;
+  blockRun(nr, T.tiles * g.nz * groups, xcd, runIdx, blk);
+  const int grp = blk % groups;
+  blk /= groups;
+  const int slot = b.runs[runIdx], z = blk / T.tiles, y0 = (blk % T.tiles) * T.ty;
+  const int f0 = grp * nc / groups, f1 = (grp + 1) * nc / groups;
+  const int inSet = stage, outSet = stage == 2 ? 0 : stage + 1;
+  const int layer = g.layer, nx = g.nx, ny = g.ny, nz = g.nz, ty = T.ty, tn = nx * ty;
+  extern __shared__ char smem[];
+  float *ql = reinterpret_cast<float *>(smem);                        // rows y0-3 .. y0+ty+2
+  R *fxl = reinterpret_cast<R *>(smem + ((size_t(ty + 6) * nx * 4 + 15) / 16) * 16);  // ty rows
+  R *fyl = fxl + tn;                                                   // faces of rows y0-1 .. y0+ty-1
+  __shared__ double red[16][3];
+  const int t = threadIdx.x;
+  const bool live = t < tn;
+  const int y = y0 + t / nx, x = t % nx, c = y * nx + x, i = z * layer + c;
+  const double *rho = b.lv(slot, L_RHO, g);
+  const R rz = R(rho[z]), hz = R(b.hz[z]), frac = R((stage == 0 ? 1.0 / 3 : stage == 1 ? 1.0 / 2 : 1.0) * dt);
+  // The node's face mass fluxes and divergence, once for all fields.
+  R mx = 0, my = 0, myd = 0, mzt = 0, mzb = 0, dv = 0;
+  if (live) {
+    mx = b.m(slot, 0, g)[i]; my = b.m(slot, 1, g)[i]; dv = b.m(slot, 3, g)[i];
+    mzt = z == nz - 1 ? R(0) : b.m(slot, 2, g)[i]; mzb = z == 0 ? R(0) : b.m(slot, 2, g)[i - layer];
+  }
+  // The y face below the strip's first row (row y0 - 1) belongs to threads t < nx.
+  const int yb = y0 == 0 ? ny - 1 : y0 - 1;
+  if (t < nx) myd = b.m(slot, 1, g)[z * layer + yb * nx + t];
+  int zc[7];
+  for (int k = 0; k < 7; k++) { int zz = z - 3 + k; zz = zz < 0 ? 0 : zz > nz - 1 ? nz - 1 : zz; zc[k] = zz * layer; }
+  for (int f = f0; f < f1; f++) {
+    const float *q = b.set(inSet, slot, f, g);
+    // u, v and theta (linear upwind5) are transported as departures from their level's base profile B(z), plus the
+    // profile's own tendency: the scheme is linear, so this is the same tendency, without float32 sums of ~25 m/s or
+    // ~300 K values (the reference's WebGPU step carries theta the same way, gpu/dynamics.ts:78-97).
+    const int lvl = f == F_U ? L_UENV : f == F_V ? L_VENV : f == F_THETA ? L_THETA : -1;
+    const double *B = lvl >= 0 ? b.lv(slot, lvl, g) : nullptr;
+    const double Bz = B ? B[z] : 0;
+    const float *qz = q + size_t(z) * layer;
+    for (int k = t; k < (ty + 6) * nx; k += blockDim.x) {
+      int r = k / nx, yy = y0 - 3 + r; yy += yy < 0 ? ny : 0; yy -= yy >= ny ? ny : 0;
+      float v = qz[yy * nx + (k - r * nx)];
+      ql[k] = B ? float(double(v) - Bz) : v;
+    }
+    __syncthreads();
+    const bool mono = monotoneField(f);
+    auto Q = [&](int row, int xx) { xx += xx < 0 ? nx : 0; xx -= xx >= nx ? nx : 0; return R(ql[row * nx + xx]); };
+    if (live) {
+      int r = t / nx + 3;  // the node's row in ql
+      fxl[t] = faceFlux(mono, mx, Q(r, x - 2), Q(r, x - 1), Q(r, x), Q(r, x + 1), Q(r, x + 2), Q(r, x + 3));
+      fyl[nx + t] = faceFlux(mono, my, Q(r - 2, x), Q(r - 1, x), Q(r, x), Q(r + 1, x), Q(r + 2, x), Q(r + 3, x));
+    }
+    if (t < nx) fyl[t] = faceFlux(mono, myd, Q(0, t), Q(1, t), Q(2, t), Q(3, t), Q(4, t), Q(5, t));
+    __syncthreads();
+    double fill[3] = {0, 0, 0};
+    if (live) {
+      double Bc[7];
+      for (int k = 0; k < 7; k++) Bc[k] = B ? B[zc[k] / layer] : 0;
+      R col[7];
+      const R qn = R(ql[(t / nx + 3) * nx + x]);
+      for (int k = 0; k < 7; k++) col[k] = k == 3 ? qn : B ? R(double(q[zc[k] + c]) - Bc[k]) : R(q[zc[k] + c]);
+      // Top face of this node (none through the model top) and of the node below (none into the ground).
+      R fzt = z == nz - 1 ? R(0) : faceFlux(mono, mzt, col[1], col[2], col[3], col[4], col[5], col[6]);
+      R fzb = z == 0 ? R(0) : faceFlux(mono, mzb, col[0], col[1], col[2], col[3], col[4], col[5]);
+      const int xm = x == 0 ? nx - 1 : x - 1;
+      R flux = (fxl[t] - fxl[t - x + xm]) / R(g.dx) + (fyl[nx + t] - fyl[t]) / R(g.dy) + (fzt - fzb) / hz;
+      R tend = -(flux - qn * dv) / rz;
+      if (B) {
+        // The base profile's face values at the top and bottom faces for the flow's direction (float64), minus B(z):
+        // its tendency is the vertical flux of these, the horizontal fluxes cancelling against -B div.
+        R pt = R((mzt >= 0 ? upwind5d(Bc[1], Bc[2], Bc[3], Bc[4], Bc[5]) : upwind5d(Bc[6], Bc[5], Bc[4], Bc[3], Bc[2])) - Bz);
+        R pb = R((mzb >= 0 ? upwind5d(Bc[0], Bc[1], Bc[2], Bc[3], Bc[4]) : upwind5d(Bc[5], Bc[4], Bc[3], Bc[2], Bc[1])) - Bz);
+        tend -= (mzt * pt - mzb * pb) / (hz * rz);
+      }
+      if (f == F_W) {
+        // g B with the fields of this stage; the departure of theta_v from the environment in float64 (it is a small
+        // difference of numbers near 300 K).
+        const double te = b.lv(slot, L_THETA, g)[z], qe = b.lv(slot, L_Q, g)[z], thvEnv = te * (1 + .61 * qe);
+        const float th = b.set(inSet, slot, F_THETA, g)[i], qv = b.set(inSet, slot, F_Q, g)[i];
+        R loaded = R(b.set(inSet, slot, F_CLOUD, g)[i]) + R(b.set(inSet, slot, F_ICE, g)[i])
+          + (R(b.set(inSet, slot, F_RAIN, g)[i]) + R(b.set(inSet, slot, F_SNOW, g)[i]) + R(b.set(inSet, slot, F_GRAUPEL, g)[i]) + R(b.set(inSet, slot, F_HAIL, g)[i]));
+        double dthv = (double(th) * (1 + .61 * double(qv)) - thvEnv) / thvEnv;
+        tend += R(G) * (R(dthv) - loaded);
+      }
+      float val = float(R(b.f(slot, f, g)[i]) + frac * tend);
+      if (f == F_W && stage < 2 && (z == 0 || z == nz - 1)) val = 0;
+      b.set(outSet, slot, f, g)[i] = val;
+      if (stage == 2) {
+        double xv = val, wz = b.lv(slot, L_LW, g)[z];
+        if (xv < TINY_WATER) { if (xv != 0) { fill[0] = wz * xv; if (xv < 0) fill[2] = -wz * xv; } }
+        else fill[1] = wz * xv;
+      }
+    }
+    const int cf = conservedIndex(f);
+    if (stage == 2 && cf >= 0) {
+      // Block sums of removed, kept and negative mass (level weights), into the run's positivity-fix sums.
+      for (int j = 0; j < 3; j++) fill[j] = waveSum(fill[j]);
+      if ((t & 63) == 0) for (int j = 0; j < 3; j++) red[t >> 6][j] = fill[j];
+      __syncthreads();
+      if (t < 3) {
+        double s = 0;
+        for (int w = 0; w < int(blockDim.x) / 64; w++) s += red[w][t];
+        if (s != 0) atomicAdd(b.sc(slot) + S_FILL + 3 * cf + t, s);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// The other transport layout: one block of 256 threads per (level, field, run), the whole level in LDS (periodic in x
+// and y, so the level is its own halo), each thread a few nodes; the mass fluxes are read once per field.
+__host__ __device__ inline size_t stageLevelLdsBytes(const Geo &g, size_t rb) { return ((g.layer * 4 + 15) / 16) * 16 + 2 * size_t(g.layer) * rb; }
+template <class R> __global__ __launch_bounds__(TPB) void k_stage_level(Geo g, Bufs<R> b, int stage, double dt, int nr, int nc, int xcd) {
+  int runIdx, blk;
+  blockRun(nr, g.nz * nc, xcd, runIdx, blk);
+  const int slot = b.runs[runIdx], z = blk / nc, f = blk % nc;
+  const int inSet = stage, outSet = stage == 2 ? 0 : stage + 1;
   const float *q = b.set(inSet, slot, f, g);
   float *out = b.set(outSet, slot, f, g);
   const float *start = b.f(slot, f, g);
@@ -278,9 +429,8 @@ template <class R> __global__ __launch_bounds__(TPB) void k_stage(Geo g, Bufs<R>
   extern __shared__ char smem[];
   float *ql = reinterpret_cast<float *>(smem);
   R *fxl = reinterpret_cast<R *>(smem + ((layer * sizeof(float) + 15) / 16) * 16), *fyl = fxl + layer;
-  // u, v and theta (linear upwind5) are transported as departures from their level's base profile B(z), plus the
-  // profile's own tendency: the scheme is linear, so this is the same tendency, without float32 sums of ~25 m/s or
-  // ~300 K values (the reference's WebGPU step carries theta the same way, gpu/dynamics.ts:78-97).
+  __shared__ double red[TPB / 64][3];
+  // Departures from the level's base profile for u, v and theta, as in k_stage.
   const int lvl = f == F_U ? L_UENV : f == F_V ? L_VENV : f == F_THETA ? L_THETA : -1;
   const double *B = lvl >= 0 ? b.lv(slot, lvl, g) : nullptr;
   const double Bz = B ? B[z] : 0;
@@ -306,19 +456,18 @@ template <class R> __global__ __launch_bounds__(TPB) void k_stage(Geo g, Bufs<R>
   int zc[7];
   double Bc[7];
   for (int k = 0; k < 7; k++) { int zz = z - 3 + k; zz = zz < 0 ? 0 : zz > nz - 1 ? nz - 1 : zz; zc[k] = zz * layer; Bc[k] = B ? B[zz] : 0; }
-  // The base profile's face values at the top and bottom faces of the level for either flow direction (float64),
-  // minus B(z): its tendency is the vertical flux of these, the horizontal fluxes cancelling against -B div.
   R pt[2] = {0, 0}, pb[2] = {0, 0};
   if (B) {
     pt[0] = R(upwind5d(Bc[1], Bc[2], Bc[3], Bc[4], Bc[5]) - Bz); pt[1] = R(upwind5d(Bc[6], Bc[5], Bc[4], Bc[3], Bc[2]) - Bz);
     pb[0] = R(upwind5d(Bc[0], Bc[1], Bc[2], Bc[3], Bc[4]) - Bz); pb[1] = R(upwind5d(Bc[5], Bc[4], Bc[3], Bc[2], Bc[1]) - Bz);
   }
+  const double wz = b.lv(slot, L_LW, g)[z];
+  double fill[3] = {0, 0, 0};
   for (int c = threadIdx.x; c < layer; c += TPB) {
     int y = c / nx, x = c - y * nx;
     int i = z * layer + c;
     R col[7];
     for (int k = 0; k < 7; k++) col[k] = k == 3 ? R(ql[c]) : B ? R(double(q[zc[k] + c]) - Bc[k]) : R(q[zc[k] + c]);
-    // Top face of this node (none through the model top) and of the node below (none into the ground).
     R mzt = z == nz - 1 ? R(0) : mz[i], mzb = z == 0 ? R(0) : mz[i - layer];
     R fzt = z == nz - 1 ? R(0) : faceFlux(mono, mzt, col[1], col[2], col[3], col[4], col[5], col[6]);
     R fzb = z == 0 ? R(0) : faceFlux(mono, mzb, col[0], col[1], col[2], col[3], col[4], col[5]);
@@ -327,8 +476,6 @@ template <class R> __global__ __launch_bounds__(TPB) void k_stage(Geo g, Bufs<R>
     R tend = -(flux - R(ql[c]) * div[i]) / rz;
     if (B) tend -= (mzt * pt[mzt >= 0 ? 0 : 1] - mzb * pb[mzb >= 0 ? 0 : 1]) / (hz * rz);
     if (isW) {
-      // g B with the fields of this stage; the departure of theta_v from the environment in float64 (it is a small
-      // difference of numbers near 300 K).
       const float th = b.set(inSet, slot, F_THETA, g)[i], qv = b.set(inSet, slot, F_Q, g)[i];
       R loaded = R(b.set(inSet, slot, F_CLOUD, g)[i]) + R(b.set(inSet, slot, F_ICE, g)[i])
         + (R(b.set(inSet, slot, F_RAIN, g)[i]) + R(b.set(inSet, slot, F_SNOW, g)[i]) + R(b.set(inSet, slot, F_GRAUPEL, g)[i]) + R(b.set(inSet, slot, F_HAIL, g)[i]));
@@ -338,6 +485,22 @@ template <class R> __global__ __launch_bounds__(TPB) void k_stage(Geo g, Bufs<R>
     float val = float(R(start[i]) + frac * tend);
     if (isW && stage < 2 && (z == 0 || z == nz - 1)) val = 0;
     out[i] = val;
+    if (stage == 2) {
+      double xv = val;
+      if (xv < TINY_WATER) { if (xv != 0) { fill[0] += wz * xv; if (xv < 0) fill[2] -= wz * xv; } }
+      else fill[1] += wz * xv;
+    }
+  }
+  const int cf = conservedIndex(f);
+  if (stage == 2 && cf >= 0) {
+    for (int j = 0; j < 3; j++) fill[j] = waveSum(fill[j]);
+    if ((threadIdx.x & 63) == 0) for (int j = 0; j < 3; j++) red[threadIdx.x >> 6][j] = fill[j];
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      double s = 0;
+      for (int w = 0; w < TPB / 64; w++) s += red[w][threadIdx.x];
+      if (s != 0) atomicAdd(b.sc(slot) + S_FILL + 3 * cf + threadIdx.x, s);
+    }
   }
 }
 
@@ -408,7 +571,6 @@ template <class R> __global__ __launch_bounds__(TPB) void k_pcorrect(Geo g, Bufs
 
 // ---- positivity fix sums (model.ts:606-617 fillNegative): removed, kept, negative of each conserved field ------------
 __device__ inline int conservedField(int k) { const int c[N_CONSERVED] = {F_RAIN, F_SNOW, F_GRAUPEL, F_HAIL, F_Q, F_CLOUD, F_ICE}; return c[k]; }
-constexpr double TINY_WATER = 1e-10;
 template <class R> __global__ __launch_bounds__(TPB) void k_fill_sums(Geo g, Bufs<R> b) {
   int slot = b.runs[blockIdx.y];
   int i = blockIdx.x * blockDim.x + threadIdx.x;

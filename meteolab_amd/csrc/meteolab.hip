@@ -27,12 +27,18 @@ namespace {
 
 constexpr int MAX_POOLS = 64, SLOT_BITS = 16;
 
+// The transport kernel's launch: layout 0 is k_stage (strips of a level, `groups` field groups per strip), layout 1 is
+// k_stage_level (one whole level of one field per block); `xcd` keeps each run's blocks on one XCD.
+struct Tuning { int layout = 0, groups = 1, xcd = 0; float ms = 0; };
+
 struct Batch {
   std::vector<int> runs;
   int *dev = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double dt = 0;
+  Tuning tune{};   // the transport kernel's layout, picked by the autotuner when the batch is first stepped
+  bool tuned = false;
 };
 
 struct Pool {
@@ -132,9 +138,19 @@ template <class R> void spectralSolve(const Pool &p, const Bufs<R> &b, int nr, h
   if (t) t->mark("k_spec_inv");
 }
 
+// One RK3 stage of the transport of every carried field, in the batch's tuned layout.
+template <class R> void launchStage(const Geo &g, const Bufs<R> &b, int nr, int nc, int stage, double dt, hipStream_t s, const Tuning &tu) {
+  if (tu.layout == 1) {
+    hipLaunchKernelGGL(k_stage_level<R>, dim3(unsigned(g.nz) * nc * nr), dim3(TPB), stageLevelLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, tu.xcd);
+  } else {
+    const StageTile st = stageTile(g);
+    hipLaunchKernelGGL(k_stage<R>, dim3(unsigned(st.tiles) * g.nz * nr * tu.groups), dim3(st.threads), stageLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, tu.groups, tu.xcd);
+  }
+}
+
 // One model step of the runs in `list` (device array of nr slots): AtmosphereModel.step (model.ts:873-877). With a
 // timer, an event follows every kernel (named by the kernel).
-template <class R> void launchStep(const Pool &p, const int *list, int nr, double dt, hipStream_t s, Timer *t) {
+template <class R> void launchStep(const Pool &p, const int *list, int nr, double dt, hipStream_t s, Timer *t, const Tuning &tu) {
   const Geo &g = p.g;
   Bufs<R> b = bufs<R>(p, list);
   const int el = p.electrified, nc = el ? N_CARRIED : F_COLD + 1;
@@ -154,9 +170,8 @@ template <class R> void launchStep(const Pool &p, const int *list, int nr, doubl
   spectralSolve<R>(p, b, nr, s, 0, g.nz, 0, t);
   hipLaunchKernelGGL(k_faces<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 1);
   mark("k_faces");
-  const size_t stageLds = ((g.layer * sizeof(float) + 15) / 16) * 16 + 2 * g.layer * sizeof(R);
   for (int stage = 0; stage < 3; stage++) {
-    hipLaunchKernelGGL(k_stage<R>, dim3(g.nz, nc, nr), dim3(TPB), stageLds, s, g, b, stage, dt);
+    launchStage<R>(g, b, nr, nc, stage, dt, s, tu);
     mark("k_stage");
     if (stage < 2) {
       hipLaunchKernelGGL(k_pdiv<R>, dim3(cb, nr), dim3(TPB), 0, s, g, b, stage + 1, dt);
@@ -167,8 +182,6 @@ template <class R> void launchStep(const Pool &p, const int *list, int nr, doubl
     }
   }
   // endStep: positivity fix + physics (physicsStep), turbulence + drag (mixStep), projection (model.ts:1103-1113).
-  hipLaunchKernelGGL(k_fill_sums<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b);
-  mark("k_fill_sums");
   hipLaunchKernelGGL((k_physics<R, double>), dim3(nb, nr), dim3(TPB), 0, s, g, b, dt, el);
   mark("k_physics");
   hipLaunchKernelGGL(k_viscosity<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt);
@@ -319,7 +332,7 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
   const Geo &g = p->g;
   p->cap = capacity; p->electrified = electrified ? 1 : 0; p->rb = c->precision == 64 ? 8 : 4;
   // LDS the kernels need must fit the device's block limit.
-  size_t stageLds = ((g.layer * sizeof(float) + 15) / 16) * 16 + 2 * g.layer * p->rb;
+  size_t stageLds = stageLdsBytes(g, p->rb);
   if (stageLds > c->prop.sharedMemPerBlock || specLdsBytes(g, 8) + 4096 > c->prop.sharedMemPerBlock) { delete p; return fail(c, ML_E_UNSUPPORTED, "grid too large for a level in LDS"); }
   Sizes s = sizesOf(g, p->rb);
   size_t cap = capacity;
@@ -503,6 +516,57 @@ int ml_run_flashes(ml_ctx *c, int run, double *out, int max) {
   return n;
 }
 
+}  // extern "C"
+
+namespace {
+// The autotuner: the transport kernel's layout (k_stage with each field-group split, or k_stage_level; each with and
+// without the XCD-per-run block order), by timing the stage-0 transport of this batch (it reads the state and writes
+// only stage buffer X, which every step rewrites) for each candidate, fastest of 3 launches.
+template <class R> float timeStage(ml_ctx *c, const Pool &p, const Batch &bt, const Tuning &tu) {
+  const Geo &g = p.g;
+  Bufs<R> b = bufs<R>(p, bt.dev);
+  const int nr = int(bt.runs.size()), nc = p.electrified ? N_CARRIED : F_COLD + 1;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0); (void)hipEventCreate(&e1);
+  float best = 1e30f;
+  for (int rep = 0; rep < 3; rep++) {
+    (void)hipEventRecord(e0, c->stream);
+    launchStage<R>(g, b, nr, nc, 0, 3.0, c->stream, tu);
+    (void)hipEventRecord(e1, c->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    best = std::min(best, ms);
+  }
+  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+  return best;
+}
+static int tune(ml_ctx *c, Pool *p, Batch *bt) {
+  HIPCHK(c, hipSetDevice(c->device));
+  const int options[] = {1, 2, 3, 4, 6, 9, 17};
+  const int nc = p->electrified ? N_CARRIED : F_COLD + 1, nr = int(bt->runs.size());
+  std::vector<Tuning> cands;
+  for (int xcd = 0; xcd < (nr % 8 == 0 ? 2 : 1); xcd++) {
+    Tuning lv; lv.layout = 1; lv.xcd = xcd; cands.push_back(lv);
+    for (int gr : options) {
+      if (gr > nc) continue;
+      Tuning tl; tl.layout = 0; tl.groups = gr; tl.xcd = xcd; cands.push_back(tl);
+    }
+  }
+  Tuning best; best.ms = 1e30f;
+  for (Tuning &t : cands) {
+    t.ms = p->rb == 8 ? timeStage<double>(c, *p, *bt, t) : timeStage<float>(c, *p, *bt, t);
+    if (t.ms < best.ms) best = t;
+  }
+  HIPCHK(c, hipGetLastError());
+  bt->tune = best;
+  bt->tuned = true;
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
 static int batchOf(ml_ctx *c, const int *runs, int count, Pool *&p, Batch *&bt) {
   if (!c || !runs || count < 1) return fail(c, ML_E_ARG, "step: no runs");
   std::vector<int> slots(count);
@@ -520,7 +584,7 @@ static int batchOf(ml_ctx *c, const int *runs, int count, Pool *&p, Batch *&bt) 
   HIPCHK(c, hipMalloc((void **)&bt->dev, count * sizeof(int)));
   HIPCHK(c, hipMemcpy(bt->dev, slots.data(), count * sizeof(int), hipMemcpyHostToDevice));
   p->batches[slots] = bt;
-  return 0;
+  return tune(c, p, bt);
 }
 
 int ml_step(ml_ctx *c, const int *runs, int count, int steps, double dt) {
@@ -532,14 +596,35 @@ int ml_step(ml_ctx *c, const int *runs, int count, int steps, double dt) {
     if (bt->exec) { hipGraphExecDestroy(bt->exec); bt->exec = nullptr; }
     if (bt->graph) { hipGraphDestroy(bt->graph); bt->graph = nullptr; }
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    if (p->rb == 8) launchStep<double>(*p, bt->dev, count, dt, c->stream, nullptr);
-    else launchStep<float>(*p, bt->dev, count, dt, c->stream, nullptr);
+    if (p->rb == 8) launchStep<double>(*p, bt->dev, count, dt, c->stream, nullptr, bt->tune);
+    else launchStep<float>(*p, bt->dev, count, dt, c->stream, nullptr, bt->tune);
     HIPCHK(c, hipStreamEndCapture(c->stream, &bt->graph));
     HIPCHK(c, hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0));
     bt->dt = dt;
   }
   for (int k = 0; k < steps; k++) HIPCHK(c, hipGraphLaunch(bt->exec, c->stream));
   HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int ml_batch_tuning(ml_ctx *c, const int *runs, int count, int *shape, float *ms) {
+  Pool *p; Batch *bt;
+  if (int r = batchOf(c, runs, count, p, bt)) return r;
+  if (shape) { shape[0] = bt->tune.layout; shape[1] = bt->tune.groups; shape[2] = bt->tune.xcd; }
+  if (ms) *ms = bt->tune.ms;
+  return 0;
+}
+
+int ml_batch_set_tuning(ml_ctx *c, const int *runs, int count, const int *shape) {
+  Pool *p; Batch *bt;
+  if (int r = batchOf(c, runs, count, p, bt)) return r;
+  const int nc = p->electrified ? N_CARRIED : F_COLD + 1;
+  if (!shape || shape[0] < 0 || shape[0] > 1 || shape[1] < 1 || shape[1] > nc || shape[2] < 0 || shape[2] > 1)
+    return fail(c, ML_E_ARG, "set tuning: layout 0 or 1, 1 <= field groups <= carried fields, xcd 0 or 1");
+  bt->tune.layout = shape[0]; bt->tune.groups = shape[0] == 0 ? shape[1] : 1; bt->tune.xcd = shape[2]; bt->tune.ms = 0;
+  // The captured step holds the old launch shape.
+  if (bt->exec) { hipGraphExecDestroy(bt->exec); bt->exec = nullptr; }
+  if (bt->graph) { hipGraphDestroy(bt->graph); bt->graph = nullptr; }
   return 0;
 }
 
@@ -614,7 +699,7 @@ int ml_profile_step(ml_ctx *c, const int *runs, int count, int steps, double dt,
   std::vector<std::string> first;
   for (int k = 0; k < steps; k++) {
     events.clear(); labels.clear();
-    if (p->rb == 8) launchStep<double>(*p, bt->dev, count, dt, c->stream, &t); else launchStep<float>(*p, bt->dev, count, dt, c->stream, &t);
+    if (p->rb == 8) launchStep<double>(*p, bt->dev, count, dt, c->stream, &t, bt->tune); else launchStep<float>(*p, bt->dev, count, dt, c->stream, &t, bt->tune);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (acc.empty()) { acc.assign(events.size(), 0.0); first = labels; }
     for (size_t e = 1; e < events.size(); e++) { float v = 0; hipEventElapsedTime(&v, events[e - 1], events[e]); acc[e] += v; }

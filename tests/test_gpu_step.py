@@ -142,3 +142,40 @@ def test_graph_replay_equals_eager_launches(ctx):
     M.profile_step([b], 4)         # the same kernels launched one by one
     for name in M.CARRIED:
         assert rel_l2(a.field(name), b.field(name)) <= 1e-6, name
+
+
+@pytest.fixture(scope="module")
+def mature_small():
+    return mature(C.small_grid(), 650)
+
+
+TRANSPORT_SHAPES = [("strip", 1, False), ("strip", 4, True), ("strip", 17, True), ("level", 1, False), ("level", 1, True)]
+
+
+@pytest.mark.parametrize("layout,groups,xcd", TRANSPORT_SHAPES)
+def test_every_transport_layout_keeps_parity(ctx, mature_small, layout, groups, xcd):
+    # Each launch shape the autotuner may pick: one step of the mature storm against the oracle, and eight different
+    # runs batched in that shape (the XCD block order needs a multiple of eight) against each run stepped alone.
+    g = C.small_grid()
+    ref = OracleRun(g, C.profile(), C.run(g, seed=1))
+    for name in M.CARRIED + ["fieldZ"]:
+        ref.set_field(name, mature_small.field(name))
+    for name in M.COLUMNS:
+        ref.set_column(name, mature_small.column(name))
+    ref.set_time(mature_small.diag()["time"])
+    r = restart(ctx, mature_small, g)
+    M.set_batch_tuning([r], layout, groups, xcd)
+    ref.step(1); r.step(1)
+    compare(r, ref)
+    specs = [C.run(g, seed=s, amp=1.5 + .2 * s, offset=(500 * s - 2000, 300 * s - 1000)) for s in range(8)]
+    pool = ctx.pool(g, 8)
+    mixed = [pool.run(C.profile(), s) for s in specs]
+    M.set_batch_tuning(mixed, layout, groups, xcd)
+    assert M.batch_tuning(mixed)["transport_layout"] == layout
+    M.step(mixed, 10)
+    ctx.sync()
+    for s, rm in zip(specs, mixed):
+        alone = ctx.pool(g, 1).run(C.profile(), s)
+        alone.step(10)
+        for name in M.CARRIED:
+            assert rel_l2(rm.field(name), alone.field(name)) <= 1e-6, (name, s["seed"])
