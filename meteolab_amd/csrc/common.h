@@ -52,6 +52,9 @@ enum : int {
 };
 // Lightning channel limits (lightning.ts:32-37) and the per-run flash log.
 constexpr int MAX_CHANNEL = 3000, STROKES = 4, FLASH_LOG = 64, FLASH_FIELDS = 8;
+// The lightning's per-node int arrays and per-front-entry arrays of a slot.
+enum : int { LM_MARK = 0, LM_FAILED, LM_SEEN, LM_POS, LM_FNODE, LM_FLINK, LM_FFLAG, N_LMARK };
+enum : int { LN_FSCALE = 0, LN_FPHI, LN_FWEIGHT, LN_NSCALE, N_LNODE };
 // A flash log entry: time, CG (1) or IC (0), start node, channel nodes, charge (C), potential (V), charge moment (C m).
 enum : int { FL_TIME = 0, FL_CG, FL_START, FL_NODES, FL_CHARGE, FL_POTENTIAL, FL_MOMENT };
 // The falls of a step in order (model.ts:928-932, sedimentStep): water into the gauge, then charge with its carrier's
@@ -91,8 +94,11 @@ template <class R> struct Bufs {
   const int *runs; // active slots of this launch
   // Lightning state of each slot (lightning.ts:135-170): node charge density, the channel's per-node marks and front.
   R *lrho;         // [slot][n]     charge density of the last solve, C/m3
-  int *lmark;      // [slot][5][n]  mark, failed, seen, link, frontier
-  double *lnode;   // [slot][2][n]  scale (link length times threshold) and the front's weights
+  // Per node: mark, failed, seen (flash stamps) and the node's place in the front; per front entry (the front of a
+  // flash is a list in the reference's order): node, link (channel index) and flags.
+  int *lmark;      // [slot][7][n]
+  double *lnode;   // [slot][4][n]  per front entry: scale (link length times threshold), potential, weight; per node: scale
+  const double *efac; // [3][nz - 2][nm] the electric modes' elimination: pivots, upper factors, lower coefficients
   int *chan;       // [slot][2][MAX_CHANNEL] channel nodes and their parents
   double *cdep;    // [slot][MAX_CHANNEL] charge deposited on each channel node
   double *flog;    // [slot][FLASH_LOG][FLASH_FIELDS]
@@ -110,8 +116,8 @@ template <class R> struct Bufs {
   __device__ double *lv(int slot, int k, const Geo &g) const { return lev + (size_t(slot) * N_LEVELS + k) * g.nz; }
   __device__ double *pr(int slot) const { return par + size_t(slot) * N_PARAMS; }
   __device__ double *sc(int slot) const { return scal + size_t(slot) * N_SCALARS; }
-  __device__ int *lm(int slot, int k, const Geo &g) const { return lmark + (size_t(slot) * 5 + k) * g.n; }
-  __device__ double *ln(int slot, int k, const Geo &g) const { return lnode + (size_t(slot) * 2 + k) * g.n; }
+  __device__ int *lm(int slot, int k, const Geo &g) const { return lmark + (size_t(slot) * N_LMARK + k) * g.n; }
+  __device__ double *ln(int slot, int k, const Geo &g) const { return lnode + (size_t(slot) * N_LNODE + k) * g.n; }
 };
 
 // ---- wave64 / block reductions --------------------------------------------------------------------------------

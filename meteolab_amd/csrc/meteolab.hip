@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "discharge.h"
 #include "lightning.h"
 #include "spectral.h"
 
@@ -49,7 +50,8 @@ struct Pool {
   void *fields = nullptr, *stx = nullptr, *sty = nullptr, *mass = nullptr, *phi = nullptr, *cells = nullptr, *press = nullptr,
        *spec = nullptr, *mean0 = nullptr, *km = nullptr, *kh = nullptr, *ephi = nullptr, *cols = nullptr, *lev = nullptr,
        *par = nullptr, *scal = nullptr, *zs = nullptr, *dzs = nullptr, *hz = nullptr, *sym = nullptr, *twx = nullptr, *twy = nullptr,
-       *lrho = nullptr, *lmark = nullptr, *lnode = nullptr, *chan = nullptr, *cdep = nullptr, *flog = nullptr;
+       *lrho = nullptr, *lmark = nullptr, *lnode = nullptr, *chan = nullptr, *cdep = nullptr, *flog = nullptr, *efac = nullptr;
+  int chunk = 1;          // levels per pass of the discharge's own electric solve (as many as the block's LDS holds)
   int *single = nullptr;  // single[s] = s: the run list of one slot
   double *diag = nullptr; // [cap][N_DIAG_DEV]
   std::vector<double> zsH, dzsH, hzH;
@@ -110,7 +112,7 @@ template <class R> Bufs<R> bufs(const Pool &p, const int *runs) {
   b.ephi = (double *)p.ephi; b.cols = (float *)p.cols; b.lev = (double *)p.lev; b.par = (double *)p.par; b.scal = (double *)p.scal;
   b.zs = (const double *)p.zs; b.dzs = (const double *)p.dzs; b.hz = (const double *)p.hz; b.sym = (const double *)p.sym;
   b.twx = (const double *)p.twx; b.twy = (const double *)p.twy; b.runs = runs;
-  b.lrho = (R *)p.lrho; b.lmark = (int *)p.lmark; b.lnode = (double *)p.lnode; b.chan = (int *)p.chan; b.cdep = (double *)p.cdep; b.flog = (double *)p.flog;
+  b.efac = (const double *)p.efac; b.lrho = (R *)p.lrho; b.lmark = (int *)p.lmark; b.lnode = (double *)p.lnode; b.chan = (int *)p.chan; b.cdep = (double *)p.cdep; b.flog = (double *)p.flog;
   return b;
 }
 
@@ -203,7 +205,7 @@ template <class R> void launchStep(const Pool &p, const int *list, int nr, doubl
     mark("k_strongest");
     // flashLimit (lightning.ts:46): 10 flashes per 3 s of model time.
     int limit = std::max(1, int(std::floor(10 * dt / 3 + .5)));
-    hipLaunchKernelGGL(k_discharge<R>, dim3(nr), dim3(DISCHARGE_THREADS), specLdsBytes(g, sizeof(double)), s, g, b, p.plan, dt, limit);
+    hipLaunchKernelGGL(k_discharge<R>, dim3(nr), dim3(DISCHARGE_THREADS), solveLdsBytes(g, p.chunk), s, g, b, p.plan, dt, limit, p.chunk);
     mark("k_discharge");
     hipLaunchKernelGGL(k_conduct<R>, dim3(ib, nr), dim3(256), 0, s, g, b, dt);
     mark("k_conduct");
@@ -243,7 +245,7 @@ Sizes sizesOf(const Geo &g, size_t rb) {
   s.fields = size_t(N_FIELDS) * g.n * 4; s.stage = size_t(N_CARRIED) * g.n * 4; s.mass = 4 * size_t(g.n) * rb; s.node = size_t(g.n) * rb; s.node8 = size_t(g.n) * 8;
   s.spec = size_t(g.nz) * g.nm * 2 * 8; s.mean0 = size_t(g.nz) * 8; s.cols = size_t(N_COLUMNS) * g.layer * 4;
   s.lev = size_t(N_LEVELS) * g.nz * 8; s.par = N_PARAMS * 8; s.scal = N_SCALARS * 8;
-  s.lmark = 5 * size_t(g.n) * 4; s.lnode = 2 * size_t(g.n) * 8; s.chan = 2 * size_t(MAX_CHANNEL) * 4; s.cdep = size_t(MAX_CHANNEL) * 8;
+  s.lmark = size_t(N_LMARK) * g.n * 4; s.lnode = size_t(N_LNODE) * g.n * 8; s.chan = 2 * size_t(MAX_CHANNEL) * 4; s.cdep = size_t(MAX_CHANNEL) * 8;
   s.flog = size_t(FLASH_LOG) * FLASH_FIELDS * 8;
   return s;
 }
@@ -301,7 +303,7 @@ void ml_ctx_destroy(ml_ctx *c) {
     dropBatches(p, -1);
     void *all[] = {p->fields, p->stx, p->sty, p->mass, p->phi, p->cells, p->press, p->spec, p->mean0, p->km, p->kh, p->ephi, p->cols,
                    p->lev, p->par, p->scal, p->zs, p->dzs, p->hz, p->sym, p->twx, p->twy, p->single, p->diag, p->lrho, p->lmark,
-                   p->lnode, p->chan, p->cdep, p->flog};
+                   p->lnode, p->chan, p->cdep, p->flog, p->efac};
     for (void *a : all) if (a) hipFree(a);
     delete p;
   }
@@ -363,6 +365,22 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
     HIPCHK(c, hipMalloc(t.ptr, t.bytes));
     HIPCHK(c, hipMemcpy(*t.ptr, t.src, t.bytes, hipMemcpyHostToDevice));
   }
+  // The electric modes' elimination, once for the grid; the discharge's solve takes as many levels a pass as the
+  // block's LDS holds beside its own state.
+  const int L = g.nz - 2;
+  HIPCHK(c, hipMalloc(&p->efac, size_t(3) * L * g.nm * 8));
+  hipLaunchKernelGGL(k_electric_factors, dim3(blocks(g.nm, 256)), dim3(256), 0, c->stream, g, (const double *)p->sym, (const double *)p->dzs,
+                     (const double *)p->hz, (double *)p->efac);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t dischargeStatic = 8192, ldsMax = c->prop.sharedMemPerBlock;
+  int chunk = 1;
+  while (chunk < L && solveLdsBytes(g, chunk + 1) + dischargeStatic <= ldsMax) chunk++;
+  if (solveLdsBytes(g, chunk) + dischargeStatic > ldsMax) { delete p; return fail(c, ML_E_UNSUPPORTED, "grid too large for the discharge's level in LDS"); }
+  p->chunk = chunk;
+  const int dyn = int(solveLdsBytes(g, chunk));
+  if (p->rb == 8) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_discharge<double>), hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+  else HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_discharge<float>), hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
   p->used.assign(capacity, 0);
   c->pools.push_back(p);
   *pool = int(c->pools.size()) - 1;

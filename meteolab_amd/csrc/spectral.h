@@ -22,48 +22,66 @@ struct Plan { int rx[8], nrx, ry[8], nry; };
 
 constexpr int SPEC_THREADS = 256;
 
-// `count` sequences of length N in LDS, element k of sequence s at s * sStride + k * kStride; tw[k] = exp(-2 pi i k/N)
-// (sign -1, forward) or its conjugate (sign +1). Stockham stages ping-pong between a and b; returns the buffer that
-// holds the result. Ends with a barrier.
-template <class R> __device__ cx<R> *fftStages(cx<R> *a, cx<R> *b, int N, int count, int sStride, int kStride, const cx<R> *tw, const int *rad, int nrad, int sign) {
+// `count` sequences of length N in LDS, element k of sequence s at base(s) + k * kStride, base(s) = (s / per) * outer +
+// (s % per) * sStride (several levels' sequences at once: `per` sequences a level, levels `outer` apart);
+// tw[k] = exp(-2 pi i k/N) (sign -1, forward) or its conjugate (sign +1). Stockham stages ping-pong between a and b;
+// returns the buffer that holds the result. Ends with a barrier.
+// One Stockham pass of radix P (a compile-time radix, so a butterfly's values stay in registers).
+template <int P, class R> __device__ inline void fftPass(const cx<R> *src, cx<R> *dst, int N, int Ns, int count, int sStride, int kStride, const cx<R> *tw, int sign,
+                                                         int per, int outer) {
+  const int span = N / P, total = count * span, tstep = N / (Ns * P), wp = N / P;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int s = t / span, j = t - s * span, k = j % Ns, lvl = s / per, base = lvl * outer + (s - lvl * per) * sStride;
+    cx<R> v[P], o[P];
+#pragma unroll
+    for (int r = 0; r < P; r++) {
+      cx<R> x = src[base + (j + r * span) * kStride];
+      cx<R> w = tw[k * r * tstep];
+      if (sign > 0) w.im = -w.im;
+      v[r] = r == 0 ? x : cmul(x, w);
+    }
+    if constexpr (P == 2) {
+      o[0] = {v[0].re + v[1].re, v[0].im + v[1].im}; o[1] = {v[0].re - v[1].re, v[0].im - v[1].im};
+    } else if constexpr (P == 4) {
+      // W_4 = -i forward, +i inverse.
+      cx<R> s02 = {v[0].re + v[2].re, v[0].im + v[2].im}, d02 = {v[0].re - v[2].re, v[0].im - v[2].im};
+      cx<R> s13 = {v[1].re + v[3].re, v[1].im + v[3].im}, d13 = {v[1].re - v[3].re, v[1].im - v[3].im};
+      cx<R> jd13 = sign < 0 ? cx<R>{d13.im, -d13.re} : cx<R>{-d13.im, d13.re};  // d13 times W_4
+      o[0] = {s02.re + s13.re, s02.im + s13.im}; o[2] = {s02.re - s13.re, s02.im - s13.im};
+      o[1] = {d02.re + jd13.re, d02.im + jd13.im}; o[3] = {d02.re - jd13.re, d02.im - jd13.im};
+    } else {
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        cx<R> acc = v[0];
+#pragma unroll
+        for (int r = 1; r < P; r++) {
+          cx<R> w = tw[((r * q) % P) * wp];
+          if (sign > 0) w.im = -w.im;
+          cx<R> m = cmul(v[r], w);
+          acc.re += m.re; acc.im += m.im;
+        }
+        o[q] = acc;
+      }
+    }
+    const int d = (j / Ns) * Ns * P + k;
+#pragma unroll
+    for (int q = 0; q < P; q++) dst[base + (d + q * Ns) * kStride] = o[q];
+  }
+  __syncthreads();
+}
+
+template <class R> __device__ cx<R> *fftStages(cx<R> *a, cx<R> *b, int N, int count, int sStride, int kStride, const cx<R> *tw, const int *rad, int nrad, int sign,
+                                               int per = 1 << 30, int outer = 0) {
   cx<R> *src = a, *dst = b;
   int Ns = 1;
   for (int st = 0; st < nrad; st++) {
-    const int p = rad[st], span = N / p, total = count * span, tstep = N / (Ns * p), wp = N / p;
-    for (int t = threadIdx.x; t < total; t += blockDim.x) {
-      int s = t / span, j = t - s * span, k = j % Ns;
-      cx<R> v[5], o[5];
-      for (int r = 0; r < p; r++) {
-        cx<R> x = src[s * sStride + (j + r * span) * kStride];
-        cx<R> w = tw[k * r * tstep];
-        if (sign > 0) w.im = -w.im;
-        v[r] = r == 0 ? x : cmul(x, w);
-      }
-      if (p == 2) {
-        o[0] = {v[0].re + v[1].re, v[0].im + v[1].im}; o[1] = {v[0].re - v[1].re, v[0].im - v[1].im};
-      } else if (p == 4) {
-        // W_4 = -i forward, +i inverse.
-        cx<R> s02 = {v[0].re + v[2].re, v[0].im + v[2].im}, d02 = {v[0].re - v[2].re, v[0].im - v[2].im};
-        cx<R> s13 = {v[1].re + v[3].re, v[1].im + v[3].im}, d13 = {v[1].re - v[3].re, v[1].im - v[3].im};
-        cx<R> jd13 = sign < 0 ? cx<R>{d13.im, -d13.re} : cx<R>{-d13.im, d13.re};  // d13 times W_4
-        o[0] = {s02.re + s13.re, s02.im + s13.im}; o[2] = {s02.re - s13.re, s02.im - s13.im};
-        o[1] = {d02.re + jd13.re, d02.im + jd13.im}; o[3] = {d02.re - jd13.re, d02.im - jd13.im};
-      } else {
-        for (int q = 0; q < p; q++) {
-          cx<R> acc = v[0];
-          for (int r = 1; r < p; r++) {
-            cx<R> w = tw[((r * q) % p) * wp];
-            if (sign > 0) w.im = -w.im;
-            cx<R> m = cmul(v[r], w);
-            acc.re += m.re; acc.im += m.im;
-          }
-          o[q] = acc;
-        }
-      }
-      int d = (j / Ns) * Ns * p + k;
-      for (int q = 0; q < p; q++) dst[s * sStride + (d + q * Ns) * kStride] = o[q];
+    const int p = rad[st];
+    switch (p) {
+      case 2: fftPass<2>(src, dst, N, Ns, count, sStride, kStride, tw, sign, per, outer); break;
+      case 3: fftPass<3>(src, dst, N, Ns, count, sStride, kStride, tw, sign, per, outer); break;
+      case 4: fftPass<4>(src, dst, N, Ns, count, sStride, kStride, tw, sign, per, outer); break;
+      default: fftPass<5>(src, dst, N, Ns, count, sStride, kStride, tw, sign, per, outer); break;
     }
-    __syncthreads();
     cx<R> *t = src; src = dst; dst = t;
     Ns *= p;
   }

@@ -157,12 +157,7 @@ def test_every_transport_layout_keeps_parity(ctx, mature_small, layout, groups, 
     # Each launch shape the autotuner may pick: one step of the mature storm against the oracle, and eight different
     # runs batched in that shape (the XCD block order needs a multiple of eight) against each run stepped alone.
     g = C.small_grid()
-    ref = OracleRun(g, C.profile(), C.run(g, seed=1))
-    for name in M.CARRIED + ["fieldZ"]:
-        ref.set_field(name, mature_small.field(name))
-    for name in M.COLUMNS:
-        ref.set_column(name, mature_small.column(name))
-    ref.set_time(mature_small.diag()["time"])
+    ref = oracle_restart(mature_small, g)
     r = restart(ctx, mature_small, g)
     M.set_batch_tuning([r], layout, groups, xcd)
     ref.step(1); r.step(1)
@@ -179,3 +174,34 @@ def test_every_transport_layout_keeps_parity(ctx, mature_small, layout, groups, 
         alone.step(10)
         for name in M.CARRIED:
             assert rel_l2(rm.field(name), alone.field(name)) <= 1e-6, (name, s["seed"])
+
+
+def oracle_restart(o, g, seed=1):
+    """A fresh oracle run holding o's state (fields, fieldZ, columns, time)."""
+    ref = OracleRun(g, C.profile(), C.run(g, seed=seed))
+    for name in M.CARRIED + ["fieldZ"]:
+        ref.set_field(name, o.field(name))
+    for name in M.COLUMNS:
+        ref.set_column(name, o.column(name))
+    ref.set_time(o.diag()["time"])
+    return ref
+
+
+def test_lightning_matches_the_oracle_flash_for_flash(ctx, mature_small):
+    # From 32.5 min into the small-grid storm the first flash comes at step 656 and 24 more follow by step 700 (the
+    # oracle's counts): the same flashes, step by step, from the same seeded streams, and the charge fields after them.
+    g = C.small_grid()
+    ref = oracle_restart(mature_small, g)
+    r = restart(ctx, mature_small, g)
+    counts = []
+    for k in range(50):
+        ref.step(1); r.step(1)
+        d, od = r.diagnostics(), ref.diag()
+        counts.append((int(od["flashesIC"]), int(od["flashesCG"])))
+        assert (d["flashesIC"], d["flashesCG"]) == counts[-1], (k, counts[-1], d["flashesIC"], d["flashesCG"])
+    assert sum(counts[-1]) >= 20, counts[-1]
+    assert r.diagnostics()["firstFlash"] == ref.diag()["firstFlash"] == 1968.0
+    log = r.flashes()
+    assert len(log) == sum(counts[-1])
+    w = compare(r, ref, tol=1e-4)
+    print("after 50 steps and", counts[-1], "flashes:", {k: v for k, v in w.items() if k.startswith("charge")})
