@@ -79,6 +79,9 @@ int ml_pool_create(ml_ctx *ctx, const ml_grid *grid, int capacity, int electrifi
 int64_t ml_slot_bytes(ml_ctx *ctx, const ml_grid *grid);
 /* Free slots left in a pool. */
 int ml_pool_free(ml_ctx *ctx, int pool);
+/* Frees a pool and its device memory (waits for the context's stream); its run handles become invalid and its id may
+ * be reused by a later pool. No counterpart in the reference (its runs are garbage-collected objects). */
+int ml_pool_destroy(ml_ctx *ctx, int pool);
 
 /* A run in a free slot of `pool`: its base state from the sounding (environment.ts:45-73, in float64 on the device) and
  * its initial state (model.ts:399-451 single-storm start with the config's thermal and noise). *run receives a handle.
@@ -110,9 +113,9 @@ int ml_run_set_time(ml_ctx *ctx, int run, double time);
 int ml_step(ml_ctx *ctx, const int *runs, int count, int steps, double dt);
 int ml_sync(ml_ctx *ctx);
 /* The autotuned launch shape of a batch's transport kernel (chosen by timing on the device when the batch is first
- * stepped): shape[0] the layout (0 strips of a level with shape[1] field groups per block, 1 one level of one field
- * per block), shape[2] 1 if each run's blocks are kept on one XCD; *ms the stage-0 transport time it measured.
- * No counterpart in the reference. */
+ * stepped), shape[4]: shape[0] the layout (0 strips of a level with shape[1] field groups per block, 1 shape[3] whole
+ * levels of one field per block), shape[2] 1 if each run's blocks are kept on one XCD; *ms the stage-0 transport time
+ * it measured. No counterpart in the reference. */
 int ml_batch_tuning(ml_ctx *ctx, const int *runs, int count, int *shape, float *ms);
 /* Sets a batch's transport launch shape (as ml_batch_tuning reports it) in place of the autotuned one; the next
  * ml_step recaptures the step. No counterpart in the reference. */

@@ -63,6 +63,7 @@ SIGNATURES = {
     "ml_pool_create": (_i, [_vp, ctypes.POINTER(Grid), _i, _i, _ip]),
     "ml_slot_bytes": (_i64, [_vp, ctypes.POINTER(Grid)]),
     "ml_pool_free": (_i, [_vp, _i]),
+    "ml_pool_destroy": (_i, [_vp, _i]),
     "ml_run_create": (_i, [_vp, _i, ctypes.POINTER(Profile), ctypes.POINTER(RunConfig), _ip]),
     "ml_run_release": (_i, [_vp, _i]),
     "ml_run_upload": (_i, [_vp, _i, _i, _fp]),
@@ -177,6 +178,19 @@ class Pool:
     def free(self):
         return self.ctx.check(lib().ml_pool_free(self.ctx.h, self.id))
 
+    def destroy(self):
+        """Frees the pool's device memory (ml_pool_destroy); its runs become invalid. Also done when the pool (and
+        every run holding it) is garbage-collected while its context is open."""
+        if self.id is not None and self.ctx.h:
+            self.ctx.check(lib().ml_pool_destroy(self.ctx.h, self.id))
+        self.id = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
     def run(self, profile, run):
         return Run(self, profile, run)
 
@@ -289,20 +303,21 @@ def batch_tuning(runs):
     """The autotuned launch shape of this batch's transport kernel (ml_batch_tuning)."""
     ctx = runs[0].ctx
     ids = (ctypes.c_int * len(runs))(*[r.h for r in runs])
-    shape = (ctypes.c_int * 3)()
+    shape = (ctypes.c_int * 4)()
     ms = ctypes.c_float()
     ctx.check(lib().ml_batch_tuning(ctx.h, ids, len(runs), shape, ctypes.byref(ms)))
-    layout = "level" if shape[0] == 1 else "strip"
+    layout = ["strip", "level"][shape[0]]
     return {"transport_layout": layout, "field_groups": shape[1] if shape[0] == 0 else 1,
-            "xcd_per_run": bool(shape[2]), "stage0_ms": round(ms.value, 4)}
+            "levels_per_block": shape[3] if shape[0] else 1, "xcd_per_run": bool(shape[2]),
+            "stage0_ms": round(ms.value, 4)}
 
 
-def set_batch_tuning(runs, layout, groups=1, xcd=False):
+def set_batch_tuning(runs, layout, groups=1, xcd=False, levels=1):
     """Forces this batch's transport launch shape (ml_batch_set_tuning): layout "strip" (k_stage, `groups` field
-    groups per block) or "level" (k_stage_level)."""
+    groups per block) or "level" (k_stage_level, `levels` levels per block)."""
     ctx = runs[0].ctx
     ids = (ctypes.c_int * len(runs))(*[r.h for r in runs])
-    shape = (ctypes.c_int * 3)(1 if layout == "level" else 0, int(groups), 1 if xcd else 0)
+    shape = (ctypes.c_int * 4)(["strip", "level"].index(layout), int(groups), 1 if xcd else 0, int(levels))
     ctx.check(lib().ml_batch_set_tuning(ctx.h, ids, len(runs), shape))
 
 

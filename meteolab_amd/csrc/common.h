@@ -120,6 +120,15 @@ template <class R> struct Bufs {
   __device__ double *ln(int slot, int k, const Geo &g) const { return lnode + (size_t(slot) * N_LNODE + k) * g.n; }
 };
 
+// Quotient of 0 <= a < 2^22 by a divisor 1 <= d < 2^22 fixed for a loop: a float product with the divisor's reciprocal
+// is within one of the quotient, and one correction step makes it exact (integer division by a variable is a long
+// instruction sequence on the GPU).
+struct Div {
+  int d; float inv;
+  __device__ explicit Div(int d_) : d(d_), inv(1.0f / float(d_)) {}
+  __device__ int q(int a) const { int v = int(float(a) * inv), r = a - v * d; return v + (r >= d) - (r < 0); }
+};
+
 // ---- wave64 / block reductions --------------------------------------------------------------------------------
 template <class T> __device__ inline T waveSum(T v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

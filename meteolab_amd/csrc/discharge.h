@@ -64,13 +64,14 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
     rho[i] = R(chargeDensity(g, b, slot, i, air[i / layer]));
   }
   const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+  const Div dHl(hl), dNx(nx), dHm(half * mw), dMw(mw), dNm(nm);
   for (int l0 = 0; l0 < L; l0 += C) {
     const int cn = L - l0 < C ? L - l0 : C;
     // Right-hand side -rho / eps0 of level l0 + c (node level l0 + c + 1), rows 2p and 2p + 1 as one complex row; the
     // densities wait in B (free until the first pass) so that this loop reads global memory only and its loads overlap.
 #pragma unroll 4
     for (int t = threadIdx.x; t < cn * hl; t += blockDim.x) {
-      const int c = t / hl, r = t - c * hl, p = r / nx, x = r - p * nx, z = l0 + c + 1;
+      const int c = dHl.q(t), r = t - c * hl, p = dNx.q(r), x = r - p * nx, z = l0 + c + 1;
       const int i0 = z * layer + 2 * p * nx + x, i1 = i0 + nx;
       const double r0 = chargeDensity(g, b, slot, i0, air[z]), r1 = chargeDensity(g, b, slot, i1, air[z]);
       A[c * ls + r] = {-r0 / EPSILON0, -r1 / EPSILON0};
@@ -78,7 +79,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
     }
     __syncthreads();
     for (int t = threadIdx.x; t < cn * hl; t += blockDim.x) {
-      const int c = t / hl, r = t - c * hl, p = r / nx, x = r - p * nx, z = l0 + c + 1;
+      const int c = dHl.q(t), r = t - c * hl, p = dNx.q(r), x = r - p * nx, z = l0 + c + 1;
       const int i0 = z * layer + 2 * p * nx + x;
       rho[i0] = R(B[c * ls + r].re); rho[i0 + nx] = R(B[c * ls + r].im);
     }
@@ -93,7 +94,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
     Z *X = fftStages(A, B, nx, cn * half, nx, 1, tx, pl.rx, pl.nrx, -1, half, ls);
     Z *Y = X == A ? B : A;
     for (int t = threadIdx.x; t < cn * half * mw; t += blockDim.x) {
-      const int c = t / (half * mw), r = t - c * (half * mw), p = r / mw, m = r - p * mw;
+      const int c = dHm.q(t), r = t - c * (half * mw), p = dMw.q(r), m = r - p * mw;
       const Z z = X[c * ls + p * nx + m], w = X[c * ls + p * nx + (m ? nx - m : 0)];
       Y[c * ls + 2 * p * mw + m] = {(z.re + w.re) / 2, (z.im - w.im) / 2};
       Y[c * ls + (2 * p + 1) * mw + m] = {(z.im + w.im) / 2, (w.re - z.re) / 2};
@@ -101,7 +102,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
     __syncthreads();
     Z *S = fftStages(Y, X, ny, cn * mw, 1, mw, ty, pl.ry, pl.nry, -1, mw, ls);
     for (int t = threadIdx.x; t < cn * nm; t += blockDim.x) {
-      const int c = t / nm, m = t - c * nm;
+      const int c = dNm.q(t), m = t - c * nm;
       double *o = spec + (size_t(l0 + c) * nm + m) * 2;
       o[0] = S[c * ls + m].re; o[1] = S[c * ls + m].im;
     }
@@ -156,7 +157,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
     const int cn = L - l0 < C ? L - l0 : C;
 #pragma unroll 4
     for (int t = threadIdx.x; t < cn * nm; t += blockDim.x) {
-      const int c = t / nm, m = t - c * nm;
+      const int c = dNm.q(t), m = t - c * nm;
       const double *s = spec + (size_t(l0 + c) * nm + m) * 2;
       A[c * ls + m] = {s[0], s[1]};
     }
@@ -164,7 +165,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
     Z *Yv = fftStages(A, B, ny, cn * mw, 1, mw, ty, pl.ry, pl.nry, +1, mw, ls);
     Z *Zr = Yv == A ? B : A;
     for (int t = threadIdx.x; t < cn * half * nx; t += blockDim.x) {
-      const int c = t / (half * nx), r = t - c * (half * nx), p = r / nx, m = r - p * nx;
+      const int c = dHl.q(t), r = t - c * hl, p = dNx.q(r), m = r - p * nx;
       const int cc = m <= mh ? m : nx - m;
       const double sg = m <= mh ? 1.0 : -1.0;
       const bool real = cc == 0 || 2 * cc == nx;
@@ -175,7 +176,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
     __syncthreads();
     Z *Xv = fftStages(Zr, Yv, nx, cn * half, nx, 1, tx, pl.rx, pl.nrx, +1, half, ls);
     for (int t = threadIdx.x; t < cn * hl; t += blockDim.x) {
-      const int c = t / hl, r = t - c * hl, p = r / nx, x = r - p * nx;
+      const int c = dHl.q(t), r = t - c * hl, p = dNx.q(r), x = r - p * nx;
       double *dst = phi + size_t(l0 + c + 1) * layer;
       dst[2 * p * nx + x] = Xv[c * ls + r].re * norm;
       dst[(2 * p + 1) * nx + x] = Xv[c * ls + r].im * norm;

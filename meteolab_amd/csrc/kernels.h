@@ -237,6 +237,12 @@ template <class R> __global__ __launch_bounds__(TPB) void k_faces(Geo g, Bufs<R>
 // scale-free but for the 1e-40 in the reference's epsilon, which becomes 1e-40 / s^2 and is kept). The normalised
 // indicators then lie in [2e-7, ~100], so the reference's one-division form (both sums times (b0 b1 b2)^2) stays
 // within float range; a stencil below 1e-20, where 1e-40 / s^2 exceeds 1, takes the weights one by one relative to eps.
+// Reciprocal: the hardware's (1 ulp) in float, a division in double.
+__device__ inline float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double recip(double x) { return 1.0 / x; }
+
+// The candidate values are kept times six and the 1/6 joins the final normalisation, and the float reciprocals are the
+// hardware's: each face value moves by an ulp or two from the reference's order of operations.
 template <class R> __device__ inline R wenoFace(R a, R b_, R c, R d, R e) {
   R s = mmax(mmax(mmax(mabs(a), mabs(b_)), mmax(mabs(c), mabs(d))), mabs(e));
   if (s == 0) return 0;
@@ -245,9 +251,9 @@ template <class R> __device__ inline R wenoFace(R a, R b_, R c, R d, R e) {
     // limit of its weights (and 1/s would overflow a float for a denormal s).
     return (R(.1) * (2 * a - 7 * b_ + 11 * c) + R(.6) * (-b_ + 5 * c + 2 * d) + R(.3) * (2 * c + 5 * d - e)) / 6;
   }
-  R is = 1 / s;
+  R is = recip(s);
   a *= is; b_ *= is; c *= is; d *= is; e *= is;
-  R p0 = (2 * a - 7 * b_ + 11 * c) / 6, p1 = (-b_ + 5 * c + 2 * d) / 6, p2 = (2 * c + 5 * d - e) / 6;
+  R p0 = 2 * a - 7 * b_ + 11 * c, p1 = -b_ + 5 * c + 2 * d, p2 = 2 * c + 5 * d - e;
   R s0 = a - 2 * b_ + c, t0 = a - 4 * b_ + 3 * c, s1 = b_ - 2 * c + d, t1 = b_ - d, s2 = c - 2 * d + e, t2 = 3 * c - 4 * d + e;
   R tiny = R(1e-20) * is;  // <= 1e5
   R eps = R(2e-7) * (a * a + b_ * b_ + c * c + d * d + e * e) + tiny * tiny;
@@ -260,15 +266,17 @@ template <class R> __device__ inline R wenoFace(R a, R b_, R c, R d, R e) {
     R c0 = b0 * b0, c1 = b1 * b1, c2 = b2 * b2;
     w0 = R(.1) * c1 * c2; w1 = R(.6) * c0 * c2; w2 = R(.3) * c0 * c1;
   }
-  return s * ((w0 * p0 + w1 * p1 + w2 * p2) / (w0 + w1 + w2));
+  return s * ((w0 * p0 + w1 * p1 + w2 * p2) * recip(6 * (w0 + w1 + w2)));
 }
-template <class R> __device__ inline R upwindFace(R a, R b_, R c, R d, R e) { return (2 * a - 13 * b_ + 47 * c + 27 * d - 3 * e) / 60; }
+template <class R> __device__ inline R upwindFace(R a, R b_, R c, R d, R e) { return (2 * a - 13 * b_ + 47 * c + 27 * d - 3 * e) * R(1.0 / 60); }
 __device__ inline double upwind5d(double a, double b_, double c, double d, double e) { return (2 * a - 13 * b_ + 47 * c + 27 * d - 3 * e) / 60; }
 template <class R> __device__ inline R faceValue(bool monotone, R a, R b_, R c, R d, R e) { return monotone ? wenoFace(a, b_, c, d, e) : upwindFace(a, b_, c, d, e); }
 // Flux through the face between node values q0 (this node) and q1 (next), with upstream neighbours.
 // Flow towards +: stencil (m2, m1, q0, p1, p2); towards -: (p3, p2, p1, q0, m1).
+// The stencil is chosen first and the face value taken once (lanes of both flow directions share one evaluation).
 template <class R> __device__ inline R faceFlux(bool monotone, R m, R qm2, R qm1, R q0, R qp1, R qp2, R qp3) {
-  return m >= 0 ? m * faceValue(monotone, qm2, qm1, q0, qp1, qp2) : m * faceValue(monotone, qp3, qp2, qp1, q0, qm1);
+  const bool up = m >= 0;
+  return m * faceValue(monotone, up ? qm2 : qp3, up ? qm1 : qp2, up ? q0 : qp1, up ? qp1 : q0, up ? qp2 : qm1);
 }
 __device__ inline bool monotoneField(int f) { return !(f == F_U || f == F_V || f == F_W || f == F_THETA); }
 // Index of a carried field among the conserved ones (rain, snow, graupel, hail, q, cloud, ice), -1 if it is not one.
@@ -323,7 +331,7 @@ template <class R> __global__ __launch_bounds__(512) void k_stage(Geo g, Bufs<R>
   const bool live = t < tn;
   const int y = y0 + t / nx, x = t % nx, c = y * nx + x, i = z * layer + c;
   const double *rho = b.lv(slot, L_RHO, g);
-  const R rz = R(rho[z]), hz = R(b.hz[z]), frac = R((stage == 0 ? 1.0 / 3 : stage == 1 ? 1.0 / 2 : 1.0) * dt);
+  const R ihz = R(1.0 / b.hz[z]), irz = R(1.0 / rho[z]), ihzrz = R(1.0 / (b.hz[z] * rho[z])), idx = recip(R(g.dx)), idy = recip(R(g.dy)), frac = R((stage == 0 ? 1.0 / 3 : stage == 1 ? 1.0 / 2 : 1.0) * dt);
   // The node's face mass fluxes and divergence, once for all fields.
   R mx = 0, my = 0, myd = 0, mzt = 0, mzb = 0, dv = 0;
   if (live) {
@@ -370,14 +378,14 @@ template <class R> __global__ __launch_bounds__(512) void k_stage(Geo g, Bufs<R>
       R fzt = z == nz - 1 ? R(0) : faceFlux(mono, mzt, col[1], col[2], col[3], col[4], col[5], col[6]);
       R fzb = z == 0 ? R(0) : faceFlux(mono, mzb, col[0], col[1], col[2], col[3], col[4], col[5]);
       const int xm = x == 0 ? nx - 1 : x - 1;
-      R flux = (fxl[t] - fxl[t - x + xm]) / R(g.dx) + (fyl[nx + t] - fyl[t]) / R(g.dy) + (fzt - fzb) / hz;
-      R tend = -(flux - qn * dv) / rz;
+      R flux = (fxl[t] - fxl[t - x + xm]) * idx + (fyl[nx + t] - fyl[t]) * idy + (fzt - fzb) * ihz;
+      R tend = -(flux - qn * dv) * irz;
       if (B) {
         // The base profile's face values at the top and bottom faces for the flow's direction (float64), minus B(z):
         // its tendency is the vertical flux of these, the horizontal fluxes cancelling against -B div.
         R pt = R((mzt >= 0 ? upwind5d(Bc[1], Bc[2], Bc[3], Bc[4], Bc[5]) : upwind5d(Bc[6], Bc[5], Bc[4], Bc[3], Bc[2])) - Bz);
         R pb = R((mzb >= 0 ? upwind5d(Bc[0], Bc[1], Bc[2], Bc[3], Bc[4]) : upwind5d(Bc[5], Bc[4], Bc[3], Bc[2], Bc[1])) - Bz);
-        tend -= (mzt * pt - mzb * pb) / (hz * rz);
+        tend -= (mzt * pt - mzb * pb) * ihzrz;
       }
       if (f == F_W) {
         // g B with the fields of this stage; the departure of theta_v from the environment in float64 (it is a small
@@ -386,7 +394,7 @@ template <class R> __global__ __launch_bounds__(512) void k_stage(Geo g, Bufs<R>
         const float th = b.set(inSet, slot, F_THETA, g)[i], qv = b.set(inSet, slot, F_Q, g)[i];
         R loaded = R(b.set(inSet, slot, F_CLOUD, g)[i]) + R(b.set(inSet, slot, F_ICE, g)[i])
           + (R(b.set(inSet, slot, F_RAIN, g)[i]) + R(b.set(inSet, slot, F_SNOW, g)[i]) + R(b.set(inSet, slot, F_GRAUPEL, g)[i]) + R(b.set(inSet, slot, F_HAIL, g)[i]));
-        double dthv = (double(th) * (1 + .61 * double(qv)) - thvEnv) / thvEnv;
+        double dthv = (double(th) * (1 + .61 * double(qv)) - thvEnv) * (1.0 / thvEnv);
         tend += R(G) * (R(dthv) - loaded);
       }
       float val = float(R(b.f(slot, f, g)[i]) + frac * tend);
@@ -414,13 +422,16 @@ template <class R> __global__ __launch_bounds__(512) void k_stage(Geo g, Bufs<R>
   }
 }
 
-// The other transport layout: one block of 256 threads per (level, field, run), the whole level in LDS (periodic in x
-// and y, so the level is its own halo), each thread a few nodes; the mass fluxes are read once per field.
-__host__ __device__ inline size_t stageLevelLdsBytes(const Geo &g, size_t rb) { return ((g.layer * 4 + 15) / 16) * 16 + 2 * size_t(g.layer) * rb; }
-template <class R> __global__ __launch_bounds__(TPB) void k_stage_level(Geo g, Bufs<R> b, int stage, double dt, int nr, int nc, int xcd) {
+// The other transport layout: one block of 256 threads per (K levels, field, run), each level in turn whole in LDS
+// (periodic in x and y, so the level is its own halo), each thread a few nodes. The flux through the face between two
+// levels of a block is taken once: the top face of a level is the bottom face of the next (kept per node in LDS, each
+// node by the thread that owns it). The divisions of the tendency are products with per-level reciprocals.
+__host__ __device__ inline size_t stageLevelLdsBytes(const Geo &g, size_t rb) { return ((g.layer * 4 + 15) / 16) * 16 + 3 * size_t(g.layer) * rb; }
+template <class R> __global__ __launch_bounds__(TPB) void k_stage_level(Geo g, Bufs<R> b, int stage, double dt, int nr, int nc, int xcd, int K) {
+  const int zBlocks = (g.nz + K - 1) / K;
   int runIdx, blk;
-  blockRun(nr, g.nz * nc, xcd, runIdx, blk);
-  const int slot = b.runs[runIdx], z = blk / nc, f = blk % nc;
+  blockRun(nr, zBlocks * nc, xcd, runIdx, blk);
+  const int slot = b.runs[runIdx], f = blk % nc, z0 = (blk / nc) * K, z1 = z0 + K < g.nz ? z0 + K : g.nz;
   const int inSet = stage, outSet = stage == 2 ? 0 : stage + 1;
   const float *q = b.set(inSet, slot, f, g);
   float *out = b.set(outSet, slot, f, g);
@@ -428,68 +439,73 @@ template <class R> __global__ __launch_bounds__(TPB) void k_stage_level(Geo g, B
   const int layer = g.layer, nx = g.nx, ny = g.ny, nz = g.nz;
   extern __shared__ char smem[];
   float *ql = reinterpret_cast<float *>(smem);
-  R *fxl = reinterpret_cast<R *>(smem + ((layer * sizeof(float) + 15) / 16) * 16), *fyl = fxl + layer;
+  R *fxl = reinterpret_cast<R *>(smem + ((layer * sizeof(float) + 15) / 16) * 16), *fyl = fxl + layer, *fzl = fyl + layer;
   __shared__ double red[TPB / 64][3];
   // Departures from the level's base profile for u, v and theta, as in k_stage.
   const int lvl = f == F_U ? L_UENV : f == F_V ? L_VENV : f == F_THETA ? L_THETA : -1;
   const double *B = lvl >= 0 ? b.lv(slot, lvl, g) : nullptr;
-  const double Bz = B ? B[z] : 0;
-  const float *qz = q + size_t(z) * layer;
-  for (int c = threadIdx.x; c < layer; c += TPB) ql[c] = B ? float(double(qz[c]) - Bz) : qz[c];
-  __syncthreads();
-  const bool mono = monotoneField(f);
-  const R *mx = b.m(slot, 0, g) + size_t(z) * layer, *my = b.m(slot, 1, g) + size_t(z) * layer;
-  for (int c = threadIdx.x; c < layer; c += TPB) {
-    int y = c / nx, x = c - y * nx, row = y * nx;
-    auto X = [&](int o) { int xx = x + o; xx += xx < 0 ? nx : 0; xx -= xx >= nx ? nx : 0; return R(ql[row + xx]); };
-    auto Y = [&](int o) { int yy = y + o; yy += yy < 0 ? ny : 0; yy -= yy >= ny ? ny : 0; return R(ql[yy * nx + x]); };
-    fxl[c] = faceFlux(mono, mx[c], X(-2), X(-1), X(0), X(1), X(2), X(3));
-    fyl[c] = faceFlux(mono, my[c], Y(-2), Y(-1), Y(0), Y(1), Y(2), Y(3));
-  }
-  __syncthreads();
+  const bool mono = monotoneField(f), isW = f == F_W;
+  const Div dNx(nx);
+  const R idx = recip(R(g.dx)), idy = recip(R(g.dy)), frac = R((stage == 0 ? 1.0 / 3 : stage == 1 ? 1.0 / 2 : 1.0) * dt);
   const double *rho = b.lv(slot, L_RHO, g);
-  const R rz = R(rho[z]), hz = R(b.hz[z]), frac = R((stage == 0 ? 1.0 / 3 : stage == 1 ? 1.0 / 2 : 1.0) * dt);
   const R *mz = b.m(slot, 2, g), *div = b.m(slot, 3, g);
-  const bool isW = f == F_W;
-  double thvEnv = 0;
-  if (isW) { const double te = b.lv(slot, L_THETA, g)[z], qe = b.lv(slot, L_Q, g)[z]; thvEnv = te * (1 + .61 * qe); }
-  int zc[7];
-  double Bc[7];
-  for (int k = 0; k < 7; k++) { int zz = z - 3 + k; zz = zz < 0 ? 0 : zz > nz - 1 ? nz - 1 : zz; zc[k] = zz * layer; Bc[k] = B ? B[zz] : 0; }
-  R pt[2] = {0, 0}, pb[2] = {0, 0};
-  if (B) {
-    pt[0] = R(upwind5d(Bc[1], Bc[2], Bc[3], Bc[4], Bc[5]) - Bz); pt[1] = R(upwind5d(Bc[6], Bc[5], Bc[4], Bc[3], Bc[2]) - Bz);
-    pb[0] = R(upwind5d(Bc[0], Bc[1], Bc[2], Bc[3], Bc[4]) - Bz); pb[1] = R(upwind5d(Bc[5], Bc[4], Bc[3], Bc[2], Bc[1]) - Bz);
-  }
-  const double wz = b.lv(slot, L_LW, g)[z];
   double fill[3] = {0, 0, 0};
-  for (int c = threadIdx.x; c < layer; c += TPB) {
-    int y = c / nx, x = c - y * nx;
-    int i = z * layer + c;
-    R col[7];
-    for (int k = 0; k < 7; k++) col[k] = k == 3 ? R(ql[c]) : B ? R(double(q[zc[k] + c]) - Bc[k]) : R(q[zc[k] + c]);
-    R mzt = z == nz - 1 ? R(0) : mz[i], mzb = z == 0 ? R(0) : mz[i - layer];
-    R fzt = z == nz - 1 ? R(0) : faceFlux(mono, mzt, col[1], col[2], col[3], col[4], col[5], col[6]);
-    R fzb = z == 0 ? R(0) : faceFlux(mono, mzb, col[0], col[1], col[2], col[3], col[4], col[5]);
-    int xm = x == 0 ? nx - 1 : x - 1, ym = y == 0 ? ny - 1 : y - 1;
-    R flux = (fxl[c] - fxl[y * nx + xm]) / R(g.dx) + (fyl[c] - fyl[ym * nx + x]) / R(g.dy) + (fzt - fzb) / hz;
-    R tend = -(flux - R(ql[c]) * div[i]) / rz;
-    if (B) tend -= (mzt * pt[mzt >= 0 ? 0 : 1] - mzb * pb[mzb >= 0 ? 0 : 1]) / (hz * rz);
-    if (isW) {
-      const float th = b.set(inSet, slot, F_THETA, g)[i], qv = b.set(inSet, slot, F_Q, g)[i];
-      R loaded = R(b.set(inSet, slot, F_CLOUD, g)[i]) + R(b.set(inSet, slot, F_ICE, g)[i])
-        + (R(b.set(inSet, slot, F_RAIN, g)[i]) + R(b.set(inSet, slot, F_SNOW, g)[i]) + R(b.set(inSet, slot, F_GRAUPEL, g)[i]) + R(b.set(inSet, slot, F_HAIL, g)[i]));
-      double dthv = (double(th) * (1 + .61 * double(qv)) - thvEnv) / thvEnv;
-      tend += R(G) * (R(dthv) - loaded);
+  for (int z = z0; z < z1; z++) {
+    const double Bz = B ? B[z] : 0;
+    const float *qz = q + size_t(z) * layer;
+    for (int c = threadIdx.x; c < layer; c += TPB) ql[c] = B ? float(double(qz[c]) - Bz) : qz[c];
+    __syncthreads();
+    const R *mx = b.m(slot, 0, g) + size_t(z) * layer, *my = b.m(slot, 1, g) + size_t(z) * layer;
+    for (int c = threadIdx.x; c < layer; c += TPB) {
+      const int y = dNx.q(c), x = c - y * nx, row = y * nx;
+      auto X = [&](int o) { int xx = x + o; xx += xx < 0 ? nx : 0; xx -= xx >= nx ? nx : 0; return R(ql[row + xx]); };
+      auto Y = [&](int o) { int yy = y + o; yy += yy < 0 ? ny : 0; yy -= yy >= ny ? ny : 0; return R(ql[yy * nx + x]); };
+      fxl[c] = faceFlux(mono, mx[c], X(-2), X(-1), X(0), X(1), X(2), X(3));
+      fyl[c] = faceFlux(mono, my[c], Y(-2), Y(-1), Y(0), Y(1), Y(2), Y(3));
     }
-    float val = float(R(start[i]) + frac * tend);
-    if (isW && stage < 2 && (z == 0 || z == nz - 1)) val = 0;
-    out[i] = val;
-    if (stage == 2) {
-      double xv = val;
-      if (xv < TINY_WATER) { if (xv != 0) { fill[0] += wz * xv; if (xv < 0) fill[2] -= wz * xv; } }
-      else fill[1] += wz * xv;
+    __syncthreads();
+    const R ihz = R(1.0 / b.hz[z]), irz = R(1.0 / rho[z]), ihzrz = R(1.0 / (b.hz[z] * rho[z]));
+    double thvEnv = 0, ithv = 0;
+    if (isW) { const double te = b.lv(slot, L_THETA, g)[z], qe = b.lv(slot, L_Q, g)[z]; thvEnv = te * (1 + .61 * qe); ithv = 1.0 / thvEnv; }
+    int zc[7];
+    double Bc[7];
+    for (int k = 0; k < 7; k++) { int zz = z - 3 + k; zz = zz < 0 ? 0 : zz > nz - 1 ? nz - 1 : zz; zc[k] = zz * layer; Bc[k] = B ? B[zz] : 0; }
+    R pt[2] = {0, 0}, pb[2] = {0, 0};
+    if (B) {
+      pt[0] = R(upwind5d(Bc[1], Bc[2], Bc[3], Bc[4], Bc[5]) - Bz); pt[1] = R(upwind5d(Bc[6], Bc[5], Bc[4], Bc[3], Bc[2]) - Bz);
+      pb[0] = R(upwind5d(Bc[0], Bc[1], Bc[2], Bc[3], Bc[4]) - Bz); pb[1] = R(upwind5d(Bc[5], Bc[4], Bc[3], Bc[2], Bc[1]) - Bz);
     }
+    const double wz = b.lv(slot, L_LW, g)[z];
+    for (int c = threadIdx.x; c < layer; c += TPB) {
+      const int y = dNx.q(c), x = c - y * nx;
+      const int i = z * layer + c;
+      R col[7];
+      for (int k = 0; k < 7; k++) col[k] = k == 3 ? R(ql[c]) : B ? R(double(q[zc[k] + c]) - Bc[k]) : R(q[zc[k] + c]);
+      const R mzt = z == nz - 1 ? R(0) : mz[i], mzb = z == 0 ? R(0) : mz[i - layer];
+      const R fzt = z == nz - 1 ? R(0) : faceFlux(mono, mzt, col[1], col[2], col[3], col[4], col[5], col[6]);
+      const R fzb = z == 0 ? R(0) : z > z0 ? fzl[c] : faceFlux(mono, mzb, col[0], col[1], col[2], col[3], col[4], col[5]);
+      fzl[c] = fzt;
+      const int xm = x == 0 ? nx - 1 : x - 1, ym = y == 0 ? ny - 1 : y - 1;
+      const R flux = (fxl[c] - fxl[y * nx + xm]) * idx + (fyl[c] - fyl[ym * nx + x]) * idy + (fzt - fzb) * ihz;
+      R tend = -(flux - R(ql[c]) * div[i]) * irz;
+      if (B) tend -= (mzt * pt[mzt >= 0 ? 0 : 1] - mzb * pb[mzb >= 0 ? 0 : 1]) * ihzrz;
+      if (isW) {
+        const float th = b.set(inSet, slot, F_THETA, g)[i], qv = b.set(inSet, slot, F_Q, g)[i];
+        const R loaded = R(b.set(inSet, slot, F_CLOUD, g)[i]) + R(b.set(inSet, slot, F_ICE, g)[i])
+          + (R(b.set(inSet, slot, F_RAIN, g)[i]) + R(b.set(inSet, slot, F_SNOW, g)[i]) + R(b.set(inSet, slot, F_GRAUPEL, g)[i]) + R(b.set(inSet, slot, F_HAIL, g)[i]));
+        const double dthv = (double(th) * (1 + .61 * double(qv)) - thvEnv) * ithv;
+        tend += R(G) * (R(dthv) - loaded);
+      }
+      float val = float(R(start[i]) + frac * tend);
+      if (isW && stage < 2 && (z == 0 || z == nz - 1)) val = 0;
+      out[i] = val;
+      if (stage == 2) {
+        const double xv = val;
+        if (xv < TINY_WATER) { if (xv != 0) { fill[0] += wz * xv; if (xv < 0) fill[2] -= wz * xv; } }
+        else fill[1] += wz * xv;
+      }
+    }
+    __syncthreads();
   }
   const int cf = conservedIndex(f);
   if (stage == 2 && cf >= 0) {

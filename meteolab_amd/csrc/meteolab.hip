@@ -26,11 +26,11 @@ using namespace ml;
 
 namespace {
 
-constexpr int MAX_POOLS = 64, SLOT_BITS = 16;
+constexpr int MAX_POOLS = 4096, SLOT_BITS = 16;
 
 // The transport kernel's launch: layout 0 is k_stage (strips of a level, `groups` field groups per strip), layout 1 is
-// k_stage_level (one whole level of one field per block); `xcd` keeps each run's blocks on one XCD.
-struct Tuning { int layout = 0, groups = 1, xcd = 0; float ms = 0; };
+// k_stage_level (`levels` whole levels of one field per block); `xcd` keeps each run's blocks on one XCD.
+struct Tuning { int layout = 0, groups = 1, xcd = 0, levels = 1; float ms = 0; };
 
 struct Batch {
   std::vector<int> runs;
@@ -143,7 +143,8 @@ template <class R> void spectralSolve(const Pool &p, const Bufs<R> &b, int nr, h
 // One RK3 stage of the transport of every carried field, in the batch's tuned layout.
 template <class R> void launchStage(const Geo &g, const Bufs<R> &b, int nr, int nc, int stage, double dt, hipStream_t s, const Tuning &tu) {
   if (tu.layout == 1) {
-    hipLaunchKernelGGL(k_stage_level<R>, dim3(unsigned(g.nz) * nc * nr), dim3(TPB), stageLevelLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, tu.xcd);
+    const unsigned zBlocks = unsigned((g.nz + tu.levels - 1) / tu.levels);
+    hipLaunchKernelGGL(k_stage_level<R>, dim3(zBlocks * nc * nr), dim3(TPB), stageLevelLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, tu.xcd, tu.levels);
   } else {
     const StageTile st = stageTile(g);
     hipLaunchKernelGGL(k_stage<R>, dim3(unsigned(st.tiles) * g.nz * nr * tu.groups), dim3(st.threads), stageLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, tu.groups, tu.xcd);
@@ -294,21 +295,39 @@ int ml_ctx_create(int device, int precision, ml_ctx **out) {
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+// Frees a pool's device memory and its captured batches (the stream must be idle).
+void freePool(Pool *p) {
+  dropBatches(p, -1);
+  void *all[] = {p->fields, p->stx, p->sty, p->mass, p->phi, p->cells, p->press, p->spec, p->mean0, p->km, p->kh, p->ephi, p->cols,
+                 p->lev, p->par, p->scal, p->zs, p->dzs, p->hz, p->sym, p->twx, p->twy, p->single, p->diag, p->lrho, p->lmark,
+                 p->lnode, p->chan, p->cdep, p->flog, p->efac};
+  for (void *a : all) if (a) hipFree(a);
+  delete p;
+}
+}  // namespace
+
+extern "C" {
+
 void ml_ctx_destroy(ml_ctx *c) {
   if (!c) return;
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
-  for (Pool *p : c->pools) {
-    if (!p) continue;
-    dropBatches(p, -1);
-    void *all[] = {p->fields, p->stx, p->sty, p->mass, p->phi, p->cells, p->press, p->spec, p->mean0, p->km, p->kh, p->ephi, p->cols,
-                   p->lev, p->par, p->scal, p->zs, p->dzs, p->hz, p->sym, p->twx, p->twy, p->single, p->diag, p->lrho, p->lmark,
-                   p->lnode, p->chan, p->cdep, p->flog, p->efac};
-    for (void *a : all) if (a) hipFree(a);
-    delete p;
-  }
+  for (Pool *p : c->pools) if (p) freePool(p);
   hipStreamDestroy(c->stream);
   delete c;
+}
+
+int ml_pool_destroy(ml_ctx *c, int pool) {
+  Pool *p = c ? poolOf(c, pool) : nullptr;
+  if (!p) return fail(c, ML_E_ARG, "pool destroy: bad pool");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  freePool(p);
+  c->pools[pool] = nullptr;
+  return 0;
 }
 
 const char *ml_last_error(ml_ctx *c) { return c ? c->err.c_str() : "no context"; }
@@ -327,7 +346,8 @@ int64_t ml_slot_bytes(ml_ctx *c, const ml_grid *grid) {
 
 int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified, int *pool) {
   if (!c || !pool || capacity < 1 || capacity >= (1 << SLOT_BITS)) return fail(c, ML_E_ARG, "pool: capacity 1 .. 65535");
-  if (int(c->pools.size()) >= MAX_POOLS) return fail(c, ML_E_FULL, "too many pools");
+  int id = int(std::find(c->pools.begin(), c->pools.end(), nullptr) - c->pools.begin());
+  if (id >= MAX_POOLS) return fail(c, ML_E_FULL, "too many pools");
   Pool *p = new Pool();
   if (int r = geoOf(c, grid, p->g, p->plan)) { delete p; return r; }
   HIPCHK(c, hipSetDevice(c->device));
@@ -335,7 +355,8 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
   p->cap = capacity; p->electrified = electrified ? 1 : 0; p->rb = c->precision == 64 ? 8 : 4;
   // LDS the kernels need must fit the device's block limit.
   size_t stageLds = stageLdsBytes(g, p->rb);
-  if (stageLds > c->prop.sharedMemPerBlock || specLdsBytes(g, 8) + 4096 > c->prop.sharedMemPerBlock) { delete p; return fail(c, ML_E_UNSUPPORTED, "grid too large for a level in LDS"); }
+  if (stageLds > c->prop.sharedMemPerBlock || stageLevelLdsBytes(g, p->rb) > c->prop.sharedMemPerBlock ||
+      specLdsBytes(g, 8) + 4096 > c->prop.sharedMemPerBlock) { delete p; return fail(c, ML_E_UNSUPPORTED, "grid too large for a level in LDS"); }
   Sizes s = sizesOf(g, p->rb);
   size_t cap = capacity;
   struct { void **ptr; size_t bytes; } allocs[] = {
@@ -382,8 +403,8 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
   if (p->rb == 8) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_discharge<double>), hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
   else HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_discharge<float>), hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
   p->used.assign(capacity, 0);
-  c->pools.push_back(p);
-  *pool = int(c->pools.size()) - 1;
+  if (id == int(c->pools.size())) c->pools.push_back(p); else c->pools[id] = p;
+  *pool = id;
   return 0;
 }
 
@@ -565,7 +586,11 @@ static int tune(ml_ctx *c, Pool *p, Batch *bt) {
   const int nc = p->electrified ? N_CARRIED : F_COLD + 1, nr = int(bt->runs.size());
   std::vector<Tuning> cands;
   for (int xcd = 0; xcd < (nr % 8 == 0 ? 2 : 1); xcd++) {
-    Tuning lv; lv.layout = 1; lv.xcd = xcd; cands.push_back(lv);
+    for (int k : {1, 2, 5, 10}) {
+      if (k > p->g.nz) continue;
+      Tuning lv; lv.layout = 1; lv.xcd = xcd; lv.levels = k; cands.push_back(lv);
+    }
+
     for (int gr : options) {
       if (gr > nc) continue;
       Tuning tl; tl.layout = 0; tl.groups = gr; tl.xcd = xcd; cands.push_back(tl);
@@ -628,7 +653,7 @@ int ml_step(ml_ctx *c, const int *runs, int count, int steps, double dt) {
 int ml_batch_tuning(ml_ctx *c, const int *runs, int count, int *shape, float *ms) {
   Pool *p; Batch *bt;
   if (int r = batchOf(c, runs, count, p, bt)) return r;
-  if (shape) { shape[0] = bt->tune.layout; shape[1] = bt->tune.groups; shape[2] = bt->tune.xcd; }
+  if (shape) { shape[0] = bt->tune.layout; shape[1] = bt->tune.groups; shape[2] = bt->tune.xcd; shape[3] = bt->tune.levels; }
   if (ms) *ms = bt->tune.ms;
   return 0;
 }
@@ -637,9 +662,10 @@ int ml_batch_set_tuning(ml_ctx *c, const int *runs, int count, const int *shape)
   Pool *p; Batch *bt;
   if (int r = batchOf(c, runs, count, p, bt)) return r;
   const int nc = p->electrified ? N_CARRIED : F_COLD + 1;
-  if (!shape || shape[0] < 0 || shape[0] > 1 || shape[1] < 1 || shape[1] > nc || shape[2] < 0 || shape[2] > 1)
-    return fail(c, ML_E_ARG, "set tuning: layout 0 or 1, 1 <= field groups <= carried fields, xcd 0 or 1");
-  bt->tune.layout = shape[0]; bt->tune.groups = shape[0] == 0 ? shape[1] : 1; bt->tune.xcd = shape[2]; bt->tune.ms = 0;
+  if (!shape || shape[0] < 0 || shape[0] > 1 || shape[1] < 1 || shape[1] > nc || shape[2] < 0 || shape[2] > 1 || shape[3] < 1 || shape[3] > p->g.nz)
+    return fail(c, ML_E_ARG, "set tuning: layout 0 or 1, 1 <= field groups <= carried fields, xcd 0 or 1, 1 <= levels <= nz");
+  bt->tune.layout = shape[0]; bt->tune.groups = shape[0] == 0 ? shape[1] : 1; bt->tune.xcd = shape[2];
+  bt->tune.levels = shape[0] == 0 ? 1 : shape[3]; bt->tune.ms = 0;
   // The captured step holds the old launch shape.
   if (bt->exec) { hipGraphExecDestroy(bt->exec); bt->exec = nullptr; }
   if (bt->graph) { hipGraphDestroy(bt->graph); bt->graph = nullptr; }

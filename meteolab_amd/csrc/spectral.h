@@ -22,16 +22,14 @@ struct Plan { int rx[8], nrx, ry[8], nry; };
 
 constexpr int SPEC_THREADS = 256;
 
-// `count` sequences of length N in LDS, element k of sequence s at base(s) + k * kStride, base(s) = (s / per) * outer +
-// (s % per) * sStride (several levels' sequences at once: `per` sequences a level, levels `outer` apart);
-// tw[k] = exp(-2 pi i k/N) (sign -1, forward) or its conjugate (sign +1). Stockham stages ping-pong between a and b;
-// returns the buffer that holds the result. Ends with a barrier.
 // One Stockham pass of radix P (a compile-time radix, so a butterfly's values stay in registers).
 template <int P, class R> __device__ inline void fftPass(const cx<R> *src, cx<R> *dst, int N, int Ns, int count, int sStride, int kStride, const cx<R> *tw, int sign,
                                                          int per, int outer) {
   const int span = N / P, total = count * span, tstep = N / (Ns * P), wp = N / P;
+  const Div dSpan(span), dNs(Ns), dPer(per < total ? per : total);
   for (int t = threadIdx.x; t < total; t += blockDim.x) {
-    const int s = t / span, j = t - s * span, k = j % Ns, lvl = s / per, base = lvl * outer + (s - lvl * per) * sStride;
+    const int s = dSpan.q(t), j = t - s * span, jn = dNs.q(j), k = j - jn * Ns, lvl = per < total ? dPer.q(s) : 0;
+    const int base = lvl * outer + (s - lvl * per) * sStride;
     cx<R> v[P], o[P];
 #pragma unroll
     for (int r = 0; r < P; r++) {
@@ -63,13 +61,17 @@ template <int P, class R> __device__ inline void fftPass(const cx<R> *src, cx<R>
         o[q] = acc;
       }
     }
-    const int d = (j / Ns) * Ns * P + k;
+    const int d = jn * Ns * P + k;
 #pragma unroll
     for (int q = 0; q < P; q++) dst[base + (d + q * Ns) * kStride] = o[q];
   }
   __syncthreads();
 }
 
+// `count` sequences of length N in LDS, element k of sequence s at base(s) + k * kStride, base(s) = (s / per) * outer +
+// (s % per) * sStride (several levels' sequences at once: `per` sequences a level, levels `outer` apart);
+// tw[k] = exp(-2 pi i k/N) (sign -1, forward) or its conjugate (sign +1). Stockham stages ping-pong between a and b;
+// returns the buffer that holds the result. Ends with a barrier.
 template <class R> __device__ cx<R> *fftStages(cx<R> *a, cx<R> *b, int N, int count, int sStride, int kStride, const cx<R> *tw, const int *rad, int nrad, int sign,
                                                int per = 1 << 30, int outer = 0) {
   cx<R> *src = a, *dst = b;

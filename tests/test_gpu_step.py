@@ -149,23 +149,28 @@ def mature_small():
     return mature(C.small_grid(), 650)
 
 
-TRANSPORT_SHAPES = [("strip", 1, False), ("strip", 4, True), ("strip", 17, True), ("level", 1, False), ("level", 1, True)]
+MARCH_OFF = pytest.mark.skip(reason="the column-march transport layout was taken out after a whole-step run with it faulted "
+                                   "the GPU, cause not found on the CPU; set_batch_tuning rejects it")
+TRANSPORT_SHAPES = [("strip", 1, False, 1), ("strip", 4, True, 1), ("strip", 17, True, 1), ("level", 1, False, 1),
+                    ("level", 1, True, 1), ("level", 1, True, 5), ("level", 1, False, 10),
+                    pytest.param("march", 1, True, 5, marks=MARCH_OFF), pytest.param("march", 1, False, 24, marks=MARCH_OFF),
+                    pytest.param("march", 1, True, 10, marks=MARCH_OFF)]
 
 
-@pytest.mark.parametrize("layout,groups,xcd", TRANSPORT_SHAPES)
-def test_every_transport_layout_keeps_parity(ctx, mature_small, layout, groups, xcd):
+@pytest.mark.parametrize("layout,groups,xcd,levels", TRANSPORT_SHAPES)
+def test_every_transport_layout_keeps_parity(ctx, mature_small, layout, groups, xcd, levels):
     # Each launch shape the autotuner may pick: one step of the mature storm against the oracle, and eight different
     # runs batched in that shape (the XCD block order needs a multiple of eight) against each run stepped alone.
     g = C.small_grid()
     ref = oracle_restart(mature_small, g)
     r = restart(ctx, mature_small, g)
-    M.set_batch_tuning([r], layout, groups, xcd)
+    M.set_batch_tuning([r], layout, groups, xcd, levels)
     ref.step(1); r.step(1)
     compare(r, ref)
     specs = [C.run(g, seed=s, amp=1.5 + .2 * s, offset=(500 * s - 2000, 300 * s - 1000)) for s in range(8)]
     pool = ctx.pool(g, 8)
     mixed = [pool.run(C.profile(), s) for s in specs]
-    M.set_batch_tuning(mixed, layout, groups, xcd)
+    M.set_batch_tuning(mixed, layout, groups, xcd, levels)
     assert M.batch_tuning(mixed)["transport_layout"] == layout
     M.step(mixed, 10)
     ctx.sync()
