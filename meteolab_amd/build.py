@@ -37,5 +37,26 @@ def build(verbose=False):
     return LIB
 
 
+NODE_SRC = os.path.join(ROOT, "node", "meteolab_node.cc")
+NODE_ADDON = os.path.join(ROOT, "node", "meteolab.node")
+NODE_INCLUDE = os.environ.get("NODE_INCLUDE", "/usr/include/node")
+
+
+def build_node():
+    """Compiles the N-API addon of the C ABI (node/meteolab.node) against libmeteolab.so with g++, when Node's
+    headers are installed; returns its path, or None without them."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        return None
+    cmd = ["g++", "-std=c++17", "-O2", "-shared", "-fPIC", "-Wall", "-DNODE_GYP_MODULE_NAME=meteolab", "-I", NODE_INCLUDE,
+           "-o", NODE_ADDON + ".tmp", NODE_SRC, "-L", HERE, "-lmeteolab", "-Wl,-rpath,$ORIGIN/../meteolab_amd"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        print(res.stdout + res.stderr, file=sys.stderr)
+        raise subprocess.CalledProcessError(res.returncode, cmd, output=res.stdout + res.stderr)
+    os.replace(NODE_ADDON + ".tmp", NODE_ADDON)
+    return NODE_ADDON
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_node())

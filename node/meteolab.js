@@ -1,0 +1,125 @@
+'use strict'
+// Node host of the MI355X StormLab step: the classes a caller of AtmosphereModel.step (src/core/model.ts:873-877) or of
+// the GPU plugin GpuDynamics (src/gpu/dynamics.ts:250-504) uses, over the N-API addon of include/meteolab.h.
+//
+//   const { MeteolabDevice, STANDARD, profile, run } = require('./meteolab')
+//   const dev = MeteolabDevice.create()            // GpuDynamics.create: a device and its stream
+//   const pool = dev.pool(STANDARD, 64)            // slots for 64 standard runs, resident in HBM
+//   const r = pool.run(profile(), run(STANDARD))   // new AtmosphereModel(...) for the step's state
+//   pool.step([r], 20, 3)                          // 20 x (step(dt); time += dt), one launch per kernel for the batch
+//   const w = r.field('w')                         // Float32Array, node i = x + nx (y + ny z)
+//
+// Errors of the device are thrown as Error and kept in `errors` (GpuDynamics.errors, gpu/dynamics.ts:252).
+const path = require('path')
+const addon = require(path.join(__dirname, 'meteolab.node'))
+
+// The carried order of AtmosphereModel.carried() (model.ts:1319-1335), then the arrays a step recomputes; the ground
+// columns of STATE_COLUMNS (src/core/state.ts:10).
+const FIELDS = ['rain', 'snow', 'graupel', 'hail', 'u', 'v', 'w', 'theta', 'q', 'cloud', 'ice', 'iceNumber', 'cold',
+  'chargeIce', 'chargeGraupel', 'chargeRain', 'chargeAir', 'fieldZ', 'fallRain', 'fallSnow', 'fallGraupel', 'fallHail',
+  'fallChargeIce', 'fallChargeGraupel', 'condensate']
+const COLUMNS = ['precipitation', 'hailGround', 'hailSize']
+const DIAG = ['time', 'maxW', 'minW', 'maxThetaP', 'rainTotal', 'water', 'charge', 'clipped', 'negativeFilled',
+  'chargeGround', 'flashesIC', 'flashesCG', 'firstFlash', 'error']
+const LEVELS = ['p', 'exner', 'theta', 'q', 'u', 'v', 'rho']
+
+// src/core/grid.ts:31 DEFAULT_GRID and :34 LARGE_GRID.
+const STANDARD = { nx: 40, ny: 32, nz: 50, width: 48000, depth: 36000, height: 18900, bottom: 100 }
+const LARGE = { nx: 80, ny: 64, nz: 50, width: 96000, depth: 72000, height: 18900, bottom: 100 }
+
+// The Weisman-Klemp sounding (src/core/profiles.ts:61-73) with the wind u = shear tanh(z / shear depth).
+function profile (qvMax = 0.014, shear = 25, shearDepth = 3000) {
+  return { theta0: 300, dtheta: 43, ztr: 12000, ttr: 213, rh_drop: 0.75, rh_top: 0.25, qv_max: qvMax, shear, shear_depth: shearDepth }
+}
+
+// A run: the cos^2 thermal of model.ts:94 (ELLIPSOID) of `amp` K, a third of the way in from the west, shifted by
+// `offset` m, and +-0.1 K theta noise below 1 km from mulberry32(seed).
+function run (grid, seed = 1, amp = 2, offset = [0, 0]) {
+  return { seed, latitude: 45, turbulence: 0.55, tropopause_km: 12, roughness: 0.03, frame_u: 0, frame_v: 0, bubble_amp: amp,
+    bubble_x: grid.width / 3 + offset[0], bubble_y: grid.depth / 2 + offset[1], bubble_z: 1400, bubble_rh: 10000, bubble_rz: 1400,
+    noise_seed: seed, noise_amp: 0.1, noise_top: 1000 }
+}
+
+class MeteolabDevice {
+  constructor (ctx) { this.ctx = ctx; this.errors = []; this.adapter = addon.deviceName(ctx) }
+  /** A context on HIP device `device`; precision 32 (production kernels) or 64 (the same in float64). */
+  static create (device = 0, precision = 32) { return new MeteolabDevice(addon.ctxCreate(device, precision)) }
+  /** Slots for `capacity` runs of one grid; electrified runs carry the charges and the lightning. */
+  pool (grid, capacity, electrified = true) { return new MeteolabPool(this, grid, capacity, electrified) }
+  /** Device bytes one run of this grid takes (admission by HBM footprint). */
+  slotBytes (grid) { return addon.slotBytes(this.ctx, grid) }
+  sync () { this.call(() => addon.sync(this.ctx)) }
+  call (f) {
+    try { return f() } catch (e) { this.errors.push(e.message); throw e }
+  }
+  destroy () { if (this.ctx) { addon.ctxDestroy(this.ctx); this.ctx = null } }
+}
+
+class MeteolabPool {
+  constructor (device, grid, capacity, electrified) {
+    this.device = device; this.grid = { ...grid }
+    this.n = grid.nx * grid.ny * grid.nz; this.layer = grid.nx * grid.ny
+    this.id = device.call(() => addon.poolCreate(device.ctx, grid, capacity, electrified ? 1 : 0))
+  }
+  run (prof, config) { return new MeteolabRun(this, prof, config) }
+  /** Advances the runs (all of this pool) by `steps` steps of dt s; asynchronous, sync() waits. */
+  step (runs, steps = 1, dt = 3) {
+    const ids = Int32Array.from(runs.map(r => r.handle))
+    this.device.call(() => addon.step(this.device.ctx, ids, steps, dt))
+  }
+  destroy () { if (this.id !== null) { this.device.call(() => addon.poolDestroy(this.device.ctx, this.id)); this.id = null } }
+}
+
+class MeteolabRun {
+  constructor (pool, prof, config) {
+    this.pool = pool; this.device = pool.device
+    this.handle = this.device.call(() => addon.runCreate(this.device.ctx, pool.id, prof, config))
+  }
+  /** A field (or fieldZ, a fall speed, the condensate) as a new Float32Array. */
+  field (name) {
+    const out = new Float32Array(this.pool.n)
+    this.device.call(() => addon.download(this.device.ctx, this.handle, FIELDS.indexOf(name), out))
+    return out
+  }
+  setField (name, values) {
+    const a = Float32Array.from(values)
+    if (a.length !== this.pool.n) throw new Error(`${name}: ${a.length} values for ${this.pool.n} nodes`)
+    this.device.call(() => addon.upload(this.device.ctx, this.handle, FIELDS.indexOf(name), a))
+  }
+  column (name) {
+    const out = new Float32Array(this.pool.layer)
+    this.device.call(() => addon.downloadColumn(this.device.ctx, this.handle, COLUMNS.indexOf(name), out))
+    return out
+  }
+  setColumn (name, values) {
+    const a = Float32Array.from(values)
+    if (a.length !== this.pool.layer) throw new Error(`${name}: ${a.length} values for ${this.pool.layer} columns`)
+    this.device.call(() => addon.uploadColumn(this.device.ctx, this.handle, COLUMNS.indexOf(name), a))
+  }
+  /** model.ts:1672-1696 diagnostics and the run metrics (max w, max theta', rain total, flashes, first flash). */
+  diagnostics () {
+    const out = new Float64Array(DIAG.length)
+    this.device.call(() => addon.diag(this.device.ctx, this.handle, out))
+    return Object.fromEntries(DIAG.map((k, i) => [k, out[i]]))
+  }
+  /** The base state per level (environment.ts:124-130). */
+  levels () {
+    const nz = this.pool.grid.nz, out = new Float64Array(LEVELS.length * nz)
+    this.device.call(() => addon.levels(this.device.ctx, this.handle, out))
+    return Object.fromEntries(LEVELS.map((k, i) => [k, out.subarray(i * nz, (i + 1) * nz)]))
+  }
+  /** The most recent flashes (lightning.ts Flash: time, kind, start, nodes, charge, potential, moment). */
+  flashes (max = 64) {
+    const out = new Float64Array(8 * max), n = this.device.call(() => addon.flashes(this.device.ctx, this.handle, out))
+    const list = []
+    for (let k = 0; k < n; k++) {
+      const e = out.subarray(8 * k, 8 * k + 7)
+      list.push({ time: e[0], kind: e[1] ? 'CG' : 'IC', start: e[2], nodes: e[3], charge: e[4], potential: e[5], moment: e[6] })
+    }
+    return list
+  }
+  set time (t) { this.device.call(() => addon.setTime(this.device.ctx, this.handle, t)) }
+  release () { if (this.handle !== null) { this.device.call(() => addon.runRelease(this.device.ctx, this.handle)); this.handle = null } }
+}
+
+module.exports = { MeteolabDevice, MeteolabPool, MeteolabRun, FIELDS, COLUMNS, DIAG, STANDARD, LARGE, profile, run, addon }

@@ -1,0 +1,71 @@
+"""The boundary as a Node caller sees it: the N-API addon of the C ABI (node/meteolab_node.cc) and the Node host
+(node/meteolab.js), which replace the reference's GPU plugin and step calls on its Node worker
+(src/simulation.worker.ts:104-112, src/gpu/dynamics.ts:250-504)."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import meteolab_amd as M
+from meteolab_amd import config as C
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADDON = os.path.join(ROOT, "node", "meteolab.node")
+NODE = shutil.which("node")
+needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reason="node or the built addon is absent")
+ABI_FUNCTIONS = ["ctxCreate", "ctxDestroy", "deviceName", "poolCreate", "poolDestroy", "slotBytes", "runCreate", "runRelease",
+                 "upload", "download", "uploadColumn", "downloadColumn", "step", "sync", "diag", "levels", "flashes", "setTime"]
+
+
+def node(code):
+    return subprocess.run([NODE, "-e", code], capture_output=True, text=True, cwd=ROOT, timeout=120)
+
+
+@needs_node
+def test_addon_loads_and_exports_the_abi():
+    res = node("console.log(JSON.stringify(Object.keys(require('./node/meteolab.node'))))")
+    assert res.returncode == 0, res.stderr
+    assert json.loads(res.stdout) == ABI_FUNCTIONS
+
+
+@needs_node
+def test_host_classes_and_field_order_match_the_abi():
+    res = node("const m = require('./node/meteolab'); console.log(JSON.stringify({fields: m.FIELDS, diag: m.DIAG, "
+               "columns: m.COLUMNS, std: m.STANDARD}))")
+    assert res.returncode == 0, res.stderr
+    d = json.loads(res.stdout)
+    assert d["fields"] == M.FIELDS and d["diag"] == M.DIAG and d["columns"] == M.COLUMNS
+    assert d["std"] == C.STANDARD
+
+
+@needs_node
+def test_node_host_fails_loudly_without_a_device_or_runs_with_one():
+    # Without a GPU the device cannot be created and the host throws; there is no CPU path behind it.
+    res = node("const m = require('./node/meteolab'); try { const d = m.MeteolabDevice.create(); console.log('device', d.adapter) }"
+               " catch (e) { console.log('error', e.message) }")
+    assert res.returncode == 0, res.stderr
+    assert res.stdout.startswith("device") or "no usable HIP device" in res.stdout
+
+
+@pytest.mark.gpu
+@needs_node
+def test_node_host_step_matches_the_oracle(tmp_path):
+    from oracle.binding import OracleRun
+    g = C.small_grid()
+    out = tmp_path / "fields.bin"
+    res = subprocess.run([NODE, os.path.join(ROOT, "node", "step_fields.js"), json.dumps(g), "10", str(out)],
+                         capture_output=True, text=True, cwd=ROOT, timeout=300)
+    assert res.returncode == 0, res.stderr
+    info = json.loads(res.stdout.strip().splitlines()[-1])
+    assert info["errors"] == [] and info["diagnostics"]["time"] == 30.0
+    fields = np.fromfile(out, np.float32).reshape(len(M.CARRIED), -1)
+    o = OracleRun(g, C.profile(), C.run(g))
+    o.step(10)
+    for k, name in enumerate(M.CARRIED):
+        b = o.field(name).astype(np.float64)
+        nb = np.linalg.norm(b)
+        err = np.linalg.norm(fields[k] - b) / nb if nb > 0 else np.linalg.norm(fields[k])
+        assert err <= 1e-5, (name, err)
