@@ -56,7 +56,9 @@ struct Pool {
   double *diag = nullptr; // [cap][N_DIAG_DEV]
   std::vector<double> zsH, dzsH, hzH;
   std::vector<char> used;
-  std::map<std::vector<int>, Batch *> batches;
+  // Captured steps by batch size: the graph of a size is reused for any runs of the pool, its device run list rewritten
+  // when the runs change (the continuous-batching case: runs join and leave between calls).
+  std::map<int, Batch *> batches;
 };
 
 }  // namespace
@@ -272,11 +274,9 @@ void freeBatch(Batch *bt) {
   if (bt->dev) hipFree(bt->dev);
   delete bt;
 }
-void dropBatches(Pool *p, int slot) {
-  for (auto it = p->batches.begin(); it != p->batches.end();) {
-    if (slot < 0 || std::find(it->first.begin(), it->first.end(), slot) != it->first.end()) { freeBatch(it->second); it = p->batches.erase(it); }
-    else ++it;
-  }
+void dropBatches(Pool *p) {
+  for (auto &kv : p->batches) freeBatch(kv.second);
+  p->batches.clear();
 }
 
 }  // namespace
@@ -300,7 +300,7 @@ int ml_ctx_create(int device, int precision, ml_ctx **out) {
 namespace {
 // Frees a pool's device memory and its captured batches (the stream must be idle).
 void freePool(Pool *p) {
-  dropBatches(p, -1);
+  dropBatches(p);
   void *all[] = {p->fields, p->stx, p->sty, p->mass, p->phi, p->cells, p->press, p->spec, p->mean0, p->km, p->kh, p->ephi, p->cols,
                  p->lev, p->par, p->scal, p->zs, p->dzs, p->hz, p->sym, p->twx, p->twy, p->single, p->diag, p->lrho, p->lmark,
                  p->lnode, p->chan, p->cdep, p->flog, p->efac};
@@ -465,7 +465,6 @@ int ml_run_release(ml_ctx *c, int run) {
   Pool *p; int slot;
   if (!c || !decode(c, run, p, slot)) return fail(c, ML_E_ARG, "bad run");
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  dropBatches(p, slot);
   p->used[slot] = 0;
   return 0;
 }
@@ -620,13 +619,25 @@ static int batchOf(ml_ctx *c, const int *runs, int count, Pool *&p, Batch *&bt) 
     if (p && q != p) return fail(c, ML_E_ARG, "step: runs of one pool per call");
     p = q; slots[k] = slot;
   }
-  auto it = p->batches.find(slots);
-  if (it != p->batches.end()) { bt = it->second; return 0; }
+  std::vector<int> sorted(slots);
+  std::sort(sorted.begin(), sorted.end());
+  if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) return fail(c, ML_E_ARG, "step: a run twice in one batch");
+  auto it = p->batches.find(count);
+  if (it != p->batches.end()) {
+    bt = it->second;
+    if (bt->runs != slots) {
+      // The same launch shape for other runs: the new run list goes ahead of the next replay in the stream.
+      HIPCHK(c, hipSetDevice(c->device));
+      HIPCHK(c, hipMemcpyAsync(bt->dev, slots.data(), count * sizeof(int), hipMemcpyHostToDevice, c->stream));
+      bt->runs = slots;
+    }
+    return 0;
+  }
   bt = new Batch();
   bt->runs = slots;
   HIPCHK(c, hipMalloc((void **)&bt->dev, count * sizeof(int)));
   HIPCHK(c, hipMemcpy(bt->dev, slots.data(), count * sizeof(int), hipMemcpyHostToDevice));
-  p->batches[slots] = bt;
+  p->batches[count] = bt;
   return tune(c, p, bt);
 }
 

@@ -1,0 +1,46 @@
+"""The continuous-batching serving loop (meteolab_amd/serve.py): admission into free slots, rounds of one model
+minute, release; every request ends where it would alone."""
+import numpy as np
+import pytest
+
+import meteolab_amd as M
+from meteolab_amd import config as C
+from meteolab_amd import serve as S
+
+
+def test_workload_is_seeded_and_shaped():
+    a, b = S.workload(200, rate=5.0, seed=3), S.workload(200, rate=5.0, seed=3)
+    assert [(r.size, r.minutes, r.arrival) for r in a] == [(r.size, r.minutes, r.arrival) for r in b]
+    arrivals = np.array([r.arrival for r in a])
+    assert np.all(np.diff(arrivals) > 0)
+    assert 0.15 < np.mean([r.size == "large" for r in a]) < 0.45
+    assert min(r.minutes for r in a) >= 10 and max(r.minutes for r in a) <= 120
+    gaps = np.diff(np.concatenate([[0], arrivals]))
+    assert gaps.mean() == pytest.approx(1 / 5.0, rel=0.25)
+
+
+@pytest.mark.gpu
+def test_requests_served_together_end_as_alone():
+    # Six small-grid requests of 1-3 model minutes arriving together into 3 slots of one class: runs join and leave
+    # between rounds, so the captured step of each batch size is replayed over changing runs.
+    g = C.small_grid()
+    grids = {"small": g}
+    reqs = [S.Request(k, "small", 1 + k % 3, 0.0, C.profile(qv_max=0.012 + 0.0005 * k), C.run(g, seed=k + 1, amp=2.0 + 0.1 * k))
+            for k in range(6)]
+    with M.Context(0, 32) as ctx:
+        finals = {}
+        server = S.Server(ctx, {"small": 3}, grids=grids,
+                          on_finish=lambda r: finals.setdefault(r.id, {name: r.handle.field(name) for name in M.CARRIED}))
+        done, clock = server.serve(reqs)
+        assert sorted(r.id for r in done) == list(range(6))
+        assert all(len(r.minute_ms) == r.minutes for r in done)
+        out = S.summary(done, clock, grids)
+        assert out["latency_per_model_minute_ms"]["p50"] > 0
+        for r in reqs:
+            alone = ctx.pool(g, 1).run(r.profile, r.run)
+            alone.step(r.minutes * S.MINUTE_STEPS)
+            for name in M.CARRIED:
+                a, b = finals[r.id][name].astype(np.float64), alone.field(name).astype(np.float64)
+                nb = np.linalg.norm(b)
+                err = np.linalg.norm(a - b) / nb if nb > 0 else np.linalg.norm(a)
+                assert err <= 1e-6, (r.id, name, err)
