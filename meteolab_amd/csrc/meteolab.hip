@@ -177,7 +177,7 @@ template <class R> void launchStep(const Pool &p, const int *list, int nr, doubl
   mark("k_faces");
   for (int stage = 0; stage < 3; stage++) {
     launchStage<R>(g, b, nr, nc, stage, dt, s, tu);
-    mark("k_stage");
+    mark(tu.layout == 1 ? "k_stage_level" : "k_stage");
     if (stage < 2) {
       hipLaunchKernelGGL(k_pdiv<R>, dim3(cb, nr), dim3(TPB), 0, s, g, b, stage + 1, dt);
       mark("k_pdiv");
@@ -595,12 +595,16 @@ static int tune(ml_ctx *c, Pool *p, Batch *bt) {
       Tuning tl; tl.layout = 0; tl.groups = gr; tl.xcd = xcd; cands.push_back(tl);
     }
   }
-  Tuning best; best.ms = 1e30f;
+  float fastest = 1e30f;
   for (Tuning &t : cands) {
     t.ms = p->rb == 8 ? timeStage<double>(c, *p, *bt, t) : timeStage<float>(c, *p, *bt, t);
-    if (t.ms < best.ms) best = t;
+    fastest = std::min(fastest, t.ms);
   }
   HIPCHK(c, hipGetLastError());
+  // The first candidate within 3 % of the fastest: shapes that time alike within the launches' noise resolve the same
+  // way on every run of the same batch.
+  Tuning best = cands[0];
+  for (const Tuning &t : cands) if (t.ms <= fastest * 1.03f) { best = t; break; }
   bt->tune = best;
   bt->tuned = true;
   return 0;

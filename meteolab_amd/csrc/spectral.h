@@ -47,6 +47,27 @@ template <int P, class R> __device__ inline void fftPass(const cx<R> *src, cx<R>
       cx<R> jd13 = sign < 0 ? cx<R>{d13.im, -d13.re} : cx<R>{-d13.im, d13.re};  // d13 times W_4
       o[0] = {s02.re + s13.re, s02.im + s13.im}; o[2] = {s02.re - s13.re, s02.im - s13.im};
       o[1] = {d02.re + jd13.re, d02.im + jd13.im}; o[3] = {d02.re - jd13.re, d02.im - jd13.im};
+    } else if constexpr (P == 3) {
+      // X1,2 = v0 - (v1 + v2)/2 -+ i sin(2 pi/3) (v1 - v2) forward (+- inverse).
+      const R h = R(0.86602540378443864676);
+      cx<R> t1 = {v[1].re + v[2].re, v[1].im + v[2].im}, t2 = {v[1].re - v[2].re, v[1].im - v[2].im};
+      cx<R> a = {v[0].re - t1.re / 2, v[0].im - t1.im / 2}, bq = {h * t2.re, h * t2.im};
+      cx<R> ib = sign < 0 ? cx<R>{bq.im, -bq.re} : cx<R>{-bq.im, bq.re};  // -i b forward, +i b inverse
+      o[0] = {v[0].re + t1.re, v[0].im + t1.im};
+      o[1] = {a.re + ib.re, a.im + ib.im}; o[2] = {a.re - ib.re, a.im - ib.im};
+    } else if constexpr (P == 5) {
+      // With t1 = v1 + v4, t2 = v2 + v3, t3 = v1 - v4, t4 = v2 - v3 and c_k = cos(2 pi k/5), s_k = sin(2 pi k/5):
+      // X1,4 = v0 + c1 t1 + c2 t2 -+ i (s1 t3 + s2 t4), X2,3 = v0 + c2 t1 + c1 t2 -+ i (s2 t3 - s1 t4) forward.
+      const R c1 = R(0.30901699437494742410), c2 = R(-0.80901699437494742410), s1 = R(0.95105651629515357212), s2 = R(0.58778525229247312917);
+      cx<R> t1 = {v[1].re + v[4].re, v[1].im + v[4].im}, t2 = {v[2].re + v[3].re, v[2].im + v[3].im};
+      cx<R> t3 = {v[1].re - v[4].re, v[1].im - v[4].im}, t4 = {v[2].re - v[3].re, v[2].im - v[3].im};
+      cx<R> a1 = {v[0].re + c1 * t1.re + c2 * t2.re, v[0].im + c1 * t1.im + c2 * t2.im};
+      cx<R> a2 = {v[0].re + c2 * t1.re + c1 * t2.re, v[0].im + c2 * t1.im + c1 * t2.im};
+      cx<R> b1 = {s1 * t3.re + s2 * t4.re, s1 * t3.im + s2 * t4.im}, b2 = {s2 * t3.re - s1 * t4.re, s2 * t3.im - s1 * t4.im};
+      cx<R> ib1 = sign < 0 ? cx<R>{b1.im, -b1.re} : cx<R>{-b1.im, b1.re}, ib2 = sign < 0 ? cx<R>{b2.im, -b2.re} : cx<R>{-b2.im, b2.re};
+      o[0] = {v[0].re + t1.re + t2.re, v[0].im + t1.im + t2.im};
+      o[1] = {a1.re + ib1.re, a1.im + ib1.im}; o[4] = {a1.re - ib1.re, a1.im - ib1.im};
+      o[2] = {a2.re + ib2.re, a2.im + ib2.im}; o[3] = {a2.re - ib2.re, a2.im - ib2.im};
     } else {
 #pragma unroll
       for (int q = 0; q < P; q++) {
