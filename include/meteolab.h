@@ -111,6 +111,27 @@ int ml_run_set_time(ml_ctx *ctx, int run, double time);
  * (model.ts:873-877 and its callers, e.g. tests/bench/threads.ts:38-42). One launch per kernel covers all the runs;
  * the launch sequence of a batch is captured once into a hipGraph and replayed. Asynchronous: ml_sync waits. */
 int ml_step(ml_ctx *ctx, const int *runs, int count, int steps, double dt);
+
+/* The parts of one step, split where the reference's GPU step splits it (src/gpu/step.ts:12-31), for a host that runs
+ * some parts itself. ml_dynamics, ml_physics, ml_mix and ml_after_step in that order are one ml_step (the same
+ * kernels, eagerly launched). Asynchronous; the runs' state stays on the device between the calls.
+ *   ml_dynamics:   beginStep's fall speeds, hail swath and sedimentation, the face projection and the three RK3 stages
+ *                  with buoyancy and the pressure in the stages: GpuDynamics.step(fields, env, dt, timed, falls)
+ *                  (gpu/dynamics.ts:419); the ground deposits go into the run's precipitation column.
+ *   ml_physics:    physicsStep (model.ts:1116-1127): the positivity fix with the sums of the last ml_dynamics, then
+ *                  the column physics. A host that changed the fields after ml_dynamics runs its own physicsStep.
+ *   ml_mix:        mixStep, the projection and clipWind: GpuDynamics.mix(fields, mixingInputs, dt) (:356) and
+ *                  endGpuStep (model.ts:1145-1150); the condensate is the run's `condensate` field.
+ *   ml_after_step: afterStep's lightning (model.ts:1130-1133), then time += dt. */
+int ml_dynamics(ml_ctx *ctx, const int *runs, int count, double dt);
+int ml_physics(ml_ctx *ctx, const int *runs, int count, double dt);
+int ml_mix(ml_ctx *ctx, const int *runs, int count, double dt);
+int ml_after_step(ml_ctx *ctx, const int *runs, int count, double dt);
+/* The sub-steps of each fall of the run's last ml_dynamics (or step): fallInputs(dt)'s `parts` (model.ts:992-998,
+ * fallParts :575-580), in the order rain, snow, graupel, hail, then the charges of ice, graupel and rain when the
+ * run is electrified; the fall speeds are the fields fallRain .. fallChargeGraupel. Returns the number of falls
+ * (stream-synchronising); `max` below it is ML_E_ARG. */
+int ml_fall_inputs(ml_ctx *ctx, int run, int *parts, int max);
 int ml_sync(ml_ctx *ctx);
 /* The autotuned launch shape of a batch's transport kernel (chosen by timing on the device when the batch is first
  * stepped), shape[4]: shape[0] the layout (0 strips of a level with shape[1] field groups per block, 1 shape[3] whole

@@ -75,6 +75,11 @@ SIGNATURES = {
     "ml_run_set_time": (_i, [_vp, _i, _d]),
     "ml_run_flashes": (_i, [_vp, _i, _dp, _i]),
     "ml_step": (_i, [_vp, _ip, _i, _i, _d]),
+    "ml_dynamics": (_i, [_vp, _ip, _i, _d]),
+    "ml_physics": (_i, [_vp, _ip, _i, _d]),
+    "ml_mix": (_i, [_vp, _ip, _i, _d]),
+    "ml_after_step": (_i, [_vp, _ip, _i, _d]),
+    "ml_fall_inputs": (_i, [_vp, _i, _ip, _i]),
     "ml_sync": (_i, [_vp]),
     "ml_step_timed": (_i, [_vp, _ip, _i, _i, _d, _dp]),
     "ml_copy_probe": (_i, [_vp, _i64, _i, _dp]),
@@ -251,6 +256,13 @@ class Run:
         keys = ["time", "cg", "start", "nodes", "charge", "potential", "moment"]
         return [dict(zip(keys, out[8 * j:8 * j + 7])) for j in range(k)]
 
+    def fall_parts(self):
+        """The sub-steps of each fall of the last step or ml_dynamics (ml_fall_inputs): rain, snow, graupel, hail,
+        then the charges of ice, graupel and rain."""
+        out = (ctypes.c_int * 8)()
+        k = self.ctx.check(lib().ml_fall_inputs(self.ctx.h, self.h, out, 8))
+        return list(out[:k])
+
     def set_time(self, t):
         self.ctx.check(lib().ml_run_set_time(self.ctx.h, self.h, float(t)))
 
@@ -277,6 +289,20 @@ def step(runs, n=1, dt=3.0):
     ctx = runs[0].ctx
     ids = (ctypes.c_int * len(runs))(*[r.h for r in runs])
     ctx.check(lib().ml_step(ctx.h, ids, len(runs), int(n), float(dt)))
+
+
+# The parts of a step (ml_dynamics, ml_physics, ml_mix, ml_after_step), in the order a step runs them.
+STEP_PARTS = ["dynamics", "physics", "mix", "after_step"]
+
+
+def step_part(runs, part, dt=3.0):
+    """One part of a step of the runs (one pool), named as in STEP_PARTS; asynchronous. The four parts in order are
+    one step: the split of the reference's GPU step (src/gpu/step.ts:12-31), for a host that runs some parts itself."""
+    if part not in STEP_PARTS:
+        raise ValueError(f"step part {part!r}: one of {STEP_PARTS}")
+    ctx = runs[0].ctx
+    ids = (ctypes.c_int * len(runs))(*[r.h for r in runs])
+    ctx.check(getattr(lib(), "ml_" + part)(ctx.h, ids, len(runs), float(dt)))
 
 
 def step_timed(runs, n, dt=3.0):

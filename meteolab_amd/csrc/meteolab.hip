@@ -153,51 +153,64 @@ template <class R> void launchStage(const Geo &g, const Bufs<R> &b, int nr, int 
   }
 }
 
-// One model step of the runs in `list` (device array of nr slots): AtmosphereModel.step (model.ts:873-877). With a
-// timer, an event follows every kernel (named by the kernel).
-template <class R> void launchStep(const Pool &p, const int *list, int nr, double dt, hipStream_t s, Timer *t, const Tuning &tu) {
+// The parts of a model step, split where the reference's GPU step splits it (gpu/step.ts:12-31):
+//   PART_DYNAMICS  beginStep's fall speeds, hail swath and sedimentation, then the transport (GpuDynamics.step);
+//   PART_PHYSICS   physicsStep: the positivity fix from the last transport's sums, the column physics;
+//   PART_MIX       mixStep and the projection with the wind clipping (GpuDynamics.mix, endGpuStep);
+//   PART_AFTER     afterStep's lightning, then time += dt.
+enum StepPart { PART_DYNAMICS, PART_PHYSICS, PART_MIX, PART_AFTER, N_PARTS };
+
+// One part of a model step of the runs in `list` (device array of nr slots). With a timer, an event follows every
+// kernel (named by the kernel).
+template <class R> void launchPart(const Pool &p, const int *list, int nr, double dt, hipStream_t s, Timer *t, const Tuning &tu, int part) {
   const Geo &g = p.g;
   Bufs<R> b = bufs<R>(p, list);
   const int el = p.electrified, nc = el ? N_CARRIED : F_COLD + 1;
   const unsigned nb = blocks(g.n, TPB), cb = blocks(size_t(g.layer) * (g.nz - 1), TPB);
   auto mark = [&](const char *n) { if (t) t->mark(n); };
-  mark("start");
-  // beginStep: fall speeds, hail swath, sedimentation (model.ts:893-902).
-  hipLaunchKernelGGL(k_begin<R>, dim3(nr), dim3(64), 0, s, g, b);
-  mark("k_begin");
-  hipLaunchKernelGGL(k_fall<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt, el);
-  mark("k_fall");
-  hipLaunchKernelGGL(k_sediment<R>, dim3(blocks(g.layer, SED_COLS), nr), dim3(SED_COLS), size_t(2) * SED_COLS * g.nz * sizeof(float), s, g, b, dt, el);
-  mark("k_sediment");
-  // dynamics: face projection (FluxTransport.setVelocity) and the three RK3 stages with the stage projections.
-  hipLaunchKernelGGL(k_faces<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 0);
-  mark("k_faces");
-  spectralSolve<R>(p, b, nr, s, 0, g.nz, 0, t);
-  hipLaunchKernelGGL(k_faces<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 1);
-  mark("k_faces");
-  for (int stage = 0; stage < 3; stage++) {
-    launchStage<R>(g, b, nr, nc, stage, dt, s, tu);
-    mark(tu.layout == 1 ? "k_stage_level" : "k_stage");
-    if (stage < 2) {
-      hipLaunchKernelGGL(k_pdiv<R>, dim3(cb, nr), dim3(TPB), 0, s, g, b, stage + 1, dt);
-      mark("k_pdiv");
-      spectralSolve<R>(p, b, nr, s, 1, g.nz - 1, 1, t);
-      hipLaunchKernelGGL(k_pcorrect<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, stage + 1, dt, 0);
-      mark("k_pcorrect");
+  if (part == PART_DYNAMICS) {
+    // beginStep: fall speeds, hail swath, sedimentation (model.ts:893-902).
+    hipLaunchKernelGGL(k_begin<R>, dim3(nr), dim3(64), 0, s, g, b);
+    mark("k_begin");
+    hipLaunchKernelGGL(k_fall<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt, el);
+    mark("k_fall");
+    hipLaunchKernelGGL(k_sediment<R>, dim3(blocks(g.layer, SED_COLS), nr), dim3(SED_COLS), size_t(2) * SED_COLS * g.nz * sizeof(float), s, g, b, dt, el);
+    mark("k_sediment");
+    // dynamics: face projection (FluxTransport.setVelocity) and the three RK3 stages with the stage projections.
+    hipLaunchKernelGGL(k_faces<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 0);
+    mark("k_faces");
+    spectralSolve<R>(p, b, nr, s, 0, g.nz, 0, t);
+    hipLaunchKernelGGL(k_faces<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 1);
+    mark("k_faces");
+    for (int stage = 0; stage < 3; stage++) {
+      launchStage<R>(g, b, nr, nc, stage, dt, s, tu);
+      mark(tu.layout == 1 ? "k_stage_level" : "k_stage");
+      if (stage < 2) {
+        hipLaunchKernelGGL(k_pdiv<R>, dim3(cb, nr), dim3(TPB), 0, s, g, b, stage + 1, dt);
+        mark("k_pdiv");
+        spectralSolve<R>(p, b, nr, s, 1, g.nz - 1, 1, t);
+        hipLaunchKernelGGL(k_pcorrect<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, stage + 1, dt, 0);
+        mark("k_pcorrect");
+      }
     }
   }
   // endStep: positivity fix + physics (physicsStep), turbulence + drag (mixStep), projection (model.ts:1103-1113).
-  hipLaunchKernelGGL((k_physics<R, double>), dim3(nb, nr), dim3(TPB), 0, s, g, b, dt, el);
-  mark("k_physics");
-  hipLaunchKernelGGL(k_viscosity<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt);
-  mark("k_viscosity");
-  hipLaunchKernelGGL(k_mix<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt);
-  mark("k_mix");
-  hipLaunchKernelGGL(k_pdiv<R>, dim3(cb, nr), dim3(TPB), 0, s, g, b, 1, dt);
-  mark("k_pdiv");
-  spectralSolve<R>(p, b, nr, s, 1, g.nz - 1, 1, t);
-  hipLaunchKernelGGL(k_pcorrect<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 1, dt, 1);
-  mark("k_pcorrect");
+  if (part == PART_PHYSICS) {
+    hipLaunchKernelGGL((k_physics<R, double>), dim3(nb, nr), dim3(TPB), 0, s, g, b, dt, el);
+    mark("k_physics");
+  }
+  if (part == PART_MIX) {
+    hipLaunchKernelGGL(k_viscosity<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt);
+    mark("k_viscosity");
+    hipLaunchKernelGGL(k_mix<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt);
+    mark("k_mix");
+    hipLaunchKernelGGL(k_pdiv<R>, dim3(cb, nr), dim3(TPB), 0, s, g, b, 1, dt);
+    mark("k_pdiv");
+    spectralSolve<R>(p, b, nr, s, 1, g.nz - 1, 1, t);
+    hipLaunchKernelGGL(k_pcorrect<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 1, dt, 1);
+    mark("k_pcorrect");
+  }
+  if (part != PART_AFTER) return;
   // afterStep: lightning (model.ts:1537-1552).
   if (el) {
     unsigned ib = blocks(size_t(g.layer) * (g.nz - 2), 256);
@@ -215,6 +228,12 @@ template <class R> void launchStep(const Pool &p, const int *list, int nr, doubl
   }
   hipLaunchKernelGGL(k_end<R>, dim3(nr), dim3(64), 0, s, g, b, dt);
   mark("k_end");
+}
+
+// One model step: AtmosphereModel.step (model.ts:873-877), the parts in order.
+template <class R> void launchStep(const Pool &p, const int *list, int nr, double dt, hipStream_t s, Timer *t, const Tuning &tu) {
+  if (t) t->mark("start");
+  for (int part = 0; part < N_PARTS; part++) launchPart<R>(p, list, nr, dt, s, t, tu, part);
 }
 
 void fillTables(Pool &p, std::vector<char> &sym, std::vector<char> &twx, std::vector<char> &twy) {
@@ -663,6 +682,35 @@ int ml_step(ml_ctx *c, const int *runs, int count, int steps, double dt) {
   for (int k = 0; k < steps; k++) HIPCHK(c, hipGraphLaunch(bt->exec, c->stream));
   HIPCHK(c, hipGetLastError());
   return 0;
+}
+
+static int stepPart(ml_ctx *c, const int *runs, int count, double dt, int part) {
+  Pool *p; Batch *bt;
+  if (int r = batchOf(c, runs, count, p, bt)) return r;
+  if (!(dt > 0)) return fail(c, ML_E_ARG, "step part: dt > 0");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (p->rb == 8) launchPart<double>(*p, bt->dev, count, dt, c->stream, nullptr, bt->tune, part);
+  else launchPart<float>(*p, bt->dev, count, dt, c->stream, nullptr, bt->tune, part);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+int ml_dynamics(ml_ctx *c, const int *runs, int count, double dt) { return stepPart(c, runs, count, dt, PART_DYNAMICS); }
+int ml_physics(ml_ctx *c, const int *runs, int count, double dt) { return stepPart(c, runs, count, dt, PART_PHYSICS); }
+int ml_mix(ml_ctx *c, const int *runs, int count, double dt) { return stepPart(c, runs, count, dt, PART_MIX); }
+int ml_after_step(ml_ctx *c, const int *runs, int count, double dt) { return stepPart(c, runs, count, dt, PART_AFTER); }
+
+int ml_fall_inputs(ml_ctx *c, int run, int *parts, int max) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot) || (!parts && max > 0)) return fail(c, ML_E_ARG, "fall inputs: bad run");
+  const int falls = p->electrified ? N_FALLS : 4;
+  if (max < falls) return fail(c, ML_E_ARG, "fall inputs: room for " + std::to_string(falls) + " falls");
+  double sc[N_SCALARS];
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(sc, (double *)p->scal + size_t(slot) * N_SCALARS, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // fallParts (model.ts:575-580): sub-steps from the largest Courant number of the fall's non-zero nodes.
+  for (int k = 0; k < falls; k++) parts[k] = std::max(1, int(std::ceil(sc[S_COURANT + k] / .8)));
+  return falls;
 }
 
 int ml_batch_tuning(ml_ctx *c, const int *runs, int count, int *shape, float *ms) {

@@ -22,6 +22,9 @@ const COLUMNS = ['precipitation', 'hailGround', 'hailSize']
 const DIAG = ['time', 'maxW', 'minW', 'maxThetaP', 'rainTotal', 'water', 'charge', 'clipped', 'negativeFilled',
   'chargeGround', 'flashesIC', 'flashesCG', 'firstFlash', 'error']
 const LEVELS = ['p', 'exner', 'theta', 'q', 'u', 'v', 'rho']
+// The parts of a step where the reference's GPU step splits it (src/gpu/step.ts:12-31).
+const PARTS = ['dynamics', 'physics', 'mix', 'afterStep']
+const CARRIED = FIELDS.slice(0, 17)
 
 // src/core/grid.ts:31 DEFAULT_GRID and :34 LARGE_GRID.
 const STANDARD = { nx: 40, ny: 32, nz: 50, width: 48000, depth: 36000, height: 18900, bottom: 100 }
@@ -66,6 +69,12 @@ class MeteolabPool {
   step (runs, steps = 1, dt = 3) {
     const ids = Int32Array.from(runs.map(r => r.handle))
     this.device.call(() => addon.step(this.device.ctx, ids, steps, dt))
+  }
+  /** One part of a step of the runs: 'dynamics', 'physics', 'mix' or 'afterStep' (in that order they are a step). */
+  part (runs, name, dt = 3) {
+    if (!PARTS.includes(name)) throw new Error(`step part ${name}: one of ${PARTS.join(', ')}`)
+    const ids = Int32Array.from(runs.map(r => r.handle))
+    this.device.call(() => addon[name](this.device.ctx, ids, dt))
   }
   destroy () { if (this.id !== null) { this.device.call(() => addon.poolDestroy(this.device.ctx, this.id)); this.id = null } }
 }
@@ -118,8 +127,86 @@ class MeteolabRun {
     }
     return list
   }
+  /** fallInputs' sub-steps of each fall of the last step or dynamics part (rain, snow, graupel, hail, then charges). */
+  fallParts () {
+    const out = new Int32Array(8), k = this.device.call(() => addon.fallParts(this.device.ctx, this.handle, out))
+    return Array.from(out.subarray(0, k))
+  }
   set time (t) { this.device.call(() => addon.setTime(this.device.ctx, this.handle, t)) }
   release () { if (this.handle !== null) { this.device.call(() => addon.runRelease(this.device.ctx, this.handle)); this.handle = null } }
 }
 
-module.exports = { MeteolabDevice, MeteolabPool, MeteolabRun, FIELDS, COLUMNS, DIAG, STANDARD, LARGE, profile, run, addon }
+/**
+ * The reference's GPU plugin interface (GpuDynamics, src/gpu/dynamics.ts:250-504) over one resident run, so that
+ * gpuStep (src/gpu/step.ts:12-31) and the worker's fallback (simulation.worker.ts:104-112) drive this device unchanged:
+ * step() is the sedimentation and the transport (ml_dynamics), mix() the turbulence, friction and projection (ml_mix);
+ * the physics and afterStep stay with the model on the host between them. Errors reject the promise and are kept in
+ * `errors`, as the reference's are. stepN() is a whole step on the device (the state stays resident).
+ *
+ * The device computes the fall speeds and sub-steps of `falls` itself from the same fields (the model's fallInputs
+ * are the same numbers); the ground deposits of all falls that land in the rain gauge come back as the first such
+ * fall's array, the charge carried to the ground as the first charge fall's column 0 (landed() sums them the same).
+ * The baseline carried set (17 fields, AtmosphereModel.carried() order) is the only one supported.
+ */
+class GpuDynamics {
+  constructor (device, grid, env, prof, config) {
+    this.device = device; this.grid = { ...grid }; this.env = env
+    this.pool = device.pool(grid, 1); this.run = this.pool.run(prof, config)
+    this.errors = []; this.adapter = device.adapter
+  }
+  /** GpuDynamics.create(grid, env): a device and one run slot; `options` names the run's sounding and thermal. */
+  static async create (grid, env, options = {}) {
+    const device = MeteolabDevice.create(options.device || 0, options.precision || 32)
+    return new GpuDynamics(device, grid, env, options.profile || profile(), options.config || run(grid))
+  }
+  async step (fields, env, dt, timed = false, falls = []) {
+    try {
+      const t0 = Date.now(), r = this.run, layer = this.pool.layer
+      this.load(fields)
+      r.setColumn('precipitation', new Float32Array(layer))
+      const charged = r.diagnostics().chargeGround
+      this.pool.part([r], 'dynamics', dt)
+      const out = fields.map((_, k) => r.field(CARRIED[k]))
+      const rain = r.column('precipitation'), toGround = r.diagnostics().chargeGround - charged
+      const deposit = falls.findIndex(f => f.deposit !== false), charge = falls.findIndex(f => f.deposit === false)
+      const ground = falls.map((_, j) => {
+        if (j === deposit) return rain
+        const g = new Float32Array(layer)
+        if (j === charge) g[0] = toGround / (this.grid.dx * this.grid.dy)
+        return g
+      })
+      return { fields: out, ground, gpuMs: Date.now() - t0 }
+    } catch (e) { this.errors.push(String(e && e.message ? e.message : e)); throw e }
+  }
+  async mix (fields, mix, dt) {
+    try {
+      const t0 = Date.now(), r = this.run
+      for (const { k } of mix.mixed) r.setField(CARRIED[k], fields[k].q)
+      r.setField('condensate', mix.condensate)
+      this.pool.part([r], 'mix', dt)
+      // The projection's pressure stays on the device; endGpuStep takes null as "no pressure".
+      return { fields: mix.mixed.map(({ k }) => r.field(CARRIED[k])), pressure: null, gpuMs: Date.now() - t0 }
+    } catch (e) { this.errors.push(String(e && e.message ? e.message : e)); throw e }
+  }
+  /** Whole steps on the device from the resident state (or `fields` first): pool.step. */
+  stepN (n, dt, fields = null) {
+    if (fields) this.load(fields)
+    this.pool.step([this.run], n, dt)
+    this.device.sync()
+  }
+  /** Whether this instance fits a grid and base state: the same dimensions and spacing, and the device's base state
+   * equal to the model's within float64 rounding of the same formulas. */
+  matches (grid, env) {
+    const g = this.grid, lv = this.run.levels()
+    const close = (a, b) => a.length === b.length && Array.from(a).every((x, k) => Math.abs(x - b[k]) <= 1e-9 * Math.abs(b[k]))
+    return g.nx === grid.nx && g.ny === grid.ny && g.nz === grid.nz && g.width === grid.width && g.depth === grid.depth &&
+      close(lv.rho, env.rho) && close(lv.theta, env.theta) && close(lv.exner, env.exner)
+  }
+  destroy () { this.run.release(); this.pool.destroy(); this.device.destroy() }
+  load (fields) {
+    if (fields.length !== CARRIED.length) throw new Error(`GpuDynamics: ${fields.length} carried fields, the baseline set has ${CARRIED.length}`)
+    fields.forEach((f, k) => this.run.setField(CARRIED[k], f.q || f))
+  }
+}
+
+module.exports = { MeteolabDevice, MeteolabPool, MeteolabRun, GpuDynamics, FIELDS, CARRIED, COLUMNS, DIAG, PARTS, STANDARD, LARGE, profile, run, addon }

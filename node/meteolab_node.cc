@@ -181,6 +181,27 @@ napi_value Step(napi_env env, napi_callback_info info) {
   return status(env, ctx, ml_step(ctx, static_cast<const int *>(raw), int(len), steps, dt));
 }
 
+// dynamics / physics / mix / afterStep (ctx, Int32Array runs, dt): one part of a step of a batch (ml_dynamics,
+// ml_physics, ml_mix, ml_after_step; asynchronous).
+template <int (*F)(ml_ctx *, const int *, int, double)> napi_value Part(napi_env env, napi_callback_info info) {
+  napi_value a[3]; if (!args(env, info, 3, a)) return nullptr;
+  ml_ctx *ctx = ctxOf(env, a[0]); if (!ctx) return nullptr;
+  napi_typedarray_type t; size_t len; void *raw; napi_value buf; size_t off;
+  if (napi_get_typedarray_info(env, a[1], &t, &len, &raw, &buf, &off) != napi_ok || t != napi_int32_array) { napi_throw_type_error(env, nullptr, "runs: Int32Array"); return nullptr; }
+  double dt; if (!numberOf(env, a[2], dt)) return nullptr;
+  return status(env, ctx, F(ctx, static_cast<const int *>(raw), int(len), dt));
+}
+
+// fallParts(ctx, run, Int32Array(8)) -> number of falls written (ml_fall_inputs).
+napi_value FallParts(napi_env env, napi_callback_info info) {
+  napi_value a[3]; if (!args(env, info, 3, a)) return nullptr;
+  ml_ctx *ctx = ctxOf(env, a[0]); if (!ctx) return nullptr;
+  int32_t run; if (!intOf(env, a[1], run)) return nullptr;
+  napi_typedarray_type t; size_t len; void *raw; napi_value buf; size_t off;
+  if (napi_get_typedarray_info(env, a[2], &t, &len, &raw, &buf, &off) != napi_ok || t != napi_int32_array) { napi_throw_type_error(env, nullptr, "Int32Array"); return nullptr; }
+  return status(env, ctx, ml_fall_inputs(ctx, run, static_cast<int *>(raw), int(len)));
+}
+
 napi_value Sync(napi_env env, napi_callback_info info) {
   napi_value a[1]; if (!args(env, info, 1, a)) return nullptr;
   ml_ctx *ctx = ctxOf(env, a[0]); if (!ctx) return nullptr;
@@ -219,7 +240,8 @@ napi_value Init(napi_env env, napi_value exports) {
     {"poolDestroy", PoolDestroy}, {"slotBytes", SlotBytes}, {"runCreate", RunCreate}, {"runRelease", RunRelease},
     {"upload", Transfer<true, false>}, {"download", Transfer<false, false>}, {"uploadColumn", Transfer<true, true>},
     {"downloadColumn", Transfer<false, true>}, {"step", Step}, {"sync", Sync}, {"diag", Doubles<ml_run_diag>},
-    {"levels", Doubles<ml_run_levels>}, {"flashes", Flashes}, {"setTime", SetTime},
+    {"levels", Doubles<ml_run_levels>}, {"flashes", Flashes}, {"setTime", SetTime}, {"dynamics", Part<ml_dynamics>},
+    {"physics", Part<ml_physics>}, {"mix", Part<ml_mix>}, {"afterStep", Part<ml_after_step>}, {"fallParts", FallParts},
   };
   for (auto &f : fns) {
     napi_value v;

@@ -60,6 +60,8 @@ def lib():
         L.orc_create.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(Profile), ctypes.POINTER(Run)]
         L.orc_destroy.argtypes = [vp]
         L.orc_step.argtypes = [vp, d, i]
+        L.orc_step_part.argtypes = [vp, i, d]
+        L.orc_step_part.restype = i
         L.orc_set_lightning.argtypes = [vp, i]
         for name in ("orc_get_field", "orc_get_column"):
             getattr(L, name).argtypes = [vp, i, fp]
@@ -141,6 +143,11 @@ class OracleRun:
 
     def step(self, n=1, dt=3.0):
         lib().orc_step(self.h, dt, n)
+
+    def step_part(self, part, dt=3.0):
+        """One part of a step as the reference's GPU step splits it: 0 sedimentation + transport, 1 physics,
+        2 mixing + projection, 3 lightning and time += dt."""
+        assert lib().orc_step_part(self.h, int(part), dt) == 0
 
     def set_lightning(self, on):
         lib().orc_set_lightning(self.h, 1 if on else 0)

@@ -1293,6 +1293,20 @@ extern "C" {
 void *orc_create(const orc_grid *grid, const orc_profile *profile, const orc_run *run) { return new Model(*grid, *profile, *run); }
 void orc_destroy(void *m) { delete static_cast<Model *>(m); }
 void orc_step(void *m, double dt, int n) { Model *mm = static_cast<Model *>(m); for (int k = 0; k < n; k++) { mm->step(dt); mm->time += dt; } }
+// One part of a step, split where the reference's GPU step splits it (src/gpu/step.ts:12-31): 0 beginStep with the
+// sedimentation and the transport (GpuDynamics.step), 1 physicsStep (on the CPU there), 2 mixStep and the projection
+// with the wind clipping (GpuDynamics.mix, endGpuStep), 3 afterStep's lightning and time += dt. Parts 0-3 in order
+// are orc_step.
+int orc_step_part(void *m, int part, double dt) {
+  Model *mm = static_cast<Model *>(m);
+  switch (part) {
+    case 0: mm->fallPhase(); mm->collectHail(dt); mm->sedimentStep(dt); mm->dynamics(dt); return 0;
+    case 1: mm->physicsStep(dt); return 0;
+    case 2: mm->mixStep(dt); mm->project(dt, true); return 0;
+    case 3: if (mm->electrified) mm->discharge(dt); mm->time += dt; return 0;
+    default: return -1;
+  }
+}
 void orc_set_lightning(void *m, int on) { static_cast<Model *>(m)->electrified = on != 0; }
 int orc_get_field(void *m, int field, float *out) {
   Model *mm = static_cast<Model *>(m); if (field < 0 || field >= ORC_FIELDS) return -1;

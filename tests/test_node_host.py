@@ -17,7 +17,8 @@ ADDON = os.path.join(ROOT, "node", "meteolab.node")
 NODE = shutil.which("node")
 needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reason="node or the built addon is absent")
 ABI_FUNCTIONS = ["ctxCreate", "ctxDestroy", "deviceName", "poolCreate", "poolDestroy", "slotBytes", "runCreate", "runRelease",
-                 "upload", "download", "uploadColumn", "downloadColumn", "step", "sync", "diag", "levels", "flashes", "setTime"]
+                 "upload", "download", "uploadColumn", "downloadColumn", "step", "sync", "diag", "levels", "flashes", "setTime",
+                 "dynamics", "physics", "mix", "afterStep", "fallParts"]
 
 
 def node(code):
@@ -48,6 +49,11 @@ def test_node_host_fails_loudly_without_a_device_or_runs_with_one():
                " catch (e) { console.log('error', e.message) }")
     assert res.returncode == 0, res.stderr
     assert res.stdout.startswith("device") or "no usable HIP device" in res.stdout
+    # GpuDynamics.create rejects the same way, so the worker stays on its CPU step (simulation.worker.ts:63-66).
+    res = node("const m = require('./node/meteolab'); m.GpuDynamics.create(m.STANDARD, {}).then(g => { console.log('device', g.adapter);"
+               " g.destroy() }, e => console.log('rejected', e.message))")
+    assert res.returncode == 0, res.stderr
+    assert res.stdout.startswith("device") or "no usable HIP device" in res.stdout
 
 
 @pytest.mark.gpu
@@ -64,6 +70,31 @@ def test_node_host_step_matches_the_oracle(tmp_path):
     fields = np.fromfile(out, np.float32).reshape(len(M.CARRIED), -1)
     o = OracleRun(g, C.profile(), C.run(g))
     o.step(10)
+    for k, name in enumerate(M.CARRIED):
+        b = o.field(name).astype(np.float64)
+        nb = np.linalg.norm(b)
+        err = np.linalg.norm(fields[k] - b) / nb if nb > 0 else np.linalg.norm(fields[k])
+        assert err <= 1e-5, (name, err)
+
+
+@pytest.mark.gpu
+@needs_node
+def test_gpu_dynamics_interface_steps_as_the_oracle(tmp_path):
+    # gpuStep's calls (step, the physics between, mix, afterStep) through the GpuDynamics class for 3 steps, against
+    # the oracle's 3 steps; a wrong-size field set rejects and is kept in errors (the CPU-completion path).
+    from oracle.binding import OracleRun
+    g = C.small_grid()
+    out = tmp_path / "fields.bin"
+    res = subprocess.run([NODE, os.path.join(ROOT, "node", "gpu_dynamics_steps.js"), json.dumps(g), "3", str(out)],
+                         capture_output=True, text=True, cwd=ROOT, timeout=300)
+    assert res.returncode == 0, res.stderr
+    info = json.loads(res.stdout.strip().splitlines()[-1])
+    assert info["errors"] == [] and info["matches"] and info["diagnostics"]["time"] == 9.0
+    assert info["rejected"] and info["errorsAfter"] == 1
+    assert len(info["fallParts"]) == 7 and min(info["fallParts"]) >= 1
+    fields = np.fromfile(out, np.float32).reshape(len(M.CARRIED), -1)
+    o = OracleRun(g, C.profile(), C.run(g))
+    o.step(3)
     for k, name in enumerate(M.CARRIED):
         b = o.field(name).astype(np.float64)
         nb = np.linalg.norm(b)
