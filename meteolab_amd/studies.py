@@ -86,9 +86,12 @@ def main():
     ap.add_argument("--grid", default="large")
     a = ap.parse_args()
     if a.study == "precision":
-        print(json.dumps(precision(a.steps or 1200, a.grid)))
+        out = precision(a.steps or 1200, a.grid)
+        cmd = f"python -m meteolab_amd.studies precision --steps {out['steps']} --grid {a.grid}"
     else:
-        print(json.dumps(regime(a.steps or 600)))
+        out = regime(a.steps or 600)
+        cmd = f"python -m meteolab_amd.studies regime --steps {out['steps']}"
+    print(json.dumps({"command": cmd, **out}))
 
 
 if __name__ == "__main__":
