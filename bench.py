@@ -121,7 +121,7 @@ def main():
         M.step(eruns, a.warmup, DT)
     kt = M.kernel_times(M.profile_step(eruns, a.steps, DT))
     epool.destroy()
-    stage_kernel = "k_stage_level" if "k_stage_level" in kt else "k_stage"
+    stage_kernel = next(k for k in ("k_stage_plane", "k_stage_level", "k_stage") if k in kt)
     stage_ms = kt[stage_kernel][1]
     stage_b = sum(stage_bytes(n, s) for s in range(3)) * a.runs
     achieved = stage_b / (stage_ms / 1e3)

@@ -325,6 +325,10 @@ def profile_step(runs, n=3, dt=3.0, max_launches=128):
     return list(zip(names.value.decode().split("\n")[:k], ms[:k]))
 
 
+# The transport kernel's layouts (ml_batch_tuning's shape[0]).
+LAYOUTS = ["strip", "level", "plane"]
+
+
 def batch_tuning(runs):
     """The autotuned launch shape of this batch's transport kernel (ml_batch_tuning)."""
     ctx = runs[0].ctx
@@ -332,7 +336,7 @@ def batch_tuning(runs):
     shape = (ctypes.c_int * 4)()
     ms = ctypes.c_float()
     ctx.check(lib().ml_batch_tuning(ctx.h, ids, len(runs), shape, ctypes.byref(ms)))
-    layout = ["strip", "level"][shape[0]]
+    layout = LAYOUTS[shape[0]]
     return {"transport_layout": layout, "field_groups": shape[1] if shape[0] == 0 else 1,
             "levels_per_block": shape[3] if shape[0] else 1, "xcd_per_run": bool(shape[2]),
             "stage0_ms": round(ms.value, 4)}
@@ -340,10 +344,11 @@ def batch_tuning(runs):
 
 def set_batch_tuning(runs, layout, groups=1, xcd=False, levels=1):
     """Forces this batch's transport launch shape (ml_batch_set_tuning): layout "strip" (k_stage, `groups` field
-    groups per block) or "level" (k_stage_level, `levels` levels per block)."""
+    groups per block), "level" (k_stage_level, `levels` levels per block) or "plane" (k_stage_plane, the same with
+    each level in a padded LDS plane)."""
     ctx = runs[0].ctx
     ids = (ctypes.c_int * len(runs))(*[r.h for r in runs])
-    shape = (ctypes.c_int * 4)(["strip", "level"].index(layout), int(groups), 1 if xcd else 0, int(levels))
+    shape = (ctypes.c_int * 4)(LAYOUTS.index(layout), int(groups), 1 if xcd else 0, int(levels))
     ctx.check(lib().ml_batch_set_tuning(ctx.h, ids, len(runs), shape))
 
 

@@ -520,6 +520,156 @@ template <class R> __global__ __launch_bounds__(TPB) void k_stage_level(Geo g, B
   }
 }
 
+// The third transport layout: K levels of one field per block as in k_stage_level, each level staged in LDS as a plane
+// padded by three periodic rows and columns on every side, so a node's six-point stencils are LDS reads at fixed
+// distances from its own place. The x and y face fluxes go to LDS planes whose west and south edges also hold the
+// periodic faces, so the tendency takes its faces without wrap-around arithmetic; a thread steps through its nodes
+// by (x, y) increments. The node's other inputs are read at a level offset held in scalar registers. The scheme
+// (WENO5 or upwind5) and the base-profile departure are compile-time paths. The arithmetic and its order are those
+// of k_stage_level, so both give the same values.
+__host__ __device__ inline int align16i(int bytes) { return (bytes + 15) & ~15; }
+__host__ __device__ inline size_t stagePlaneLdsBytes(const Geo &g, size_t rb) {
+  return size_t(align16i((g.nx + 6) * (g.ny + 6) * 4)) + align16i(int((g.nx + 1) * g.ny * rb)) + align16i(int(g.nx * (g.ny + 1) * rb)) +
+         size_t(g.layer) * rb;
+}
+// Element i of an array whose base is uniform over the block: a 32-bit offset from a base in scalar registers.
+template <class T> __device__ inline T at(const T *base, int i) { return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + unsigned(i) * unsigned(sizeof(T))); }
+template <class T> __device__ inline void put(T *base, int i, T v) { *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + unsigned(i) * unsigned(sizeof(T))) = v; }
+template <class R, bool MONO> __device__ inline R faceFluxT(R m, R qm2, R qm1, R q0, R qp1, R qp2, R qp3) {
+  const bool up = m >= 0;
+  const R a = up ? qm2 : qp3, b_ = up ? qm1 : qp2, c = up ? q0 : qp1, d = up ? qp1 : q0, e = up ? qp2 : qm1;
+  return m * (MONO ? wenoFace(a, b_, c, d, e) : upwindFace(a, b_, c, d, e));
+}
+// Levels z0 .. z1-1 of field f of a run. BASE: u, v or theta, transported as departures from the base profile.
+template <class R, bool BASE, bool MONO> __device__ void stagePlane(const Geo &g, const Bufs<R> &b, int slot, int f, int z0, int z1, int stage, double dt, char *smem,
+                                                                     double *fill) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz, layer = g.layer, W = nx + 6, X1 = nx + 1, T = TPB, planeN = W * (ny + 6);
+  const int inSet = stage, outSet = stage == 2 ? 0 : stage + 1;
+  const float *q = b.set(inSet, slot, f, g), *start = b.f(slot, f, g);
+  float *out = b.set(outSet, slot, f, g);
+  float *ql = reinterpret_cast<float *>(smem);
+  R *fxl = reinterpret_cast<R *>(smem + align16i(planeN * 4));
+  R *fyl = reinterpret_cast<R *>(reinterpret_cast<char *>(fxl) + align16i(int(X1 * ny * sizeof(R))));
+  R *fzl = reinterpret_cast<R *>(reinterpret_cast<char *>(fyl) + align16i(int(nx * (ny + 1) * sizeof(R))));
+  const double *B = BASE ? b.lv(slot, f == F_U ? L_UENV : f == F_V ? L_VENV : L_THETA, g) : nullptr;
+  const bool isW = !BASE && !MONO && f == F_W;
+  const R idx = recip(R(g.dx)), idy = recip(R(g.dy)), frac = R((stage == 0 ? 1.0 / 3 : stage == 1 ? 1.0 / 2 : 1.0) * dt);
+  const double *rho = b.lv(slot, L_RHO, g);
+  // This thread's first node and place in the plane, and the (x, y) steps between its nodes and places.
+  const Div dNx(nx), dW(W);
+  const int y00 = dNx.q(threadIdx.x), x00 = threadIdx.x - y00 * nx, sy = T / nx, sx = T - sy * nx;
+  const int py00 = dW.q(threadIdx.x), px00 = threadIdx.x - py00 * W, spy = T / W, spx = T - spy * W;
+  for (int z = z0; z < z1; z++) {
+    const size_t lz = size_t(z) * layer;
+    const float *qz = q + lz;
+    const double Bz = BASE ? B[z] : 0;
+    // Place (px, py) of the plane holds node ((px - 3) mod nx, (py - 3) mod ny).
+    {
+      int px = px00, py = py00;
+#pragma unroll 4
+      for (int p = threadIdx.x; p < planeN; p += T) {
+        int xs = px - 3, ys = py - 3;
+        xs += xs < 0 ? nx : 0; xs -= xs >= nx ? nx : 0;
+        ys += ys < 0 ? ny : 0; ys -= ys >= ny ? ny : 0;
+        const float v = at(qz, ys * nx + xs);
+        ql[p] = BASE ? float(double(v) - Bz) : v;
+        px += spx; py += spy; if (px >= W) { px -= W; py++; }
+      }
+    }
+    __syncthreads();
+    {
+      const R *mxz = b.m(slot, 0, g) + lz, *myz = b.m(slot, 1, g) + lz;
+      int x = x00, y = y00;
+      for (int c = threadIdx.x; c < layer; c += T) {
+        const float *s = ql + (y + 3) * W + x + 3;
+        const R fx = faceFluxT<R, MONO>(at(mxz, c), R(s[-2]), R(s[-1]), R(s[0]), R(s[1]), R(s[2]), R(s[3]));
+        const R fy = faceFluxT<R, MONO>(at(myz, c), R(s[-2 * W]), R(s[-W]), R(s[0]), R(s[W]), R(s[2 * W]), R(s[3 * W]));
+        R *ex = fxl + y * X1 + x + 1;  // face x + 1/2; the row's first slot is the face west of x = 0
+        ex[0] = fx;
+        if (x == nx - 1) ex[-nx] = fx;
+        fyl[(y + 1) * nx + x] = fy;    // face y + 1/2; the first row is the face south of y = 0
+        if (y == ny - 1) fyl[x] = fy;
+        x += sx; y += sy; if (x >= nx) { x -= nx; y++; }
+      }
+    }
+    __syncthreads();
+    const R ihz = R(1.0 / b.hz[z]), irz = R(1.0 / rho[z]), ihzrz = R(1.0 / (b.hz[z] * rho[z]));
+    double thvEnv = 0, ithv = 0;
+    if (isW) { const double te = b.lv(slot, L_THETA, g)[z], qe = b.lv(slot, L_Q, g)[z]; thvEnv = te * (1 + .61 * qe); ithv = 1.0 / thvEnv; }
+    const float *qc[7];
+    double Bc[7];
+    for (int k = 0; k < 7; k++) { int zz = z - 3 + k; zz = zz < 0 ? 0 : zz > nz - 1 ? nz - 1 : zz; qc[k] = q + size_t(zz) * layer; Bc[k] = BASE ? B[zz] : 0; }
+    R pt[2] = {0, 0}, pb[2] = {0, 0};
+    if (BASE) {
+      pt[0] = R(upwind5d(Bc[1], Bc[2], Bc[3], Bc[4], Bc[5]) - Bz); pt[1] = R(upwind5d(Bc[6], Bc[5], Bc[4], Bc[3], Bc[2]) - Bz);
+      pb[0] = R(upwind5d(Bc[0], Bc[1], Bc[2], Bc[3], Bc[4]) - Bz); pb[1] = R(upwind5d(Bc[5], Bc[4], Bc[3], Bc[2], Bc[1]) - Bz);
+    }
+    const double wz = b.lv(slot, L_LW, g)[z];
+    const R *mzz = b.m(slot, 2, g) + lz, *divz = b.m(slot, 3, g) + lz;
+    const R *mzb_ = z > 0 ? mzz - layer : mzz;
+    const float *startz = start + lz;
+    float *outz = out + lz;
+    auto colAt = [&](int k, int c) { const float v = at(qc[k], c); return BASE ? R(double(v) - Bc[k]) : R(v); };
+    int x = x00, y = y00;
+    for (int c = threadIdx.x; c < layer; c += T) {
+      const R q0 = R(ql[(y + 3) * W + x + 3]);
+      const R c1 = colAt(1, c), c2 = colAt(2, c), c4 = colAt(4, c), c5 = colAt(5, c), c6 = colAt(6, c);
+      const R mzt = z == nz - 1 ? R(0) : at(mzz, c), mzb = z == 0 ? R(0) : at(mzb_, c);
+      const R fzt = z == nz - 1 ? R(0) : faceFluxT<R, MONO>(mzt, c1, c2, q0, c4, c5, c6);
+      R fzb;
+      if (z == 0) fzb = 0;
+      else if (z > z0) fzb = fzl[c];
+      else fzb = faceFluxT<R, MONO>(mzb, colAt(0, c), c1, c2, q0, c4, c5);
+      fzl[c] = fzt;
+      const R *fxw = fxl + y * X1 + x, *fys = fyl + y * nx + x;
+      const R flux = (fxw[1] - fxw[0]) * idx + (fys[nx] - fys[0]) * idy + (fzt - fzb) * ihz;
+      R tend = -(flux - q0 * at(divz, c)) * irz;
+      if (BASE) tend -= (mzt * pt[mzt >= 0 ? 0 : 1] - mzb * pb[mzb >= 0 ? 0 : 1]) * ihzrz;
+      if (isW) {
+        const int i = int(lz) + c;
+        const float th = b.set(inSet, slot, F_THETA, g)[i], qv = b.set(inSet, slot, F_Q, g)[i];
+        const R loaded = R(b.set(inSet, slot, F_CLOUD, g)[i]) + R(b.set(inSet, slot, F_ICE, g)[i])
+          + (R(b.set(inSet, slot, F_RAIN, g)[i]) + R(b.set(inSet, slot, F_SNOW, g)[i]) + R(b.set(inSet, slot, F_GRAUPEL, g)[i]) + R(b.set(inSet, slot, F_HAIL, g)[i]));
+        const double dthv = (double(th) * (1 + .61 * double(qv)) - thvEnv) * ithv;
+        tend += R(G) * (R(dthv) - loaded);
+      }
+      float val = float(R(at(startz, c)) + frac * tend);
+      if (isW && stage < 2 && (z == 0 || z == nz - 1)) val = 0;
+      put(outz, c, val);
+      if (MONO && stage == 2) {
+        const double xv = val;
+        if (xv < TINY_WATER) { if (xv != 0) { fill[0] += wz * xv; if (xv < 0) fill[2] -= wz * xv; } }
+        else fill[1] += wz * xv;
+      }
+      x += sx; y += sy; if (x >= nx) { x -= nx; y++; }
+    }
+    __syncthreads();
+  }
+}
+template <class R> __global__ __launch_bounds__(TPB) void k_stage_plane(Geo g, Bufs<R> b, int stage, double dt, int nr, int nc, int xcd, int K) {
+  const int zBlocks = (g.nz + K - 1) / K;
+  int runIdx, blk;
+  blockRun(nr, zBlocks * nc, xcd, runIdx, blk);
+  const int slot = b.runs[runIdx], f = blk % nc, z0 = (blk / nc) * K, z1 = z0 + K < g.nz ? z0 + K : g.nz;
+  extern __shared__ char smem[];
+  __shared__ double red[TPB / 64][3];
+  double fill[3] = {0, 0, 0};
+  if (f == F_U || f == F_V || f == F_THETA) stagePlane<R, true, false>(g, b, slot, f, z0, z1, stage, dt, smem, fill);
+  else if (f == F_W) stagePlane<R, false, false>(g, b, slot, f, z0, z1, stage, dt, smem, fill);
+  else stagePlane<R, false, true>(g, b, slot, f, z0, z1, stage, dt, smem, fill);
+  const int cf = conservedIndex(f);
+  if (stage == 2 && cf >= 0) {
+    for (int j = 0; j < 3; j++) fill[j] = waveSum(fill[j]);
+    if ((threadIdx.x & 63) == 0) for (int j = 0; j < 3; j++) red[threadIdx.x >> 6][j] = fill[j];
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      double s = 0;
+      for (int w = 0; w < TPB / 64; w++) s += red[w][threadIdx.x];
+      if (s != 0) atomicAdd(b.sc(slot) + S_FILL + 3 * cf + threadIdx.x, s);
+    }
+  }
+}
+
 // ---- pressure projection: dual-cell divergence and the node-velocity correction (pressure.ts:50-92) --------------------
 // set: which buffer set holds u, v, w (0 state, 1 X, 2 Y).
 template <class R> __global__ __launch_bounds__(TPB) void k_pdiv(Geo g, Bufs<R> b, int set, double dt) {

@@ -29,7 +29,8 @@ namespace {
 constexpr int MAX_POOLS = 4096, SLOT_BITS = 16;
 
 // The transport kernel's launch: layout 0 is k_stage (strips of a level, `groups` field groups per strip), layout 1 is
-// k_stage_level (`levels` whole levels of one field per block); `xcd` keeps each run's blocks on one XCD.
+// k_stage_level (`levels` whole levels of one field per block), layout 2 is k_stage_plane (as layout 1, each level in a
+// padded LDS plane); `xcd` keeps each run's blocks on one XCD.
 struct Tuning { int layout = 0, groups = 1, xcd = 0, levels = 1; float ms = 0; };
 
 struct Batch {
@@ -147,6 +148,9 @@ template <class R> void launchStage(const Geo &g, const Bufs<R> &b, int nr, int 
   if (tu.layout == 1) {
     const unsigned zBlocks = unsigned((g.nz + tu.levels - 1) / tu.levels);
     hipLaunchKernelGGL(k_stage_level<R>, dim3(zBlocks * nc * nr), dim3(TPB), stageLevelLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, tu.xcd, tu.levels);
+  } else if (tu.layout == 2) {
+    const unsigned zBlocks = unsigned((g.nz + tu.levels - 1) / tu.levels);
+    hipLaunchKernelGGL(k_stage_plane<R>, dim3(zBlocks * nc * nr), dim3(TPB), stagePlaneLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, tu.xcd, tu.levels);
   } else {
     const StageTile st = stageTile(g);
     hipLaunchKernelGGL(k_stage<R>, dim3(unsigned(st.tiles) * g.nz * nr * tu.groups), dim3(st.threads), stageLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, tu.groups, tu.xcd);
@@ -184,7 +188,7 @@ template <class R> void launchPart(const Pool &p, const int *list, int nr, doubl
     mark("k_faces");
     for (int stage = 0; stage < 3; stage++) {
       launchStage<R>(g, b, nr, nc, stage, dt, s, tu);
-      mark(tu.layout == 1 ? "k_stage_level" : "k_stage");
+      mark(tu.layout == 2 ? "k_stage_plane" : tu.layout == 1 ? "k_stage_level" : "k_stage");
       if (stage < 2) {
         hipLaunchKernelGGL(k_pdiv<R>, dim3(cb, nr), dim3(TPB), 0, s, g, b, stage + 1, dt);
         mark("k_pdiv");
@@ -421,6 +425,15 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
   const int dyn = int(solveLdsBytes(g, chunk));
   if (p->rb == 8) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_discharge<double>), hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
   else HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_discharge<float>), hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+  // The transport layouts' level planes in LDS (the plane layout only where it fits; the autotuner skips it otherwise).
+  const int levelLds = int(stageLevelLdsBytes(g, p->rb)), planeLds = int(stagePlaneLdsBytes(g, p->rb));
+  if (p->rb == 8) {
+    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_level<double>), hipFuncAttributeMaxDynamicSharedMemorySize, levelLds));
+    if (size_t(planeLds) <= ldsMax) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_plane<double>), hipFuncAttributeMaxDynamicSharedMemorySize, planeLds));
+  } else {
+    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_level<float>), hipFuncAttributeMaxDynamicSharedMemorySize, levelLds));
+    if (size_t(planeLds) <= ldsMax) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_plane<float>), hipFuncAttributeMaxDynamicSharedMemorySize, planeLds));
+  }
   p->used.assign(capacity, 0);
   if (id == int(c->pools.size())) c->pools.push_back(p); else c->pools[id] = p;
   *pool = id;
@@ -604,9 +617,12 @@ static int tune(ml_ctx *c, Pool *p, Batch *bt) {
   const int nc = p->electrified ? N_CARRIED : F_COLD + 1, nr = int(bt->runs.size());
   std::vector<Tuning> cands;
   for (int xcd = 0; xcd < (nr % 8 == 0 ? 2 : 1); xcd++) {
-    for (int k : {1, 2, 5, 10}) {
-      if (k > p->g.nz) continue;
-      Tuning lv; lv.layout = 1; lv.xcd = xcd; lv.levels = k; cands.push_back(lv);
+    for (int layout : {1, 2}) {
+      if (layout == 2 && stagePlaneLdsBytes(p->g, p->rb) > c->prop.sharedMemPerBlock) continue;
+      for (int k : {1, 2, 5, 10}) {
+        if (k > p->g.nz) continue;
+        Tuning lv; lv.layout = layout; lv.xcd = xcd; lv.levels = k; cands.push_back(lv);
+      }
     }
 
     for (int gr : options) {
@@ -725,8 +741,9 @@ int ml_batch_set_tuning(ml_ctx *c, const int *runs, int count, const int *shape)
   Pool *p; Batch *bt;
   if (int r = batchOf(c, runs, count, p, bt)) return r;
   const int nc = p->electrified ? N_CARRIED : F_COLD + 1;
-  if (!shape || shape[0] < 0 || shape[0] > 1 || shape[1] < 1 || shape[1] > nc || shape[2] < 0 || shape[2] > 1 || shape[3] < 1 || shape[3] > p->g.nz)
-    return fail(c, ML_E_ARG, "set tuning: layout 0 or 1, 1 <= field groups <= carried fields, xcd 0 or 1, 1 <= levels <= nz");
+  if (!shape || shape[0] < 0 || shape[0] > 2 || shape[1] < 1 || shape[1] > nc || shape[2] < 0 || shape[2] > 1 || shape[3] < 1 || shape[3] > p->g.nz)
+    return fail(c, ML_E_ARG, "set tuning: layout 0, 1 or 2, 1 <= field groups <= carried fields, xcd 0 or 1, 1 <= levels <= nz");
+  if (shape[0] == 2 && stagePlaneLdsBytes(p->g, p->rb) > c->prop.sharedMemPerBlock) return fail(c, ML_E_UNSUPPORTED, "set tuning: the plane layout's LDS exceeds a block's");
   bt->tune.layout = shape[0]; bt->tune.groups = shape[0] == 0 ? shape[1] : 1; bt->tune.xcd = shape[2];
   bt->tune.levels = shape[0] == 0 ? 1 : shape[3]; bt->tune.ms = 0;
   // The captured step holds the old launch shape.
