@@ -25,7 +25,10 @@ enum : int { C_PRECIPITATION = 0, C_HAIL_GROUND, C_HAIL_SIZE, N_COLUMNS };
 // Per-level tables of a run (double, nz each): the base state (src/core/environment.ts:124-130) and what the step
 // derives from it.
 enum : int {
-  L_P = 0, L_EXNER, L_THETA, L_Q, L_U, L_V, L_RHO, L_UENV, L_VENV, L_LW, L_EINIT, L_CAPACITANCE, N_LEVELS
+  L_P = 0, L_EXNER, L_THETA, L_Q, L_U, L_V, L_RHO, L_UENV, L_VENV, L_LW, L_EINIT, L_CAPACITANCE,
+  // The level coefficients of the vertical operators of the face projection (F0) and the pressure (P1): a mode's row
+  // is these with its horizontal symbols (modeSolve).
+  L_F0LO, L_F0UP, L_F0RH, L_P1A, L_P1B, L_P1C, L_P1D, L_P1E, N_LEVELS
 };
 // Per-run parameters (double).
 enum : int {

@@ -54,6 +54,24 @@ template <class R> __global__ void k_base_state(Geo g, Bufs<R> b) {
     double L = cbrt(g.dx * g.dy * b.hz[z]);
     b.lv(slot, L_CAPACITANCE, g)[z] = .25 * 2 * PI * EPSILON0 * L / log(L / 1.0);  // lightning.ts:175
   }
+  __syncthreads();
+  // The level coefficients of the face projection's operator (advection.ts:222-241, nz levels) and the pressure's
+  // (pressure.ts:104-140, nz - 1 levels), with the divisions done once per run.
+  const double *rho = b.lv(slot, L_RHO, g), *dzs = b.dzs, *hz = b.hz;
+  const int L0 = g.nz, L1 = g.nz - 1;
+  for (int k = threadIdx.x; k < g.nz; k += blockDim.x) {
+    auto face = [&](int j) { return j < 0 || j >= L0 - 1 ? 0.0 : (rho[j] + rho[j + 1]) / 2; };
+    b.lv(slot, L_F0LO, g)[k] = k > 0 ? face(k - 1) / dzs[k - 1] : 0.0;
+    b.lv(slot, L_F0UP, g)[k] = k < L0 - 1 ? face(k) / dzs[k] : 0.0;
+    b.lv(slot, L_F0RH, g)[k] = rho[k] * hz[k];
+    if (k < L1) {
+      b.lv(slot, L_P1A, g)[k] = rho[k] * dzs[k] / (2 * hz[k]) + rho[k + 1] * dzs[k] / (2 * hz[k + 1]);
+      b.lv(slot, L_P1B, g)[k] = k > 0 ? rho[k] * dzs[k - 1] / (2 * hz[k]) : 0.0;
+      b.lv(slot, L_P1C, g)[k] = k > 0 ? rho[k] / (hz[k] * dzs[k]) : 0.0;
+      b.lv(slot, L_P1D, g)[k] = k < L1 - 1 ? rho[k + 1] * dzs[k + 1] / (2 * hz[k + 1]) : 0.0;
+      b.lv(slot, L_P1E, g)[k] = k < L1 - 1 ? rho[k + 1] / (hz[k + 1] * dzs[k]) : 0.0;
+    }
+  }
 }
 
 // ---- initial state (model.ts:399-451 in single-storm mode), the run's thermal and its theta noise -----------------

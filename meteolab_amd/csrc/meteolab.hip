@@ -276,8 +276,9 @@ Sizes sizesOf(const Geo &g, size_t rb) {
   return s;
 }
 int geoOf(ml_ctx *c, const ml_grid *o, Geo &g, Plan &pl) {
-  if (!o || o->nx < 2 || o->ny < 2 || o->nz < 4 || o->nz > 64 || o->nx % 2 || o->ny % 2 || !(o->width > 0) || !(o->depth > 0) || !(o->height > 0))
-    return fail(c, ML_E_ARG, "grid: nx, ny even and >= 2, 4 <= nz <= 64, positive extents");
+  // The six-point stencils take one periodic wrap, so a level has at least four columns and rows.
+  if (!o || o->nx < 4 || o->ny < 4 || o->nz < 4 || o->nz > 64 || o->nx % 2 || o->ny % 2 || !(o->width > 0) || !(o->depth > 0) || !(o->height > 0))
+    return fail(c, ML_E_ARG, "grid: nx, ny even and >= 4, 4 <= nz <= 64, positive extents");
   if (!radices(o->nx, pl.rx, pl.nrx) || !radices(o->ny, pl.ry, pl.nry)) return fail(c, ML_E_UNSUPPORTED, "grid: nx and ny must factor into 2, 3 and 5");
   g.nx = o->nx; g.ny = o->ny; g.nz = o->nz; g.layer = g.nx * g.ny; g.n = g.layer * g.nz;
   g.mw = g.nx / 2 + 1; g.nm = g.mw * g.ny;

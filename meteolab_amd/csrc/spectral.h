@@ -224,30 +224,46 @@ template <class T> __device__ inline void modeRow(int kind, int k, int L, T a, T
 }
 
 constexpr int MODE_THREADS = 64;
-// Solves the column of `mode` in place in the spectra (float64); cp[k * cpStride] holds the elimination's factors.
+// Solves the column of `mode` in place in the spectra (float64) by the Thomas algorithm; cp[k * cpStride] holds the
+// elimination's factors. The rows come from the run's level coefficients (face projection, pressure) or the pool's
+// elimination of the electric operator, so a level costs one reciprocal of its pivot.
 template <class R> __device__ void modeSolve(const Geo &g, const Bufs<R> &b, int slot, int mode, int kind, int L, double *cpv, int cpStride) {
-  const double *rho = b.lv(slot, L_RHO, g), *dzs = b.dzs, *hz = b.hz;
   double *spec = b.sp(slot, g);
-  const size_t ls = size_t(g.nm) * 2;
-  const double a = b.sym[(kind == 0 ? 0 : kind == 1 ? 1 : 3) * g.nm + mode], bsym = kind == 1 ? b.sym[2 * g.nm + mode] : 0.0;
+  const int nm = g.nm;
+  const size_t ls = size_t(nm) * 2;
+  const double a = b.sym[(kind == 0 ? 0 : kind == 1 ? 1 : 3) * nm + mode], bsym = kind == 1 ? b.sym[2 * nm + mode] : 0.0;
   if (kind == 1 && ((a == 0 && bsym == 0) || (L == 1 && a == 0))) {
     for (int k = 0; k < L; k++) { spec[k * ls + 2 * mode] = 0; spec[k * ls + 2 * mode + 1] = 0; }
     return;
   }
   const bool pinned = kind != 2 && a == 0;
+  const double *c0 = b.lv(slot, kind == 0 ? L_F0LO : L_P1A, g);  // the run's level coefficients: [5][nz] from here
+  const double *ep = b.efac, *eu = b.efac + size_t(L) * nm, *el = b.efac + size_t(2 * L) * nm;
+  const int nz = g.nz;
+  const double h = -a / 2;
   // The m = n = 0 mode (the level means) takes its right-hand side from the float64 level sums (its imaginary part is 0).
   const double *m0 = mode == 0 ? b.m0(slot, g) : nullptr;
-  double pr = 0, pi = 0;
+  double pr = 0, pi = 0, cprev = 0;
   for (int k = 0; k < L; k++) {
-    double lo, di, up, rf;
-    modeRow<double>(kind, k, L, a, bsym, rho, dzs, hz, lo, di, up, rf);
+    double lo, di, up, rf = 1;
+    if (kind == 0) {
+      lo = c0[k]; up = c0[nz + k]; di = -c0[2 * nz + k] * a - lo - up; rf = b.hz[k];
+    } else if (kind == 1) {
+      di = h * c0[k]; lo = 0; up = 0;
+      if (k > 0) { const double v = bsym * c0[2 * nz + k]; lo = h * c0[nz + k] + v; di -= v; }
+      if (k < L - 1) { const double v = bsym * c0[4 * nz + k]; up = h * c0[3 * nz + k] + v; di -= v; }
+    } else {
+      lo = el[size_t(k) * nm + mode]; up = 0; di = 0;
+    }
     double r = rf * (m0 ? m0[k] : spec[k * ls + 2 * mode]), s = m0 ? 0.0 : rf * spec[k * ls + 2 * mode + 1];
     if (pinned && k == 0) { lo = 0; di = 1; up = 0; r = 0; s = 0; }
     if (pinned && k == 1) lo = 0;
-    double piv = k == 0 ? di : di - lo * cpv[(k - 1) * cpStride];
-    cpv[k * cpStride] = up / piv;
-    pr = (r - (k == 0 ? 0.0 : lo * pr)) / piv;
-    pi = (s - (k == 0 ? 0.0 : lo * pi)) / piv;
+    double inv, cp;
+    if (kind == 2) { inv = 1.0 / ep[size_t(k) * nm + mode]; cp = eu[size_t(k) * nm + mode]; }
+    else { inv = 1.0 / (k == 0 ? di : di - lo * cprev); cp = up * inv; }
+    cpv[k * cpStride] = cprev = cp;
+    pr = (r - (k == 0 ? 0.0 : lo * pr)) * inv;
+    pi = (s - (k == 0 ? 0.0 : lo * pi)) * inv;
     spec[k * ls + 2 * mode] = pr; spec[k * ls + 2 * mode + 1] = pi;
   }
   for (int k = L - 2; k >= 0; k--) {
