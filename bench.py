@@ -87,6 +87,9 @@ def main():
     ap.add_argument("--plain-members", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--transport", default="auto",
+                    help="the transport kernel's launch shape: auto (the autotuner's), or layout:n:xcd with layout strip "
+                         "(n field groups) or level / plane (n levels a block) and xcd 0 or 1")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {a.gpus} was asked): the batched runs are independent and a "
@@ -101,6 +104,10 @@ def main():
     # Batched: R runs in one pool, one launch per kernel for all of them.
     pool = ctx.pool(grid, a.runs)
     runs = [pool.run(prof, C.ensemble_member(grid, k)) for k in range(a.runs)]
+    if a.transport != "auto":
+        layout, count, xcd = a.transport.split(":")
+        M.set_batch_tuning(runs, layout, groups=int(count) if layout == "strip" else 1, xcd=xcd == "1",
+                           levels=int(count) if layout != "strip" else 1)
     if a.warmup:
         M.step(runs, a.warmup, DT)
         ctx.sync()
