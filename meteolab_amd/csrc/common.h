@@ -83,6 +83,7 @@ template <class R> struct Bufs {
   double *cells;   // [slot][n]     right-hand side of a spectral solve (cells or interior layers) (float64)
   double *press;   // [slot][n]     pressure of the projection (model.ts:358), cells (float64)
   double *spec;    // [slot][nz][nm][2] Hermitian half spectra, re/im interleaved (float64: see spectral.h)
+  double *ehat;    // [slot][nz][nm][2] the discharge's spectra of the electric right-hand side (kept between its solves)
   double *mean0;   // [slot][nz]    float64 layer sums of the right-hand side (the m = n = 0 mode)
   R *km, *kh;      // [slot][n]     eddy viscosities
   double *ephi;    // [slot][n]     electric potential (electric.ts:36-60) (float64)
@@ -114,6 +115,7 @@ template <class R> struct Bufs {
   __device__ R *m(int slot, int k, const Geo &g) const { return mass + (size_t(slot) * 4 + k) * g.n; }
   template <class T> __device__ T *arr(T *base, int slot, const Geo &g) const { return base + size_t(slot) * g.n; }
   __device__ double *sp(int slot, const Geo &g) const { return spec + size_t(slot) * g.nz * g.nm * 2; }
+  __device__ double *eh(int slot, const Geo &g) const { return ehat + size_t(slot) * g.nz * g.nm * 2; }
   __device__ double *m0(int slot, const Geo &g) const { return mean0 + size_t(slot) * g.nz; }
   __device__ float *col(int slot, int k, const Geo &g) const { return cols + (size_t(slot) * N_COLUMNS + k) * g.layer; }
   __device__ double *lv(int slot, int k, const Geo &g) const { return lev + (size_t(slot) * N_LEVELS + k) * g.nz; }

@@ -48,7 +48,12 @@ __host__ __device__ inline size_t solveLdsBytes(const Geo &g, int C) { return (s
 // density of every node into lrho, the interior levels' transforms C levels at a time (the transforms of
 // forwardLevel / inverseLevel, several levels per Stockham pass), the modes' columns with the pool's elimination
 // (float64, KB levels' loads in flight per mode). Ends with a barrier.
-template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, const Plan &pl, int slot, char *smem, int C) {
+// The right-hand side's spectra stay in ehat between the solves of a discharge. With `incremental` they are not
+// transformed again: a deposit changes the charge of the channel's nodes only, and the change of node (x, y) of
+// interior level l adds d exp(-2 pi i (m x / nx + n y / ny)) to mode (m, n) of level l (the transform is linear);
+// chan[c] and delta[c] (c < count) are the channel's nodes and their right-hand sides' changes (lrho already new).
+template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, const Plan &pl, int slot, char *smem, int C, const int *chan,
+                                              const double *delta, int count, bool incremental) {
   typedef cx<double> Z;
   const int nx = g.nx, ny = g.ny, mw = g.mw, nm = g.nm, layer = g.layer, half = ny / 2, hl = layer / 2, L = g.nz - 2;
   const int ls = solveStride(g);
@@ -57,14 +62,71 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
   for (int k = threadIdx.x; k < ny; k += blockDim.x) ty[k] = {b.twy[2 * k], b.twy[2 * k + 1]};
   const double *air = b.lv(slot, L_RHO, g);
   R *rho = b.arr(b.lrho, slot, g);
-  double *spec = b.sp(slot, g), *m0 = b.m0(slot, g), *phi = b.arr(b.ephi, slot, g);
+  double *spec = b.sp(slot, g), *hat = b.eh(slot, g), *m0 = b.m0(slot, g), *phi = b.arr(b.ephi, slot, g);
+  const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+  const Div dHl(hl), dNx(nx), dHm(half * mw), dMw(mw), dNm(nm), dNy(ny), dLayer(layer);
+  // The changed nodes as (level, y, x) keys and changes, compacted in channel order into the transform buffers.
+  int *keys = reinterpret_cast<int *>(A);
+  double *ds = reinterpret_cast<double *>(keys + MAX_CHANNEL);
+  if (incremental && size_t(2) * C * ls * sizeof(Z) < size_t(MAX_CHANNEL) * (sizeof(int) + sizeof(double))) incremental = false;
+  if (incremental) {
+    __shared__ int waveCount[DISCHARGE_THREADS / 64];
+    int total = 0;
+    for (int base = 0; base < count; base += blockDim.x) {
+      const int c = base + threadIdx.x;
+      bool keep = false;
+      int key = 0;
+      double d = 0;
+      if (c < count) {
+        d = delta[c];
+        if (d != 0) {
+          const int i = chan[c], z = dLayer.q(i), r = i - z * layer, y = dNx.q(r), x = r - y * nx;
+          key = ((z - 1) << 24) | (y << 12) | x;
+          keep = true;
+        }
+      }
+      const unsigned long long bal = __ballot(keep);
+      if (lane == 0) waveCount[threadIdx.x >> 6] = __popcll(bal);
+      __syncthreads();
+      int at = total, all = 0;
+      for (int v = 0; v < waves; v++) { if (v < int(threadIdx.x >> 6)) at += waveCount[v]; all += waveCount[v]; }
+      if (keep) { at += __popcll(bal & ((1ull << lane) - 1)); keys[at] = key; ds[at] = d; }
+      total += all;
+      __syncthreads();
+    }
+    // Each thread its modes: the changes of a level summed in registers, then added to the level's spectrum.
+    for (int mode = threadIdx.x; mode < nm; mode += blockDim.x) {
+      const int n = dMw.q(mode), m = mode - n * mw;
+      int cur = -1;
+      double ar = 0, ai = 0, s0 = 0;
+      for (int e = 0; e < total; e++) {
+        const int key = keys[e], l = key >> 24, y = (key >> 12) & 0xFFF, x = key & 0xFFF;
+        const double d = ds[e];
+        if (l != cur) {
+          if (cur >= 0) {
+            double *o = hat + (size_t(cur) * nm + mode) * 2;
+            o[0] += ar; o[1] += ai;
+            if (mode == 0) m0[cur] += s0;
+          }
+          cur = l; ar = 0; ai = 0; s0 = 0;
+        }
+        const int mx = m * x - dNx.q(m * x) * nx, nyy = n * y - dNy.q(n * y) * ny;
+        const Z w = cmul(tx[mx], ty[nyy]);
+        ar += d * w.re; ai += d * w.im; s0 += d;
+      }
+      if (cur >= 0) {
+        double *o = hat + (size_t(cur) * nm + mode) * 2;
+        o[0] += ar; o[1] += ai;
+        if (mode == 0) m0[cur] += s0;
+      }
+    }
+    __syncthreads();
+  } else {
   // The bottom and top levels' charge density (the interior levels' comes with their transforms).
   for (int t = threadIdx.x; t < 2 * layer; t += blockDim.x) {
     const int i = t < layer ? t : (g.nz - 1) * layer + (t - layer);
     rho[i] = R(chargeDensity(g, b, slot, i, air[i / layer]));
   }
-  const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
-  const Div dHl(hl), dNx(nx), dHm(half * mw), dMw(mw), dNm(nm);
   for (int l0 = 0; l0 < L; l0 += C) {
     const int cn = L - l0 < C ? L - l0 : C;
     // Right-hand side -rho / eps0 of level l0 + c (node level l0 + c + 1), rows 2p and 2p + 1 as one complex row; the
@@ -103,12 +165,14 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
     Z *S = fftStages(Y, X, ny, cn * mw, 1, mw, ty, pl.ry, pl.nry, -1, mw, ls);
     for (int t = threadIdx.x; t < cn * nm; t += blockDim.x) {
       const int c = dNm.q(t), m = t - c * nm;
-      double *o = spec + (size_t(l0 + c) * nm + m) * 2;
+      double *o = hat + (size_t(l0 + c) * nm + m) * 2;
       o[0] = S[c * ls + m].re; o[1] = S[c * ls + m].im;
     }
     __syncthreads();
   }
-  // The modes' columns: forward elimination, then back substitution.
+  }
+  // The modes' columns: forward elimination from the right-hand side's spectra into the potential's, then back
+  // substitution.
   constexpr int KB = 8;
   const double *fp = b.efac, *fu = b.efac + size_t(L) * nm, *fl = b.efac + size_t(2 * L) * nm;
   for (int mode = threadIdx.x; mode < nm; mode += blockDim.x) {
@@ -118,7 +182,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
 #pragma unroll
       for (int q = 0; q < KB; q++) {
         const int k = k0 + q < L ? k0 + q : L - 1;
-        const double *s = spec + (size_t(k) * nm + mode) * 2;
+        const double *s = hat + (size_t(k) * nm + mode) * 2;
         vr[q] = mode ? s[0] : m0[k]; vi[q] = mode ? s[1] : 0.0;
         pv[q] = fp[size_t(k) * nm + mode]; lo[q] = fl[size_t(k) * nm + mode];
       }
@@ -191,6 +255,7 @@ struct FrontEntry { int node, link, flag; double phi, scale; };
 
 struct FlashState {
   int size, count, ground, touched, left, brk, event, id, pickN, best, prune;
+  int hat;  // the right-hand side's spectra of the last solve are in ehat (the next solve updates them)
   uint32_t rng;
   double sum, potential, before, moment, pick, ratio;
   FrontEntry picked, last;  // the picked entry and the front's last one (which takes the picked one's place)
@@ -235,7 +300,8 @@ template <class R> struct Discharge {
   __device__ int flagsOf(int i) const { return (clear(i) ? FF_CLEAR : 0) | (i < g.layer ? FF_GROUND : 0); }
   // The potential of the carriers as they now stand, then the front's and the channel's potentials and flags from it.
   __device__ void solve() {
-    solveBlock(g, b, pl, slot, smem, chunk);
+    solveBlock(g, b, pl, slot, smem, chunk, nodes, dep, st.count, st.hat != 0);
+    if (threadIdx.x == 0) st.hat = 1;
     for (int f = threadIdx.x; f < st.size; f += blockDim.x) { const int j = fnode[f]; fphi[f] = phi[j]; fflag[f] = flagsOf(j); }
     for (int c = threadIdx.x; c < st.count; c += blockDim.x) cflag[c] = clear(nodes[c]) ? 1 : 0;
     __syncthreads();
@@ -387,7 +453,17 @@ template <class R> struct Discharge {
       moment = blockSum(moment, red);
       if (threadIdx.x == 0) { st.moment += moment; b.sc(slot)[S_FROM_GROUND] -= charge; }
     }
-    for (int c = threadIdx.x; c < st.count; c += blockDim.x) if (dep[c] != 0) neutralize(nodes[c], dep[c]);
+    // The carriers' change, and for the next solve each node's new charge density (lrho) and the change of its
+    // right-hand side -rho / eps0 (interior levels; 0 elsewhere) in place of its deposit.
+    for (int c = threadIdx.x; c < st.count; c += blockDim.x) {
+      if (dep[c] == 0) continue;
+      const int i = nodes[c], z = i / g.layer;
+      const double before = chargeDensity(g, b, slot, i, air[z]);
+      neutralize(i, dep[c]);
+      const double after = chargeDensity(g, b, slot, i, air[z]);
+      rho[i] = R(after);
+      dep[c] = z >= 1 && z <= g.nz - 2 ? -(after - before) / EPSILON0 : 0.0;
+    }
     __syncthreads();
     return charge;
   }
@@ -531,7 +607,7 @@ template <class R> __global__ __launch_bounds__(DISCHARGE_THREADS) void k_discha
   const double time = P[P_TIME] + dt;
   if (threadIdx.x == 0) {
     long long id; memcpy(&id, S + S_FLASH_ID, 8); id++; memcpy(S + S_FLASH_ID, &id, 8);
-    st.id = int(id); st.size = 0; st.count = 0;
+    st.id = int(id); st.size = 0; st.count = 0; st.hat = 0;
     // The stream of this step: the seed (SimConfig.seed + 1) xor the model time in tenths of a second times 0x9E3779B1.
     uint32_t seed = uint32_t(int(P[P_SEED]) + 1), tt = uint32_t((long long)floor(time * 10 + .5));
     st.rng = seed ^ (tt * 0x9E3779B1u);

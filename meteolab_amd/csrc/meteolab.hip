@@ -51,7 +51,8 @@ struct Pool {
   void *fields = nullptr, *stx = nullptr, *sty = nullptr, *mass = nullptr, *phi = nullptr, *cells = nullptr, *press = nullptr,
        *spec = nullptr, *mean0 = nullptr, *km = nullptr, *kh = nullptr, *ephi = nullptr, *cols = nullptr, *lev = nullptr,
        *par = nullptr, *scal = nullptr, *zs = nullptr, *dzs = nullptr, *hz = nullptr, *sym = nullptr, *twx = nullptr, *twy = nullptr,
-       *lrho = nullptr, *lmark = nullptr, *lnode = nullptr, *chan = nullptr, *cdep = nullptr, *flog = nullptr, *efac = nullptr;
+       *lrho = nullptr, *lmark = nullptr, *lnode = nullptr, *chan = nullptr, *cdep = nullptr, *flog = nullptr, *efac = nullptr,
+       *ehat = nullptr;
   int chunk = 1;          // levels per pass of the discharge's own electric solve (as many as the block's LDS holds)
   int *single = nullptr;  // single[s] = s: the run list of one slot
   double *diag = nullptr; // [cap][N_DIAG_DEV]
@@ -70,6 +71,10 @@ struct ml_ctx {
   hipDeviceProp_t prop{};
   std::string err;
   std::vector<Pool *> pools;
+  // The transport layout of each grid (nx, ny, nz, electrified), fixed by its first autotuning: the layouts round
+  // differently in float32, so a run's result must not depend on the size of the batch it is stepped in. Levels per
+  // block, field groups and block order are tuned per batch (they leave the arithmetic unchanged).
+  std::map<std::vector<int>, int> layouts;
 };
 
 namespace {
@@ -116,6 +121,7 @@ template <class R> Bufs<R> bufs(const Pool &p, const int *runs) {
   b.zs = (const double *)p.zs; b.dzs = (const double *)p.dzs; b.hz = (const double *)p.hz; b.sym = (const double *)p.sym;
   b.twx = (const double *)p.twx; b.twy = (const double *)p.twy; b.runs = runs;
   b.efac = (const double *)p.efac; b.lrho = (R *)p.lrho; b.lmark = (int *)p.lmark; b.lnode = (double *)p.lnode; b.chan = (int *)p.chan; b.cdep = (double *)p.cdep; b.flog = (double *)p.flog;
+  b.ehat = (double *)p.ehat;
   return b;
 }
 
@@ -264,7 +270,7 @@ void fillTables(Pool &p, std::vector<char> &sym, std::vector<char> &twx, std::ve
 
 struct Sizes {
   size_t fields, stage, mass, node, node8, spec, mean0, cols, lev, par, scal, lmark, lnode, chan, cdep, flog;
-  size_t slot() const { return fields + 2 * stage + mass + 3 * node + 4 * node8 + spec + mean0 + cols + lev + par + scal + lmark + lnode + chan + cdep + flog; }
+  size_t slot() const { return fields + 2 * stage + mass + 3 * node + 4 * node8 + 2 * spec + mean0 + cols + lev + par + scal + lmark + lnode + chan + cdep + flog; }
 };
 Sizes sizesOf(const Geo &g, size_t rb) {
   Sizes s;
@@ -327,7 +333,7 @@ void freePool(Pool *p) {
   dropBatches(p);
   void *all[] = {p->fields, p->stx, p->sty, p->mass, p->phi, p->cells, p->press, p->spec, p->mean0, p->km, p->kh, p->ephi, p->cols,
                  p->lev, p->par, p->scal, p->zs, p->dzs, p->hz, p->sym, p->twx, p->twy, p->single, p->diag, p->lrho, p->lmark,
-                 p->lnode, p->chan, p->cdep, p->flog, p->efac};
+                 p->lnode, p->chan, p->cdep, p->flog, p->efac, p->ehat};
   for (void *a : all) if (a) hipFree(a);
   delete p;
 }
@@ -385,7 +391,7 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
   size_t cap = capacity;
   struct { void **ptr; size_t bytes; } allocs[] = {
     {&p->fields, s.fields * cap}, {&p->stx, s.stage * cap}, {&p->sty, s.stage * cap}, {&p->mass, s.mass * cap}, {&p->phi, s.node8 * cap},
-    {&p->cells, s.node8 * cap}, {&p->press, s.node8 * cap}, {&p->spec, s.spec * cap}, {&p->mean0, s.mean0 * cap}, {&p->km, s.node * cap},
+    {&p->cells, s.node8 * cap}, {&p->press, s.node8 * cap}, {&p->spec, s.spec * cap}, {&p->ehat, s.spec * cap}, {&p->mean0, s.mean0 * cap}, {&p->km, s.node * cap},
     {&p->kh, s.node * cap}, {&p->ephi, s.node8 * cap}, {&p->cols, s.cols * cap}, {&p->lev, s.lev * cap}, {&p->par, s.par * cap},
     {&p->scal, s.scal * cap}, {(void **)&p->diag, cap * N_DIAG_DEV * 8}, {&p->lrho, s.node * cap}, {&p->lmark, s.lmark * cap},
     {&p->lnode, s.lnode * cap}, {&p->chan, s.chan * cap}, {&p->cdep, s.cdep * cap}, {&p->flog, s.flog * cap}};
@@ -631,6 +637,13 @@ static int tune(ml_ctx *c, Pool *p, Batch *bt) {
       Tuning tl; tl.layout = 0; tl.groups = gr; tl.xcd = xcd; cands.push_back(tl);
     }
   }
+  const std::vector<int> key = {p->g.nx, p->g.ny, p->g.nz, p->electrified};
+  auto fixed = c->layouts.find(key);
+  if (fixed != c->layouts.end()) {
+    std::vector<Tuning> same;
+    for (const Tuning &t : cands) if (t.layout == fixed->second) same.push_back(t);
+    if (!same.empty()) cands.swap(same);
+  }
   float fastest = 1e30f;
   for (Tuning &t : cands) {
     t.ms = p->rb == 8 ? timeStage<double>(c, *p, *bt, t) : timeStage<float>(c, *p, *bt, t);
@@ -641,6 +654,7 @@ static int tune(ml_ctx *c, Pool *p, Batch *bt) {
   // way on every run of the same batch.
   Tuning best = cands[0];
   for (const Tuning &t : cands) if (t.ms <= fastest * 1.03f) { best = t; break; }
+  if (fixed == c->layouts.end()) c->layouts[key] = best.layout;
   bt->tune = best;
   bt->tuned = true;
   return 0;

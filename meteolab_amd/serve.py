@@ -166,7 +166,10 @@ def main():
         out = summary(done, clock)
         out["slot_bytes"] = server.slot_bytes()
         out["workload"] = {"requests": a.requests, "rate_per_s": a.rate, "minutes": [a.min_minutes, a.max_minutes],
-                           "large_share": 0.3, "slots": server.capacity}
+                           "large_share": 0.3, "slots": server.capacity, "seed": a.seed}
+        out["command"] = (f"python -m meteolab_amd.serve --requests {a.requests} --seed {a.seed} --min-minutes {a.min_minutes} "
+                          f"--max-minutes {a.max_minutes} --standard-slots {a.standard_slots} --large-slots {a.large_slots}"
+                          + (f" --rate {a.rate}" if a.rate else ""))
         print(json.dumps(out))
 
 
