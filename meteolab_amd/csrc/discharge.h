@@ -255,7 +255,7 @@ struct FrontEntry { int node, link, flag; double phi, scale; };
 
 struct FlashState {
   int size, count, ground, touched, left, brk, event, id, pickN, best, prune;
-  int hat;  // the right-hand side's spectra of the last solve are in ehat (the next solve updates them)
+  int hat;  // the right-hand side's spectra of the charge before the last deposit are in ehat (the next solve updates them)
   uint32_t rng;
   double sum, potential, before, moment, pick, ratio;
   FrontEntry picked, last;  // the picked entry and the front's last one (which takes the picked one's place)
@@ -607,7 +607,8 @@ template <class R> __global__ __launch_bounds__(DISCHARGE_THREADS) void k_discha
   const double time = P[P_TIME] + dt;
   if (threadIdx.x == 0) {
     long long id; memcpy(&id, S + S_FLASH_ID, 8); id++; memcpy(S + S_FLASH_ID, &id, 8);
-    st.id = int(id); st.size = 0; st.count = 0; st.hat = 0;
+    // The step's electric solve kept its right-hand side's spectra and level sums: every solve here is an update.
+    st.id = int(id); st.size = 0; st.count = 0; st.hat = 1;
     // The stream of this step: the seed (SimConfig.seed + 1) xor the model time in tenths of a second times 0x9E3779B1.
     uint32_t seed = uint32_t(int(P[P_SEED]) + 1), tt = uint32_t((long long)floor(time * 10 + .5));
     st.rng = seed ^ (tt * 0x9E3779B1u);

@@ -141,7 +141,7 @@ struct Timer {
 template <class R> void spectralSolve(const Pool &p, const Bufs<R> &b, int nr, hipStream_t s, int kind, int L, int invKind, Timer *t) {
   const Geo &g = p.g;
   size_t lds = specLdsBytes(g, sizeof(double));
-  hipLaunchKernelGGL(k_spec_fwd<R>, dim3(L, nr), dim3(SPEC_THREADS), lds, s, g, b, p.plan, L);
+  hipLaunchKernelGGL(k_spec_fwd<R>, dim3(L, nr), dim3(SPEC_THREADS), lds, s, g, b, p.plan, L, kind == 2 ? 1 : 0);
   if (t) t->mark("k_spec_fwd");
   hipLaunchKernelGGL(k_modes<R>, dim3(blocks(g.nm, MODE_THREADS), nr), dim3(MODE_THREADS), size_t(MODE_THREADS) * L * sizeof(double), s, g, b, kind, L);
   if (t) t->mark("k_modes");

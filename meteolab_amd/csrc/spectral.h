@@ -119,7 +119,9 @@ __host__ __device__ inline size_t specLdsBytes(const Geo &g, size_t realBytes) {
 
 // Forward transform of level `lev` of the right-hand side in `cells` into the spectra, and its float64 sum (the block
 // works together; `smem` holds the LDS of specLdsBytes, `red` 16 doubles).
-template <class R> __device__ void forwardLevel(const Geo &g, const Bufs<R> &b, const Plan &pl, int slot, int lev, char *smem, double *red) {
+// keep: also write the spectrum to the discharge's right-hand-side spectra (the electric solve, whose right-hand side
+// the discharge then updates node by node).
+template <class R> __device__ void forwardLevel(const Geo &g, const Bufs<R> &b, const Plan &pl, int slot, int lev, char *smem, double *red, bool keep) {
   typedef double S;
   const int nx = g.nx, ny = g.ny, mw = g.mw, layer = g.layer, half = ny / 2;
   const int buf = layer / 2 > g.nm ? layer / 2 : g.nm;
@@ -149,8 +151,11 @@ template <class R> __device__ void forwardLevel(const Geo &g, const Bufs<R> &b, 
   __syncthreads();
   // Columns: mw sequences of length ny, element n of column m at n * mw + m.
   cx<S> *Y = fftStages(C, Z, ny, mw, 1, mw, ty, pl.ry, pl.nry, -1);
-  S *out = b.sp(slot, g) + size_t(lev) * g.nm * 2;
-  for (int t = threadIdx.x; t < g.nm; t += blockDim.x) { out[2 * t] = Y[t].re; out[2 * t + 1] = Y[t].im; }
+  S *out = b.sp(slot, g) + size_t(lev) * g.nm * 2, *kept = b.eh(slot, g) + size_t(lev) * g.nm * 2;
+  for (int t = threadIdx.x; t < g.nm; t += blockDim.x) {
+    out[2 * t] = Y[t].re; out[2 * t + 1] = Y[t].im;
+    if (keep) { kept[2 * t] = Y[t].re; kept[2 * t + 1] = Y[t].im; }
+  }
   __syncthreads();
 }
 
@@ -190,10 +195,10 @@ template <class R> __device__ void inverseLevel(const Geo &g, const Bufs<R> &b, 
   __syncthreads();
 }
 
-template <class R> __global__ __launch_bounds__(SPEC_THREADS) void k_spec_fwd(Geo g, Bufs<R> b, Plan pl, int L) {
+template <class R> __global__ __launch_bounds__(SPEC_THREADS) void k_spec_fwd(Geo g, Bufs<R> b, Plan pl, int L, int keep) {
   extern __shared__ char smem[];
   __shared__ double red[SPEC_THREADS / 64];
-  forwardLevel(g, b, pl, b.runs[blockIdx.y], blockIdx.x, smem, red);
+  forwardLevel(g, b, pl, b.runs[blockIdx.y], blockIdx.x, smem, red, keep != 0);
 }
 template <class R> __global__ __launch_bounds__(SPEC_THREADS) void k_spec_inv(Geo g, Bufs<R> b, Plan pl, int L, int kind) {
   extern __shared__ char smem[];
