@@ -550,6 +550,8 @@ __host__ __device__ inline size_t stagePlaneLdsBytes(const Geo &g, size_t rb) {
   return size_t(align16i((g.nx + 6) * (g.ny + 6) * 4)) + align16i(int((g.nx + 1) * g.ny * rb)) + align16i(int(g.nx * (g.ny + 1) * rb)) +
          size_t(g.layer) * rb;
 }
+// The plane layout's halo copies are one periodic image per side, so a level has at least six columns and rows.
+__host__ __device__ inline bool planeFits(const Geo &g, size_t rb, size_t ldsMax) { return g.nx >= 6 && g.ny >= 6 && stagePlaneLdsBytes(g, rb) <= ldsMax; }
 // Element i of an array whose base is uniform over the block: a 32-bit offset from a base in scalar registers.
 template <class T> __device__ inline T at(const T *base, int i) { return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + unsigned(i) * unsigned(sizeof(T))); }
 template <class T> __device__ inline void put(T *base, int i, T v) { *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + unsigned(i) * unsigned(sizeof(T))) = v; }

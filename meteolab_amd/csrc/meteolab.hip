@@ -436,10 +436,10 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
   const int levelLds = int(stageLevelLdsBytes(g, p->rb)), planeLds = int(stagePlaneLdsBytes(g, p->rb));
   if (p->rb == 8) {
     HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_level<double>), hipFuncAttributeMaxDynamicSharedMemorySize, levelLds));
-    if (size_t(planeLds) <= ldsMax) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_plane<double>), hipFuncAttributeMaxDynamicSharedMemorySize, planeLds));
+    if (planeFits(g, p->rb, ldsMax)) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_plane<double>), hipFuncAttributeMaxDynamicSharedMemorySize, planeLds));
   } else {
     HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_level<float>), hipFuncAttributeMaxDynamicSharedMemorySize, levelLds));
-    if (size_t(planeLds) <= ldsMax) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_plane<float>), hipFuncAttributeMaxDynamicSharedMemorySize, planeLds));
+    if (planeFits(g, p->rb, ldsMax)) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_plane<float>), hipFuncAttributeMaxDynamicSharedMemorySize, planeLds));
   }
   p->used.assign(capacity, 0);
   if (id == int(c->pools.size())) c->pools.push_back(p); else c->pools[id] = p;
@@ -625,7 +625,7 @@ static int tune(ml_ctx *c, Pool *p, Batch *bt) {
   std::vector<Tuning> cands;
   for (int xcd = 0; xcd < (nr % 8 == 0 ? 2 : 1); xcd++) {
     for (int layout : {1, 2}) {
-      if (layout == 2 && stagePlaneLdsBytes(p->g, p->rb) > c->prop.sharedMemPerBlock) continue;
+      if (layout == 2 && !planeFits(p->g, p->rb, c->prop.sharedMemPerBlock)) continue;
       for (int k : {1, 2, 5, 10}) {
         if (k > p->g.nz) continue;
         Tuning lv; lv.layout = layout; lv.xcd = xcd; lv.levels = k; cands.push_back(lv);
@@ -758,7 +758,7 @@ int ml_batch_set_tuning(ml_ctx *c, const int *runs, int count, const int *shape)
   const int nc = p->electrified ? N_CARRIED : F_COLD + 1;
   if (!shape || shape[0] < 0 || shape[0] > 2 || shape[1] < 1 || shape[1] > nc || shape[2] < 0 || shape[2] > 1 || shape[3] < 1 || shape[3] > p->g.nz)
     return fail(c, ML_E_ARG, "set tuning: layout 0, 1 or 2, 1 <= field groups <= carried fields, xcd 0 or 1, 1 <= levels <= nz");
-  if (shape[0] == 2 && stagePlaneLdsBytes(p->g, p->rb) > c->prop.sharedMemPerBlock) return fail(c, ML_E_UNSUPPORTED, "set tuning: the plane layout's LDS exceeds a block's");
+  if (shape[0] == 2 && !planeFits(p->g, p->rb, c->prop.sharedMemPerBlock)) return fail(c, ML_E_UNSUPPORTED, "set tuning: the plane layout needs nx, ny >= 6 and its LDS within a block's");
   bt->tune.layout = shape[0]; bt->tune.groups = shape[0] == 0 ? shape[1] : 1; bt->tune.xcd = shape[2];
   bt->tune.levels = shape[0] == 0 ? 1 : shape[3]; bt->tune.ms = 0;
   // The captured step holds the old launch shape.
