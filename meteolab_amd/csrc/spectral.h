@@ -248,33 +248,57 @@ template <class R> __device__ void modeSolve(const Geo &g, const Bufs<R> &b, int
   const double h = -a / 2;
   // The m = n = 0 mode (the level means) takes its right-hand side from the float64 level sums (its imaginary part is 0).
   const double *m0 = mode == 0 ? b.m0(slot, g) : nullptr;
+  // KB levels' loads in flight at a time: the right-hand sides first, then their elimination.
+  constexpr int KB = 8;
   double pr = 0, pi = 0, cprev = 0;
-  for (int k = 0; k < L; k++) {
-    double lo, di, up, rf = 1;
-    if (kind == 0) {
-      lo = c0[k]; up = c0[nz + k]; di = -c0[2 * nz + k] * a - lo - up; rf = b.hz[k];
-    } else if (kind == 1) {
-      di = h * c0[k]; lo = 0; up = 0;
-      if (k > 0) { const double v = bsym * c0[2 * nz + k]; lo = h * c0[nz + k] + v; di -= v; }
-      if (k < L - 1) { const double v = bsym * c0[4 * nz + k]; up = h * c0[3 * nz + k] + v; di -= v; }
-    } else {
-      lo = el[size_t(k) * nm + mode]; up = 0; di = 0;
+  for (int k0 = 0; k0 < L; k0 += KB) {
+    double vr[KB], vi[KB];
+#pragma unroll
+    for (int q = 0; q < KB; q++) {
+      const int k = k0 + q < L ? k0 + q : L - 1;
+      vr[q] = m0 ? m0[k] : spec[k * ls + 2 * mode]; vi[q] = m0 ? 0.0 : spec[k * ls + 2 * mode + 1];
     }
-    double r = rf * (m0 ? m0[k] : spec[k * ls + 2 * mode]), s = m0 ? 0.0 : rf * spec[k * ls + 2 * mode + 1];
-    if (pinned && k == 0) { lo = 0; di = 1; up = 0; r = 0; s = 0; }
-    if (pinned && k == 1) lo = 0;
-    double inv, cp;
-    if (kind == 2) { inv = 1.0 / ep[size_t(k) * nm + mode]; cp = eu[size_t(k) * nm + mode]; }
-    else { inv = 1.0 / (k == 0 ? di : di - lo * cprev); cp = up * inv; }
-    cpv[k * cpStride] = cprev = cp;
-    pr = (r - (k == 0 ? 0.0 : lo * pr)) * inv;
-    pi = (s - (k == 0 ? 0.0 : lo * pi)) * inv;
-    spec[k * ls + 2 * mode] = pr; spec[k * ls + 2 * mode + 1] = pi;
+#pragma unroll
+    for (int q = 0; q < KB; q++) {
+      const int k = k0 + q;
+      if (k >= L) break;
+      double lo, di, up, rf = 1;
+      if (kind == 0) {
+        lo = c0[k]; up = c0[nz + k]; di = -c0[2 * nz + k] * a - lo - up; rf = b.hz[k];
+      } else if (kind == 1) {
+        di = h * c0[k]; lo = 0; up = 0;
+        if (k > 0) { const double v = bsym * c0[2 * nz + k]; lo = h * c0[nz + k] + v; di -= v; }
+        if (k < L - 1) { const double v = bsym * c0[4 * nz + k]; up = h * c0[3 * nz + k] + v; di -= v; }
+      } else {
+        lo = el[size_t(k) * nm + mode]; up = 0; di = 0;
+      }
+      double r = rf * vr[q], s_ = rf * vi[q];
+      if (pinned && k == 0) { lo = 0; di = 1; up = 0; r = 0; s_ = 0; }
+      if (pinned && k == 1) lo = 0;
+      double inv, cp;
+      if (kind == 2) { inv = 1.0 / ep[size_t(k) * nm + mode]; cp = eu[size_t(k) * nm + mode]; }
+      else { inv = 1.0 / (k == 0 ? di : di - lo * cprev); cp = up * inv; }
+      cpv[k * cpStride] = cprev = cp;
+      pr = (r - (k == 0 ? 0.0 : lo * pr)) * inv;
+      pi = (s_ - (k == 0 ? 0.0 : lo * pi)) * inv;
+      spec[k * ls + 2 * mode] = pr; spec[k * ls + 2 * mode + 1] = pi;
+    }
   }
-  for (int k = L - 2; k >= 0; k--) {
-    double c = cpv[k * cpStride];
-    pr = spec[k * ls + 2 * mode] - c * pr; pi = spec[k * ls + 2 * mode + 1] - c * pi;
-    spec[k * ls + 2 * mode] = pr; spec[k * ls + 2 * mode + 1] = pi;
+  for (int k1 = L - 2; k1 >= 0; k1 -= KB) {
+    double vr[KB], vi[KB];
+#pragma unroll
+    for (int q = 0; q < KB; q++) {
+      const int k = k1 - q >= 0 ? k1 - q : 0;
+      vr[q] = spec[k * ls + 2 * mode]; vi[q] = spec[k * ls + 2 * mode + 1];
+    }
+#pragma unroll
+    for (int q = 0; q < KB; q++) {
+      const int k = k1 - q;
+      if (k < 0) break;
+      const double c = cpv[k * cpStride];
+      pr = vr[q] - c * pr; pi = vi[q] - c * pi;
+      spec[k * ls + 2 * mode] = pr; spec[k * ls + 2 * mode + 1] = pi;
+    }
   }
 }
 // One thread per (mode, run); the elimination's factors stay in LDS, its right-hand sides overwrite the spectra.
