@@ -106,6 +106,10 @@ int ml_run_diag(ml_ctx *ctx, int run, double *out);
 int ml_run_flashes(ml_ctx *ctx, int run, double *out, int max);
 /* Model time of the run, s (AtmosphereModel.time). */
 int ml_run_set_time(ml_ctx *ctx, int run, double time);
+/* The run's lightning totals as a saved state carries them (applyState, src/core/state.ts:84): the charge carried to the
+ * ground (C, header.chargeGround) and the IC and CG flash counts (header.flashCount). The flash log and the first-flash
+ * time are left as they are (a new run has none: applyState forgets the flashes after the state's time). */
+int ml_run_set_counts(ml_ctx *ctx, int run, double charge_ground, int64_t flashes_ic, int64_t flashes_cg);
 
 /* Advances `count` runs of one pool by `steps` model steps of dt s each: AtmosphereModel.step(dt) then time += dt
  * (model.ts:873-877 and its callers, e.g. tests/bench/threads.ts:38-42). One launch per kernel covers all the runs;

@@ -73,6 +73,7 @@ SIGNATURES = {
     "ml_run_levels": (_i, [_vp, _i, _dp]),
     "ml_run_diag": (_i, [_vp, _i, _dp]),
     "ml_run_set_time": (_i, [_vp, _i, _d]),
+    "ml_run_set_counts": (_i, [_vp, _i, _d, _i64, _i64]),
     "ml_run_flashes": (_i, [_vp, _i, _dp, _i]),
     "ml_step": (_i, [_vp, _ip, _i, _i, _d]),
     "ml_dynamics": (_i, [_vp, _ip, _i, _d]),
@@ -211,6 +212,7 @@ class Run:
         h = ctypes.c_int()
         self.ctx.check(lib().ml_run_create(self.ctx.h, pool.id, ctypes.byref(p), ctypes.byref(r), ctypes.byref(h)))
         self.h = h.value
+        self.spec = {"grid": dict(pool.grid), "profile": dict(profile), "run": dict(run)}
 
     def release(self):
         if self.h is not None:
@@ -265,6 +267,10 @@ class Run:
 
     def set_time(self, t):
         self.ctx.check(lib().ml_run_set_time(self.ctx.h, self.h, float(t)))
+
+    def set_counts(self, charge_ground, flashes_ic, flashes_cg):
+        """The lightning totals of a saved state (ml_run_set_counts)."""
+        self.ctx.check(lib().ml_run_set_counts(self.ctx.h, self.h, float(charge_ground), int(flashes_ic), int(flashes_cg)))
 
     def state(self):
         return {name: self.field(name) for name in CARRIED}

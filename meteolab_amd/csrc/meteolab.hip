@@ -555,6 +555,18 @@ int ml_run_set_time(ml_ctx *c, int run, double time) {
   return 0;
 }
 
+int ml_run_set_counts(ml_ctx *c, int run, double charge_ground, int64_t flashes_ic, int64_t flashes_cg) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot) || flashes_ic < 0 || flashes_cg < 0) return fail(c, ML_E_ARG, "set counts: bad run or count");
+  double v[3];
+  v[0] = charge_ground; std::memcpy(v + 1, &flashes_ic, 8); std::memcpy(v + 2, &flashes_cg, 8);
+  static_assert(S_FLASH_IC == S_CHARGE_GROUND + 1 && S_FLASH_CG == S_CHARGE_GROUND + 2, "the counts are adjacent scalars");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync((double *)p->scal + size_t(slot) * N_SCALARS + S_CHARGE_GROUND, v, sizeof(v), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 int ml_run_diag(ml_ctx *c, int run, double *out) {
   Pool *p; int slot;
   if (!c || !decode(c, run, p, slot) || !out) return fail(c, ML_E_ARG, "diag: bad run");

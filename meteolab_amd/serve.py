@@ -19,6 +19,7 @@ import time
 import numpy as np
 
 from . import config as C
+from .state import apply_state, read_state, save_state
 
 MINUTE_STEPS = 20  # model minute at dt = 3 s
 
@@ -32,6 +33,7 @@ class Request:
         self.finished = None
         self.minute_ms = []
         self.handle = None
+        self.saved = None  # (.meteolab bytes, fieldZ) while evicted
 
 
 def workload(requests=1000, rate=None, seed=1, min_minutes=10, max_minutes=120, large_share=0.3):
@@ -69,6 +71,24 @@ class Server:
 
     def admit(self, req):
         req.handle = self.pools[req.size].run(req.profile, req.run)
+        self.active[req.size].append(req)
+
+    def evict(self, req):
+        """Takes a running request off the device to free its slot: its state as a .meteolab file (save_state) and
+        the vertical field of its last discharge (fieldZ, which the file does not hold and the next step's charging
+        reads before it is recomputed)."""
+        req.saved = (save_state(req.handle, config=req.handle.spec), req.handle.field("fieldZ"))
+        self.active[req.size].remove(req)
+        req.handle.release()
+        req.handle = None
+
+    def readmit(self, req):
+        """Puts an evicted request back into a free slot of its class from what evict kept; it goes on as if it had
+        never left."""
+        req.handle = self.pools[req.size].run(req.profile, req.run)
+        apply_state(req.handle, read_state(req.saved[0]))
+        req.handle.set_field("fieldZ", req.saved[1])
+        req.saved = None
         self.active[req.size].append(req)
 
     def round(self):

@@ -83,6 +83,7 @@ class MeteolabRun {
   constructor (pool, prof, config) {
     this.pool = pool; this.device = pool.device
     this.handle = this.device.call(() => addon.runCreate(this.device.ctx, pool.id, prof, config))
+    this.spec = { grid: { ...pool.grid }, profile: { ...prof }, run: { ...config } }
   }
   /** A field (or fieldZ, a fall speed, the condensate) as a new Float32Array. */
   field (name) {
@@ -133,6 +134,8 @@ class MeteolabRun {
     return Array.from(out.subarray(0, k))
   }
   set time (t) { this.device.call(() => addon.setTime(this.device.ctx, this.handle, t)) }
+  /** A saved state's lightning totals: charge carried to the ground (C) and the IC and CG flash counts. */
+  setCounts (chargeGround, ic, cg) { this.device.call(() => addon.setCounts(this.device.ctx, this.handle, chargeGround, ic, cg)) }
   release () { if (this.handle !== null) { this.device.call(() => addon.runRelease(this.device.ctx, this.handle)); this.handle = null } }
 }
 
@@ -209,4 +212,62 @@ class GpuDynamics {
   }
 }
 
-module.exports = { MeteolabDevice, MeteolabPool, MeteolabRun, GpuDynamics, FIELDS, CARRIED, COLUMNS, DIAG, PARTS, STANDARD, LARGE, profile, run, addon }
+// The .meteolab state file (src/core/state.ts:1-110): 'MLAB', the header's length (uint32 LE), the JSON header,
+// padding to 4 bytes, float32 fields in STATE_FIELDS order, then the STATE_COLUMNS. Version 5 is written; the
+// two-moment numbers and volumes (not carried by this step) and the water and charge budgets (not kept) are zeros.
+const STATE_FIELDS = ['u', 'v', 'w', 'theta', 'q', 'cloud', 'rain', 'cold', 'ice', 'snow', 'graupel', 'hail', 'chargeIce',
+  'chargeGraupel', 'chargeRain', 'chargeAir', 'iceNumber', 'rainNumber', 'graupelNumber', 'graupelVolume', 'hailNumber', 'hailVolume']
+const STATE_COLUMNS = ['precipitation', 'hailGround', 'hailSize']
+const STATE_VERSION = 5
+const FIELD_COUNT = { 1: 12, 2: 16, 3: 17, 4: 18, 5: 22 }
+const UNCARRIED = new Set(['rainNumber', 'graupelNumber', 'graupelVolume', 'hailNumber', 'hailVolume'])
+
+/** saveState (state.ts:34-45): a run's state as the bytes of a .meteolab file (a Buffer). */
+function saveState (r, config = null) {
+  const g = r.pool.grid, n = r.pool.n, layer = r.pool.layer, d = r.diagnostics()
+  const header = { format: 'meteolab-state', version: STATE_VERSION, config: config === null ? r.spec : config, time: d.time, persisted: 0,
+    nx: g.nx, ny: g.ny, nz: g.nz, chargeGround: d.chargeGround, flashCount: { ic: d.flashesIC, cg: d.flashesCG },
+    water: { condensed: 0, evaporated: 0, rainEvaporated: 0, surface: 0, start: 0 }, added: { heat: 0, vapour: 0, count: 0 },
+    charge: { separated: 0, ic: 0, cgNegative: 0, cgPositive: 0, precipitation: 0, conduction: 0 } }
+  const json = Buffer.from(JSON.stringify(header)), start = Math.ceil((8 + json.length) / 4) * 4
+  const buffer = Buffer.alloc(start + (STATE_FIELDS.length * n + STATE_COLUMNS.length * layer) * 4)
+  buffer.write('MLAB', 0, 'latin1'); buffer.writeUInt32LE(json.length, 4); json.copy(buffer, 8)
+  const data = new Float32Array(buffer.buffer, buffer.byteOffset + start, STATE_FIELDS.length * n + STATE_COLUMNS.length * layer)
+  STATE_FIELDS.forEach((k, j) => { if (!UNCARRIED.has(k)) data.set(r.field(k), j * n) })
+  STATE_COLUMNS.forEach((k, j) => data.set(r.column(k), STATE_FIELDS.length * n + j * layer))
+  return buffer
+}
+
+/** readState (state.ts:48-68): { header, data } of a .meteolab file; throws with the reference's messages. */
+function readState (buffer) {
+  if (buffer.length < 8 || buffer.toString('latin1', 0, 4) !== 'MLAB') throw new Error('this is not a MeteoLab state file')
+  const length = buffer.readUInt32LE(4)
+  if (8 + length > buffer.length) throw new Error('the file is truncated')
+  let header
+  try { header = JSON.parse(buffer.toString('utf8', 8, 8 + length)) } catch (e) { throw new Error('the file’s header is damaged') }
+  if (!header || header.format !== 'meteolab-state' || !(header.version in FIELD_COUNT)) throw new Error('unknown file format or version')
+  for (const k of ['time', 'persisted', 'nx', 'ny', 'nz']) if (!Number.isFinite(header[k]) || header[k] < 0) throw new Error('the file’s header is damaged')
+  const start = Math.ceil((8 + length) / 4) * 4, cells = header.nx * header.ny, size = FIELD_COUNT[header.version] * cells * header.nz + STATE_COLUMNS.length * cells
+  if (buffer.length !== start + size * 4) throw new Error('the file’s size does not match the grid')
+  const data = new Float32Array(size)
+  for (let k = 0; k < size; k++) data[k] = buffer.readFloatLE(start + 4 * k)
+  return { header, data }
+}
+
+/** applyState (state.ts:74-110) into a run of the same grid: fields, columns, model time, lightning totals. */
+function applyState (r, state) {
+  const g = r.pool.grid, h = state.header, n = r.pool.n, layer = r.pool.layer
+  if (h.nx !== g.nx || h.ny !== g.ny || h.nz !== g.nz) throw new Error(`the file’s grid is ${h.nx}×${h.ny}×${h.nz}, the model’s ${g.nx}×${g.ny}×${g.nz}`)
+  const held = FIELD_COUNT[h.version]
+  STATE_FIELDS.forEach((k, j) => {
+    if (UNCARRIED.has(k)) return
+    r.setField(k, j < held ? state.data.subarray(j * n, (j + 1) * n) : new Float32Array(n))
+  })
+  STATE_COLUMNS.forEach((k, j) => r.setColumn(k, state.data.subarray(held * n + j * layer, held * n + (j + 1) * layer)))
+  r.time = h.time
+  const c = h.flashCount || {}
+  r.setCounts(h.chargeGround || 0, c.ic || 0, c.cg || 0)
+}
+
+module.exports = { MeteolabDevice, MeteolabPool, MeteolabRun, GpuDynamics, FIELDS, CARRIED, COLUMNS, DIAG, PARTS, STANDARD, LARGE, profile, run, addon,
+  STATE_FIELDS, STATE_COLUMNS, saveState, readState, applyState }

@@ -234,6 +234,15 @@ napi_value SetTime(napi_env env, napi_callback_info info) {
   return status(env, ctx, ml_run_set_time(ctx, run, t));
 }
 
+// setCounts(ctx, run, chargeGround, flashesIC, flashesCG): a saved state's lightning totals (ml_run_set_counts).
+napi_value SetCounts(napi_env env, napi_callback_info info) {
+  napi_value a[5]; if (!args(env, info, 5, a)) return nullptr;
+  ml_ctx *ctx = ctxOf(env, a[0]); if (!ctx) return nullptr;
+  int32_t run; double q, ic, cg;
+  if (!intOf(env, a[1], run) || !numberOf(env, a[2], q) || !numberOf(env, a[3], ic) || !numberOf(env, a[4], cg)) return nullptr;
+  return status(env, ctx, ml_run_set_counts(ctx, run, q, int64_t(ic), int64_t(cg)));
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct { const char *name; napi_callback fn; } fns[] = {
     {"ctxCreate", CtxCreate}, {"ctxDestroy", CtxDestroy}, {"deviceName", DeviceName}, {"poolCreate", PoolCreate},
@@ -242,6 +251,7 @@ napi_value Init(napi_env env, napi_value exports) {
     {"downloadColumn", Transfer<false, true>}, {"step", Step}, {"sync", Sync}, {"diag", Doubles<ml_run_diag>},
     {"levels", Doubles<ml_run_levels>}, {"flashes", Flashes}, {"setTime", SetTime}, {"dynamics", Part<ml_dynamics>},
     {"physics", Part<ml_physics>}, {"mix", Part<ml_mix>}, {"afterStep", Part<ml_after_step>}, {"fallParts", FallParts},
+    {"setCounts", SetCounts},
   };
   for (auto &f : fns) {
     napi_value v;

@@ -44,3 +44,34 @@ def test_requests_served_together_end_as_alone():
                 nb = np.linalg.norm(b)
                 err = np.linalg.norm(a - b) / nb if nb > 0 else np.linalg.norm(a)
                 assert err <= 1e-6, (r.id, name, err)
+
+
+@pytest.mark.gpu
+def test_an_evicted_request_readmitted_from_its_state_file_ends_as_alone():
+    # Checkpoint and restore of a resident run: after a round a request leaves the device as a .meteolab file (and its
+    # discharge's vertical field), a round goes on without it, and it comes back and finishes as it would alone.
+    g = C.small_grid()
+    grids = {"small": g}
+    a, b = [S.Request(k, "small", 3, 0.0, C.profile(qv_max=0.013), C.run(g, seed=k + 3, amp=2.5)) for k in range(2)]
+    with M.Context(0, 32) as ctx:
+        finals = {}
+        server = S.Server(ctx, {"small": 2}, grids=grids,
+                          on_finish=lambda r: finals.setdefault(r.id, {name: r.handle.field(name) for name in M.CARRIED}))
+        server.admit(a)
+        server.admit(b)
+        server.round()
+        server.evict(a)
+        assert a.handle is None and a.saved[0][:4] == b"MLAB"
+        server.round()           # b alone
+        server.readmit(a)
+        server.round()           # b finishes
+        server.round()           # a finishes
+        assert a.done_minutes == 3 and b.done_minutes == 3 and set(finals) == {0, 1}
+        for r in (a, b):
+            alone = ctx.pool(g, 1).run(r.profile, r.run)
+            alone.step(r.minutes * S.MINUTE_STEPS)
+            for name in M.CARRIED:
+                x, y = finals[r.id][name].astype(np.float64), alone.field(name).astype(np.float64)
+                ny = np.linalg.norm(y)
+                err = np.linalg.norm(x - y) / ny if ny > 0 else np.linalg.norm(x)
+                assert err <= 1e-6, (r.id, name, err)
