@@ -41,6 +41,12 @@ enum {
   ML_D_NEGFILLED, ML_D_CHARGEGROUND, ML_D_FLASHES_IC, ML_D_FLASHES_CG, ML_D_FIRSTFLASH, ML_D_ERROR, ML_D_COUNT
 };
 
+/* Rotation diagnostics of a run (ml_run_rotation). */
+enum {
+  ML_ROT_UH = 0, ML_ROT_X, ML_ROT_Y, ML_ROT_ANTICYCLONIC, ML_ROT_PERSISTED, ML_ROT_UH01, ML_ROT_UH03, ML_ROT_LOW,
+  ML_ROT_CORE_ZETA, ML_ROT_CORE_UH, ML_ROT_CORE_ANTI, ML_ROT_COUNT
+};
+
 /* Grid (src/core/grid.ts GridOptions, createGrid :99-121): periodic nx x ny, nz levels stretched geometrically from
  * `bottom` m at the ground to `height` (bottom = 0: uniform). */
 typedef struct { int nx, ny, nz; double width, depth, height, bottom; } ml_grid;
@@ -106,10 +112,20 @@ int ml_run_diag(ml_ctx *ctx, int run, double *out);
 int ml_run_flashes(ml_ctx *ctx, int run, double *out, int max);
 /* Model time of the run, s (AtmosphereModel.time). */
 int ml_run_set_time(ml_ctx *ctx, int run, double time);
-/* The run's lightning totals as a saved state carries them (applyState, src/core/state.ts:84): the charge carried to the
- * ground (C, header.chargeGround) and the IC and CG flash counts (header.flashCount). The flash log and the first-flash
- * time are left as they are (a new run has none: applyState forgets the flashes after the state's time). */
-int ml_run_set_counts(ml_ctx *ctx, int run, double charge_ground, int64_t flashes_ic, int64_t flashes_cg);
+/* The totals a saved state's header carries (applyState, src/core/state.ts:84-95): the charge carried to the ground
+ * (C, header.chargeGround), the IC and CG flash counts (header.flashCount) and how long the mesocyclone criterion has
+ * held (s, header.persisted, rotation.persisted). The flash log and the first-flash time are left as they are (a new
+ * run has none: applyState forgets the flashes after the state's time). */
+int ml_run_set_totals(ml_ctx *ctx, int run, double charge_ground, int64_t flashes_ic, int64_t flashes_cg, double persisted);
+/* The rotation diagnostics of the run's last step (updateRotation, model.ts:1609-1648; RotationState :142-158),
+ * ML_ROT_COUNT float64: the largest updraught helicity 2-5 km (m2/s2) and its column (x, y), the largest
+ * anticyclonic one, how long the mesocyclone criterion has held (s), the largest 0-1 km and 0-3 km helicity and the
+ * column of the 0-1 km one, the most cyclonic updraught core's mean vorticity (s-1) and its strongest helicity, and the
+ * most anticyclonic core's mean vorticity. */
+int ml_run_rotation(ml_ctx *ctx, int run, double *out);
+/* Recomputes the rotation diagnostics from the run's fields as they are, keeping how long the criterion has held
+ * (refreshRotation, model.ts:1607; applyState calls it, src/core/state.ts:73-74). */
+int ml_run_refresh_rotation(ml_ctx *ctx, int run);
 
 /* Advances `count` runs of one pool by `steps` model steps of dt s each: AtmosphereModel.step(dt) then time += dt
  * (model.ts:873-877 and its callers, e.g. tests/bench/threads.ts:38-42). One launch per kernel covers all the runs;

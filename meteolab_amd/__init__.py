@@ -24,6 +24,8 @@ COLUMNS = ["precipitation", "hailGround", "hailSize"]
 LEVELS = ["p", "exner", "theta", "q", "u", "v", "rho"]
 DIAG = ["time", "maxW", "minW", "maxThetaP", "rainTotal", "water", "charge", "clipped", "negativeFilled",
         "chargeGround", "flashesIC", "flashesCG", "firstFlash", "error"]
+# RotationState (model.ts:142-158) in ml_run_rotation's order.
+ROTATION = ["uh", "x", "y", "anticyclonic", "persisted", "uh01", "uh03", "low", "coreZeta", "coreUH", "coreAnti"]
 # STATE_FIELDS of src/core/state.ts:9 that this step carries (two-moment numbers are not part of the baseline step).
 STATE_FIELDS = ["u", "v", "w", "theta", "q", "cloud", "rain", "cold", "ice", "snow", "graupel", "hail", "chargeIce",
                 "chargeGraupel", "chargeRain", "chargeAir", "iceNumber"]
@@ -73,7 +75,9 @@ SIGNATURES = {
     "ml_run_levels": (_i, [_vp, _i, _dp]),
     "ml_run_diag": (_i, [_vp, _i, _dp]),
     "ml_run_set_time": (_i, [_vp, _i, _d]),
-    "ml_run_set_counts": (_i, [_vp, _i, _d, _i64, _i64]),
+    "ml_run_set_totals": (_i, [_vp, _i, _d, _i64, _i64, _d]),
+    "ml_run_rotation": (_i, [_vp, _i, _dp]),
+    "ml_run_refresh_rotation": (_i, [_vp, _i]),
     "ml_run_flashes": (_i, [_vp, _i, _dp, _i]),
     "ml_step": (_i, [_vp, _ip, _i, _i, _d]),
     "ml_dynamics": (_i, [_vp, _ip, _i, _d]),
@@ -268,9 +272,20 @@ class Run:
     def set_time(self, t):
         self.ctx.check(lib().ml_run_set_time(self.ctx.h, self.h, float(t)))
 
-    def set_counts(self, charge_ground, flashes_ic, flashes_cg):
-        """The lightning totals of a saved state (ml_run_set_counts)."""
-        self.ctx.check(lib().ml_run_set_counts(self.ctx.h, self.h, float(charge_ground), int(flashes_ic), int(flashes_cg)))
+    def set_totals(self, charge_ground, flashes_ic, flashes_cg, persisted=0.0):
+        """The totals of a saved state's header (ml_run_set_totals)."""
+        self.ctx.check(lib().ml_run_set_totals(self.ctx.h, self.h, float(charge_ground), int(flashes_ic), int(flashes_cg), float(persisted)))
+
+    def rotation(self):
+        """The rotation diagnostics of the last step (ml_run_rotation), by the names of ROTATION."""
+        out = np.empty(len(ROTATION), np.float64)
+        self.ctx.check(lib().ml_run_rotation(self.ctx.h, self.h, _dptr(out)))
+        return dict(zip(ROTATION, out))
+
+    def refresh_rotation(self):
+        """Recomputes the rotation diagnostics from the fields as they are, keeping how long the rotation has held
+        (ml_run_refresh_rotation, refreshRotation of model.ts:1607)."""
+        self.ctx.check(lib().ml_run_refresh_rotation(self.ctx.h, self.h))
 
     def state(self):
         return {name: self.field(name) for name in CARRIED}

@@ -51,8 +51,10 @@ enum : int {
   S_FLASH_ID = 39,       // int64: the lightning's stamp counter (lightning.ts flashId)
   S_FROM_GROUND = 40,    // double: charge the CG flashes took from the ground (lightning.ts fromGround)
   S_FLASH_LOG = 41,      // int64: flashes recorded in the run's flash log (a ring of FLASH_LOG entries)
-  N_SCALARS = 48
+  S_ROT = 48,            // the rotation diagnostics (model.ts:142-158 RotationState), ROT_* from here
+  N_SCALARS = 64
 };
+enum : int { ROT_UH = 0, ROT_X, ROT_Y, ROT_ANTI, ROT_PERSISTED, ROT_UH01, ROT_UH03, ROT_LOW, ROT_CORE_ZETA, ROT_CORE_UH, ROT_CORE_ANTI, N_ROT };
 // Lightning channel limits (lightning.ts:32-37) and the per-run flash log.
 constexpr int MAX_CHANNEL = 3000, STROKES = 4, FLASH_LOG = 64, FLASH_FIELDS = 8;
 // The lightning's per-node int arrays and per-front-entry arrays of a slot.

@@ -21,6 +21,7 @@
 #include "discharge.h"
 #include "lightning.h"
 #include "spectral.h"
+#include "rotation.h"
 
 using namespace ml;
 
@@ -54,6 +55,7 @@ struct Pool {
        *lrho = nullptr, *lmark = nullptr, *lnode = nullptr, *chan = nullptr, *cdep = nullptr, *flog = nullptr, *efac = nullptr,
        *ehat = nullptr;
   int chunk = 1;          // levels per pass of the discharge's own electric solve (as many as the block's LDS holds)
+  RotLevels rot{};        // the rotation diagnostics' levels and threshold
   int *single = nullptr;  // single[s] = s: the run list of one slot
   double *diag = nullptr; // [cap][N_DIAG_DEV]
   std::vector<double> zsH, dzsH, hzH;
@@ -221,7 +223,9 @@ template <class R> void launchPart(const Pool &p, const int *list, int nr, doubl
     mark("k_pcorrect");
   }
   if (part != PART_AFTER) return;
-  // afterStep: lightning (model.ts:1537-1552).
+  // afterStep: the rotation diagnostics (model.ts:1609-1648), then the lightning (model.ts:1537-1552).
+  hipLaunchKernelGGL(k_rotation<R>, dim3(nr), dim3(ROTATION_THREADS), rotationLdsBytes(g), s, g, b, dt, p.rot, 0);
+  mark("k_rotation");
   if (el) {
     unsigned ib = blocks(size_t(g.layer) * (g.nz - 2), 256);
     hipLaunchKernelGGL(k_charge_rhs<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b);
@@ -427,11 +431,29 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
   const size_t dischargeStatic = 8192, ldsMax = c->prop.sharedMemPerBlock;
   int chunk = 1;
   while (chunk < L && solveLdsBytes(g, chunk + 1) + dischargeStatic <= ldsMax) chunk++;
-  if (solveLdsBytes(g, chunk) + dischargeStatic > ldsMax) { delete p; return fail(c, ML_E_UNSUPPORTED, "grid too large for the discharge's level in LDS"); }
+  if (solveLdsBytes(g, chunk) + dischargeStatic > ldsMax) { freePool(p); return fail(c, ML_E_UNSUPPORTED, "grid too large for the discharge's level in LDS"); }
   p->chunk = chunk;
   const int dyn = int(solveLdsBytes(g, chunk));
   if (p->rb == 8) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_discharge<double>), hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
   else HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_discharge<float>), hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+  // The rotation diagnostics: the first level at or above 2 km to the last at or below 5 km (model.ts:389), the last
+  // levels at or below 1 and 3 km (:1612), and the core-vorticity threshold of this spacing (mesoCoreZeta, :129).
+  {
+    RotLevels &r = p->rot;
+    r.z0 = g.nz; r.z1 = -1; r.top1 = -1; r.top3 = -1;
+    for (int z = 0; z < g.nz; z++) {
+      const double h = p->zsH[z];
+      if (h >= 2000 && r.z0 == g.nz) r.z0 = z;
+      if (h <= 5000) r.z1 = z;
+      if (h <= 1000) r.top1 = z;
+      if (h <= 3000) r.top3 = z;
+    }
+    r.meso = 3e-3 * std::max(1.0, 1200.0 / g.dx);
+    const int rotLds = int(rotationLdsBytes(g));
+    if (size_t(rotLds) > ldsMax) { freePool(p); return fail(c, ML_E_UNSUPPORTED, "grid too large for the rotation diagnostics' columns in LDS"); }
+    if (p->rb == 8) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_rotation<double>), hipFuncAttributeMaxDynamicSharedMemorySize, rotLds));
+    else HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_rotation<float>), hipFuncAttributeMaxDynamicSharedMemorySize, rotLds));
+  }
   // The transport layouts' level planes in LDS (the plane layout only where it fits; the autotuner skips it otherwise).
   const int levelLds = int(stageLevelLdsBytes(g, p->rb)), planeLds = int(stagePlaneLdsBytes(g, p->rb));
   if (p->rb == 8) {
@@ -555,14 +577,38 @@ int ml_run_set_time(ml_ctx *c, int run, double time) {
   return 0;
 }
 
-int ml_run_set_counts(ml_ctx *c, int run, double charge_ground, int64_t flashes_ic, int64_t flashes_cg) {
+int ml_run_set_totals(ml_ctx *c, int run, double charge_ground, int64_t flashes_ic, int64_t flashes_cg, double persisted) {
   Pool *p; int slot;
-  if (!c || !decode(c, run, p, slot) || flashes_ic < 0 || flashes_cg < 0) return fail(c, ML_E_ARG, "set counts: bad run or count");
+  if (!c || !decode(c, run, p, slot) || flashes_ic < 0 || flashes_cg < 0 || !(persisted >= 0)) return fail(c, ML_E_ARG, "set totals: bad run or value");
   double v[3];
   v[0] = charge_ground; std::memcpy(v + 1, &flashes_ic, 8); std::memcpy(v + 2, &flashes_cg, 8);
   static_assert(S_FLASH_IC == S_CHARGE_GROUND + 1 && S_FLASH_CG == S_CHARGE_GROUND + 2, "the counts are adjacent scalars");
+  double *sc = (double *)p->scal + size_t(slot) * N_SCALARS;
   HIPCHK(c, hipSetDevice(c->device));
-  HIPCHK(c, hipMemcpyAsync((double *)p->scal + size_t(slot) * N_SCALARS + S_CHARGE_GROUND, v, sizeof(v), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(sc + S_CHARGE_GROUND, v, sizeof(v), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(sc + S_ROT + ROT_PERSISTED, &persisted, 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ml_run_rotation(ml_ctx *c, int run, double *out) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot) || !out) return fail(c, ML_E_ARG, "rotation: bad run");
+  static_assert(N_ROT == ML_ROT_COUNT, "the ABI's rotation record is the scalars'");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(out, (double *)p->scal + size_t(slot) * N_SCALARS + S_ROT, N_ROT * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ml_run_refresh_rotation(ml_ctx *c, int run) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot)) return fail(c, ML_E_ARG, "refresh rotation: bad run");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int *one = p->single + slot;
+  if (p->rb == 8) hipLaunchKernelGGL(k_rotation<double>, dim3(1), dim3(ROTATION_THREADS), rotationLdsBytes(p->g), c->stream, p->g, bufs<double>(*p, one), 0.0, p->rot, 1);
+  else hipLaunchKernelGGL(k_rotation<float>, dim3(1), dim3(ROTATION_THREADS), rotationLdsBytes(p->g), c->stream, p->g, bufs<float>(*p, one), 0.0, p->rot, 1);
+  HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }

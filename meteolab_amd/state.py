@@ -9,7 +9,7 @@ save_state writes version 5 (state.ts:18): the two-moment numbers and volumes, w
 zeros, and the water and charge budgets, which it does not keep, are zeros. read_state takes versions 1-5 and refuses
 what the reference refuses, with its messages (state.ts:53-68). apply_state puts a state into a run of the same grid
 (state.ts:74-110): the fields the file holds, zeros for those it does not, the columns, the model time and the
-lightning totals; the flash log starts empty. Everything else a step needs it recomputes from these, except the
+lightning totals and how long the rotation has held; the flash log starts empty. Everything else a step needs it recomputes from these, except the
 vertical field of the last discharge, which a step computes at its end: a restored run's first step charges without
 it, as the reference's applyState leaves it.
 """
@@ -48,7 +48,7 @@ def save_state(run, config=None):
     n, layer = g["nx"] * g["ny"] * g["nz"], g["nx"] * g["ny"]
     d = run.diagnostics()
     header = {"format": "meteolab-state", "version": VERSION, "config": run.spec if config is None else config,
-              "time": _number(d["time"]), "persisted": 0, "nx": g["nx"], "ny": g["ny"], "nz": g["nz"],
+              "time": _number(d["time"]), "persisted": _number(run.rotation()["persisted"]), "nx": g["nx"], "ny": g["ny"], "nz": g["nz"],
               "chargeGround": _number(d["chargeGround"]),
               "flashCount": {"ic": int(d["flashesIC"]), "cg": int(d["flashesCG"])},
               "water": {"condensed": 0, "evaporated": 0, "rainEvaporated": 0, "surface": 0, "start": 0},
@@ -112,4 +112,6 @@ def apply_state(run, state):
         run.set_column(name, data[o + k * layer:o + (k + 1) * layer])
     run.set_time(header["time"])
     count = header.get("flashCount") or {}
-    run.set_counts(header.get("chargeGround") or 0.0, count.get("ic") or 0, count.get("cg") or 0)
+    run.set_totals(header.get("chargeGround") or 0.0, count.get("ic") or 0, count.get("cg") or 0, header["persisted"])
+    # The rotation diagnostics are not in the file: recomputed from its fields (state.ts:73-74).
+    run.refresh_rotation()

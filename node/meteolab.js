@@ -22,6 +22,8 @@ const COLUMNS = ['precipitation', 'hailGround', 'hailSize']
 const DIAG = ['time', 'maxW', 'minW', 'maxThetaP', 'rainTotal', 'water', 'charge', 'clipped', 'negativeFilled',
   'chargeGround', 'flashesIC', 'flashesCG', 'firstFlash', 'error']
 const LEVELS = ['p', 'exner', 'theta', 'q', 'u', 'v', 'rho']
+// RotationState (model.ts:142-158) in ml_run_rotation's order.
+const ROTATION = ['uh', 'x', 'y', 'anticyclonic', 'persisted', 'uh01', 'uh03', 'low', 'coreZeta', 'coreUH', 'coreAnti']
 // The parts of a step where the reference's GPU step splits it (src/gpu/step.ts:12-31).
 const PARTS = ['dynamics', 'physics', 'mix', 'afterStep']
 const CARRIED = FIELDS.slice(0, 17)
@@ -134,8 +136,16 @@ class MeteolabRun {
     return Array.from(out.subarray(0, k))
   }
   set time (t) { this.device.call(() => addon.setTime(this.device.ctx, this.handle, t)) }
-  /** A saved state's lightning totals: charge carried to the ground (C) and the IC and CG flash counts. */
-  setCounts (chargeGround, ic, cg) { this.device.call(() => addon.setCounts(this.device.ctx, this.handle, chargeGround, ic, cg)) }
+  /** A saved state's header totals: charge carried to the ground (C), IC and CG flash counts, rotation held (s). */
+  setTotals (chargeGround, ic, cg, persisted = 0) { this.device.call(() => addon.setTotals(this.device.ctx, this.handle, chargeGround, ic, cg, persisted)) }
+  /** The rotation diagnostics of the last step (model.ts RotationState): uh, x, y, anticyclonic, persisted, ... */
+  rotation () {
+    const out = new Float64Array(ROTATION.length)
+    this.device.call(() => addon.rotation(this.device.ctx, this.handle, out))
+    return Object.fromEntries(ROTATION.map((k, i) => [k, out[i]]))
+  }
+  /** The rotation diagnostics of the fields as they are, keeping how long the rotation has held (refreshRotation). */
+  refreshRotation () { this.device.call(() => addon.refreshRotation(this.device.ctx, this.handle)) }
   release () { if (this.handle !== null) { this.device.call(() => addon.runRelease(this.device.ctx, this.handle)); this.handle = null } }
 }
 
@@ -225,7 +235,7 @@ const UNCARRIED = new Set(['rainNumber', 'graupelNumber', 'graupelVolume', 'hail
 /** saveState (state.ts:34-45): a run's state as the bytes of a .meteolab file (a Buffer). */
 function saveState (r, config = null) {
   const g = r.pool.grid, n = r.pool.n, layer = r.pool.layer, d = r.diagnostics()
-  const header = { format: 'meteolab-state', version: STATE_VERSION, config: config === null ? r.spec : config, time: d.time, persisted: 0,
+  const header = { format: 'meteolab-state', version: STATE_VERSION, config: config === null ? r.spec : config, time: d.time, persisted: r.rotation().persisted,
     nx: g.nx, ny: g.ny, nz: g.nz, chargeGround: d.chargeGround, flashCount: { ic: d.flashesIC, cg: d.flashesCG },
     water: { condensed: 0, evaporated: 0, rainEvaporated: 0, surface: 0, start: 0 }, added: { heat: 0, vapour: 0, count: 0 },
     charge: { separated: 0, ic: 0, cgNegative: 0, cgPositive: 0, precipitation: 0, conduction: 0 } }
@@ -266,8 +276,10 @@ function applyState (r, state) {
   STATE_COLUMNS.forEach((k, j) => r.setColumn(k, state.data.subarray(held * n + j * layer, held * n + (j + 1) * layer)))
   r.time = h.time
   const c = h.flashCount || {}
-  r.setCounts(h.chargeGround || 0, c.ic || 0, c.cg || 0)
+  r.setTotals(h.chargeGround || 0, c.ic || 0, c.cg || 0, h.persisted)
+  // The rotation diagnostics are not in the file: recomputed from its fields (state.ts:73-74).
+  r.refreshRotation()
 }
 
 module.exports = { MeteolabDevice, MeteolabPool, MeteolabRun, GpuDynamics, FIELDS, CARRIED, COLUMNS, DIAG, PARTS, STANDARD, LARGE, profile, run, addon,
-  STATE_FIELDS, STATE_COLUMNS, saveState, readState, applyState }
+  STATE_FIELDS, STATE_COLUMNS, ROTATION, saveState, readState, applyState }

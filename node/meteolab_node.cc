@@ -151,11 +151,12 @@ napi_value RunCreate(napi_env env, napi_callback_info info) {
   napi_value r; napi_create_int32(env, run, &r); return r;
 }
 
-napi_value RunRelease(napi_env env, napi_callback_info info) {
+// runRelease(ctx, run) / refreshRotation(ctx, run): entry points that take a run and nothing else.
+template <int (*F)(ml_ctx *, int)> napi_value RunOp(napi_env env, napi_callback_info info) {
   napi_value a[2]; if (!args(env, info, 2, a)) return nullptr;
   ml_ctx *ctx = ctxOf(env, a[0]); if (!ctx) return nullptr;
   int32_t run; if (!intOf(env, a[1], run)) return nullptr;
-  return status(env, ctx, ml_run_release(ctx, run));
+  return status(env, ctx, F(ctx, run));
 }
 
 // upload(ctx, run, field, Float32Array) / download(ctx, run, field, Float32Array); the same for ground columns.
@@ -234,24 +235,24 @@ napi_value SetTime(napi_env env, napi_callback_info info) {
   return status(env, ctx, ml_run_set_time(ctx, run, t));
 }
 
-// setCounts(ctx, run, chargeGround, flashesIC, flashesCG): a saved state's lightning totals (ml_run_set_counts).
-napi_value SetCounts(napi_env env, napi_callback_info info) {
-  napi_value a[5]; if (!args(env, info, 5, a)) return nullptr;
+// setTotals(ctx, run, chargeGround, flashesIC, flashesCG, persisted): a saved state's header totals (ml_run_set_totals).
+napi_value SetTotals(napi_env env, napi_callback_info info) {
+  napi_value a[6]; if (!args(env, info, 6, a)) return nullptr;
   ml_ctx *ctx = ctxOf(env, a[0]); if (!ctx) return nullptr;
-  int32_t run; double q, ic, cg;
-  if (!intOf(env, a[1], run) || !numberOf(env, a[2], q) || !numberOf(env, a[3], ic) || !numberOf(env, a[4], cg)) return nullptr;
-  return status(env, ctx, ml_run_set_counts(ctx, run, q, int64_t(ic), int64_t(cg)));
+  int32_t run; double q, ic, cg, held;
+  if (!intOf(env, a[1], run) || !numberOf(env, a[2], q) || !numberOf(env, a[3], ic) || !numberOf(env, a[4], cg) || !numberOf(env, a[5], held)) return nullptr;
+  return status(env, ctx, ml_run_set_totals(ctx, run, q, int64_t(ic), int64_t(cg), held));
 }
 
 napi_value Init(napi_env env, napi_value exports) {
   struct { const char *name; napi_callback fn; } fns[] = {
     {"ctxCreate", CtxCreate}, {"ctxDestroy", CtxDestroy}, {"deviceName", DeviceName}, {"poolCreate", PoolCreate},
-    {"poolDestroy", PoolDestroy}, {"slotBytes", SlotBytes}, {"runCreate", RunCreate}, {"runRelease", RunRelease},
+    {"poolDestroy", PoolDestroy}, {"slotBytes", SlotBytes}, {"runCreate", RunCreate}, {"runRelease", RunOp<ml_run_release>},
     {"upload", Transfer<true, false>}, {"download", Transfer<false, false>}, {"uploadColumn", Transfer<true, true>},
     {"downloadColumn", Transfer<false, true>}, {"step", Step}, {"sync", Sync}, {"diag", Doubles<ml_run_diag>},
     {"levels", Doubles<ml_run_levels>}, {"flashes", Flashes}, {"setTime", SetTime}, {"dynamics", Part<ml_dynamics>},
     {"physics", Part<ml_physics>}, {"mix", Part<ml_mix>}, {"afterStep", Part<ml_after_step>}, {"fallParts", FallParts},
-    {"setCounts", SetCounts},
+    {"setTotals", SetTotals}, {"rotation", Doubles<ml_run_rotation>}, {"refreshRotation", RunOp<ml_run_refresh_rotation>},
   };
   for (auto &f : fns) {
     napi_value v;

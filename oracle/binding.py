@@ -21,6 +21,8 @@ EXTRA = ["fieldZ", "fallRain", "fallSnow", "fallGraupel", "fallHail", "fallCharg
 FIELDS = CARRIED + EXTRA
 FIELD_ID = {name: k for k, name in enumerate(FIELDS)}
 COLUMNS = ["precipitation", "hailGround", "hailSize"]
+# RotationState (model.ts:142-158) in the order orc_rotation writes it.
+ROTATION = ["uh", "x", "y", "anticyclonic", "persisted", "uh01", "uh03", "low", "coreZeta", "coreUH", "coreAnti"]
 DIAG = ["time", "maxW", "maxThetaP", "rainTotal", "firstFlash", "flashesIC", "flashesCG", "clipped", "negativeFilled",
         "chargeGround", "minW", "water"]
 
@@ -74,6 +76,8 @@ def lib():
         L.orc_diag.argtypes = [vp, dp]
         L.orc_pressure.argtypes = [vp, dp]
         L.orc_set_time.argtypes = [vp, d]
+        L.orc_rotation.argtypes = [vp, dp]
+        L.orc_update_rotation.argtypes = [vp, d, i]
         for name in ("orc_weno", "orc_upwind5"):
             getattr(L, name).restype = d
             getattr(L, name).argtypes = [d] * 5
@@ -195,6 +199,16 @@ class OracleRun:
 
     def set_time(self, t):
         lib().orc_set_time(self.h, t)
+
+    def rotation(self):
+        """The rotation diagnostics of the last step (updateRotation), by the names of ROTATION."""
+        out = np.empty(len(ROTATION), np.float64)
+        lib().orc_rotation(self.h, dptr(out))
+        return dict(zip(ROTATION, out))
+
+    def update_rotation(self, dt=0.0, keep=True):
+        """updateRotation(dt) on the fields as they are; keep: refreshRotation (how long it held stays)."""
+        lib().orc_update_rotation(self.h, dt, 1 if keep else 0)
 
     def state(self):
         return {name: self.field(name) for name in CARRIED}

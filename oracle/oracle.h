@@ -71,6 +71,15 @@ void orc_diag(void *model, double *out);
 /* The model's projection pressure (cells) after the last step, (nz-1) * layer float64. */
 void orc_pressure(void *model, double *out);
 void orc_set_time(void *model, double t);
+/* Rotation diagnostics of the last step (updateRotation, model.ts:1609-1648), ORC_ROT float64 in this order. */
+enum {
+  ORC_ROT_UH = 0, ORC_ROT_X, ORC_ROT_Y, ORC_ROT_ANTI, ORC_ROT_PERSISTED, ORC_ROT_UH01, ORC_ROT_UH03, ORC_ROT_LOW,
+  ORC_ROT_CORE_ZETA, ORC_ROT_CORE_UH, ORC_ROT_CORE_ANTI, ORC_ROT
+};
+void orc_rotation(void *model, double *out);
+/* updateRotation(dt) on the fields as they are; keep != 0 then restores how long the rotation had held
+ * (refreshRotation, model.ts:1607). */
+void orc_update_rotation(void *model, double dt, int keep);
 
 /* Known-answer surface of the pieces (for tests). */
 double orc_weno(double a, double b, double c, double d, double e);

@@ -964,6 +964,9 @@ struct Model {
   double time = 0, negativeFilled = 0, chargeGround = 0, firstFlash = -1; long clipped = 0; int flashesIC = 0, flashesCG = 0;
   bool electrified = true;
   double cs, dragCd, f_coriolis;
+  // RotationState (model.ts:142-158) in the order of ORC_ROT_*, and the levels updateRotation sums over.
+  double rotation[ORC_ROT] = {0};
+  int uhZ0 = 0, uhZ1 = -1, top1 = -1, top3 = -1;
 
   Model(const orc_grid &go, const orc_profile &pr, const orc_run &r) : g(go), env(g, pr), run(r), flux(g, env.rho), solver(g, env.rho) {
     for (int k = 0; k < ORC_FIELDS; k++) f[k].assign(g.n, 0.0f);
@@ -976,6 +979,14 @@ struct Model {
     cs = .18 * run.turbulence / .55;
     dragCd = std::pow(.4 / std::log(2 * g.hz[0] / 4 / run.roughness), 2);
     f_coriolis = 2 * OMEGA * std::sin(run.latitude * PI / 180);
+    // model.ts:389 (uhLevels) and :1612 (the last levels at or below 1 and 3 km).
+    uhZ0 = g.nz;
+    for (int z = 0; z < g.nz; z++) {
+      if (g.zs[z] >= 2000 && uhZ0 == g.nz) uhZ0 = z;
+      if (g.zs[z] <= 5000) uhZ1 = z;
+      if (g.zs[z] <= 1000) top1 = z;
+      if (g.zs[z] <= 3000) top3 = z;
+    }
     initialize();
   }
   float *F(int k) { return f[k].data(); }
@@ -1256,6 +1267,59 @@ struct Model {
     for (int z = 1; z < g.nz - 1; z++) { double h = g.zs[z + 1] - g.zs[z - 1]; for (int c = 0, i = z * g.layer; c < g.layer; c++, i++) ez[i] = float(-(phi[i + g.layer] - phi[i - g.layer]) / h); }
   }
   // step (model.ts:873-877) = beginStep, dynamics, endStep.
+  // zeta (model.ts:1598-1601): vertical vorticity at a node from the periodic neighbours.
+  double zeta(int x, int y, int z) const {
+    const int nx = g.nx, l = z * g.layer, xp = (x + 1) % nx, xm = (x + nx - 1) % nx, yp = ((y + 1) % g.ny) * nx, ym = ((y + g.ny - 1) % g.ny) * nx;
+    const float *u = f[ORC_U].data(), *v = f[ORC_V].data();
+    return (double(v[xp + nx * y + l]) - double(v[xm + nx * y + l])) / (2 * g.dx) - (double(u[x + yp + l]) - double(u[x + ym + l])) / (2 * g.dy);
+  }
+  // updateRotation (model.ts:1609-1648): the columns' updraught helicity, the updraught cores by a depth-first fill,
+  // and how long the core's rotation has exceeded mesoCoreZeta (model.ts:129).
+  void updateRotation(double dt) {
+    const int nx = g.nx, ny = g.ny, layer = g.layer;
+    const float *w = f[ORC_W].data();
+    std::vector<float> uhc(layer);
+    std::vector<double> coreW(layer), coreWZ(layer);
+    double max = 0, min = 0, max01 = 0, max03 = 0; int bx = 0, by = 0, low01 = 0;
+    for (int y = 0; y < ny; y++) for (int x = 0; x < nx; x++) {
+      double uh = 0, low = 0, sw = 0, swz = 0;
+      for (int z = 1; z <= top3; z++) { low += double(w[x + nx * y + z * layer]) * zeta(x, y, z) * g.hz[z]; if (z == top1 && low > max01) { max01 = low; low01 = x + nx * y; } }
+      max03 = std::max(max03, low);
+      for (int z = uhZ0; z <= uhZ1; z++) {
+        const double wn = w[x + nx * y + z * layer], wz = wn * zeta(x, y, z) * g.hz[z];
+        uh += wz;
+        if (wn > 10) { sw += wn * g.hz[z]; swz += wz; }
+      }
+      uhc[x + nx * y] = float(uh); coreW[x + nx * y] = sw; coreWZ[x + nx * y] = swz;
+      if (uh > max) { max = uh; bx = x; by = y; }
+      min = std::min(min, uh);
+    }
+    std::vector<unsigned char> label(layer, 0);
+    std::vector<int> stack;
+    double coreZeta = 0, coreUH = 0, coreAnti = 0;
+    for (int start = 0; start < layer; start++) {
+      if (coreW[start] <= 0 || label[start]) continue;
+      double sw = 0, swz = 0, peak = 0;
+      label[start] = 1; stack.push_back(start);
+      while (!stack.empty()) {
+        const int a = stack.back(); stack.pop_back();
+        const int x = a % nx, y = (a - x) / nx;
+        sw += coreW[a]; swz += coreWZ[a]; peak = std::max(peak, double(uhc[a]));
+        for (int oy = -1; oy <= 1; oy++) for (int ox = -1; ox <= 1; ox++) {
+          const int b = (x + ox + nx) % nx + nx * ((y + oy + ny) % ny);
+          if (coreW[b] > 0 && !label[b]) { label[b] = 1; stack.push_back(b); }
+        }
+      }
+      if (swz / sw > coreZeta) { coreZeta = swz / sw; coreUH = peak; }
+      coreAnti = std::max(coreAnti, -swz / sw);
+    }
+    const double meso = 3e-3 * std::max(1.0, 1200 / g.dx);
+    double *r = rotation;
+    r[ORC_ROT_UH] = max; r[ORC_ROT_X] = bx; r[ORC_ROT_Y] = by; r[ORC_ROT_ANTI] = -min;
+    r[ORC_ROT_PERSISTED] = coreZeta >= meso ? r[ORC_ROT_PERSISTED] + dt : 0;
+    r[ORC_ROT_UH01] = max01; r[ORC_ROT_UH03] = max03; r[ORC_ROT_LOW] = low01;
+    r[ORC_ROT_CORE_ZETA] = coreZeta; r[ORC_ROT_CORE_UH] = coreUH; r[ORC_ROT_CORE_ANTI] = coreAnti;
+  }
   void step(double dt) {
     fallPhase();
     collectHail(dt);
@@ -1264,6 +1328,7 @@ struct Model {
     physicsStep(dt);
     mixStep(dt);
     project(dt, true);
+    updateRotation(dt);
     if (electrified) discharge(dt);
   }
   void diag(double *out) {
@@ -1303,7 +1368,7 @@ int orc_step_part(void *m, int part, double dt) {
     case 0: mm->fallPhase(); mm->collectHail(dt); mm->sedimentStep(dt); mm->dynamics(dt); return 0;
     case 1: mm->physicsStep(dt); return 0;
     case 2: mm->mixStep(dt); mm->project(dt, true); return 0;
-    case 3: if (mm->electrified) mm->discharge(dt); mm->time += dt; return 0;
+    case 3: mm->updateRotation(dt); if (mm->electrified) mm->discharge(dt); mm->time += dt; return 0;
     default: return -1;
   }
 }
@@ -1332,6 +1397,13 @@ void orc_grid_levels(void *m, double *zs, double *hz) { Model *mm = static_cast<
 void orc_diag(void *m, double *out) { static_cast<Model *>(m)->diag(out); }
 void orc_pressure(void *m, double *out) { Model *mm = static_cast<Model *>(m); std::memcpy(out, mm->pressure.data(), sizeof(double) * mm->pressure.size()); }
 void orc_set_time(void *m, double t) { static_cast<Model *>(m)->time = t; }
+void orc_update_rotation(void *m, double dt, int keep) {
+  Model *mm = static_cast<Model *>(m);
+  const double held = mm->rotation[ORC_ROT_PERSISTED];
+  mm->updateRotation(dt);
+  if (keep) mm->rotation[ORC_ROT_PERSISTED] = held;
+}
+void orc_rotation(void *m, double *out) { std::memcpy(out, static_cast<Model *>(m)->rotation, sizeof(double) * ORC_ROT); }
 
 double orc_weno(double a, double b, double c, double d, double e) { return weno(a, b, c, d, e); }
 double orc_upwind5(double a, double b, double c, double d, double e) { return upwind5(a, b, c, d, e); }

@@ -18,7 +18,8 @@ NODE = shutil.which("node")
 needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reason="node or the built addon is absent")
 ABI_FUNCTIONS = ["ctxCreate", "ctxDestroy", "deviceName", "poolCreate", "poolDestroy", "slotBytes", "runCreate", "runRelease",
                  "upload", "download", "uploadColumn", "downloadColumn", "step", "sync", "diag", "levels", "flashes", "setTime",
-                 "dynamics", "physics", "mix", "afterStep", "fallParts", "setCounts"]
+                 "dynamics", "physics", "mix", "afterStep", "fallParts", "setTotals",
+                 "rotation", "refreshRotation"]
 
 
 def node(code):

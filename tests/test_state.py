@@ -35,7 +35,7 @@ class FakeRun:
         rng = np.random.default_rng(seed)
         self.fields = {k: rng.standard_normal(n).astype(np.float32) for k in STATE_FIELDS}
         self.columns = {k: rng.standard_normal(layer).astype(np.float32) for k in STATE_COLUMNS}
-        self.time, self.counts = 612.0, (1.5, 3, 1)
+        self.time, self.counts = 612.0, (1.5, 3, 1, 90.0)
 
     def diagnostics(self):
         return {"time": self.time, "chargeGround": self.counts[0], "flashesIC": self.counts[1], "flashesCG": self.counts[2]}
@@ -55,8 +55,15 @@ class FakeRun:
     def set_time(self, t):
         self.time = t
 
-    def set_counts(self, q, ic, cg):
-        self.counts = (q, ic, cg)
+    def rotation(self):
+        return {"persisted": self.counts[3]}
+
+    def set_totals(self, q, ic, cg, persisted):
+        self.counts = (q, ic, cg, persisted)
+        self.refreshed = False
+
+    def refresh_rotation(self):
+        self.refreshed = True
 
 
 def test_saved_layout_is_the_references():
@@ -68,7 +75,7 @@ def test_saved_layout_is_the_references():
     header = json.loads(buf[8:8 + length])
     assert header["format"] == "meteolab-state" and header["version"] == 5 and header["config"] == {"seed": 7}
     assert (header["nx"], header["ny"], header["nz"], header["time"]) == (g["nx"], g["ny"], g["nz"], 612.0)
-    assert header["flashCount"] == {"ic": 3, "cg": 1} and header["chargeGround"] == 1.5
+    assert header["flashCount"] == {"ic": 3, "cg": 1} and header["chargeGround"] == 1.5 and header["persisted"] == 90
     start = math.ceil((8 + length) / 4) * 4
     n, layer = g["nx"] * g["ny"] * g["nz"], g["nx"] * g["ny"]
     assert len(buf) == start + (22 * n + 3 * layer) * 4
@@ -89,7 +96,7 @@ def test_apply_puts_fields_columns_time_and_counts_into_the_run():
         assert np.array_equal(dst.fields[k], src.fields[k]), k
     for k in STATE_COLUMNS:
         assert np.array_equal(dst.columns[k], src.columns[k]), k
-    assert dst.time == 612.0 and dst.counts == (1.5, 3, 1)
+    assert dst.time == 612.0 and dst.counts == (1.5, 3, 1, 90) and dst.refreshed
 
 
 def test_opens_a_version_1_file_without_charge():
@@ -102,7 +109,7 @@ def test_opens_a_version_1_file_without_charge():
     run.fields["chargeIce"][:] = 5
     apply_state(run, read_state(buf))
     assert run.time == 60 and run.fields["hail"][0] == 1 and not run.fields["chargeIce"].any()
-    assert run.counts == (0.0, 0, 0)
+    assert run.counts == (0.0, 0, 0, 0)
     assert FIELD_COUNT[1] == 12
 
 
@@ -139,6 +146,7 @@ const s = m.readState(fs.readFileSync('{py}'))
 const g = s.header, n = g.nx * g.ny * g.nz, layer = g.nx * g.ny
 const fake = {{ pool: {{ grid: {{ nx: g.nx, ny: g.ny, nz: g.nz }}, n, layer }}, spec: {{ seed: 3 }},
   diagnostics: () => ({{ time: g.time, chargeGround: g.chargeGround, flashesIC: g.flashCount.ic, flashesCG: g.flashCount.cg }}),
+  rotation: () => ({{ persisted: g.persisted }}),
   field: k => s.data.slice(m.STATE_FIELDS.indexOf(k) * n, (m.STATE_FIELDS.indexOf(k) + 1) * n),
   column: k => s.data.slice(22 * n + m.STATE_COLUMNS.indexOf(k) * layer, 22 * n + (m.STATE_COLUMNS.indexOf(k) + 1) * layer) }}
 fs.writeFileSync('{tmp_path / "node.meteolab"}', m.saveState(fake, {{ seed: 3 }}))
@@ -168,6 +176,8 @@ def test_a_run_continued_from_its_saved_state_steps_exactly_as_the_original():
         restored = ctx.pool(g, 1).run(C.profile(), C.run(g, seed=7))
         apply_state(restored, state)
         assert restored.diagnostics()["time"] == 600.0
+        # The rotation diagnostics are recomputed from the restored fields: those the last step left.
+        assert restored.rotation() == original.rotation()
         original.step(5)
         restored.step(5)
         for name in ("w", "theta", "rain", "q"):
