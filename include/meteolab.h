@@ -41,6 +41,12 @@ enum {
   ML_D_NEGFILLED, ML_D_CHARGEGROUND, ML_D_FLASHES_IC, ML_D_FLASHES_CG, ML_D_FIRSTFLASH, ML_D_ERROR, ML_D_COUNT
 };
 
+/* AtmosphereModel.diagnostics() of a run (ml_model_diagnostics; ModelDiagnostics, model.ts:161-180), float64. */
+enum {
+  ML_MD_UPDRAFT = 0, ML_MD_DOWNDRAFT, ML_MD_RAINRATE, ML_MD_RAINTOTAL, ML_MD_CLOUDTOP, ML_MD_THERMALTOP, ML_MD_MAXCLOUD,
+  ML_MD_COLDMAX, ML_MD_CORES, ML_MD_SHEAR06, ML_MD_OUTFLOW, ML_MD_OUTFLOWLOW, ML_MD_GUST, ML_MD_HAIL, ML_MD_CLIPPED,
+  ML_MD_COUNT
+};
 /* Rotation diagnostics of a run (ml_run_rotation). */
 enum {
   ML_ROT_UH = 0, ML_ROT_X, ML_ROT_Y, ML_ROT_ANTICYCLONIC, ML_ROT_PERSISTED, ML_ROT_UH01, ML_ROT_UH03, ML_ROT_LOW,
@@ -126,6 +132,15 @@ int ml_run_rotation(ml_ctx *ctx, int run, double *out);
 /* Recomputes the rotation diagnostics from the run's fields as they are, keeping how long the criterion has held
  * (refreshRotation, model.ts:1607; applyState calls it, src/core/state.ts:73-74). */
 int ml_run_refresh_rotation(ml_ctx *ctx, int run);
+/* AtmosphereModel.diagnostics() (model.ts:1672-1696) of `count` runs of one pool from their fields as they are,
+ * ML_MD_COUNT float64 a run into out (count * ML_MD_COUNT): the largest up- and downdraught (m/s), the heaviest
+ * precipitation reaching the ground (mm/h, surfaceflux.ts:22-37 at the ground nodes) and the largest rain total (mm),
+ * the cloud top and the top of w > 0.6 m/s (km), the most cloud water + ice (kg/kg), the largest cold-pool indicator
+ * below 1.5 km, the cloudy updraught cores (countCores, model.ts:1650-1670), the environment's 0-6 km shear (m/s), the
+ * outflow's largest virtual-temperature deficit and the largest within 2 km of the low-level rotation (K, rkw.ts:72-90),
+ * the strongest ground wind (m/s), the largest hailstones at the ground (mm), and the wind components clipped since the
+ * start. One launch for the runs, the ensemble's sampling of every member at once. */
+int ml_model_diagnostics(ml_ctx *ctx, const int *runs, int count, double *out);
 
 /* Advances `count` runs of one pool by `steps` model steps of dt s each: AtmosphereModel.step(dt) then time += dt
  * (model.ts:873-877 and its callers, e.g. tests/bench/threads.ts:38-42). One launch per kernel covers all the runs;

@@ -24,6 +24,9 @@ COLUMNS = ["precipitation", "hailGround", "hailSize"]
 LEVELS = ["p", "exner", "theta", "q", "u", "v", "rho"]
 DIAG = ["time", "maxW", "minW", "maxThetaP", "rainTotal", "water", "charge", "clipped", "negativeFilled",
         "chargeGround", "flashesIC", "flashesCG", "firstFlash", "error"]
+# ModelDiagnostics (model.ts:161-180) in ml_model_diagnostics' order.
+MODEL_DIAGNOSTICS = ["updraft", "downdraft", "rainRate", "rainTotal", "cloudTop", "thermalTop", "maxCloud", "coldMax",
+                     "cores", "shear06", "outflow", "outflowLow", "gust", "hail", "clipped"]
 # RotationState (model.ts:142-158) in ml_run_rotation's order.
 ROTATION = ["uh", "x", "y", "anticyclonic", "persisted", "uh01", "uh03", "low", "coreZeta", "coreUH", "coreAnti"]
 # STATE_FIELDS of src/core/state.ts:9 that this step carries (two-moment numbers are not part of the baseline step).
@@ -78,6 +81,7 @@ SIGNATURES = {
     "ml_run_set_totals": (_i, [_vp, _i, _d, _i64, _i64, _d]),
     "ml_run_rotation": (_i, [_vp, _i, _dp]),
     "ml_run_refresh_rotation": (_i, [_vp, _i]),
+    "ml_model_diagnostics": (_i, [_vp, _ip, _i, _dp]),
     "ml_run_flashes": (_i, [_vp, _i, _dp, _i]),
     "ml_step": (_i, [_vp, _ip, _i, _i, _d]),
     "ml_dynamics": (_i, [_vp, _ip, _i, _d]),
@@ -301,6 +305,20 @@ class Run:
     def step(self, n=1, dt=3.0):
         step([self], n, dt)
         self.ctx.sync()
+
+    def model_diagnostics(self):
+        """AtmosphereModel.diagnostics() of the run's fields as they are (ml_model_diagnostics)."""
+        return model_diagnostics([self])[0]
+
+
+def model_diagnostics(runs):
+    """AtmosphereModel.diagnostics() of runs of one pool in one launch (ml_model_diagnostics): a dict by the names of
+    MODEL_DIAGNOSTICS per run."""
+    ctx = runs[0].ctx
+    ids = (ctypes.c_int * len(runs))(*[r.h for r in runs])
+    out = np.empty((len(runs), len(MODEL_DIAGNOSTICS)), np.float64)
+    ctx.check(lib().ml_model_diagnostics(ctx.h, ids, len(runs), _dptr(out)))
+    return [dict(zip(MODEL_DIAGNOSTICS, row)) for row in out]
 
 
 def step(runs, n=1, dt=3.0):

@@ -22,6 +22,7 @@
 #include "lightning.h"
 #include "spectral.h"
 #include "rotation.h"
+#include "diagnostics.h"
 
 using namespace ml;
 
@@ -57,7 +58,9 @@ struct Pool {
   int chunk = 1;          // levels per pass of the discharge's own electric solve (as many as the block's LDS holds)
   RotLevels rot{};        // the rotation diagnostics' levels and threshold
   int *single = nullptr;  // single[s] = s: the run list of one slot
-  double *diag = nullptr; // [cap][N_DIAG_DEV]
+  DiagLevels dlev{};      // the levels the model diagnostics look at
+  double *diag = nullptr; // [cap][DIAG_ROW]: rows of diagnostics
+  int *dlist = nullptr;   // [cap]: the run list of a diagnostics call
   std::vector<double> zsH, dzsH, hzH;
   std::vector<char> used;
   // Captured steps by batch size: the graph of a size is reused for any runs of the pool, its device run list rewritten
@@ -336,8 +339,8 @@ namespace {
 void freePool(Pool *p) {
   dropBatches(p);
   void *all[] = {p->fields, p->stx, p->sty, p->mass, p->phi, p->cells, p->press, p->spec, p->mean0, p->km, p->kh, p->ephi, p->cols,
-                 p->lev, p->par, p->scal, p->zs, p->dzs, p->hz, p->sym, p->twx, p->twy, p->single, p->diag, p->lrho, p->lmark,
-                 p->lnode, p->chan, p->cdep, p->flog, p->efac, p->ehat};
+                 p->lev, p->par, p->scal, p->zs, p->dzs, p->hz, p->sym, p->twx, p->twy, p->single, p->diag, p->dlist, p->lrho,
+                 p->lmark, p->lnode, p->chan, p->cdep, p->flog, p->efac, p->ehat};
   for (void *a : all) if (a) hipFree(a);
   delete p;
 }
@@ -397,7 +400,7 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
     {&p->fields, s.fields * cap}, {&p->stx, s.stage * cap}, {&p->sty, s.stage * cap}, {&p->mass, s.mass * cap}, {&p->phi, s.node8 * cap},
     {&p->cells, s.node8 * cap}, {&p->press, s.node8 * cap}, {&p->spec, s.spec * cap}, {&p->ehat, s.spec * cap}, {&p->mean0, s.mean0 * cap}, {&p->km, s.node * cap},
     {&p->kh, s.node * cap}, {&p->ephi, s.node8 * cap}, {&p->cols, s.cols * cap}, {&p->lev, s.lev * cap}, {&p->par, s.par * cap},
-    {&p->scal, s.scal * cap}, {(void **)&p->diag, cap * N_DIAG_DEV * 8}, {&p->lrho, s.node * cap}, {&p->lmark, s.lmark * cap},
+    {&p->scal, s.scal * cap}, {(void **)&p->diag, cap * DIAG_ROW * 8}, {(void **)&p->dlist, cap * sizeof(int)}, {&p->lrho, s.node * cap}, {&p->lmark, s.lmark * cap},
     {&p->lnode, s.lnode * cap}, {&p->chan, s.chan * cap}, {&p->cdep, s.cdep * cap}, {&p->flog, s.flog * cap}};
   for (auto &a : allocs) {
     hipError_t e = hipMalloc(a.ptr, a.bytes);
@@ -453,6 +456,23 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
     if (size_t(rotLds) > ldsMax) { freePool(p); return fail(c, ML_E_UNSUPPORTED, "grid too large for the rotation diagnostics' columns in LDS"); }
     if (p->rb == 8) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_rotation<double>), hipFuncAttributeMaxDynamicSharedMemorySize, rotLds));
     else HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_rotation<float>), hipFuncAttributeMaxDynamicSharedMemorySize, rotLds));
+  }
+  // The model diagnostics: cores between the first level at or above 1.9 km and the last at or below 6.6 km, at most
+  // nz - 3 (model.ts:1653); the cold pool below 1.5 km (:1680).
+  {
+    DiagLevels &d = p->dlev;
+    d.coreLo = g.nz; d.coreHi = -1; d.coldTop = 0;
+    for (int z = 0; z < g.nz; z++) {
+      const double h = p->zsH[z];
+      if (h >= 1900 && d.coreLo == g.nz) d.coreLo = z;
+      if (h <= 6600) d.coreHi = z;
+      if (h < 1500) d.coldTop = z + 1;
+    }
+    d.coreHi = std::min(g.nz - 3, d.coreHi);
+    const int diagLds = int(modelDiagLdsBytes(g));
+    if (size_t(diagLds) > ldsMax) { freePool(p); return fail(c, ML_E_UNSUPPORTED, "grid too large for the diagnostics' columns in LDS"); }
+    if (p->rb == 8) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_model_diag<double>), hipFuncAttributeMaxDynamicSharedMemorySize, diagLds));
+    else HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_model_diag<float>), hipFuncAttributeMaxDynamicSharedMemorySize, diagLds));
   }
   // The transport layouts' level planes in LDS (the plane layout only where it fits; the autotuner skips it otherwise).
   const int levelLds = int(stageLevelLdsBytes(g, p->rb)), planeLds = int(stagePlaneLdsBytes(g, p->rb));
@@ -631,6 +651,31 @@ int ml_run_diag(ml_ctx *c, int run, double *out) {
   out[ML_D_RAINTOTAL] = d[D_RAINTOTAL]; out[ML_D_WATER] = d[D_WATER]; out[ML_D_CHARGE] = d[D_CHARGE]; out[ML_D_CLIPPED] = i64(S_CLIPPED);
   out[ML_D_NEGFILLED] = sc[S_NEGATIVE]; out[ML_D_CHARGEGROUND] = sc[S_CHARGE_GROUND]; out[ML_D_FLASHES_IC] = i64(S_FLASH_IC);
   out[ML_D_FLASHES_CG] = i64(S_FLASH_CG); out[ML_D_FIRSTFLASH] = sc[S_FIRST_FLASH]; out[ML_D_ERROR] = i64(S_ERROR);
+  return 0;
+}
+
+int ml_model_diagnostics(ml_ctx *c, const int *runs, int count, double *out) {
+  if (!c || !runs || count < 1 || !out) return fail(c, ML_E_ARG, "model diagnostics: no runs");
+  Pool *p = nullptr;
+  std::vector<int> slots(count);
+  for (int k = 0; k < count; k++) {
+    Pool *q; int slot;
+    if (!decode(c, runs[k], q, slot)) return fail(c, ML_E_ARG, "model diagnostics: bad run");
+    if (p && q != p) return fail(c, ML_E_ARG, "model diagnostics: runs of one pool per call");
+    p = q; slots[k] = slot;
+  }
+  if (count > p->cap) return fail(c, ML_E_ARG, "model diagnostics: more runs than the pool holds");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(p->dlist, slots.data(), count * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  const size_t lds = modelDiagLdsBytes(p->g);
+  if (p->rb == 8) hipLaunchKernelGGL(k_model_diag<double>, dim3(count), dim3(MDIAG_THREADS), lds, c->stream, p->g, bufs<double>(*p, p->dlist), p->dlev, p->diag);
+  else hipLaunchKernelGGL(k_model_diag<float>, dim3(count), dim3(MDIAG_THREADS), lds, c->stream, p->g, bufs<float>(*p, p->dlist), p->dlev, p->diag);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> rows(size_t(count) * DIAG_ROW);
+  HIPCHK(c, hipMemcpyAsync(rows.data(), p->diag, rows.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  static_assert(N_MD == ML_MD_COUNT, "the ABI's diagnostics record is the kernel's row");
+  for (int k = 0; k < count; k++) std::memcpy(out + size_t(k) * ML_MD_COUNT, rows.data() + size_t(k) * DIAG_ROW, ML_MD_COUNT * 8);
   return 0;
 }
 

@@ -23,6 +23,9 @@ const DIAG = ['time', 'maxW', 'minW', 'maxThetaP', 'rainTotal', 'water', 'charge
   'chargeGround', 'flashesIC', 'flashesCG', 'firstFlash', 'error']
 const LEVELS = ['p', 'exner', 'theta', 'q', 'u', 'v', 'rho']
 // RotationState (model.ts:142-158) in ml_run_rotation's order.
+// ModelDiagnostics (model.ts:161-180) in ml_model_diagnostics' order.
+const MODEL_DIAGNOSTICS = ['updraft', 'downdraft', 'rainRate', 'rainTotal', 'cloudTop', 'thermalTop', 'maxCloud', 'coldMax',
+  'cores', 'shear06', 'outflow', 'outflowLow', 'gust', 'hail', 'clipped']
 const ROTATION = ['uh', 'x', 'y', 'anticyclonic', 'persisted', 'uh01', 'uh03', 'low', 'coreZeta', 'coreUH', 'coreAnti']
 // The parts of a step where the reference's GPU step splits it (src/gpu/step.ts:12-31).
 const PARTS = ['dynamics', 'physics', 'mix', 'afterStep']
@@ -78,6 +81,12 @@ class MeteolabPool {
     const ids = Int32Array.from(runs.map(r => r.handle))
     this.device.call(() => addon[name](this.device.ctx, ids, dt))
   }
+  /** AtmosphereModel.diagnostics() of each run (one launch for all): an array of ModelDiagnostics objects. */
+  diagnostics (runs) {
+    const ids = Int32Array.from(runs.map(r => r.handle)), out = new Float64Array(runs.length * MODEL_DIAGNOSTICS.length)
+    this.device.call(() => addon.modelDiagnostics(this.device.ctx, ids, out))
+    return runs.map((_, k) => Object.fromEntries(MODEL_DIAGNOSTICS.map((name, j) => [name, out[k * MODEL_DIAGNOSTICS.length + j]])))
+  }
   destroy () { if (this.id !== null) { this.device.call(() => addon.poolDestroy(this.device.ctx, this.id)); this.id = null } }
 }
 
@@ -108,7 +117,9 @@ class MeteolabRun {
     if (a.length !== this.pool.layer) throw new Error(`${name}: ${a.length} values for ${this.pool.layer} columns`)
     this.device.call(() => addon.uploadColumn(this.device.ctx, this.handle, COLUMNS.indexOf(name), a))
   }
-  /** model.ts:1672-1696 diagnostics and the run metrics (max w, max theta', rain total, flashes, first flash). */
+  /** AtmosphereModel.diagnostics() (model.ts:1672-1696) of the run's fields as they are: a ModelDiagnostics object. */
+  modelDiagnostics () { return this.pool.diagnostics([this])[0] }
+  /** The run metrics (ml_run_diag): time, max w, max theta', rain total, water, charge, flashes, first flash. */
   diagnostics () {
     const out = new Float64Array(DIAG.length)
     this.device.call(() => addon.diag(this.device.ctx, this.handle, out))
@@ -282,4 +293,4 @@ function applyState (r, state) {
 }
 
 module.exports = { MeteolabDevice, MeteolabPool, MeteolabRun, GpuDynamics, FIELDS, CARRIED, COLUMNS, DIAG, PARTS, STANDARD, LARGE, profile, run, addon,
-  STATE_FIELDS, STATE_COLUMNS, ROTATION, saveState, readState, applyState }
+  STATE_FIELDS, STATE_COLUMNS, ROTATION, MODEL_DIAGNOSTICS, saveState, readState, applyState }

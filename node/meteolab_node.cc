@@ -182,6 +182,17 @@ napi_value Step(napi_env env, napi_callback_info info) {
   return status(env, ctx, ml_step(ctx, static_cast<const int *>(raw), int(len), steps, dt));
 }
 
+// modelDiagnostics(ctx, Int32Array runs, Float64Array out): ml_model_diagnostics, ML_MD_COUNT values a run.
+napi_value ModelDiagnostics(napi_env env, napi_callback_info info) {
+  napi_value a[3]; if (!args(env, info, 3, a)) return nullptr;
+  ml_ctx *ctx = ctxOf(env, a[0]); if (!ctx) return nullptr;
+  napi_typedarray_type t; size_t len; void *raw; napi_value buf; size_t off;
+  if (napi_get_typedarray_info(env, a[1], &t, &len, &raw, &buf, &off) != napi_ok || t != napi_int32_array) { napi_throw_type_error(env, nullptr, "runs: Int32Array"); return nullptr; }
+  napi_typedarray_type t2; size_t len2; void *raw2;
+  if (napi_get_typedarray_info(env, a[2], &t2, &len2, &raw2, &buf, &off) != napi_ok || t2 != napi_float64_array || len2 < len * ML_MD_COUNT) { napi_throw_type_error(env, nullptr, "out: Float64Array of runs x ML_MD_COUNT"); return nullptr; }
+  return status(env, ctx, ml_model_diagnostics(ctx, static_cast<const int *>(raw), int(len), static_cast<double *>(raw2)));
+}
+
 // dynamics / physics / mix / afterStep (ctx, Int32Array runs, dt): one part of a step of a batch (ml_dynamics,
 // ml_physics, ml_mix, ml_after_step; asynchronous).
 template <int (*F)(ml_ctx *, const int *, int, double)> napi_value Part(napi_env env, napi_callback_info info) {
@@ -253,6 +264,7 @@ napi_value Init(napi_env env, napi_value exports) {
     {"levels", Doubles<ml_run_levels>}, {"flashes", Flashes}, {"setTime", SetTime}, {"dynamics", Part<ml_dynamics>},
     {"physics", Part<ml_physics>}, {"mix", Part<ml_mix>}, {"afterStep", Part<ml_after_step>}, {"fallParts", FallParts},
     {"setTotals", SetTotals}, {"rotation", Doubles<ml_run_rotation>}, {"refreshRotation", RunOp<ml_run_refresh_rotation>},
+    {"modelDiagnostics", ModelDiagnostics},
   };
   for (auto &f : fns) {
     napi_value v;

@@ -21,6 +21,9 @@ EXTRA = ["fieldZ", "fallRain", "fallSnow", "fallGraupel", "fallHail", "fallCharg
 FIELDS = CARRIED + EXTRA
 FIELD_ID = {name: k for k, name in enumerate(FIELDS)}
 COLUMNS = ["precipitation", "hailGround", "hailSize"]
+# ModelDiagnostics (model.ts:161-180) in the order orc_model_diagnostics writes it.
+MODEL_DIAGNOSTICS = ["updraft", "downdraft", "rainRate", "rainTotal", "cloudTop", "thermalTop", "maxCloud", "coldMax",
+                     "cores", "shear06", "outflow", "outflowLow", "gust", "hail", "clipped"]
 # RotationState (model.ts:142-158) in the order orc_rotation writes it.
 ROTATION = ["uh", "x", "y", "anticyclonic", "persisted", "uh01", "uh03", "low", "coreZeta", "coreUH", "coreAnti"]
 DIAG = ["time", "maxW", "maxThetaP", "rainTotal", "firstFlash", "flashesIC", "flashesCG", "clipped", "negativeFilled",
@@ -78,6 +81,11 @@ def lib():
         L.orc_set_time.argtypes = [vp, d]
         L.orc_rotation.argtypes = [vp, dp]
         L.orc_update_rotation.argtypes = [vp, d, i]
+        L.orc_model_diagnostics.argtypes = [vp, dp]
+        L.orc_outflow_deficit.argtypes = [vp, fp, dp]
+        L.orc_outflow_deficit.restype = d
+        L.orc_deficit_near.argtypes = [vp, fp, i]
+        L.orc_deficit_near.restype = d
         for name in ("orc_weno", "orc_upwind5"):
             getattr(L, name).restype = d
             getattr(L, name).argtypes = [d] * 5
@@ -205,6 +213,23 @@ class OracleRun:
         out = np.empty(len(ROTATION), np.float64)
         lib().orc_rotation(self.h, dptr(out))
         return dict(zip(ROTATION, out))
+
+    def model_diagnostics(self):
+        """AtmosphereModel.diagnostics() of the fields as they are, by the names of MODEL_DIAGNOSTICS."""
+        out = np.empty(len(MODEL_DIAGNOSTICS), np.float64)
+        lib().orc_model_diagnostics(self.h, dptr(out))
+        return dict(zip(MODEL_DIAGNOSTICS, out))
+
+    def outflow_deficit(self):
+        """outflowDeficit (rkw.ts:72-80): (deficit per ground column as float32, warm, max)."""
+        deficit, warm = np.empty(self.layer, np.float32), ctypes.c_double()
+        mx = lib().orc_outflow_deficit(self.h, fptr(deficit), ctypes.byref(warm))
+        return deficit, warm.value, mx
+
+    def deficit_near(self, deficit, column):
+        """deficitNear (rkw.ts:82-90) within 2 km of a ground column."""
+        a = np.ascontiguousarray(deficit, dtype=np.float32)
+        return lib().orc_deficit_near(self.h, fptr(a), int(column))
 
     def update_rotation(self, dt=0.0, keep=True):
         """updateRotation(dt) on the fields as they are; keep: refreshRotation (how long it held stays)."""

@@ -80,6 +80,17 @@ void orc_rotation(void *model, double *out);
 /* updateRotation(dt) on the fields as they are; keep != 0 then restores how long the rotation had held
  * (refreshRotation, model.ts:1607). */
 void orc_update_rotation(void *model, double dt, int keep);
+/* AtmosphereModel.diagnostics() (model.ts:1672-1696; ModelDiagnostics :161-180), ORC_MD_COUNT float64 in this order. */
+enum {
+  ORC_MD_UPDRAFT = 0, ORC_MD_DOWNDRAFT, ORC_MD_RAINRATE, ORC_MD_RAINTOTAL, ORC_MD_CLOUDTOP, ORC_MD_THERMALTOP,
+  ORC_MD_MAXCLOUD, ORC_MD_COLDMAX, ORC_MD_CORES, ORC_MD_SHEAR06, ORC_MD_OUTFLOW, ORC_MD_OUTFLOWLOW, ORC_MD_GUST,
+  ORC_MD_HAIL, ORC_MD_CLIPPED, ORC_MD_COUNT
+};
+void orc_model_diagnostics(void *model, double *out);
+/* outflowDeficit (rkw.ts:72-80): the ground columns' deficits (layer float32, may be null) and the warm air's thv;
+ * returns the largest deficit. deficitNear (rkw.ts:82-90): the largest of `deficit` within 2 km of `column`. */
+double orc_outflow_deficit(void *model, float *deficit, double *warm);
+double orc_deficit_near(void *model, const float *deficit, int column);
 
 /* Known-answer surface of the pieces (for tests). */
 double orc_weno(double a, double b, double c, double d, double e);

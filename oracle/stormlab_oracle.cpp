@@ -1320,6 +1320,86 @@ struct Model {
     r[ORC_ROT_UH01] = max01; r[ORC_ROT_UH03] = max03; r[ORC_ROT_LOW] = low01;
     r[ORC_ROT_CORE_ZETA] = coreZeta; r[ORC_ROT_CORE_UH] = coreUH; r[ORC_ROT_CORE_ANTI] = coreAnti;
   }
+  // countCores (model.ts:1650-1670): cloudy updraught columns between ~1.9 and 6.6 km, 4-connected across the
+  // periodic edges.
+  int countCores() const {
+    const int nx = g.nx, ny = g.ny, nz = g.nz, layer = g.layer;
+    int lo = -1, last = -1;
+    for (int z = 0; z < nz; z++) { if (lo < 0 && g.zs[z] >= 1900) lo = z; if (g.zs[z] <= 6600) last = z; }
+    const int hi = std::min(nz - 3, last);
+    std::vector<unsigned char> mask(layer, 0);
+    for (int c = 0; c < layer; c++)
+      for (int z = std::max(lo, 0); lo >= 0 && z <= hi; z++) {
+        const int i = c + z * layer;
+        if (f[ORC_W][i] > 2 && double(f[ORC_CLOUD][i]) + double(f[ORC_ICE][i]) > .00007) { mask[c] = 1; break; }
+      }
+    int cores = 0; std::vector<int> stack;
+    for (int i = 0; i < layer; i++) {
+      if (mask[i] != 1) continue;
+      cores++; mask[i] = 2; stack.push_back(i);
+      while (!stack.empty()) {
+        const int a = stack.back(); stack.pop_back();
+        const int x = a % nx, y = a / nx;
+        const int nb[4] = {(x + 1) % nx + nx * y, (x - 1 + nx) % nx + nx * y, x + nx * ((y + 1) % ny), x + nx * ((y - 1 + ny) % ny)};
+        for (int b : nb) if (mask[b] == 1) { mask[b] = 2; stack.push_back(b); }
+      }
+    }
+    return cores;
+  }
+  // outflowDeficit (rkw.ts:72-80): the ground air's thv as float32, the warm air its element floor((layer-1) 0.75) in
+  // ascending order, the deficits as float32; returns the largest.
+  double outflowDeficit(std::vector<float> &deficit, double &warm) const {
+    const int layer = g.layer;
+    std::vector<float> thv(layer), sorted;
+    deficit.assign(layer, 0);
+    for (int c = 0; c < layer; c++) thv[c] = float(double(f[ORC_THETA][c]) * (1 + .61 * double(f[ORC_Q][c])));
+    sorted = thv; std::sort(sorted.begin(), sorted.end());
+    warm = sorted[size_t(std::floor((layer - 1) * .75))];
+    double max = 0;
+    for (int c = 0; c < layer; c++) { deficit[c] = float(std::max(0.0, warm - thv[c])); max = std::max(max, double(deficit[c])); }
+    return max;
+  }
+  // deficitNear (rkw.ts:82-90): the largest deficit within 2 km of column c, across the periodic edges.
+  double deficitNear(const std::vector<float> &deficit, int c) const {
+    const int nx = g.nx, ny = g.ny, x0 = c % nx, y0 = c / nx, rx = int(std::floor(2000 / g.dx)), ry = int(std::floor(2000 / g.dy));
+    double max = 0;
+    for (int j = -ry; j <= ry; j++) for (int i = -rx; i <= rx; i++) {
+      if ((i * g.dx) * (i * g.dx) + (j * g.dy) * (j * g.dy) > 2000.0 * 2000.0) continue;
+      max = std::max(max, double(deficit[((x0 + i) % nx + nx) % nx + nx * (((y0 + j) % ny + ny) % ny)]));
+    }
+    return max;
+  }
+  // diagnostics (model.ts:1672-1696) without terrain (the ground node of a column is its level 0), one-moment
+  // precipitation (surfaceflux.ts:22-37), the outflow deficit (rkw.ts:72-90), in the order of ORC_MD_*.
+  void diagnostics(double *out) const {
+    const int nz = g.nz, layer = g.layer;
+    const float *w = f[ORC_W].data(), *cloud = f[ORC_CLOUD].data(), *ice = f[ORC_ICE].data(), *cold = f[ORC_COLD].data();
+    double up = 0, down = 0, top = 0, coldMax = 0, thermalTop = 0, maxCloud = 0;
+    for (int z = 0, i = 0; z < nz; z++) for (int c = 0; c < layer; c++, i++) {
+      const double alt = g.zs[z], ci = double(cloud[i]) + double(ice[i]);
+      up = std::max(up, double(w[i])); down = std::max(down, -double(w[i])); maxCloud = std::max(maxCloud, ci);
+      if (ci > .00007) top = std::max(top, alt / 1000);
+      if (w[i] > .6) thermalTop = std::max(thermalTop, alt / 1000);
+      if (alt < 1500) coldMax = std::max(coldMax, double(cold[i]));
+    }
+    const double u6 = env.prof.shear * std::tanh(6000 / env.prof.shear_depth), u0 = env.prof.shear * std::tanh(0 / env.prof.shear_depth);
+    const double rho0 = env.rho[0];
+    double rainRate = 0, rainTotal = 0, gust = 0, hail = 0;
+    for (int c = 0; c < layer; c++) gust = std::max(gust, std::hypot(double(f[ORC_U][c]) + run.frame_u, double(f[ORC_V][c]) + run.frame_v));
+    for (int c = 0; c < layer; c++) {
+      const double r = f[ORC_RAIN][c], sn = f[ORC_SNOW][c], gr = f[ORC_GRAUPEL][c], h = f[ORC_HAIL][c];
+      const double vg = gr > 0 ? graupelFallSpeed(gr, rho0) : 0, vh = h > 0 ? hailFallSpeed(h, rho0) : 0;
+      const double flux = rho0 * r * fallSpeedRain(r, rho0, rho0) + rho0 * (sn * snowFallSpeed(sn, rho0, rho0) + gr * vg + h * vh);
+      rainRate = std::max(rainRate, flux * 3600); rainTotal = std::max(rainTotal, double(col[ORC_PRECIPITATION][c]));
+    }
+    for (int c = 0; c < layer; c++) hail = std::max(hail, hailMaxDiameter(f[ORC_HAIL][c], rho0) * 1000);
+    std::vector<float> deficit; double warm;
+    const double outflow = outflowDeficit(deficit, warm), outflowLow = deficitNear(deficit, int(rotation[ORC_ROT_LOW]));
+    out[ORC_MD_UPDRAFT] = up; out[ORC_MD_DOWNDRAFT] = down; out[ORC_MD_RAINRATE] = rainRate; out[ORC_MD_RAINTOTAL] = rainTotal;
+    out[ORC_MD_CLOUDTOP] = top; out[ORC_MD_THERMALTOP] = thermalTop; out[ORC_MD_MAXCLOUD] = maxCloud; out[ORC_MD_COLDMAX] = coldMax;
+    out[ORC_MD_CORES] = countCores(); out[ORC_MD_SHEAR06] = std::hypot(u6 - u0, 0.0); out[ORC_MD_OUTFLOW] = outflow;
+    out[ORC_MD_OUTFLOWLOW] = outflowLow; out[ORC_MD_GUST] = gust; out[ORC_MD_HAIL] = hail; out[ORC_MD_CLIPPED] = double(clipped);
+  }
   void step(double dt) {
     fallPhase();
     collectHail(dt);
@@ -1402,6 +1482,18 @@ void orc_update_rotation(void *m, double dt, int keep) {
   const double held = mm->rotation[ORC_ROT_PERSISTED];
   mm->updateRotation(dt);
   if (keep) mm->rotation[ORC_ROT_PERSISTED] = held;
+}
+void orc_model_diagnostics(void *m, double *out) { static_cast<Model *>(m)->diagnostics(out); }
+double orc_outflow_deficit(void *m, float *deficit, double *warm) {
+  std::vector<float> d; double w;
+  const double max = static_cast<Model *>(m)->outflowDeficit(d, w);
+  if (deficit) std::memcpy(deficit, d.data(), d.size() * 4);
+  if (warm) *warm = w;
+  return max;
+}
+double orc_deficit_near(void *m, const float *deficit, int column) {
+  Model *mm = static_cast<Model *>(m);
+  return mm->deficitNear(std::vector<float>(deficit, deficit + mm->g.layer), column);
 }
 void orc_rotation(void *m, double *out) { std::memcpy(out, static_cast<Model *>(m)->rotation, sizeof(double) * ORC_ROT); }
 
