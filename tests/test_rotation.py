@@ -91,6 +91,52 @@ def test_oracle_rotation_of_planted_vortices(scenario):
     check(o.rotation(), expected(g, zs, hz, scenario))
 
 
+DEFAULT = {"nx": 40, "ny": 32, "nz": 50, "width": 48000.0, "depth": 36000.0, "height": 18900.0, "bottom": 100.0}
+
+
+def rankine(g, zs, vortices):
+    """model.test.ts:36-49: a broad updraught (25 m/s peak, 5 km e-folding) at 1.5-6 km about column (20, 16) and
+    Rankine vortices (vorticity zeta inside 2 km) at the given column offsets."""
+    nx, ny, nz = g["nx"], g["ny"], g["nz"]
+    dx, dy, cx, cy = g["width"] / nx, g["depth"] / ny, 20, 16
+    u, v, w = (np.zeros((nz, ny, nx)) for _ in range(3))
+    ex = (np.arange(nx) - cx) * dx
+    ey = (np.arange(ny) - cy) * dy
+    EX, EY = np.meshgrid(ex, ey)
+    for z in range(nz):
+        if zs[z] < 1500 or zs[z] > 6000:
+            continue
+        w[z] = 25 * np.exp(-(EX * EX + EY * EY) / 5000 ** 2)
+        for ox, zeta in vortices:
+            rx = EX - ox * dx
+            r = np.hypot(rx, EY)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                speed = zeta / 2 * np.where(r < 2000, r, 2000 ** 2 / r)
+                u[z] += np.where(r > 0, -speed * EY / r, 0)
+                v[z] += np.where(r > 0, speed * rx / r, 0)
+    return [a.astype(np.float32).reshape(-1) for a in (u, v, w)]
+
+
+def rotation_of(o, g, vortices):
+    zs, _ = o.levels()
+    for name, a in zip(("u", "v", "w"), rankine(g, zs, vortices)):
+        o.set_field(name, a)
+    o.update_rotation(1.0, keep=False)
+    return o.rotation()
+
+
+def test_oracle_tells_a_mesocyclone_from_a_vortex_pair():
+    # model.test.ts:33-56: a rotating updraught has core vorticity above 0.004 s-1; an updraught with a cyclonic and an
+    # anticyclonic vortex on its flanks has strong helicity of both signs but core vorticity under 0.002 s-1.
+    g = DEFAULT
+    o = OracleRun(g, C.profile(), C.run(g, seed=1))
+    meso = rotation_of(o, g, [(0, .02)])
+    pair = rotation_of(o, g, [(-3, .02), (3, -.02)])
+    assert meso["coreZeta"] > .004
+    assert pair["uh"] > 200 and pair["anticyclonic"] > 200
+    assert abs(pair["coreZeta"]) < .002
+
+
 def test_oracle_rotation_holds_while_the_core_spins():
     # persisted adds dt while the core's vorticity is at least mesoCoreZeta(dx) (3e-3 at 1200 m) and resets otherwise;
     # refreshRotation keeps it.
@@ -143,6 +189,29 @@ def test_device_rotation_of_planted_vortices(scenario):
     for k in want:
         if k != "persisted":
             assert got[k] == pytest.approx(want[k], rel=1e-12, abs=1e-15), k
+
+
+@pytest.mark.gpu
+def test_device_tells_a_mesocyclone_from_a_vortex_pair():
+    import meteolab_amd as M
+    g = DEFAULT
+    o = OracleRun(g, C.profile(), C.run(g, seed=1))
+    zs, _ = o.levels()
+    with M.Context(0, 32) as ctx:
+        r = ctx.pool(g, 1).run(C.profile(), C.run(g, seed=1))
+        for vortices in ([(0, .02)], [(-3, .02), (3, -.02)]):
+            for name, a in zip(("u", "v", "w"), rankine(g, zs, vortices)):
+                r.set_field(name, a)
+            print("device rotation of the vortices", vortices, flush=True)
+            r.refresh_rotation()
+            got, want = r.rotation(), rotation_of(o, g, vortices)
+            for k in want:
+                if k != "persisted":
+                    assert got[k] == pytest.approx(want[k], rel=1e-12, abs=1e-15), (k, got[k], want[k])
+            if len(vortices) == 1:
+                assert got["coreZeta"] > .004
+            else:
+                assert got["uh"] > 200 and got["anticyclonic"] > 200 and abs(got["coreZeta"]) < .002
 
 
 @pytest.mark.gpu
