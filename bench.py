@@ -118,6 +118,18 @@ def main():
     per_minute = int(round(60 / DT))
     minutes = [ms[k:k + per_minute].sum() for k in range(0, len(ms) - per_minute + 1, per_minute)]
 
+    # The same members through the same steps with each step's lightning in line (ml_ctx_set_overlap off): what running
+    # a step's discharge beside the next step's dynamics gains.
+    ctx.set_overlap(False)
+    lpool = ctx.pool(grid, a.runs)
+    lruns = [lpool.run(prof, C.ensemble_member(grid, k)) for k in range(a.runs)]
+    M.set_batch_tuning(lruns, **tuning_args(M.batch_tuning(runs)))
+    if a.warmup:
+        M.step(lruns, a.warmup, DT)
+    lms = M.step_timed(lruns, a.steps, DT)
+    lpool.destroy()
+    ctx.set_overlap(True)
+
     # Kernel times over the same workload: the same members from their initial state through the same steps, launched
     # eagerly with HIP events between the kernels on the library's stream (the graph replays above cannot be split by
     # kernel). The lightning kernels cost nothing until the storms electrify, so the table is over the whole run.
@@ -166,6 +178,8 @@ def main():
         "plain": {"value": plain, "unit": "cell-steps/s", "ms_per_step": float(pms.mean()), "steps": a.steps,
                   "members": members},
         "batched_over_plain": batched / plain,
+        "lightning_in_line": {"ms_per_step": float(lms.mean()), "value": a.runs * n * a.steps / (lms.sum() / 1e3),
+                              "overlap_speedup": float(lms.sum() / ms.sum())},
         "latency_per_model_minute_ms": {"p50": percentile(minutes, 50), "p99": percentile(minutes, 99),
                                         "runs_per_batch": a.runs, "minutes": len(minutes)},
         "roofline": {"bound": "hbm", "kernel": stage_kernel, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,

@@ -146,6 +146,12 @@ int ml_model_diagnostics(ml_ctx *ctx, const int *runs, int count, double *out);
  * (model.ts:873-877 and its callers, e.g. tests/bench/threads.ts:38-42). One launch per kernel covers all the runs;
  * the launch sequence of a batch is captured once into a hipGraph and replayed. Asynchronous: ml_sync waits. */
 int ml_step(ml_ctx *ctx, const int *runs, int count, int steps, double dt);
+/* Scheduling of ml_step (no counterpart in the reference, whose step is serial): with on = 1 (the default) a step's
+ * discharge and conduction (the serial lightning, a block per run) run on a stream of their own beside the next step's
+ * sedimentation of water and transport of all but the charges, which read and write none of what the lightning does;
+ * the charges' falls and transport wait for it. A call still ends with every step complete, and the results are those
+ * of on = 0, which runs the step's parts in line. */
+int ml_ctx_set_overlap(ml_ctx *ctx, int on);
 
 /* The parts of one step, split where the reference's GPU step splits it (src/gpu/step.ts:12-31), for a host that runs
  * some parts itself. ml_dynamics, ml_physics, ml_mix and ml_after_step in that order are one ml_step (the same

@@ -84,6 +84,7 @@ SIGNATURES = {
     "ml_model_diagnostics": (_i, [_vp, _ip, _i, _dp]),
     "ml_run_flashes": (_i, [_vp, _i, _dp, _i]),
     "ml_step": (_i, [_vp, _ip, _i, _i, _d]),
+    "ml_ctx_set_overlap": (_i, [_vp, _i]),
     "ml_dynamics": (_i, [_vp, _ip, _i, _d]),
     "ml_physics": (_i, [_vp, _ip, _i, _d]),
     "ml_mix": (_i, [_vp, _ip, _i, _d]),
@@ -153,6 +154,10 @@ class Context:
 
     def sync(self):
         self.check(lib().ml_sync(self.h))
+
+    def set_overlap(self, on):
+        """A step's lightning beside the next step's dynamics (True, the default) or in line (ml_ctx_set_overlap)."""
+        self.check(lib().ml_ctx_set_overlap(self.h, 1 if on else 0))
 
     def copy_bandwidth(self, nbytes=1 << 30, iters=20):
         """Achievable HBM bandwidth of a plain copy (read + write), GB/s (ml_copy_probe)."""

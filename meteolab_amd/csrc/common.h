@@ -18,6 +18,8 @@ enum : int {
   F_RAIN = 0, F_SNOW, F_GRAUPEL, F_HAIL, F_U, F_V, F_W, F_THETA, F_Q, F_CLOUD, F_ICE, F_ICENUMBER, F_COLD,
   F_CHARGE_ICE, F_CHARGE_GRAUPEL, F_CHARGE_RAIN, F_CHARGE_AIR, N_CARRIED,
   F_FIELDZ = N_CARRIED, F_FALL_RAIN, F_FALL_SNOW, F_FALL_GRAUPEL, F_FALL_HAIL, F_FALL_CI, F_FALL_CG, F_CONDENSATE,
+  // Internal: cloud + ice + snow (float32) as the lightning sees it, written with the electric solve's right-hand side.
+  F_CLOUDY,
   N_FIELDS
 };
 // Ground columns (STATE_COLUMNS, src/core/state.ts:10).
@@ -40,7 +42,6 @@ enum : int {
 enum : int {
   S_COURANT = 0,         // 7 words: the largest Courant number of each fall (float bits, atomicMax on uint32)
   S_FILL = 8,            // 7 x 3 doubles: removed, kept, negative of the positivity fix of each conserved field
-  S_STRONGEST = 29,      // uint64 key of the largest (|E|/E_init)^2 (float bits << 32 | ~node) of the step's solve
   S_STEP_END = 30,       // [0, S_STEP_END) is cleared at the start of each step
   S_CLIPPED = 32,        // int64: velocity components clipped since the start (model.ts:1586-1595)
   S_NEGATIVE = 33,       // double: negative mass removed by the positivity fix since the start (model.ts:1119)
@@ -51,6 +52,9 @@ enum : int {
   S_FLASH_ID = 39,       // int64: the lightning's stamp counter (lightning.ts flashId)
   S_FROM_GROUND = 40,    // double: charge the CG flashes took from the ground (lightning.ts fromGround)
   S_FLASH_LOG = 41,      // int64: flashes recorded in the run's flash log (a ring of FLASH_LOG entries)
+  S_STRONGEST = 42,      // uint64 key of the largest (|E|/E_init)^2 (float bits << 32 | ~node) of the step's electric
+                         // solve; cleared with its right-hand side, read by the discharge (which may run during the
+                         // next step's dynamics)
   S_ROT = 48,            // the rotation diagnostics (model.ts:142-158 RotationState), ROT_* from here
   N_SCALARS = 64
 };
@@ -87,6 +91,8 @@ template <class R> struct Bufs {
   double *spec;    // [slot][nz][nm][2] Hermitian half spectra, re/im interleaved (float64: see spectral.h)
   double *ehat;    // [slot][nz][nm][2] the discharge's spectra of the electric right-hand side (kept between its solves)
   double *mean0;   // [slot][nz]    float64 layer sums of the right-hand side (the m = n = 0 mode)
+  double *emean0;  // [slot][nz]    the electric solve's layer sums (kept for the discharge, as ehat)
+  double *dspec;   // [slot][nz][nm][2] the discharge's own spectra (so that it can run beside the dynamics' solves)
   R *km, *kh;      // [slot][n]     eddy viscosities
   double *ephi;    // [slot][n]     electric potential (electric.ts:36-60) (float64)
   float *cols;     // [slot][N_COLUMNS][layer]
@@ -108,6 +114,11 @@ template <class R> struct Bufs {
   int *chan;       // [slot][2][MAX_CHANNEL] channel nodes and their parents
   double *cdep;    // [slot][MAX_CHANNEL] charge deposited on each channel node
   double *flog;    // [slot][FLASH_LOG][FLASH_FIELDS]
+  // Sums kept in parts so that they add in a fixed order, whatever order the blocks run in:
+  double *fillp;   // [slot][N_CONSERVED][nz][ftiles][3] the positivity-fix sums of each level (and strip tile)
+  int ftiles;      // strip tiles of a level (stageTile)
+  double *gpart;   // [slot][2][gmax] a launch's block parts of the charge carried to the ground (falls, conduction)
+  int gmax;
 
   __device__ float *f(int slot, int k, const Geo &g) const { return fields + (size_t(slot) * N_FIELDS + k) * g.n; }
   __device__ float *x(int slot, int k, const Geo &g) const { return stx + (size_t(slot) * N_CARRIED + k) * g.n; }
@@ -119,10 +130,15 @@ template <class R> struct Bufs {
   __device__ double *sp(int slot, const Geo &g) const { return spec + size_t(slot) * g.nz * g.nm * 2; }
   __device__ double *eh(int slot, const Geo &g) const { return ehat + size_t(slot) * g.nz * g.nm * 2; }
   __device__ double *m0(int slot, const Geo &g) const { return mean0 + size_t(slot) * g.nz; }
+  __device__ double *em0(int slot, const Geo &g) const { return emean0 + size_t(slot) * g.nz; }
+  __device__ double *dsp(int slot, const Geo &g) const { return dspec + size_t(slot) * g.nz * g.nm * 2; }
   __device__ float *col(int slot, int k, const Geo &g) const { return cols + (size_t(slot) * N_COLUMNS + k) * g.layer; }
   __device__ double *lv(int slot, int k, const Geo &g) const { return lev + (size_t(slot) * N_LEVELS + k) * g.nz; }
   __device__ double *pr(int slot) const { return par + size_t(slot) * N_PARAMS; }
   __device__ double *sc(int slot) const { return scal + size_t(slot) * N_SCALARS; }
+  __device__ double *fillPart(int slot, int cf, int z, int tile, const Geo &g) const {
+    return fillp + (((size_t(slot) * N_CONSERVED + cf) * g.nz + z) * ftiles + tile) * 3;
+  }
   __device__ int *lm(int slot, int k, const Geo &g) const { return lmark + (size_t(slot) * N_LMARK + k) * g.n; }
   __device__ double *ln(int slot, int k, const Geo &g) const { return lnode + (size_t(slot) * N_LNODE + k) * g.n; }
 };
@@ -165,6 +181,12 @@ template <class T> __device__ inline T blockMax(T v, T *shared) {
   T t = shared[0];
   for (int w = 1; w < waves; w++) t = shared[w] > t ? shared[w] : t;
   return t;
+}
+
+// A launch's block part of a run's sum (the run's blocks are blockIdx.x of a grid whose y is the run), kept so that a
+// later kernel (k_ground_total) adds the parts in block order: the sum does not depend on the order the blocks run in.
+template <class B> __device__ inline void storePart(const B &b, int slot, int which, double t) {
+  if (threadIdx.x == 0) b.gpart[(size_t(slot) * 2 + which) * b.gmax + blockIdx.x] = t;
 }
 
 }  // namespace ml

@@ -62,7 +62,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
   for (int k = threadIdx.x; k < ny; k += blockDim.x) ty[k] = {b.twy[2 * k], b.twy[2 * k + 1]};
   const double *air = b.lv(slot, L_RHO, g);
   R *rho = b.arr(b.lrho, slot, g);
-  double *spec = b.sp(slot, g), *hat = b.eh(slot, g), *m0 = b.m0(slot, g), *phi = b.arr(b.ephi, slot, g);
+  double *spec = b.dsp(slot, g), *hat = b.eh(slot, g), *m0 = b.em0(slot, g), *phi = b.arr(b.ephi, slot, g);
   const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
   const Div dHl(hl), dNx(nx), dHm(half * mw), dMw(mw), dNm(nm), dNy(ny), dLayer(layer);
   // The changed nodes as (level, y, x) keys and changes, compacted in channel order into the transform buffers.
@@ -274,7 +274,7 @@ template <class R> struct Discharge {
   unsigned char *cflag;  // LDS: channel node c is clear air (as of the last solve)
   int *prunes;           // LDS: the front places a prune takes off
   float *car[4];
-  const float *cloud, *ice, *snow;
+  const float *cloudy;
   const double *air, *einit, *cap;
   double *phi;
   R *rho;
@@ -284,7 +284,7 @@ template <class R> struct Discharge {
   __device__ Discharge(const Geo &g_, const Bufs<R> &b_, const Plan &p_, int s_, int chunk_, char *m_, double *r_, FlashState &st_, unsigned char *cf_, int *pr_)
       : g(g_), b(b_), pl(p_), slot(s_), chunk(chunk_), smem(m_), red(r_), st(st_), cflag(cf_), prunes(pr_) {
     car[0] = b.f(slot, F_CHARGE_ICE, g); car[1] = b.f(slot, F_CHARGE_GRAUPEL, g); car[2] = b.f(slot, F_CHARGE_RAIN, g); car[3] = b.f(slot, F_CHARGE_AIR, g);
-    cloud = b.f(slot, F_CLOUD, g); ice = b.f(slot, F_ICE, g); snow = b.f(slot, F_SNOW, g);
+    cloudy = b.f(slot, F_CLOUDY, g);
     air = b.lv(slot, L_RHO, g); einit = b.lv(slot, L_EINIT, g); cap = b.lv(slot, L_CAPACITANCE, g);
     phi = b.arr(b.ephi, slot, g); rho = b.arr(b.lrho, slot, g);
     mark = b.lm(slot, LM_MARK, g); failed = b.lm(slot, LM_FAILED, g); seen = b.lm(slot, LM_SEEN, g); pos = b.lm(slot, LM_POS, g);
@@ -294,8 +294,7 @@ template <class R> struct Discharge {
   }
   // Clear air for the channel: no charge to speak of, or no cloud (cloud + ice + snow, lightning.ts:320-322).
   __device__ bool clear(int i) const {
-    float cloudy = (cloud[i] + ice[i]) + snow[i];
-    return fabs(double(rho[i])) < RHO_MIN || cloudy <= float(CLOUDY);
+    return fabs(double(rho[i])) < RHO_MIN || cloudy[i] <= float(CLOUDY);
   }
   __device__ int flagsOf(int i) const { return (clear(i) ? FF_CLEAR : 0) | (i < g.layer ? FF_GROUND : 0); }
   // The potential of the carriers as they now stand, then the front's and the channel's potentials and flags from it.
@@ -341,14 +340,14 @@ template <class R> struct Discharge {
       // Everything the decision reads, in one round of loads.
       const int mk = mark[j], sn = seen[j], ps = pos[j];
       const double rj = double(rho[j]), sj = nscale[j], phj = phi[j];
-      const float cloudy = (cloud[j] + ice[j]) + snow[j];
+      const float cj = cloudy[j];
       // Node i itself is on the channel (its mark was just set).
       if (!(j == i || mk == id || (k >= z && fabs(rj) < RHO_MIN))) {
         const double hz = b.zs[k] - b.zs[z], threshold = F_CRIT * .5 * (einit[z] + einit[k]);
         s = sqrt((di * g.dx) * (di * g.dx) + (dj * g.dy) * (dj * g.dy) + hz * hz) * threshold;
         if (sn != id) {
           fresh = true; pj = phj;
-          fl = (fabs(rj) < RHO_MIN || cloudy <= float(CLOUDY) ? FF_CLEAR : 0) | (j < g.layer ? FF_GROUND : 0);
+          fl = (fabs(rj) < RHO_MIN || cj <= float(CLOUDY) ? FF_CLEAR : 0) | (j < g.layer ? FF_GROUND : 0);
         }
         // Seen before: a shorter link replaces the entry's (a node seen and taken off the front does not return).
         else if (ps >= 0 && s < sj) { better = true; at = ps; }
@@ -591,7 +590,8 @@ template <class R> struct Discharge {
 };
 
 // The discharge of each run of the launch (lightning.ts:261-276): one block per run, `chunk` levels a solve pass.
-template <class R> __global__ __launch_bounds__(DISCHARGE_THREADS) void k_discharge(Geo g, Bufs<R> b, Plan pl, double dt, int limit, int chunk) {
+// advanced: the run's clock has already moved on by dt (the discharge of a step run beside the next step's dynamics).
+template <class R> __global__ __launch_bounds__(DISCHARGE_THREADS) void k_discharge(Geo g, Bufs<R> b, Plan pl, double dt, int limit, int chunk, int advanced) {
   int slot = b.runs[blockIdx.x];
   double *S = b.sc(slot);
   unsigned long long key0 = reinterpret_cast<unsigned long long *>(S)[S_STRONGEST];
@@ -604,7 +604,7 @@ template <class R> __global__ __launch_bounds__(DISCHARGE_THREADS) void k_discha
   __shared__ int prunes[PRUNE_MAX];
   Discharge<R> d(g, b, pl, slot, chunk, smem, red, st, cflag, prunes);
   const double *P = b.pr(slot);
-  const double time = P[P_TIME] + dt;
+  const double time = advanced ? P[P_TIME] : P[P_TIME] + dt;
   if (threadIdx.x == 0) {
     long long id; memcpy(&id, S + S_FLASH_ID, 8); id++; memcpy(S + S_FLASH_ID, &id, 8);
     // The step's electric solve kept its right-hand side's spectra and level sums: every solve here is an update.

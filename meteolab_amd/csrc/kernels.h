@@ -97,7 +97,9 @@ template <class R> __global__ void k_init(Geo g, Bufs<R> b) {
 }
 
 // ---- fall speeds, Courant numbers and the hail swath (model.ts:1220-1235 'fall', 575-580 fallParts, 697-706) --------
-template <class R> __global__ __launch_bounds__(TPB) void k_fall(Geo g, Bufs<R> b, double dt, int electrified) {
+// chargeCourant: also the charge falls' Courant numbers (0 when k_charge_courant takes them after the lightning of the
+// step before has changed the charges).
+template <class R> __global__ __launch_bounds__(TPB) void k_fall(Geo g, Bufs<R> b, double dt, int electrified, int chargeCourant) {
   int slot = b.runs[blockIdx.y];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double cour[N_FALLS] = {0, 0, 0, 0, 0, 0, 0};
@@ -118,9 +120,11 @@ template <class R> __global__ __launch_bounds__(TPB) void k_fall(Geo g, Bufs<R> 
       float vci = float(is > 0 ? R(snow) * R(vs) / is : R(0));
       float vcg = float(gh > 0 ? (R(graupel) * R(vg) + R(hail) * R(vh)) / gh : R(0));
       b.f(slot, F_FALL_CI, g)[i] = vci; b.f(slot, F_FALL_CG, g)[i] = vcg;
-      cour[4] = b.f(slot, F_CHARGE_ICE, g)[i] != 0 ? c * vci : 0;
-      cour[5] = b.f(slot, F_CHARGE_GRAUPEL, g)[i] != 0 ? c * vcg : 0;
-      cour[6] = b.f(slot, F_CHARGE_RAIN, g)[i] != 0 ? c * vr : 0;
+      if (chargeCourant) {
+        cour[4] = b.f(slot, F_CHARGE_ICE, g)[i] != 0 ? c * vci : 0;
+        cour[5] = b.f(slot, F_CHARGE_GRAUPEL, g)[i] != 0 ? c * vcg : 0;
+        cour[6] = b.f(slot, F_CHARGE_RAIN, g)[i] != 0 ? c * vr : 0;
+      }
     }
     // collectHail: the swath at the ground node before the sedimentation moves the hail.
     if (z == 0 && hail > 1e-9f) {
@@ -142,11 +146,35 @@ template <class R> __global__ __launch_bounds__(TPB) void k_fall(Geo g, Bufs<R> 
   }
 }
 
+// The charge falls' Courant numbers from the fall speeds k_fall stored (the same values as k_fall's own).
+template <class R> __global__ __launch_bounds__(TPB) void k_charge_courant(Geo g, Bufs<R> b, double dt) {
+  int slot = b.runs[blockIdx.y];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double cour[3] = {0, 0, 0};
+  if (i < g.n) {
+    int z = i / g.layer;
+    const double c = dt * b.lv(slot, L_RHO, g)[z] / (b.lv(slot, L_LW, g)[z] * g.dz);
+    cour[0] = b.f(slot, F_CHARGE_ICE, g)[i] != 0 ? c * b.f(slot, F_FALL_CI, g)[i] : 0;
+    cour[1] = b.f(slot, F_CHARGE_GRAUPEL, g)[i] != 0 ? c * b.f(slot, F_FALL_CG, g)[i] : 0;
+    cour[2] = b.f(slot, F_CHARGE_RAIN, g)[i] != 0 ? c * b.f(slot, F_FALL_RAIN, g)[i] : 0;
+  }
+  __shared__ double part[3][TPB / 64];
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int k = 0; k < 3; k++) { double m = waveMax(cour[k]); if (lane == 0) part[k][wave] = m; }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double m = 0;
+    for (int w = 0; w < TPB / 64; w++) m = part[threadIdx.x][w] > m ? part[threadIdx.x][w] : m;
+    if (m > 0) atomicMax(reinterpret_cast<unsigned long long *>(b.sc(slot) + S_COURANT + 4 + threadIdx.x), (unsigned long long)__double_as_longlong(m));
+  }
+}
+
 // ---- sedimentation: flux-form upwind fall per column with sub-steps (model.ts:565-598, sedimentStep 928-932) -------
 // One thread per column; the column of the falling field and its speed are staged in LDS, so the bottom-up sweep of
 // each sub-step reads no memory.
 constexpr int SED_COLS = 64;
-template <class R> __global__ __launch_bounds__(SED_COLS) void k_sediment(Geo g, Bufs<R> b, double dt, int electrified) {
+// Falls k0 .. k1 - 1 of the step's order (the water falls 0-3 and the charge falls 4-6 are independent of each other).
+template <class R> __global__ __launch_bounds__(SED_COLS) void k_sediment(Geo g, Bufs<R> b, double dt, int k0, int k1) {
   int slot = b.runs[blockIdx.y];
   int col = blockIdx.x * SED_COLS + threadIdx.x;
   extern __shared__ float sed[];
@@ -156,7 +184,7 @@ template <class R> __global__ __launch_bounds__(SED_COLS) void k_sediment(Geo g,
   const int falls[N_FALLS] = {F_FALL_RAIN, F_FALL_SNOW, F_FALL_GRAUPEL, F_FALL_HAIL, F_FALL_CI, F_FALL_CG, F_FALL_RAIN};
   double charge = 0;
   bool live = col < g.layer;
-  for (int k = 0; k < (electrified ? N_FALLS : 4); k++) {
+  for (int k = k0; k < k1; k++) {
     double courant = __longlong_as_double((long long)reinterpret_cast<unsigned long long *>(b.sc(slot))[S_COURANT + k]);
     // No node moves anything (every non-zero value has zero speed): the fall changes nothing.
     if (!(courant > 0)) continue;
@@ -182,10 +210,9 @@ template <class R> __global__ __launch_bounds__(SED_COLS) void k_sediment(Geo g,
       for (int z = 0; z < g.nz; z++) a[size_t(z) * g.layer] = qa[z * SED_COLS + threadIdx.x];
     }
   }
-  if (electrified) {
+  if (k1 > 4) {
     __shared__ double red[SED_COLS / 64];
-    double t = blockSum(charge, red);
-    if (threadIdx.x == 0 && t != 0) atomicAdd(b.sc(slot) + S_CHARGE_GROUND, t);
+    storePart(b, slot, 0, blockSum(charge, red));
   }
 }
 
@@ -298,7 +325,44 @@ template <class R> __device__ inline R faceFlux(bool monotone, R m, R qm2, R qm1
 }
 __device__ inline bool monotoneField(int f) { return !(f == F_U || f == F_V || f == F_W || f == F_THETA); }
 // Index of a carried field among the conserved ones (rain, snow, graupel, hail, q, cloud, ice), -1 if it is not one.
+// The conserved (positivity-fixed) carried fields by their index (model.ts:1319-1327).
+__device__ inline int conservedField(int k) { const int c[N_CONSERVED] = {F_RAIN, F_SNOW, F_GRAUPEL, F_HAIL, F_Q, F_CLOUD, F_ICE}; return c[k]; }
 __device__ inline int conservedIndex(int f) { return f <= F_HAIL ? f : f == F_Q ? 4 : f == F_CLOUD ? 5 : f == F_ICE ? 6 : -1; }
+// The positivity-fix sums (removed, kept, negative; level-weighted) of one level (or strip tile) of conserved field cf,
+// reduced over the block in a fixed order into their part of the run's sums; k_fill_total adds the parts in level
+// order, so the sums do not depend on the order the blocks run in. Every thread of the block calls; fill is cleared.
+template <class R> __device__ inline void storeFill(const Bufs<R> &b, const Geo &g, int slot, int cf, int z, int tile, double fill[3], double (*red)[3]) {
+  for (int j = 0; j < 3; j++) fill[j] = waveSum(fill[j]);
+  if ((threadIdx.x & 63) == 0) for (int j = 0; j < 3; j++) red[threadIdx.x >> 6][j] = fill[j];
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double s = 0;
+    for (int w = 0; w < int(blockDim.x) / 64; w++) s += red[w][threadIdx.x];
+    b.fillPart(slot, cf, z, tile, g)[threadIdx.x] = s;
+  }
+  __syncthreads();
+  fill[0] = fill[1] = fill[2] = 0;
+}
+// The charge carried to the ground by a launch (which: 0 the charge falls, 1 the conduction) from its `count` block
+// parts, added in a fixed order.
+template <class R> __global__ void k_ground_total(Geo g, Bufs<R> b, int which, int count) {
+  const int slot = b.runs[blockIdx.x];
+  const double *part = b.gpart + (size_t(slot) * 2 + which) * b.gmax;
+  double s = 0;
+  for (int k = threadIdx.x; k < count; k += blockDim.x) s += part[k];
+  __shared__ double red[16];
+  s = blockSum(s, red);
+  if (threadIdx.x == 0 && s != 0) b.sc(slot)[S_CHARGE_GROUND] += s;
+}
+// The run's positivity-fix sums from the parts the last transport stage stored (tiles: parts per level).
+template <class R> __global__ void k_fill_total(Geo g, Bufs<R> b, int tiles) {
+  const int slot = b.runs[blockIdx.x], t = threadIdx.x;
+  if (t >= 3 * N_CONSERVED) return;
+  double s = 0;
+  for (int z = 0; z < g.nz; z++)
+    for (int k = 0; k < tiles; k++) s += b.fillPart(slot, t / 3, z, k, g)[t % 3];
+  b.sc(slot)[S_FILL + t] = s;
+}
 constexpr double TINY_WATER = 1e-10;
 
 // Transport blocks: a strip of TY rows of one level of one run, for all carried fields in turn, so the face mass fluxes
@@ -328,16 +392,17 @@ __device__ inline void blockRun(int nr, int perRun, int xcd, int &runIdx, int &b
   else { runIdx = p / perRun; b = p % perRun; }
 }
 
-// `groups` splits the fields among blocks (group k takes fields [k nc / groups, (k + 1) nc / groups)): more blocks in
+// The launch takes fields fbase .. fbase + nc - 1 (all carried fields, or a part of a step split around the lightning).
+// `groups` splits them among blocks (group k takes fields [k nc / groups, (k + 1) nc / groups) of them): more blocks in
 // flight against the mass fluxes read once per group; the autotuner picks it per batch.
-template <class R> __global__ __launch_bounds__(512) void k_stage(Geo g, Bufs<R> b, int stage, double dt, int nr, int nc, int groups, int xcd) {
+template <class R> __global__ __launch_bounds__(512) void k_stage(Geo g, Bufs<R> b, int stage, double dt, int nr, int nc, int fbase, int groups, int xcd) {
   const StageTile T = stageTile(g);
   int runIdx, blk;
   blockRun(nr, T.tiles * g.nz * groups, xcd, runIdx, blk);
   const int grp = blk % groups;
   blk /= groups;
   const int slot = b.runs[runIdx], z = blk / T.tiles, y0 = (blk % T.tiles) * T.ty;
-  const int f0 = grp * nc / groups, f1 = (grp + 1) * nc / groups;
+  const int f0 = fbase + grp * nc / groups, f1 = fbase + (grp + 1) * nc / groups;
   const int inSet = stage, outSet = stage == 2 ? 0 : stage + 1;
   const int layer = g.layer, nx = g.nx, ny = g.ny, nz = g.nz, ty = T.ty, tn = nx * ty;
   extern __shared__ char smem[];
@@ -425,17 +490,8 @@ template <class R> __global__ __launch_bounds__(512) void k_stage(Geo g, Bufs<R>
       }
     }
     const int cf = conservedIndex(f);
-    if (stage == 2 && cf >= 0) {
-      // Block sums of removed, kept and negative mass (level weights), into the run's positivity-fix sums.
-      for (int j = 0; j < 3; j++) fill[j] = waveSum(fill[j]);
-      if ((t & 63) == 0) for (int j = 0; j < 3; j++) red[t >> 6][j] = fill[j];
-      __syncthreads();
-      if (t < 3) {
-        double s = 0;
-        for (int w = 0; w < int(blockDim.x) / 64; w++) s += red[w][t];
-        if (s != 0) atomicAdd(b.sc(slot) + S_FILL + 3 * cf + t, s);
-      }
-    }
+    // The strip's removed, kept and negative mass (level weights): its part of the run's positivity-fix sums.
+    if (stage == 2 && cf >= 0) storeFill(b, g, slot, cf, z, blk % T.tiles, fill, red);
     __syncthreads();
   }
 }
@@ -445,11 +501,11 @@ template <class R> __global__ __launch_bounds__(512) void k_stage(Geo g, Bufs<R>
 // levels of a block is taken once: the top face of a level is the bottom face of the next (kept per node in LDS, each
 // node by the thread that owns it). The divisions of the tendency are products with per-level reciprocals.
 __host__ __device__ inline size_t stageLevelLdsBytes(const Geo &g, size_t rb) { return ((g.layer * 4 + 15) / 16) * 16 + 3 * size_t(g.layer) * rb; }
-template <class R> __global__ __launch_bounds__(TPB) void k_stage_level(Geo g, Bufs<R> b, int stage, double dt, int nr, int nc, int xcd, int K) {
+template <class R> __global__ __launch_bounds__(TPB) void k_stage_level(Geo g, Bufs<R> b, int stage, double dt, int nr, int nc, int fbase, int xcd, int K) {
   const int zBlocks = (g.nz + K - 1) / K;
   int runIdx, blk;
   blockRun(nr, zBlocks * nc, xcd, runIdx, blk);
-  const int slot = b.runs[runIdx], f = blk % nc, z0 = (blk / nc) * K, z1 = z0 + K < g.nz ? z0 + K : g.nz;
+  const int slot = b.runs[runIdx], f = fbase + blk % nc, z0 = (blk / nc) * K, z1 = z0 + K < g.nz ? z0 + K : g.nz;
   const int inSet = stage, outSet = stage == 2 ? 0 : stage + 1;
   const float *q = b.set(inSet, slot, f, g);
   float *out = b.set(outSet, slot, f, g);
@@ -463,6 +519,7 @@ template <class R> __global__ __launch_bounds__(TPB) void k_stage_level(Geo g, B
   const int lvl = f == F_U ? L_UENV : f == F_V ? L_VENV : f == F_THETA ? L_THETA : -1;
   const double *B = lvl >= 0 ? b.lv(slot, lvl, g) : nullptr;
   const bool mono = monotoneField(f), isW = f == F_W;
+  const int cf = conservedIndex(f);
   const Div dNx(nx);
   const R idx = recip(R(g.dx)), idy = recip(R(g.dy)), frac = R((stage == 0 ? 1.0 / 3 : stage == 1 ? 1.0 / 2 : 1.0) * dt);
   const double *rho = b.lv(slot, L_RHO, g);
@@ -523,18 +580,9 @@ template <class R> __global__ __launch_bounds__(TPB) void k_stage_level(Geo g, B
         else fill[1] += wz * xv;
       }
     }
+    // The level's part of the run's positivity-fix sums.
+    if (stage == 2 && cf >= 0) storeFill(b, g, slot, cf, z, 0, fill, red);
     __syncthreads();
-  }
-  const int cf = conservedIndex(f);
-  if (stage == 2 && cf >= 0) {
-    for (int j = 0; j < 3; j++) fill[j] = waveSum(fill[j]);
-    if ((threadIdx.x & 63) == 0) for (int j = 0; j < 3; j++) red[threadIdx.x >> 6][j] = fill[j];
-    __syncthreads();
-    if (threadIdx.x < 3) {
-      double s = 0;
-      for (int w = 0; w < TPB / 64; w++) s += red[w][threadIdx.x];
-      if (s != 0) atomicAdd(b.sc(slot) + S_FILL + 3 * cf + threadIdx.x, s);
-    }
   }
 }
 
@@ -561,8 +609,10 @@ template <class R, bool MONO> __device__ inline R faceFluxT(R m, R qm2, R qm1, R
   return m * (MONO ? wenoFace(a, b_, c, d, e) : upwindFace(a, b_, c, d, e));
 }
 // Levels z0 .. z1-1 of field f of a run. BASE: u, v or theta, transported as departures from the base profile.
+// cf: the conserved index of the field (-1: none); each level's positivity-fix sums go to their part of the run's.
 template <class R, bool BASE, bool MONO> __device__ void stagePlane(const Geo &g, const Bufs<R> &b, int slot, int f, int z0, int z1, int stage, double dt, char *smem,
-                                                                     double *fill) {
+                                                                     int cf, double (*red)[3]) {
+  double fill[3] = {0, 0, 0};
   const int nx = g.nx, ny = g.ny, nz = g.nz, layer = g.layer, W = nx + 6, X1 = nx + 1, T = TPB, planeN = W * (ny + 6);
   const int inSet = stage, outSet = stage == 2 ? 0 : stage + 1;
   const float *q = b.set(inSet, slot, f, g), *start = b.f(slot, f, g);
@@ -663,31 +713,21 @@ template <class R, bool BASE, bool MONO> __device__ void stagePlane(const Geo &g
       }
       x += sx; y += sy; if (x >= nx) { x -= nx; y++; }
     }
+    if (MONO && stage == 2 && cf >= 0) storeFill(b, g, slot, cf, z, 0, fill, red);
     __syncthreads();
   }
 }
-template <class R> __global__ __launch_bounds__(TPB) void k_stage_plane(Geo g, Bufs<R> b, int stage, double dt, int nr, int nc, int xcd, int K) {
+template <class R> __global__ __launch_bounds__(TPB) void k_stage_plane(Geo g, Bufs<R> b, int stage, double dt, int nr, int nc, int fbase, int xcd, int K) {
   const int zBlocks = (g.nz + K - 1) / K;
   int runIdx, blk;
   blockRun(nr, zBlocks * nc, xcd, runIdx, blk);
-  const int slot = b.runs[runIdx], f = blk % nc, z0 = (blk / nc) * K, z1 = z0 + K < g.nz ? z0 + K : g.nz;
+  const int slot = b.runs[runIdx], f = fbase + blk % nc, z0 = (blk / nc) * K, z1 = z0 + K < g.nz ? z0 + K : g.nz;
   extern __shared__ char smem[];
   __shared__ double red[TPB / 64][3];
-  double fill[3] = {0, 0, 0};
-  if (f == F_U || f == F_V || f == F_THETA) stagePlane<R, true, false>(g, b, slot, f, z0, z1, stage, dt, smem, fill);
-  else if (f == F_W) stagePlane<R, false, false>(g, b, slot, f, z0, z1, stage, dt, smem, fill);
-  else stagePlane<R, false, true>(g, b, slot, f, z0, z1, stage, dt, smem, fill);
   const int cf = conservedIndex(f);
-  if (stage == 2 && cf >= 0) {
-    for (int j = 0; j < 3; j++) fill[j] = waveSum(fill[j]);
-    if ((threadIdx.x & 63) == 0) for (int j = 0; j < 3; j++) red[threadIdx.x >> 6][j] = fill[j];
-    __syncthreads();
-    if (threadIdx.x < 3) {
-      double s = 0;
-      for (int w = 0; w < TPB / 64; w++) s += red[w][threadIdx.x];
-      if (s != 0) atomicAdd(b.sc(slot) + S_FILL + 3 * cf + threadIdx.x, s);
-    }
-  }
+  if (f == F_U || f == F_V || f == F_THETA) stagePlane<R, true, false>(g, b, slot, f, z0, z1, stage, dt, smem, cf, red);
+  else if (f == F_W) stagePlane<R, false, false>(g, b, slot, f, z0, z1, stage, dt, smem, cf, red);
+  else stagePlane<R, false, true>(g, b, slot, f, z0, z1, stage, dt, smem, cf, red);
 }
 
 // ---- pressure projection: dual-cell divergence and the node-velocity correction (pressure.ts:50-92) --------------------
@@ -752,28 +792,6 @@ template <class R> __global__ __launch_bounds__(TPB) void k_pcorrect(Geo g, Bufs
       unsigned long long t = blockSum(clipped, red);
       if (threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long *>(b.sc(slot) + S_CLIPPED), t);
     }
-  }
-}
-
-// ---- positivity fix sums (model.ts:606-617 fillNegative): removed, kept, negative of each conserved field ------------
-__device__ inline int conservedField(int k) { const int c[N_CONSERVED] = {F_RAIN, F_SNOW, F_GRAUPEL, F_HAIL, F_Q, F_CLOUD, F_ICE}; return c[k]; }
-template <class R> __global__ __launch_bounds__(TPB) void k_fill_sums(Geo g, Bufs<R> b) {
-  int slot = b.runs[blockIdx.y];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc[N_CONSERVED][3];
-  for (int k = 0; k < N_CONSERVED; k++) acc[k][0] = acc[k][1] = acc[k][2] = 0;
-  if (i < g.n) {
-    double wz = b.lv(slot, L_LW, g)[i / g.layer];
-    for (int k = 0; k < N_CONSERVED; k++) {
-      double x = b.f(slot, conservedField(k), g)[i];
-      if (x < TINY_WATER) { if (x != 0) { acc[k][0] = wz * x; if (x < 0) acc[k][2] = -wz * x; } }
-      else acc[k][1] = wz * x;
-    }
-  }
-  __shared__ double red[TPB / 64];
-  for (int k = 0; k < N_CONSERVED; k++) for (int j = 0; j < 3; j++) {
-    double t = blockSum(acc[k][j], red);
-    if (threadIdx.x == 0 && t != 0) atomicAdd(b.sc(slot) + S_FILL + 3 * k + j, t);
   }
 }
 

@@ -18,16 +18,21 @@ template <class R> __device__ inline double chargeDensity(const Geo &g, const Bu
   s += b.f(slot, F_CHARGE_ICE, g)[i]; s += b.f(slot, F_CHARGE_GRAUPEL, g)[i]; s += b.f(slot, F_CHARGE_RAIN, g)[i]; s += b.f(slot, F_CHARGE_AIR, g)[i];
   return s * air;
 }
-// rho at node i into lrho; at the interior levels -rho/eps0 into `cells` (level k at layer k - 1).
+// rho at node i into lrho; at the interior levels -rho/eps0 into `cells` (level k at layer k - 1). The cloudiness the
+// lightning reads (cloud + ice + snow, float32 as lightning.ts:320-322 adds them) goes with it into F_CLOUDY, so that
+// the discharge and the conduction of this step see this step's clouds even while the next step's dynamics runs.
 template <class R> __device__ inline void chargeAt(const Geo &g, const Bufs<R> &b, int slot, int i) {
   int z = i / g.layer;
   double rho = chargeDensity(g, b, slot, i, b.lv(slot, L_RHO, g)[z]);
   b.arr(b.lrho, slot, g)[i] = R(rho);
   if (z > 0 && z < g.nz - 1) b.arr(b.cells, slot, g)[i - g.layer] = -rho / EPSILON0;
+  b.f(slot, F_CLOUDY, g)[i] = (b.f(slot, F_CLOUD, g)[i] + b.f(slot, F_ICE, g)[i]) + b.f(slot, F_SNOW, g)[i];
 }
+// Also clears the run's strongest-field key, which k_strongest then raises after the solve.
 template <class R> __global__ __launch_bounds__(256) void k_charge_rhs(Geo g, Bufs<R> b) {
   int slot = b.runs[blockIdx.y];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) reinterpret_cast<unsigned long long *>(b.sc(slot))[S_STRONGEST] = 0;
   if (i < g.n) chargeAt(g, b, slot, i);
 }
 
@@ -62,8 +67,7 @@ template <class R> __global__ __launch_bounds__(256) void k_strongest(Geo g, Buf
 // Conductivity of a node: the clear air's, growing with height, a tenth of it in cloud (cloud + ice + snow > 1e-5).
 template <class R> __device__ inline R sigmaAt(const Bufs<R> &b, const Geo &g, int slot, int i, int z) {
   R clear = R(SIGMA_GROUND * exp(b.zs[z] / SIGMA_SCALE));
-  float cloudy = (b.f(slot, F_CLOUD, g)[i] + b.f(slot, F_ICE, g)[i]) + b.f(slot, F_SNOW, g)[i];
-  return cloudy > float(CLOUDY) ? clear * R(CLOUD_SIGMA) : clear;
+  return b.f(slot, F_CLOUDY, g)[i] > float(CLOUDY) ? clear * R(CLOUD_SIGMA) : clear;
 }
 template <class R> __device__ inline R series(R a, R c) { return 2 * a * c / (a + c); }
 
@@ -108,8 +112,7 @@ template <class R> __global__ __launch_bounds__(256) void k_conduct(Geo g, Bufs<
     b.f(slot, F_FIELDZ, g)[i] = float(-(phi[i + layer] - phi[i - layer]) / (b.zs[z + 1] - b.zs[z - 1]));
   }
   __shared__ double red[4];
-  double t = blockSum(lost, red);
-  if (threadIdx.x == 0 && t != 0) atomicAdd(b.sc(slot) + S_CHARGE_GROUND, t);
+  storePart(b, slot, 1, blockSum(lost, red));
 }
 
 }  // namespace ml

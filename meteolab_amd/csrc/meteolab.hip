@@ -40,7 +40,12 @@ struct Batch {
   int *dev = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // With lightning: the step without its discharge (first of a call), and the step beside the discharge of the step
+  // before it (the rest); a call ends with the last step's discharge.
+  hipGraph_t firstGraph = nullptr, nextGraph = nullptr;
+  hipGraphExec_t first = nullptr, next = nullptr;
   double dt = 0;
+  int overlap = -1;  // the context's overlap setting the graphs were captured with
   Tuning tune{};   // the transport kernel's layout, picked by the autotuner when the batch is first stepped
   bool tuned = false;
 };
@@ -54,7 +59,8 @@ struct Pool {
        *spec = nullptr, *mean0 = nullptr, *km = nullptr, *kh = nullptr, *ephi = nullptr, *cols = nullptr, *lev = nullptr,
        *par = nullptr, *scal = nullptr, *zs = nullptr, *dzs = nullptr, *hz = nullptr, *sym = nullptr, *twx = nullptr, *twy = nullptr,
        *lrho = nullptr, *lmark = nullptr, *lnode = nullptr, *chan = nullptr, *cdep = nullptr, *flog = nullptr, *efac = nullptr,
-       *ehat = nullptr;
+       *ehat = nullptr, *emean0 = nullptr, *dspec = nullptr, *fillp = nullptr, *gpart = nullptr;
+  int ftiles = 1, gmax = 1;  // strip tiles of a level; blocks of a run in the launches that carry charge to the ground
   int chunk = 1;          // levels per pass of the discharge's own electric solve (as many as the block's LDS holds)
   RotLevels rot{};        // the rotation diagnostics' levels and threshold
   int *single = nullptr;  // single[s] = s: the run list of one slot
@@ -73,6 +79,11 @@ struct Pool {
 struct ml_ctx {
   int device = 0, precision = 32;
   hipStream_t stream = nullptr;
+  // The lightning's stream: a step's discharge runs on it beside the next step's dynamics (default priority, as the
+  // main stream), forked from and joined back into the main stream by the two events.
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  int overlap = 1;  // a step's discharge beside the next step's dynamics (ml_ctx_set_overlap)
   hipDeviceProp_t prop{};
   std::string err;
   std::vector<Pool *> pools;
@@ -126,7 +137,8 @@ template <class R> Bufs<R> bufs(const Pool &p, const int *runs) {
   b.zs = (const double *)p.zs; b.dzs = (const double *)p.dzs; b.hz = (const double *)p.hz; b.sym = (const double *)p.sym;
   b.twx = (const double *)p.twx; b.twy = (const double *)p.twy; b.runs = runs;
   b.efac = (const double *)p.efac; b.lrho = (R *)p.lrho; b.lmark = (int *)p.lmark; b.lnode = (double *)p.lnode; b.chan = (int *)p.chan; b.cdep = (double *)p.cdep; b.flog = (double *)p.flog;
-  b.ehat = (double *)p.ehat;
+  b.ehat = (double *)p.ehat; b.emean0 = (double *)p.emean0; b.dspec = (double *)p.dspec;
+  b.fillp = (double *)p.fillp; b.ftiles = p.ftiles; b.gpart = (double *)p.gpart; b.gmax = p.gmax;
   return b;
 }
 
@@ -155,16 +167,17 @@ template <class R> void spectralSolve(const Pool &p, const Bufs<R> &b, int nr, h
 }
 
 // One RK3 stage of the transport of every carried field, in the batch's tuned layout.
-template <class R> void launchStage(const Geo &g, const Bufs<R> &b, int nr, int nc, int stage, double dt, hipStream_t s, const Tuning &tu) {
+// Fields fbase .. fbase + nc - 1.
+template <class R> void launchStage(const Geo &g, const Bufs<R> &b, int nr, int nc, int fbase, int stage, double dt, hipStream_t s, const Tuning &tu) {
   if (tu.layout == 1) {
     const unsigned zBlocks = unsigned((g.nz + tu.levels - 1) / tu.levels);
-    hipLaunchKernelGGL(k_stage_level<R>, dim3(zBlocks * nc * nr), dim3(TPB), stageLevelLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, tu.xcd, tu.levels);
+    hipLaunchKernelGGL(k_stage_level<R>, dim3(zBlocks * nc * nr), dim3(TPB), stageLevelLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, fbase, tu.xcd, tu.levels);
   } else if (tu.layout == 2) {
     const unsigned zBlocks = unsigned((g.nz + tu.levels - 1) / tu.levels);
-    hipLaunchKernelGGL(k_stage_plane<R>, dim3(zBlocks * nc * nr), dim3(TPB), stagePlaneLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, tu.xcd, tu.levels);
+    hipLaunchKernelGGL(k_stage_plane<R>, dim3(zBlocks * nc * nr), dim3(TPB), stagePlaneLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, fbase, tu.xcd, tu.levels);
   } else {
     const StageTile st = stageTile(g);
-    hipLaunchKernelGGL(k_stage<R>, dim3(unsigned(st.tiles) * g.nz * nr * tu.groups), dim3(st.threads), stageLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, tu.groups, tu.xcd);
+    hipLaunchKernelGGL(k_stage<R>, dim3(unsigned(st.tiles) * g.nz * nr * tu.groups), dim3(st.threads), stageLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, fbase, tu.groups, tu.xcd);
   }
 }
 
@@ -175,22 +188,49 @@ template <class R> void launchStage(const Geo &g, const Bufs<R> &b, int nr, int 
 //   PART_AFTER     afterStep's lightning, then time += dt.
 enum StepPart { PART_DYNAMICS, PART_PHYSICS, PART_MIX, PART_AFTER, N_PARTS };
 
+// The lightning's discharge and conduction of a step (lightning.ts:261-276, 190-225). advanced: the run's clock has
+// already moved on by dt (the discharge of a step run beside the next step's dynamics).
+template <class R> void launchDischarge(const Pool &p, const Bufs<R> &b, int nr, double dt, hipStream_t s, Timer *t, int advanced) {
+  const Geo &g = p.g;
+  const unsigned ib = blocks(size_t(g.layer) * (g.nz - 2), 256);
+  // flashLimit (lightning.ts:46): 10 flashes per 3 s of model time.
+  const int limit = std::max(1, int(std::floor(10 * dt / 3 + .5)));
+  hipLaunchKernelGGL(k_discharge<R>, dim3(nr), dim3(DISCHARGE_THREADS), solveLdsBytes(g, p.chunk), s, g, b, p.plan, dt, limit, p.chunk, advanced);
+  if (t) t->mark("k_discharge");
+  hipLaunchKernelGGL(k_conduct<R>, dim3(ib, nr), dim3(256), 0, s, g, b, dt);
+  if (t) t->mark("k_conduct");
+  hipLaunchKernelGGL(k_ground_total<R>, dim3(nr), dim3(256), 0, s, g, b, 1, int(ib));
+  if (t) t->mark("k_ground_total");
+}
+
 // One part of a model step of the runs in `list` (device array of nr slots). With a timer, an event follows every
 // kernel (named by the kernel).
-template <class R> void launchPart(const Pool &p, const int *list, int nr, double dt, hipStream_t s, Timer *t, const Tuning &tu, int part) {
+// split (with lightning): the step is split around the lightning so that a step's discharge can run beside the next
+// step's dynamics. The dynamics part then leaves out everything that reads the charges (the charge falls' Courant
+// numbers, the charge falls and the charges' transport: PART_CHARGES, after the discharge), and the after-part leaves
+// out the discharge and conduction (launchDischarge, with the step after or at the end of the call). Nothing else of a
+// step reads or writes what the discharge does: the charges, the vertical field, the flash state and the electric
+// solve's kept spectra; it reads the cloudiness the after-part snapshots (F_CLOUDY) and has spectra of its own.
+enum : int { PART_CHARGES = 4 };
+template <class R> void launchPart(const Pool &p, const int *list, int nr, double dt, hipStream_t s, Timer *t, const Tuning &tu, int part, bool split = false) {
   const Geo &g = p.g;
   Bufs<R> b = bufs<R>(p, list);
   const int el = p.electrified, nc = el ? N_CARRIED : F_COLD + 1;
+  split = split && el;
   const unsigned nb = blocks(g.n, TPB), cb = blocks(size_t(g.layer) * (g.nz - 1), TPB);
   auto mark = [&](const char *n) { if (t) t->mark(n); };
   if (part == PART_DYNAMICS) {
     // beginStep: fall speeds, hail swath, sedimentation (model.ts:893-902).
     hipLaunchKernelGGL(k_begin<R>, dim3(nr), dim3(64), 0, s, g, b);
     mark("k_begin");
-    hipLaunchKernelGGL(k_fall<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt, el);
+    hipLaunchKernelGGL(k_fall<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt, el, split ? 0 : el);
     mark("k_fall");
-    hipLaunchKernelGGL(k_sediment<R>, dim3(blocks(g.layer, SED_COLS), nr), dim3(SED_COLS), size_t(2) * SED_COLS * g.nz * sizeof(float), s, g, b, dt, el);
+    hipLaunchKernelGGL(k_sediment<R>, dim3(blocks(g.layer, SED_COLS), nr), dim3(SED_COLS), size_t(2) * SED_COLS * g.nz * sizeof(float), s, g, b, dt, 0, el && !split ? N_FALLS : 4);
     mark("k_sediment");
+    if (el && !split) {
+      hipLaunchKernelGGL(k_ground_total<R>, dim3(nr), dim3(256), 0, s, g, b, 0, int(blocks(g.layer, SED_COLS)));
+      mark("k_ground_total");
+    }
     // dynamics: face projection (FluxTransport.setVelocity) and the three RK3 stages with the stage projections.
     hipLaunchKernelGGL(k_faces<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 0);
     mark("k_faces");
@@ -198,7 +238,7 @@ template <class R> void launchPart(const Pool &p, const int *list, int nr, doubl
     hipLaunchKernelGGL(k_faces<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 1);
     mark("k_faces");
     for (int stage = 0; stage < 3; stage++) {
-      launchStage<R>(g, b, nr, nc, stage, dt, s, tu);
+      launchStage<R>(g, b, nr, split ? int(F_CHARGE_ICE) : nc, 0, stage, dt, s, tu);
       mark(tu.layout == 2 ? "k_stage_plane" : tu.layout == 1 ? "k_stage_level" : "k_stage");
       if (stage < 2) {
         hipLaunchKernelGGL(k_pdiv<R>, dim3(cb, nr), dim3(TPB), 0, s, g, b, stage + 1, dt);
@@ -209,8 +249,24 @@ template <class R> void launchPart(const Pool &p, const int *list, int nr, doubl
       }
     }
   }
+  if (part == PART_CHARGES) {
+    // The charge falls with their carriers' speeds, then the charges' three RK3 stages with the step's mass fluxes.
+    hipLaunchKernelGGL(k_charge_courant<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt);
+    mark("k_charge_courant");
+    hipLaunchKernelGGL(k_sediment<R>, dim3(blocks(g.layer, SED_COLS), nr), dim3(SED_COLS), size_t(2) * SED_COLS * g.nz * sizeof(float), s, g, b, dt, 4, N_FALLS);
+    mark("k_sediment");
+    hipLaunchKernelGGL(k_ground_total<R>, dim3(nr), dim3(256), 0, s, g, b, 0, int(blocks(g.layer, SED_COLS)));
+    mark("k_ground_total");
+    for (int stage = 0; stage < 3; stage++) {
+      launchStage<R>(g, b, nr, N_CARRIED - F_CHARGE_ICE, F_CHARGE_ICE, stage, dt, s, tu);
+      mark(tu.layout == 2 ? "k_stage_plane" : tu.layout == 1 ? "k_stage_level" : "k_stage");
+    }
+  }
   // endStep: positivity fix + physics (physicsStep), turbulence + drag (mixStep), projection (model.ts:1103-1113).
   if (part == PART_PHYSICS) {
+    // The positivity-fix sums from the last transport stage's parts, in level order.
+    hipLaunchKernelGGL(k_fill_total<R>, dim3(nr), dim3(64), 0, s, g, b, tu.layout == 0 ? stageTile(g).tiles : 1);
+    mark("k_fill_total");
     hipLaunchKernelGGL((k_physics<R, double>), dim3(nb, nr), dim3(TPB), 0, s, g, b, dt, el);
     mark("k_physics");
   }
@@ -236,15 +292,27 @@ template <class R> void launchPart(const Pool &p, const int *list, int nr, doubl
     spectralSolve<R>(p, b, nr, s, 2, g.nz - 2, 2, t);
     hipLaunchKernelGGL(k_strongest<R>, dim3(ib, nr), dim3(256), 0, s, g, b);
     mark("k_strongest");
-    // flashLimit (lightning.ts:46): 10 flashes per 3 s of model time.
-    int limit = std::max(1, int(std::floor(10 * dt / 3 + .5)));
-    hipLaunchKernelGGL(k_discharge<R>, dim3(nr), dim3(DISCHARGE_THREADS), solveLdsBytes(g, p.chunk), s, g, b, p.plan, dt, limit, p.chunk);
-    mark("k_discharge");
-    hipLaunchKernelGGL(k_conduct<R>, dim3(ib, nr), dim3(256), 0, s, g, b, dt);
-    mark("k_conduct");
+    if (!split) launchDischarge<R>(p, b, nr, dt, s, t, 0);
   }
   hipLaunchKernelGGL(k_end<R>, dim3(nr), dim3(64), 0, s, g, b, dt);
   mark("k_end");
+}
+
+// One step split around its lightning (with lightning only). prev: the discharge of the step before runs first on
+// the side stream, beside this step's dynamics, and this step's charges wait for it. The step's own discharge is left
+// to the next split step or to launchDischarge at the end of the call.
+template <class R> void launchStepSplit(const Pool &p, const int *list, int nr, double dt, ml_ctx *c, const Tuning &tu, bool prev) {
+  hipStream_t s = c->stream;
+  if (prev) {
+    hipEventRecord(c->fork, s);
+    hipStreamWaitEvent(c->side, c->fork, 0);
+    launchDischarge<R>(p, bufs<R>(p, list), nr, dt, c->side, nullptr, 1);
+    hipEventRecord(c->join, c->side);
+  }
+  launchPart<R>(p, list, nr, dt, s, nullptr, tu, PART_DYNAMICS, true);
+  if (prev) hipStreamWaitEvent(s, c->join, 0);
+  launchPart<R>(p, list, nr, dt, s, nullptr, tu, PART_CHARGES, true);
+  for (int part = PART_PHYSICS; part < N_PARTS; part++) launchPart<R>(p, list, nr, dt, s, nullptr, tu, part, true);
 }
 
 // One model step: AtmosphereModel.step (model.ts:873-877), the parts in order.
@@ -276,9 +344,14 @@ void fillTables(Pool &p, std::vector<char> &sym, std::vector<char> &twx, std::ve
 }
 
 struct Sizes {
-  size_t fields, stage, mass, node, node8, spec, mean0, cols, lev, par, scal, lmark, lnode, chan, cdep, flog;
-  size_t slot() const { return fields + 2 * stage + mass + 3 * node + 4 * node8 + 2 * spec + mean0 + cols + lev + par + scal + lmark + lnode + chan + cdep + flog; }
+  size_t fields, stage, mass, node, node8, spec, mean0, cols, lev, par, scal, lmark, lnode, chan, cdep, flog, fillp, gpart;
+  size_t slot() const {
+    return fields + 2 * stage + mass + 3 * node + 4 * node8 + 3 * spec + 2 * mean0 + cols + lev + par + scal + lmark + lnode + chan + cdep + flog +
+           fillp + gpart;
+  }
 };
+// Blocks of a run in the launches that carry charge to the ground: the charge falls and the conduction.
+int groundBlocks(const Geo &g) { return std::max(int(blocks(g.layer, SED_COLS)), int(blocks(size_t(g.layer) * (g.nz - 2), 256))); }
 Sizes sizesOf(const Geo &g, size_t rb) {
   Sizes s;
   s.fields = size_t(N_FIELDS) * g.n * 4; s.stage = size_t(N_CARRIED) * g.n * 4; s.mass = 4 * size_t(g.n) * rb; s.node = size_t(g.n) * rb; s.node8 = size_t(g.n) * 8;
@@ -286,6 +359,8 @@ Sizes sizesOf(const Geo &g, size_t rb) {
   s.lev = size_t(N_LEVELS) * g.nz * 8; s.par = N_PARAMS * 8; s.scal = N_SCALARS * 8;
   s.lmark = size_t(N_LMARK) * g.n * 4; s.lnode = size_t(N_LNODE) * g.n * 8; s.chan = 2 * size_t(MAX_CHANNEL) * 4; s.cdep = size_t(MAX_CHANNEL) * 8;
   s.flog = size_t(FLASH_LOG) * FLASH_FIELDS * 8;
+  s.fillp = size_t(N_CONSERVED) * g.nz * stageTile(g).tiles * 3 * 8;
+  s.gpart = 2 * size_t(groundBlocks(g)) * 8;
   return s;
 }
 int geoOf(ml_ctx *c, const ml_grid *o, Geo &g, Plan &pl) {
@@ -308,6 +383,10 @@ bool decode(ml_ctx *c, int run, Pool *&p, int &slot) {
 void freeBatch(Batch *bt) {
   if (bt->exec) hipGraphExecDestroy(bt->exec);
   if (bt->graph) hipGraphDestroy(bt->graph);
+  if (bt->first) hipGraphExecDestroy(bt->first);
+  if (bt->firstGraph) hipGraphDestroy(bt->firstGraph);
+  if (bt->next) hipGraphExecDestroy(bt->next);
+  if (bt->nextGraph) hipGraphDestroy(bt->nextGraph);
   if (bt->dev) hipFree(bt->dev);
   delete bt;
 }
@@ -327,7 +406,17 @@ int ml_ctx_create(int device, int precision, ml_ctx **out) {
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipGetDeviceProperties(&c->prop, device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) { delete c; return ML_E_HIP; }
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    if (c->join) hipEventDestroy(c->join);
+    if (c->fork) hipEventDestroy(c->fork);
+    if (c->side) hipStreamDestroy(c->side);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return ML_E_HIP;
+  }
   *out = c;
   return 0;
 }
@@ -340,7 +429,8 @@ void freePool(Pool *p) {
   dropBatches(p);
   void *all[] = {p->fields, p->stx, p->sty, p->mass, p->phi, p->cells, p->press, p->spec, p->mean0, p->km, p->kh, p->ephi, p->cols,
                  p->lev, p->par, p->scal, p->zs, p->dzs, p->hz, p->sym, p->twx, p->twy, p->single, p->diag, p->dlist, p->lrho,
-                 p->lmark, p->lnode, p->chan, p->cdep, p->flog, p->efac, p->ehat};
+                 p->lmark, p->lnode, p->chan, p->cdep, p->flog, p->efac, p->ehat, p->emean0, p->dspec, p->fillp,
+                 p->gpart};
   for (void *a : all) if (a) hipFree(a);
   delete p;
 }
@@ -352,7 +442,11 @@ void ml_ctx_destroy(ml_ctx *c) {
   if (!c) return;
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
+  hipStreamSynchronize(c->side);
   for (Pool *p : c->pools) if (p) freePool(p);
+  hipEventDestroy(c->join);
+  hipEventDestroy(c->fork);
+  hipStreamDestroy(c->side);
   hipStreamDestroy(c->stream);
   delete c;
 }
@@ -374,6 +468,8 @@ int ml_device_name(ml_ctx *c, char *out, int size) {
   std::snprintf(out, size, "%s (%s)", c->prop.name, c->prop.gcnArchName);
   return 0;
 }
+
+static_assert(int(F_CLOUDY) == int(ML_FIELDS) && int(N_FIELDS) == int(ML_FIELDS) + 1, "the ABI's fields, then the internal cloudiness");
 
 int64_t ml_slot_bytes(ml_ctx *c, const ml_grid *grid) {
   Geo g; Plan pl;
@@ -398,10 +494,12 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
   size_t cap = capacity;
   struct { void **ptr; size_t bytes; } allocs[] = {
     {&p->fields, s.fields * cap}, {&p->stx, s.stage * cap}, {&p->sty, s.stage * cap}, {&p->mass, s.mass * cap}, {&p->phi, s.node8 * cap},
-    {&p->cells, s.node8 * cap}, {&p->press, s.node8 * cap}, {&p->spec, s.spec * cap}, {&p->ehat, s.spec * cap}, {&p->mean0, s.mean0 * cap}, {&p->km, s.node * cap},
+    {&p->cells, s.node8 * cap}, {&p->press, s.node8 * cap}, {&p->spec, s.spec * cap}, {&p->ehat, s.spec * cap}, {&p->dspec, s.spec * cap}, {&p->mean0, s.mean0 * cap}, {&p->emean0, s.mean0 * cap}, {&p->km, s.node * cap},
     {&p->kh, s.node * cap}, {&p->ephi, s.node8 * cap}, {&p->cols, s.cols * cap}, {&p->lev, s.lev * cap}, {&p->par, s.par * cap},
     {&p->scal, s.scal * cap}, {(void **)&p->diag, cap * DIAG_ROW * 8}, {(void **)&p->dlist, cap * sizeof(int)}, {&p->lrho, s.node * cap}, {&p->lmark, s.lmark * cap},
-    {&p->lnode, s.lnode * cap}, {&p->chan, s.chan * cap}, {&p->cdep, s.cdep * cap}, {&p->flog, s.flog * cap}};
+    {&p->lnode, s.lnode * cap}, {&p->chan, s.chan * cap}, {&p->cdep, s.cdep * cap}, {&p->flog, s.flog * cap},
+    {&p->fillp, s.fillp * cap}, {&p->gpart, s.gpart * cap}};
+  p->ftiles = stageTile(g).tiles; p->gmax = groundBlocks(g);
   for (auto &a : allocs) {
     hipError_t e = hipMalloc(a.ptr, a.bytes);
     if (e == hipSuccess) e = hipMemset(*a.ptr, 0, a.bytes);
@@ -552,7 +650,7 @@ int ml_run_release(ml_ctx *c, int run) {
 
 int ml_run_upload(ml_ctx *c, int run, int field, const float *host) {
   Pool *p; int slot;
-  if (!c || !decode(c, run, p, slot) || field < 0 || field >= N_FIELDS || !host) return fail(c, ML_E_ARG, "upload: bad run or field");
+  if (!c || !decode(c, run, p, slot) || field < 0 || field >= ML_FIELDS || !host) return fail(c, ML_E_ARG, "upload: bad run or field");
   size_t n = p->g.n;
   HIPCHK(c, hipMemcpyAsync((float *)p->fields + (size_t(slot) * N_FIELDS + field) * n, host, n * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -560,7 +658,7 @@ int ml_run_upload(ml_ctx *c, int run, int field, const float *host) {
 }
 int ml_run_download(ml_ctx *c, int run, int field, float *host) {
   Pool *p; int slot;
-  if (!c || !decode(c, run, p, slot) || field < 0 || field >= N_FIELDS || !host) return fail(c, ML_E_ARG, "download: bad run or field");
+  if (!c || !decode(c, run, p, slot) || field < 0 || field >= ML_FIELDS || !host) return fail(c, ML_E_ARG, "download: bad run or field");
   size_t n = p->g.n;
   HIPCHK(c, hipMemcpyAsync(host, (float *)p->fields + (size_t(slot) * N_FIELDS + field) * n, n * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -711,7 +809,7 @@ template <class R> float timeStage(ml_ctx *c, const Pool &p, const Batch &bt, co
   float best = 1e30f;
   for (int rep = 0; rep < 3; rep++) {
     (void)hipEventRecord(e0, c->stream);
-    launchStage<R>(g, b, nr, nc, 0, 3.0, c->stream, tu);
+    launchStage<R>(g, b, nr, nc, 0, 0, 3.0, c->stream, tu);
     (void)hipEventRecord(e1, c->stream);
     (void)hipEventSynchronize(e1);
     float ms = 0;
@@ -798,22 +896,66 @@ static int batchOf(ml_ctx *c, const int *runs, int count, Pool *&p, Batch *&bt) 
   return tune(c, p, bt);
 }
 
+int ml_ctx_set_overlap(ml_ctx *c, int on) {
+  if (!c || (on != 0 && on != 1)) return fail(c, ML_E_ARG, "overlap: 0 or 1");
+  c->overlap = on;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+// Whether the batch steps split around the lightning (a pool with lightning, overlap on).
+bool overlapped(const ml_ctx *c, const Pool *p) { return c->overlap && p->electrified; }
+
+// The batch's step graphs for dt: one whole step, or with the overlap the split step alone (the first of a call) and
+// beside the discharge of the step before (the others).
+int captureSteps(ml_ctx *c, Pool *p, Batch *bt, int count, double dt) {
+  const int ov = overlapped(c, p) ? 1 : 0;
+  if (bt->dt == dt && bt->overlap == ov && (ov ? bt->first != nullptr : bt->exec != nullptr)) return 0;
+  hipGraphExec_t *execs[3] = {&bt->exec, &bt->first, &bt->next};
+  hipGraph_t *graphs[3] = {&bt->graph, &bt->firstGraph, &bt->nextGraph};
+  for (int k = 0; k < 3; k++) {
+    if (*execs[k]) { hipGraphExecDestroy(*execs[k]); *execs[k] = nullptr; }
+    if (*graphs[k]) { hipGraphDestroy(*graphs[k]); *graphs[k] = nullptr; }
+  }
+  for (int k = ov ? 1 : 0; k < (ov ? 3 : 1); k++) {
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    if (k == 0) {
+      if (p->rb == 8) launchStep<double>(*p, bt->dev, count, dt, c->stream, nullptr, bt->tune);
+      else launchStep<float>(*p, bt->dev, count, dt, c->stream, nullptr, bt->tune);
+    } else {
+      if (p->rb == 8) launchStepSplit<double>(*p, bt->dev, count, dt, c, bt->tune, k == 2);
+      else launchStepSplit<float>(*p, bt->dev, count, dt, c, bt->tune, k == 2);
+    }
+    HIPCHK(c, hipStreamEndCapture(c->stream, graphs[k]));
+    HIPCHK(c, hipGraphInstantiate(execs[k], *graphs[k], nullptr, nullptr, 0));
+  }
+  bt->dt = dt; bt->overlap = ov;
+  return 0;
+}
+
+// Step k (0-based) of a call of `steps`: the graph replay, and after the last the discharge of that step.
+int launchStepOf(ml_ctx *c, Pool *p, Batch *bt, int count, double dt, int k, int steps) {
+  if (!bt->overlap) { HIPCHK(c, hipGraphLaunch(bt->exec, c->stream)); return 0; }
+  HIPCHK(c, hipGraphLaunch(k == 0 ? bt->first : bt->next, c->stream));
+  if (k == steps - 1) {
+    if (p->rb == 8) launchDischarge<double>(*p, bufs<double>(*p, bt->dev), count, dt, c->stream, nullptr, 1);
+    else launchDischarge<float>(*p, bufs<float>(*p, bt->dev), count, dt, c->stream, nullptr, 1);
+  }
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
 int ml_step(ml_ctx *c, const int *runs, int count, int steps, double dt) {
   Pool *p; Batch *bt;
   if (int r = batchOf(c, runs, count, p, bt)) return r;
   if (steps < 0 || !(dt > 0)) return fail(c, ML_E_ARG, "step: steps >= 0, dt > 0");
   HIPCHK(c, hipSetDevice(c->device));
-  if (!bt->exec || bt->dt != dt) {
-    if (bt->exec) { hipGraphExecDestroy(bt->exec); bt->exec = nullptr; }
-    if (bt->graph) { hipGraphDestroy(bt->graph); bt->graph = nullptr; }
-    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    if (p->rb == 8) launchStep<double>(*p, bt->dev, count, dt, c->stream, nullptr, bt->tune);
-    else launchStep<float>(*p, bt->dev, count, dt, c->stream, nullptr, bt->tune);
-    HIPCHK(c, hipStreamEndCapture(c->stream, &bt->graph));
-    HIPCHK(c, hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0));
-    bt->dt = dt;
-  }
-  for (int k = 0; k < steps; k++) HIPCHK(c, hipGraphLaunch(bt->exec, c->stream));
+  if (int r = captureSteps(c, p, bt, count, dt)) return r;
+  for (int k = 0; k < steps; k++) if (int r = launchStepOf(c, p, bt, count, dt, k, steps)) return r;
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -864,23 +1006,24 @@ int ml_batch_set_tuning(ml_ctx *c, const int *runs, int count, const int *shape)
   if (shape[0] == 2 && !planeFits(p->g, p->rb, c->prop.sharedMemPerBlock)) return fail(c, ML_E_UNSUPPORTED, "set tuning: the plane layout needs nx, ny >= 6 and its LDS within a block's");
   bt->tune.layout = shape[0]; bt->tune.groups = shape[0] == 0 ? shape[1] : 1; bt->tune.xcd = shape[2];
   bt->tune.levels = shape[0] == 0 ? 1 : shape[3]; bt->tune.ms = 0;
-  // The captured step holds the old launch shape.
-  if (bt->exec) { hipGraphExecDestroy(bt->exec); bt->exec = nullptr; }
-  if (bt->graph) { hipGraphDestroy(bt->graph); bt->graph = nullptr; }
+  // The captured steps hold the old launch shape: captured again at the next step.
+  bt->dt = 0;
   return 0;
 }
 
 int ml_step_timed(ml_ctx *c, const int *runs, int count, int steps, double dt, double *ms) {
   if (!ms || steps < 1) return fail(c, ML_E_ARG, "step timed: steps >= 1 and an output array");
-  // Capture (or reuse) the batch's graph with one step, then time each replay between events on the stream.
+  // Capture (or reuse) the batch's step graphs, then time each step between events on the stream (with the overlap, a
+  // step's time holds the discharge of the step before it, and the last also its own).
   Pool *p; Batch *bt;
   if (int r = batchOf(c, runs, count, p, bt)) return r;
-  if (!bt->exec || bt->dt != dt) { if (int r = ml_step(c, runs, count, 0, dt)) return r; }
+  HIPCHK(c, hipSetDevice(c->device));
+  if (int r = captureSteps(c, p, bt, count, dt)) return r;
   std::vector<hipEvent_t> ev(steps + 1);
   for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
   HIPCHK(c, hipEventRecord(ev[0], c->stream));
   for (int k = 0; k < steps; k++) {
-    HIPCHK(c, hipGraphLaunch(bt->exec, c->stream));
+    if (int r = launchStepOf(c, p, bt, count, dt, k, steps)) return r;
     HIPCHK(c, hipEventRecord(ev[k + 1], c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));

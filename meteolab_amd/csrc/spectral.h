@@ -137,7 +137,7 @@ template <class R> __device__ void forwardLevel(const Geo &g, const Bufs<R> &b, 
     sum += v0 + v1;
   }
   sum = blockSum(sum, red);  // contains barriers
-  if (threadIdx.x == 0) b.m0(slot, g)[lev] = sum;
+  if (threadIdx.x == 0) (keep ? b.em0(slot, g) : b.m0(slot, g))[lev] = sum;
   // Row pairs: z = row_2p + i row_2p+1, complex FFT of length nx.
   cx<S> *Z = fftStages(A, B, nx, half, nx, 1, tx, pl.rx, pl.nrx, -1);
   cx<S> *C = Z == A ? B : A;
@@ -247,7 +247,7 @@ template <class R> __device__ void modeSolve(const Geo &g, const Bufs<R> &b, int
   const int nz = g.nz;
   const double h = -a / 2;
   // The m = n = 0 mode (the level means) takes its right-hand side from the float64 level sums (its imaginary part is 0).
-  const double *m0 = mode == 0 ? b.m0(slot, g) : nullptr;
+  const double *m0 = mode == 0 ? (kind == 2 ? b.em0(slot, g) : b.m0(slot, g)) : nullptr;
   // KB levels' loads in flight at a time: the right-hand sides first, then their elimination.
   constexpr int KB = 8;
   double pr = 0, pi = 0, cprev = 0;

@@ -1,0 +1,83 @@
+"""A step's lightning beside the next step's dynamics (ml_ctx_set_overlap): runs stepped with the overlap, in one call
+or in several, end bit for bit as the same runs stepped with each step's parts in line: fields, ground columns, run
+metrics, flashes, rotation and model diagnostics, through steps that flash."""
+import numpy as np
+import pytest
+
+import meteolab_amd as M
+from meteolab_amd import config as C
+from oracle.binding import OracleRun
+
+pytestmark = pytest.mark.gpu
+
+
+def charged(g, run, dry, spec):
+    """A charged cloud above the breakdown threshold (the state test's, state.test.ts:25-52): it flashes at once."""
+    nx, ny, nz = g["nx"], g["ny"], g["nz"]
+    dx, dy = g["width"] / nx, g["depth"] / ny
+    rho = run.levels()["rho"]
+    zlev, _ = OracleRun(g, dry, spec).levels()
+    x, y, z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="xy")
+    r2 = ((x - nx / 2) * dx) ** 2 + ((y - ny / 2) * dy) ** 2
+    ci = 3e-9 * np.exp(-r2 / 2500 ** 2 - ((zlev[z] - 9000) / 700) ** 2) / rho[z]
+    cg = -3e-9 * np.exp(-r2 / 2500 ** 2 - ((zlev[z] - 6500) / 700) ** 2) / rho[z]
+    run.set_field("chargeIce", np.transpose(ci, (2, 0, 1)).reshape(-1))
+    run.set_field("chargeGraupel", np.transpose(cg, (2, 0, 1)).reshape(-1))
+
+
+def setup(ctx, storm):
+    g = C.small_grid()
+    pool = ctx.pool(g, 3)
+    dry = C.profile(qv_max=0.0)
+    spec = C.run(g, seed=7, amp=0.0, noise_amp=0.0)
+    a = pool.run(dry, spec)
+    charged(g, a, dry, spec)
+    b = pool.run(C.profile(), C.run(g, seed=1))
+    for name in M.CARRIED + ["fieldZ"]:
+        b.set_field(name, storm.field(name))
+    for name in M.COLUMNS:
+        b.set_column(name, storm.column(name))
+    b.set_time(storm.diag()["time"])
+    c = pool.run(dry, dict(spec, seed=11))
+    charged(g, c, dry, spec)
+    return [a, b, c]
+
+
+def state(r):
+    return ({k: r.field(k) for k in M.CARRIED + ["fieldZ"]}, {k: r.column(k) for k in M.COLUMNS}, r.diagnostics(),
+            r.flashes(), r.rotation(), r.model_diagnostics())
+
+
+def same(s, t):
+    for name in s[0]:
+        assert np.array_equal(s[0][name], t[0][name]), name
+    for name in s[1]:
+        assert np.array_equal(s[1][name], t[1][name]), name
+    assert s[2:] == t[2:]
+
+
+@pytest.fixture(scope="module")
+def storm():
+    g = C.small_grid()
+    o = OracleRun(g, C.profile(), C.run(g, seed=1))
+    o.step(650)
+    return o
+
+
+def test_the_overlapped_steps_end_as_the_steps_in_line(storm):
+    # One context: the transport layout is fixed per grid in a context (the layouts round differently in float32).
+    results = []
+    with M.Context(0, 32) as ctx:
+        for overlap, calls in ((False, [6]), (True, [6]), (True, [1, 2, 3])):
+            ctx.set_overlap(overlap)
+            runs = setup(ctx, storm)
+            print("overlap", overlap, "calls", calls, flush=True)
+            for n in calls:
+                M.step(runs, n)
+            ctx.sync()
+            results.append([state(r) for r in runs])
+            assert results[-1][0][2]["flashesIC"] + results[-1][0][2]["flashesCG"] > 0
+            runs[0].pool.destroy()
+    for k in (1, 2):
+        for a, b in zip(results[0], results[k]):
+            same(a, b)
