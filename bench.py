@@ -29,12 +29,20 @@ DT = 3.0
 METRIC = "cell-steps/s on 1 MI355X, batched vs plain; p50/p99 latency per model-minute"
 
 
-def stage_bytes(n, stage, carried=17):
-    """Algorithmic HBM bytes of one k_stage launch (one RK3 stage of every carried field) per run of n nodes: each
-    field's stage input read, its start read (stage 0: the same array), its output written (float32), the face mass
-    fluxes and their divergence read once (4 x float32) and the 8 fields of the buoyancy read once."""
+def stage_bytes(n, stage, carried=17, buoyancy=True):
+    """Algorithmic HBM bytes of one k_stage launch (one RK3 stage of `carried` fields) per run of n nodes: each field's
+    stage input read, its start read (stage 0: the same array), its output written (float32), the face mass fluxes and
+    their divergence read once (4 x float32) and, with w among the fields, the 8 fields of the buoyancy read once."""
     per_field = 8 if stage == 0 else 12
-    return n * (carried * per_field + 16 + 8 * 4)
+    return n * (carried * per_field + 16 + (8 * 4 if buoyancy else 0))
+
+
+def stage_bytes_per_step(n, split):
+    """The transport's algorithmic bytes of a step per run: three launches of the 17 carried fields, or, with the step
+    split around its lightning, three of the 13 fields before the charges and three of the 4 charges."""
+    if split:
+        return sum(stage_bytes(n, s, 13) + stage_bytes(n, s, 4, buoyancy=False) for s in range(3))
+    return sum(stage_bytes(n, s) for s in range(3))
 
 
 # Compulsory float32 traffic of a whole step per cell (the step's fields, each phase fused, no reuse across the step's
@@ -48,16 +56,17 @@ def tuning_args(t):
     return {"layout": t["transport_layout"], "groups": t["field_groups"], "xcd": t["xcd_per_run"], "levels": t["levels_per_block"]}
 
 
-def measured_traffic(kernel, runs, grid, tuning):
-    """HBM bytes per launch of the transport kernel from the committed counter summary (rocprofv3 --pmc FETCH_SIZE and
-    WRITE_SIZE, scaled by a copy kernel of known size) when it was taken on this workload and launch shape, else None."""
+def measured_traffic(kernel, runs, grid, tuning, split):
+    """HBM bytes per launch of the transport kernel (the mean over a step's launches) from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size) when it was taken on this
+    workload, launch shape and step split, else None."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_ensemble{runs}_{grid['nx']}x{grid['ny']}x{grid['nz']}_hbm.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         rec = json.load(f)
     shape = {k: tuning[k] for k in ("transport_layout", "field_groups", "levels_per_block", "xcd_per_run")}
-    return rec["bytes_per_launch"] if rec.get("shape") == shape else None
+    return rec["bytes_per_launch"] if rec.get("shape") == shape and rec.get("split", False) == split else None
 
 
 def percentile(a, q):
@@ -87,6 +96,8 @@ def main():
     ap.add_argument("--plain-members", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--compare-in-line", action="store_true",
+                    help="also time the same runs with each step's lightning in line (ml_ctx_set_overlap off)")
     ap.add_argument("--transport", default="auto",
                     help="the transport kernel's launch shape: auto (the autotuner's), or layout:n:xcd with layout strip "
                          "(n field groups) or level / plane (n levels a block) and xcd 0 or 1")
@@ -118,17 +129,22 @@ def main():
     per_minute = int(round(60 / DT))
     minutes = [ms[k:k + per_minute].sum() for k in range(0, len(ms) - per_minute + 1, per_minute)]
 
-    # The same members through the same steps with each step's lightning in line (ml_ctx_set_overlap off): what running
-    # a step's discharge beside the next step's dynamics gains.
-    ctx.set_overlap(False)
-    lpool = ctx.pool(grid, a.runs)
-    lruns = [lpool.run(prof, C.ensemble_member(grid, k)) for k in range(a.runs)]
-    M.set_batch_tuning(lruns, **tuning_args(M.batch_tuning(runs)))
-    if a.warmup:
-        M.step(lruns, a.warmup, DT)
-    lms = M.step_timed(lruns, a.steps, DT)
-    lpool.destroy()
-    ctx.set_overlap(True)
+    # Optionally the same members through the same steps with each step's lightning in line (ml_ctx_set_overlap off):
+    # what running a step's discharge beside the next step's dynamics gains. (Off by default: its whole-field transport
+    # launches would mix into a kernel trace of this command.)
+    in_line = None
+    if a.compare_in_line:
+        ctx.set_overlap(False)
+        lpool = ctx.pool(grid, a.runs)
+        lruns = [lpool.run(prof, C.ensemble_member(grid, k)) for k in range(a.runs)]
+        M.set_batch_tuning(lruns, **tuning_args(M.batch_tuning(runs)))
+        if a.warmup:
+            M.step(lruns, a.warmup, DT)
+        lms = M.step_timed(lruns, a.steps, DT)
+        lpool.destroy()
+        ctx.set_overlap(True)
+        in_line = {"ms_per_step": float(lms.mean()), "value": a.runs * n * a.steps / (lms.sum() / 1e3),
+                   "overlap_speedup": float(lms.sum() / ms.sum())}
 
     # Kernel times over the same workload: the same members from their initial state through the same steps, launched
     # eagerly with HIP events between the kernels on the library's stream (the graph replays above cannot be split by
@@ -141,8 +157,10 @@ def main():
     kt = M.kernel_times(M.profile_step(eruns, a.steps, DT))
     epool.destroy()
     stage_kernel = next(k for k in ("k_stage_plane", "k_stage_level", "k_stage") if k in kt)
-    stage_ms = kt[stage_kernel][1]
-    stage_b = sum(stage_bytes(n, s) for s in range(3)) * a.runs
+    # Per step: the transport's launches (3, or 6 with the step split around its lightning), their time and bytes.
+    split = "k_charge_courant" in kt
+    stage_launches, stage_ms = kt[stage_kernel]
+    stage_b = stage_bytes_per_step(n, split) * a.runs
     achieved = stage_b / (stage_ms / 1e3)
     dominant = max(kt.items(), key=lambda kv: kv[1][1])[0]
 
@@ -178,14 +196,14 @@ def main():
         "plain": {"value": plain, "unit": "cell-steps/s", "ms_per_step": float(pms.mean()), "steps": a.steps,
                   "members": members},
         "batched_over_plain": batched / plain,
-        "lightning_in_line": {"ms_per_step": float(lms.mean()), "value": a.runs * n * a.steps / (lms.sum() / 1e3),
-                              "overlap_speedup": float(lms.sum() / ms.sum())},
+
         "latency_per_model_minute_ms": {"p50": percentile(minutes, 50), "p99": percentile(minutes, 99),
                                         "runs_per_batch": a.runs, "minutes": len(minutes)},
         "roofline": {"bound": "hbm", "kernel": stage_kernel, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM,
-                     "traffic": measured_traffic(stage_kernel, a.runs, grid, M.batch_tuning(runs)),
-                     "bytes_per_launch": stage_b / 3, "ms_per_launch": stage_ms / 3},
+                     "traffic": measured_traffic(stage_kernel, a.runs, grid, M.batch_tuning(runs), split),
+                     "bytes_per_launch": stage_b / stage_launches, "ms_per_launch": stage_ms / stage_launches,
+                     "launches_per_step": stage_launches},
         "step_roofline": {"bytes_per_cell_step": STEP_BYTES_PER_CELL, "achieved_GBps": batched * STEP_BYTES_PER_CELL / 1e9,
                           "frac": batched * STEP_BYTES_PER_CELL / PEAK_HBM},
         "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][1])},
@@ -194,6 +212,8 @@ def main():
         "copy_GBps": ctx.copy_bandwidth(),
         "device": ctx.adapter,
     }
+    if in_line:
+        out["lightning_in_line"] = in_line
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(grid, a.cpu_steps)
     ctx.close()

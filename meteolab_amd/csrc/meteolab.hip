@@ -298,6 +298,16 @@ template <class R> void launchPart(const Pool &p, const int *list, int nr, doubl
   mark("k_end");
 }
 
+// One step with a timer after each kernel, in line: whole, or split around its lightning with the discharge after it.
+template <class R> void launchStepTimed(const Pool &p, const int *list, int nr, double dt, hipStream_t s, Timer *t, const Tuning &tu, bool split) {
+  if (!split) { launchStep<R>(p, list, nr, dt, s, t, tu); return; }
+  t->mark("start");
+  launchPart<R>(p, list, nr, dt, s, t, tu, PART_DYNAMICS, true);
+  launchPart<R>(p, list, nr, dt, s, t, tu, PART_CHARGES, true);
+  for (int part = PART_PHYSICS; part < N_PARTS; part++) launchPart<R>(p, list, nr, dt, s, t, tu, part, true);
+  launchDischarge<R>(p, bufs<R>(p, list), nr, dt, s, t, 1);
+}
+
 // One step split around its lightning (with lightning only). prev: the discharge of the step before runs first on
 // the side stream, beside this step's dynamics, and this step's charges wait for it. The step's own discharge is left
 // to the next split step or to launchDischarge at the end of the call.
@@ -1082,9 +1092,13 @@ int ml_profile_step(ml_ctx *c, const int *runs, int count, int steps, double dt,
   Timer t{c->stream, &events, &labels};
   std::vector<double> acc;
   std::vector<std::string> first;
+  // The kernels ml_step runs, in line: with the overlap on, the step split around its lightning and the discharge
+  // right after the step (what the overlap runs beside the next step).
+  const bool split = overlapped(c, p);
   for (int k = 0; k < steps; k++) {
     events.clear(); labels.clear();
-    if (p->rb == 8) launchStep<double>(*p, bt->dev, count, dt, c->stream, &t, bt->tune); else launchStep<float>(*p, bt->dev, count, dt, c->stream, &t, bt->tune);
+    if (p->rb == 8) launchStepTimed<double>(*p, bt->dev, count, dt, c->stream, &t, bt->tune, split);
+    else launchStepTimed<float>(*p, bt->dev, count, dt, c->stream, &t, bt->tune, split);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (acc.empty()) { acc.assign(events.size(), 0.0); first = labels; }
     for (size_t e = 1; e < events.size(); e++) { float v = 0; hipEventElapsedTime(&v, events[e - 1], events[e]); acc[e] += v; }
