@@ -32,7 +32,9 @@ enum {
   ML_CONDENSATE, ML_FIELDS
 };
 /* Ground columns (STATE_COLUMNS, src/core/state.ts:10). */
-enum { ML_PRECIPITATION = 0, ML_HAIL_GROUND, ML_HAIL_SIZE, ML_COLUMNS };
+/* Then an ensemble member's swaths (src/ensemble/member.ts:19-21): the strongest surface wind (m/s) and 2-5 km
+ * updraught helicity (m2/s2) over each ground column, kept while ml_run_set_swaths is on. */
+enum { ML_PRECIPITATION = 0, ML_HAIL_GROUND, ML_HAIL_SIZE, ML_GUST_MAX, ML_UH_MAX, ML_COLUMNS };
 /* Per-level base state (environment.ts:124-130), float64, nz each, in this order. */
 enum { ML_LEVEL_P = 0, ML_LEVEL_EXNER, ML_LEVEL_THETA, ML_LEVEL_Q, ML_LEVEL_U, ML_LEVEL_V, ML_LEVEL_RHO, ML_LEVELS = 7 };
 /* Diagnostics of a run (model.ts:1672-1696 and the run metrics), float64. */
@@ -132,6 +134,15 @@ int ml_run_rotation(ml_ctx *ctx, int run, double *out);
 /* Recomputes the rotation diagnostics from the run's fields as they are, keeping how long the criterion has held
  * (refreshRotation, model.ts:1607; applyState calls it, src/core/state.ts:73-74). */
 int ml_run_refresh_rotation(ml_ctx *ctx, int run);
+/* Whether each step of the run folds its surface wind and 2-5 km updraught helicity into the ML_GUST_MAX and ML_UH_MAX
+ * columns over the ground under the moving domain (addSwaths, src/ensemble/member.ts:85-100, which runMember calls after
+ * every step, :111-114). Off for a new run; the columns start at 0. */
+int ml_run_set_swaths(ml_ctx *ctx, int run, int on);
+/* probabilityMap (src/ensemble/combine.ts:70-75) over `count` runs of one pool: per ground column (layer float32 into
+ * out) the share of the runs whose ground column `column` (ML_PRECIPITATION as the rain total, ML_HAIL_SIZE,
+ * ML_GUST_MAX, ML_UH_MAX) reached `threshold` there (for helicity the caller scales the threshold to the grid,
+ * uhScale). */
+int ml_probability_map(ml_ctx *ctx, const int *runs, int count, int column, double threshold, float *out);
 /* AtmosphereModel.diagnostics() (model.ts:1672-1696) of `count` runs of one pool from their fields as they are,
  * ML_MD_COUNT float64 a run into out (count * ML_MD_COUNT): the largest up- and downdraught (m/s), the heaviest
  * precipitation reaching the ground (mm/h, surfaceflux.ts:22-37 at the ground nodes) and the largest rain total (mm),

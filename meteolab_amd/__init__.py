@@ -20,7 +20,7 @@ CARRIED = ["rain", "snow", "graupel", "hail", "u", "v", "w", "theta", "q", "clou
 FIELDS = CARRIED + ["fieldZ", "fallRain", "fallSnow", "fallGraupel", "fallHail", "fallChargeIce",
                     "fallChargeGraupel", "condensate"]
 FIELD_ID = {name: k for k, name in enumerate(FIELDS)}
-COLUMNS = ["precipitation", "hailGround", "hailSize"]
+COLUMNS = ["precipitation", "hailGround", "hailSize", "gustMax", "uhMax"]
 LEVELS = ["p", "exner", "theta", "q", "u", "v", "rho"]
 DIAG = ["time", "maxW", "minW", "maxThetaP", "rainTotal", "water", "charge", "clipped", "negativeFilled",
         "chargeGround", "flashesIC", "flashesCG", "firstFlash", "error"]
@@ -81,6 +81,8 @@ SIGNATURES = {
     "ml_run_set_totals": (_i, [_vp, _i, _d, _i64, _i64, _d]),
     "ml_run_rotation": (_i, [_vp, _i, _dp]),
     "ml_run_refresh_rotation": (_i, [_vp, _i]),
+    "ml_run_set_swaths": (_i, [_vp, _i, _i]),
+    "ml_probability_map": (_i, [_vp, _ip, _i, _i, _d, _fp]),
     "ml_model_diagnostics": (_i, [_vp, _ip, _i, _dp]),
     "ml_run_flashes": (_i, [_vp, _i, _dp, _i]),
     "ml_step": (_i, [_vp, _ip, _i, _i, _d]),
@@ -291,6 +293,10 @@ class Run:
         self.ctx.check(lib().ml_run_rotation(self.ctx.h, self.h, _dptr(out)))
         return dict(zip(ROTATION, out))
 
+    def set_swaths(self, on):
+        """Fold each step's surface wind and helicity into the gustMax and uhMax columns (ml_run_set_swaths)."""
+        self.ctx.check(lib().ml_run_set_swaths(self.ctx.h, self.h, 1 if on else 0))
+
     def refresh_rotation(self):
         """Recomputes the rotation diagnostics from the fields as they are, keeping how long the rotation has held
         (ml_run_refresh_rotation, refreshRotation of model.ts:1607)."""
@@ -324,6 +330,16 @@ def model_diagnostics(runs):
     out = np.empty((len(runs), len(MODEL_DIAGNOSTICS)), np.float64)
     ctx.check(lib().ml_model_diagnostics(ctx.h, ids, len(runs), _dptr(out)))
     return [dict(zip(MODEL_DIAGNOSTICS, row)) for row in out]
+
+
+def probability_map(runs, column, threshold):
+    """probabilityMap over runs of one pool (ml_probability_map): per ground column the share of the runs whose column
+    reached the threshold there (float32, layer)."""
+    ctx = runs[0].ctx
+    ids = (ctypes.c_int * len(runs))(*[r.h for r in runs])
+    out = np.empty(runs[0].layer, np.float32)
+    ctx.check(lib().ml_probability_map(ctx.h, ids, len(runs), COLUMNS.index(column), float(threshold), _fptr(out)))
+    return out
 
 
 def step(runs, n=1, dt=3.0):

@@ -23,7 +23,8 @@ enum : int {
   N_FIELDS
 };
 // Ground columns (STATE_COLUMNS, src/core/state.ts:10).
-enum : int { C_PRECIPITATION = 0, C_HAIL_GROUND, C_HAIL_SIZE, N_COLUMNS };
+// Ground columns: STATE_COLUMNS (state.ts:10), then an ensemble member's swaths (ensemble/member.ts:19-21 gustMax, uhMax).
+enum : int { C_PRECIPITATION = 0, C_HAIL_GROUND, C_HAIL_SIZE, C_GUST_MAX, C_UH_MAX, N_COLUMNS };
 // Per-level tables of a run (double, nz each): the base state (src/core/environment.ts:124-130) and what the step
 // derives from it.
 enum : int {
@@ -36,7 +37,9 @@ enum : int {
 enum : int {
   P_FRAME_U = 0, P_FRAME_V, P_CORIOLIS, P_CS, P_DRAG, P_SPONGE_START, P_SEED, P_BUBBLE_AMP, P_BUBBLE_X, P_BUBBLE_Y,
   P_BUBBLE_Z, P_BUBBLE_RH, P_BUBBLE_RZ, P_NOISE_SEED, P_NOISE_AMP, P_NOISE_TOP, P_QVMAX, P_SHEAR, P_SHEAR_DEPTH,
-  P_THETA0, P_DTHETA, P_ZTR, P_TTR, P_RH_DROP, P_RH_TOP, P_ELECTRIFIED, P_TIME, N_PARAMS = 32
+  P_THETA0, P_DTHETA, P_ZTR, P_TTR, P_RH_DROP, P_RH_TOP, P_ELECTRIFIED, P_TIME,
+  P_SWATHS,  // nonzero: each step folds its surface wind and updraught helicity into the swath columns
+  N_PARAMS = 32
 };
 // Per-run scalars (8-byte words). Per step: cleared by k_begin; the rest accumulate over the run.
 enum : int {

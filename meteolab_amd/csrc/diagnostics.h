@@ -128,4 +128,14 @@ template <class R> __global__ __launch_bounds__(MDIAG_THREADS) void k_model_diag
   o[MD_OUTFLOW] = outflow; o[MD_OUTFLOWLOW] = outflowLow; o[MD_GUST] = gust; o[MD_HAIL] = hailMax; o[MD_CLIPPED] = double(clipped);
 }
 
+// probabilityMap (ensemble/combine.ts:70-75): per ground column the share of the launch's `count` runs whose column
+// `column` reached `threshold` there (a float32 count, then the quotient rounded to float32).
+template <class R> __global__ __launch_bounds__(256) void k_probability(Geo g, Bufs<R> b, int count, int column, double threshold, float *out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.layer) return;
+  float n = 0;
+  for (int k = 0; k < count; k++) if (double(b.col(b.runs[k], column, g)[c]) >= threshold) n += 1;
+  out[c] = float(double(n) / count);
+}
+
 }  // namespace ml

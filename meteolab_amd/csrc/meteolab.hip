@@ -67,6 +67,7 @@ struct Pool {
   DiagLevels dlev{};      // the levels the model diagnostics look at
   double *diag = nullptr; // [cap][DIAG_ROW]: rows of diagnostics
   int *dlist = nullptr;   // [cap]: the run list of a diagnostics call
+  void *pmap = nullptr;   // [layer] float: a probability map
   std::vector<double> zsH, dzsH, hzH;
   std::vector<char> used;
   // Captured steps by batch size: the graph of a size is reused for any runs of the pool, its device run list rewritten
@@ -298,16 +299,6 @@ template <class R> void launchPart(const Pool &p, const int *list, int nr, doubl
   mark("k_end");
 }
 
-// One step with a timer after each kernel, in line: whole, or split around its lightning with the discharge after it.
-template <class R> void launchStepTimed(const Pool &p, const int *list, int nr, double dt, hipStream_t s, Timer *t, const Tuning &tu, bool split) {
-  if (!split) { launchStep<R>(p, list, nr, dt, s, t, tu); return; }
-  t->mark("start");
-  launchPart<R>(p, list, nr, dt, s, t, tu, PART_DYNAMICS, true);
-  launchPart<R>(p, list, nr, dt, s, t, tu, PART_CHARGES, true);
-  for (int part = PART_PHYSICS; part < N_PARTS; part++) launchPart<R>(p, list, nr, dt, s, t, tu, part, true);
-  launchDischarge<R>(p, bufs<R>(p, list), nr, dt, s, t, 1);
-}
-
 // One step split around its lightning (with lightning only). prev: the discharge of the step before runs first on
 // the side stream, beside this step's dynamics, and this step's charges wait for it. The step's own discharge is left
 // to the next split step or to launchDischarge at the end of the call.
@@ -329,6 +320,16 @@ template <class R> void launchStepSplit(const Pool &p, const int *list, int nr, 
 template <class R> void launchStep(const Pool &p, const int *list, int nr, double dt, hipStream_t s, Timer *t, const Tuning &tu) {
   if (t) t->mark("start");
   for (int part = 0; part < N_PARTS; part++) launchPart<R>(p, list, nr, dt, s, t, tu, part);
+}
+
+// One step with a timer after each kernel, in line: whole, or split around its lightning with the discharge after it.
+template <class R> void launchStepTimed(const Pool &p, const int *list, int nr, double dt, hipStream_t s, Timer *t, const Tuning &tu, bool split) {
+  if (!split) { launchStep<R>(p, list, nr, dt, s, t, tu); return; }
+  t->mark("start");
+  launchPart<R>(p, list, nr, dt, s, t, tu, PART_DYNAMICS, true);
+  launchPart<R>(p, list, nr, dt, s, t, tu, PART_CHARGES, true);
+  for (int part = PART_PHYSICS; part < N_PARTS; part++) launchPart<R>(p, list, nr, dt, s, t, tu, part, true);
+  launchDischarge<R>(p, bufs<R>(p, list), nr, dt, s, t, 1);
 }
 
 void fillTables(Pool &p, std::vector<char> &sym, std::vector<char> &twx, std::vector<char> &twy) {
@@ -440,7 +441,7 @@ void freePool(Pool *p) {
   void *all[] = {p->fields, p->stx, p->sty, p->mass, p->phi, p->cells, p->press, p->spec, p->mean0, p->km, p->kh, p->ephi, p->cols,
                  p->lev, p->par, p->scal, p->zs, p->dzs, p->hz, p->sym, p->twx, p->twy, p->single, p->diag, p->dlist, p->lrho,
                  p->lmark, p->lnode, p->chan, p->cdep, p->flog, p->efac, p->ehat, p->emean0, p->dspec, p->fillp,
-                 p->gpart};
+                 p->gpart, p->pmap};
   for (void *a : all) if (a) hipFree(a);
   delete p;
 }
@@ -479,6 +480,7 @@ int ml_device_name(ml_ctx *c, char *out, int size) {
   return 0;
 }
 
+static_assert(int(N_COLUMNS) == int(ML_COLUMNS), "the ABI's ground columns");
 static_assert(int(F_CLOUDY) == int(ML_FIELDS) && int(N_FIELDS) == int(ML_FIELDS) + 1, "the ABI's fields, then the internal cloudiness");
 
 int64_t ml_slot_bytes(ml_ctx *c, const ml_grid *grid) {
@@ -508,7 +510,7 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
     {&p->kh, s.node * cap}, {&p->ephi, s.node8 * cap}, {&p->cols, s.cols * cap}, {&p->lev, s.lev * cap}, {&p->par, s.par * cap},
     {&p->scal, s.scal * cap}, {(void **)&p->diag, cap * DIAG_ROW * 8}, {(void **)&p->dlist, cap * sizeof(int)}, {&p->lrho, s.node * cap}, {&p->lmark, s.lmark * cap},
     {&p->lnode, s.lnode * cap}, {&p->chan, s.chan * cap}, {&p->cdep, s.cdep * cap}, {&p->flog, s.flog * cap},
-    {&p->fillp, s.fillp * cap}, {&p->gpart, s.gpart * cap}};
+    {&p->fillp, s.fillp * cap}, {&p->gpart, s.gpart * cap}, {&p->pmap, size_t(g.layer) * 4}};
   p->ftiles = stageTile(g).tiles; p->gmax = groundBlocks(g);
   for (auto &a : allocs) {
     hipError_t e = hipMalloc(a.ptr, a.bytes);
@@ -725,6 +727,39 @@ int ml_run_rotation(ml_ctx *c, int run, double *out) {
   static_assert(N_ROT == ML_ROT_COUNT, "the ABI's rotation record is the scalars'");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipMemcpyAsync(out, (double *)p->scal + size_t(slot) * N_SCALARS + S_ROT, N_ROT * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ml_run_set_swaths(ml_ctx *c, int run, int on) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot) || (on != 0 && on != 1)) return fail(c, ML_E_ARG, "set swaths: bad run or flag");
+  const double v = on;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync((double *)p->par + size_t(slot) * N_PARAMS + P_SWATHS, &v, 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ml_probability_map(ml_ctx *c, const int *runs, int count, int column, double threshold, float *out) {
+  if (!c || !runs || count < 1 || !out || column < 0 || column >= ML_COLUMNS) return fail(c, ML_E_ARG, "probability map: runs, a column and an output");
+  Pool *p = nullptr;
+  std::vector<int> slots(count);
+  for (int k = 0; k < count; k++) {
+    Pool *q; int slot;
+    if (!decode(c, runs[k], q, slot)) return fail(c, ML_E_ARG, "probability map: bad run");
+    if (p && q != p) return fail(c, ML_E_ARG, "probability map: runs of one pool per call");
+    p = q; slots[k] = slot;
+  }
+  if (count > p->cap) return fail(c, ML_E_ARG, "probability map: more runs than the pool holds");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(p->dlist, slots.data(), count * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  float *dev = reinterpret_cast<float *>(p->pmap);
+  const size_t need = size_t(p->g.layer) * sizeof(float);
+  if (p->rb == 8) hipLaunchKernelGGL(k_probability<double>, dim3(blocks(p->g.layer, 256)), dim3(256), 0, c->stream, p->g, bufs<double>(*p, p->dlist), count, column, threshold, dev);
+  else hipLaunchKernelGGL(k_probability<float>, dim3(blocks(p->g.layer, 256)), dim3(256), 0, c->stream, p->g, bufs<float>(*p, p->dlist), count, column, threshold, dev);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(out, dev, need, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }

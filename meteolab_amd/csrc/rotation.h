@@ -80,6 +80,20 @@ template <class R> __global__ __launch_bounds__(ROTATION_THREADS) void k_rotatio
     keepMax(vmax, cmax, uh, c);
     keepMax(vmin, cmin, -uh, c);
   }
+  // addSwaths (ensemble/member.ts:85-100) after the step: the ground under the domain is the domain shifted by its
+  // motion so far, in whole columns (the nearest), periodic; the strongest surface wind and helicity over each.
+  const double *P = b.pr(slot);
+  if (!keep && P[P_SWATHS] != 0) {
+    const double t = P[P_TIME] + dt, fu = P[P_FRAME_U], fv = P[P_FRAME_V];
+    const int sx = int(floor(fu * t / g.dx + .5)), sy = int(floor(fv * t / g.dy + .5));
+    float *gust = b.col(slot, C_GUST_MAX, g), *uhm = b.col(slot, C_UH_MAX, g);
+    for (int c = threadIdx.x; c < layer; c += blockDim.x) {
+      const int y = c / nx, x = c - y * nx, gx = ((x + sx) % nx + nx) % nx, gy = ((y + sy) % ny + ny) % ny, gc = gx + nx * gy;
+      const double wind = hypot(double(u[c]) + fu, double(v[c]) + fv);
+      if (wind > gust[gc]) gust[gc] = float(wind);
+      if (col[c] > uhm[gc]) uhm[gc] = col[c];
+    }
+  }
   blockMaxAt(vmax, cmax, redV, redC);
   blockMaxAt(vmin, cmin, redV, redC);
   blockMaxAt(v01, c01, redV, redC);

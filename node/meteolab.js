@@ -18,7 +18,7 @@ const addon = require(path.join(__dirname, 'meteolab.node'))
 const FIELDS = ['rain', 'snow', 'graupel', 'hail', 'u', 'v', 'w', 'theta', 'q', 'cloud', 'ice', 'iceNumber', 'cold',
   'chargeIce', 'chargeGraupel', 'chargeRain', 'chargeAir', 'fieldZ', 'fallRain', 'fallSnow', 'fallGraupel', 'fallHail',
   'fallChargeIce', 'fallChargeGraupel', 'condensate']
-const COLUMNS = ['precipitation', 'hailGround', 'hailSize']
+const COLUMNS = ['precipitation', 'hailGround', 'hailSize', 'gustMax', 'uhMax']
 const DIAG = ['time', 'maxW', 'minW', 'maxThetaP', 'rainTotal', 'water', 'charge', 'clipped', 'negativeFilled',
   'chargeGround', 'flashesIC', 'flashesCG', 'firstFlash', 'error']
 const LEVELS = ['p', 'exner', 'theta', 'q', 'u', 'v', 'rho']
@@ -86,6 +86,15 @@ class MeteolabPool {
     const ids = Int32Array.from(runs.map(r => r.handle)), out = new Float64Array(runs.length * MODEL_DIAGNOSTICS.length)
     this.device.call(() => addon.modelDiagnostics(this.device.ctx, ids, out))
     return runs.map((_, k) => Object.fromEntries(MODEL_DIAGNOSTICS.map((name, j) => [name, out[k * MODEL_DIAGNOSTICS.length + j]])))
+  }
+  /**
+   * probabilityMap (src/ensemble/combine.ts:70-75) on the device: per ground column the share of the runs whose swath
+   * column ('precipitation' as the rain total, 'hailSize', 'gustMax', 'uhMax') reached the threshold there.
+   */
+  probabilityMap (runs, column, threshold) {
+    const ids = Int32Array.from(runs.map(r => r.handle)), out = new Float32Array(this.layer)
+    this.device.call(() => addon.probabilityMap(this.device.ctx, ids, COLUMNS.indexOf(column), threshold, out))
+    return out
   }
   destroy () { if (this.id !== null) { this.device.call(() => addon.poolDestroy(this.device.ctx, this.id)); this.id = null } }
 }
@@ -155,6 +164,8 @@ class MeteolabRun {
     this.device.call(() => addon.rotation(this.device.ctx, this.handle, out))
     return Object.fromEntries(ROTATION.map((k, i) => [k, out[i]]))
   }
+  /** addSwaths after every step (src/ensemble/member.ts:85-100) into the gustMax and uhMax columns. */
+  setSwaths (on) { this.device.call(() => addon.setSwaths(this.device.ctx, this.handle, on ? 1 : 0)) }
   /** The rotation diagnostics of the fields as they are, keeping how long the rotation has held (refreshRotation). */
   refreshRotation () { this.device.call(() => addon.refreshRotation(this.device.ctx, this.handle)) }
   release () { if (this.handle !== null) { this.device.call(() => addon.runRelease(this.device.ctx, this.handle)); this.handle = null } }
@@ -292,5 +303,37 @@ function applyState (r, state) {
   r.refreshRotation()
 }
 
-module.exports = { MeteolabDevice, MeteolabPool, MeteolabRun, GpuDynamics, FIELDS, CARRIED, COLUMNS, DIAG, PARTS, STANDARD, LARGE, profile, run, addon,
+/**
+ * runMember (src/ensemble/member.ts:103-117) for every run of a batch of one pool at once: the swaths folded in after
+ * every step on the device, and every 60 model seconds a sample of each run's strongest values from one launch for the
+ * batch. Returns MemberResult objects (member.ts:17-22), which src/ensemble/combine.ts (memberPeaks, probabilities,
+ * probabilityMap) and src/regime/grid.ts (regimeOf) read as they are.
+ */
+function runMembers (pool, runs, minutes, dt = 3) {
+  const SAMPLE_EVERY = 60, samples = runs.map(() => []), flashes = runs.map(() => 0)
+  runs.forEach(r => r.setSwaths(true))
+  const sample = () => {
+    const ds = pool.diagnostics(runs)
+    runs.forEach((r, k) => {
+      const d = ds[k], rot = r.rotation(), m = r.diagnostics(), total = m.flashesIC + m.flashesCG
+      samples[k].push({ t: m.time, updraft: d.updraft, uh: rot.uh, rain: d.rainRate, gust: d.gust, hail: d.hail, flashes: (total - flashes[k]) * 60 / SAMPLE_EVERY,
+        rotationHeld: rot.persisted, cores: d.cores, outflow: d.outflow, outflowLow: d.outflowLow })
+      flashes[k] = total
+    })
+  }
+  sample()
+  const per = Math.round(SAMPLE_EVERY / dt), end = minutes * 60
+  let t = 0
+  while (t < end - 1e-6) {
+    const n = Math.min(per, Math.ceil((end - t) / dt - 1e-9))
+    pool.step(runs, n, dt); t += n * dt
+    pool.device.sync()
+    sample()
+  }
+  const g = pool.grid
+  return runs.map((r, k) => ({ seed: r.spec.run.seed, samples: samples[k], nx: g.nx, ny: g.ny, width: g.width, depth: g.depth,
+    rainTotal: r.column('precipitation'), hailSize: r.column('hailSize'), gustMax: r.column('gustMax'), uhMax: r.column('uhMax') }))
+}
+
+module.exports = { MeteolabDevice, runMembers, MeteolabPool, MeteolabRun, GpuDynamics, FIELDS, CARRIED, COLUMNS, DIAG, PARTS, STANDARD, LARGE, profile, run, addon,
   STATE_FIELDS, STATE_COLUMNS, ROTATION, MODEL_DIAGNOSTICS, saveState, readState, applyState }

@@ -182,6 +182,26 @@ napi_value Step(napi_env env, napi_callback_info info) {
   return status(env, ctx, ml_step(ctx, static_cast<const int *>(raw), int(len), steps, dt));
 }
 
+// setSwaths(ctx, run, on): ml_run_set_swaths.
+napi_value SetSwaths(napi_env env, napi_callback_info info) {
+  napi_value a[3]; if (!args(env, info, 3, a)) return nullptr;
+  ml_ctx *ctx = ctxOf(env, a[0]); if (!ctx) return nullptr;
+  int32_t run, on; if (!intOf(env, a[1], run) || !intOf(env, a[2], on)) return nullptr;
+  return status(env, ctx, ml_run_set_swaths(ctx, run, on));
+}
+
+// probabilityMap(ctx, Int32Array runs, column, threshold, Float32Array out): ml_probability_map.
+napi_value ProbabilityMap(napi_env env, napi_callback_info info) {
+  napi_value a[5]; if (!args(env, info, 5, a)) return nullptr;
+  ml_ctx *ctx = ctxOf(env, a[0]); if (!ctx) return nullptr;
+  napi_typedarray_type t; size_t len; void *raw; napi_value buf; size_t off;
+  if (napi_get_typedarray_info(env, a[1], &t, &len, &raw, &buf, &off) != napi_ok || t != napi_int32_array) { napi_throw_type_error(env, nullptr, "runs: Int32Array"); return nullptr; }
+  int32_t column; double threshold; if (!intOf(env, a[2], column) || !numberOf(env, a[3], threshold)) return nullptr;
+  napi_typedarray_type t2; size_t len2; void *raw2;
+  if (napi_get_typedarray_info(env, a[4], &t2, &len2, &raw2, &buf, &off) != napi_ok || t2 != napi_float32_array) { napi_throw_type_error(env, nullptr, "out: Float32Array of the layer"); return nullptr; }
+  return status(env, ctx, ml_probability_map(ctx, static_cast<const int *>(raw), int(len), column, threshold, static_cast<float *>(raw2)));
+}
+
 // modelDiagnostics(ctx, Int32Array runs, Float64Array out): ml_model_diagnostics, ML_MD_COUNT values a run.
 napi_value ModelDiagnostics(napi_env env, napi_callback_info info) {
   napi_value a[3]; if (!args(env, info, 3, a)) return nullptr;
@@ -264,7 +284,7 @@ napi_value Init(napi_env env, napi_value exports) {
     {"levels", Doubles<ml_run_levels>}, {"flashes", Flashes}, {"setTime", SetTime}, {"dynamics", Part<ml_dynamics>},
     {"physics", Part<ml_physics>}, {"mix", Part<ml_mix>}, {"afterStep", Part<ml_after_step>}, {"fallParts", FallParts},
     {"setTotals", SetTotals}, {"rotation", Doubles<ml_run_rotation>}, {"refreshRotation", RunOp<ml_run_refresh_rotation>},
-    {"modelDiagnostics", ModelDiagnostics},
+    {"modelDiagnostics", ModelDiagnostics}, {"setSwaths", SetSwaths}, {"probabilityMap", ProbabilityMap},
   };
   for (auto &f : fns) {
     napi_value v;

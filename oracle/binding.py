@@ -20,7 +20,7 @@ CARRIED = ["rain", "snow", "graupel", "hail", "u", "v", "w", "theta", "q", "clou
 EXTRA = ["fieldZ", "fallRain", "fallSnow", "fallGraupel", "fallHail", "fallChargeIce", "fallChargeGraupel"]
 FIELDS = CARRIED + EXTRA
 FIELD_ID = {name: k for k, name in enumerate(FIELDS)}
-COLUMNS = ["precipitation", "hailGround", "hailSize"]
+COLUMNS = ["precipitation", "hailGround", "hailSize", "gustMax", "uhMax"]
 # ModelDiagnostics (model.ts:161-180) in the order orc_model_diagnostics writes it.
 MODEL_DIAGNOSTICS = ["updraft", "downdraft", "rainRate", "rainTotal", "cloudTop", "thermalTop", "maxCloud", "coldMax",
                      "cores", "shear06", "outflow", "outflowLow", "gust", "hail", "clipped"]
@@ -82,6 +82,10 @@ def lib():
         L.orc_rotation.argtypes = [vp, dp]
         L.orc_update_rotation.argtypes = [vp, d, i]
         L.orc_model_diagnostics.argtypes = [vp, dp]
+        L.orc_set_swaths.argtypes = [vp, i]
+        L.orc_add_swaths.argtypes = [vp]
+        L.orc_uh_column.argtypes = [vp, fp]
+        L.orc_uh_column.restype = i
         L.orc_outflow_deficit.argtypes = [vp, fp, dp]
         L.orc_outflow_deficit.restype = d
         L.orc_deficit_near.argtypes = [vp, fp, i]
@@ -230,6 +234,20 @@ class OracleRun:
         """deficitNear (rkw.ts:82-90) within 2 km of a ground column."""
         a = np.ascontiguousarray(deficit, dtype=np.float32)
         return lib().orc_deficit_near(self.h, fptr(a), int(column))
+
+    def set_swaths(self, on):
+        """runMember's addSwaths after every step (ensemble/member.ts:85-114)."""
+        lib().orc_set_swaths(self.h, 1 if on else 0)
+
+    def add_swaths(self):
+        """addSwaths once at the model's time (ensemble/member.ts:85-100)."""
+        lib().orc_add_swaths(self.h)
+
+    def uh_column(self):
+        """The last updateRotation's 2-5 km helicity of each column (model.ts uhColumn)."""
+        out = np.empty(self.layer, np.float32)
+        assert lib().orc_uh_column(self.h, fptr(out)) == 0
+        return out
 
     def update_rotation(self, dt=0.0, keep=True):
         """updateRotation(dt) on the fields as they are; keep: refreshRotation (how long it held stays)."""

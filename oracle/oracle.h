@@ -46,7 +46,8 @@ enum {
   ORC_FIELDS = 24
 };
 /* Column ids (STATE_COLUMNS, src/core/state.ts:10). */
-enum { ORC_PRECIPITATION = 0, ORC_HAIL_GROUND, ORC_HAIL_SIZE, ORC_COLUMNS };
+/* Then an ensemble member's swaths (src/ensemble/member.ts:19-21): the strongest surface wind and 2-5 km helicity. */
+enum { ORC_PRECIPITATION = 0, ORC_HAIL_GROUND, ORC_HAIL_SIZE, ORC_GUST_MAX, ORC_UH_MAX, ORC_COLUMNS };
 
 /* Diagnostics vector (orc_diag): */
 enum {
@@ -87,6 +88,12 @@ enum {
   ORC_MD_HAIL, ORC_MD_CLIPPED, ORC_MD_COUNT
 };
 void orc_model_diagnostics(void *model, double *out);
+/* runMember's addSwaths after every step of orc_step (src/ensemble/member.ts:85-114) when on. */
+void orc_set_swaths(void *model, int on);
+/* addSwaths once, with the model's time and the last updateRotation's helicity columns; those columns (layer float32,
+ * -1 before any updateRotation). */
+void orc_add_swaths(void *model);
+int orc_uh_column(void *model, float *out);
 /* outflowDeficit (rkw.ts:72-80): the ground columns' deficits (layer float32, may be null) and the warm air's thv;
  * returns the largest deficit. deficitNear (rkw.ts:82-90): the largest of `deficit` within 2 km of `column`. */
 double orc_outflow_deficit(void *model, float *deficit, double *warm);

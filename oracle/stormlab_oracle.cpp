@@ -966,6 +966,8 @@ struct Model {
   double cs, dragCd, f_coriolis;
   // RotationState (model.ts:142-158) in the order of ORC_ROT_*, and the levels updateRotation sums over.
   double rotation[ORC_ROT] = {0};
+  std::vector<float> uhColumn;  // the columns' 2-5 km helicity of the last updateRotation (model.ts uhColumn)
+  bool swaths = false;
   int uhZ0 = 0, uhZ1 = -1, top1 = -1, top3 = -1;
 
   Model(const orc_grid &go, const orc_profile &pr, const orc_run &r) : g(go), env(g, pr), run(r), flux(g, env.rho), solver(g, env.rho) {
@@ -1319,6 +1321,24 @@ struct Model {
     r[ORC_ROT_PERSISTED] = coreZeta >= meso ? r[ORC_ROT_PERSISTED] + dt : 0;
     r[ORC_ROT_UH01] = max01; r[ORC_ROT_UH03] = max03; r[ORC_ROT_LOW] = low01;
     r[ORC_ROT_CORE_ZETA] = coreZeta; r[ORC_ROT_CORE_UH] = coreUH; r[ORC_ROT_CORE_ANTI] = coreAnti;
+    uhColumn = uhc;
+  }
+  // addSwaths (ensemble/member.ts:85-100), periodic ground (groundRepeats of periodic edges): the ground under the
+  // domain is the domain shifted by its motion so far, in whole columns (Math.round: halves up).
+  void addSwaths() {
+    const int nx = g.nx, ny = g.ny;
+    const double fu = run.frame_u, fv = run.frame_v;
+    const int sx = int(std::floor(fu * time / g.dx + .5)), sy = int(std::floor(fv * time / g.dy + .5));
+    float *gust = col[ORC_GUST_MAX].data(), *uhm = col[ORC_UH_MAX].data();
+    for (int y = 0; y < ny; y++) {
+      const int gy = ((y + sy) % ny + ny) % ny;
+      for (int x = 0; x < nx; x++) {
+        const int gx = ((x + sx) % nx + nx) % nx, c = x + nx * y, gc = gx + nx * gy;
+        const double wind = std::hypot(double(f[ORC_U][c]) + fu, double(f[ORC_V][c]) + fv);
+        if (wind > gust[gc]) gust[gc] = float(wind);
+        if (uhColumn[c] > uhm[gc]) uhm[gc] = uhColumn[c];
+      }
+    }
   }
   // countCores (model.ts:1650-1670): cloudy updraught columns between ~1.9 and 6.6 km, 4-connected across the
   // periodic edges.
@@ -1437,7 +1457,18 @@ extern "C" {
 
 void *orc_create(const orc_grid *grid, const orc_profile *profile, const orc_run *run) { return new Model(*grid, *profile, *run); }
 void orc_destroy(void *m) { delete static_cast<Model *>(m); }
-void orc_step(void *m, double dt, int n) { Model *mm = static_cast<Model *>(m); for (int k = 0; k < n; k++) { mm->step(dt); mm->time += dt; } }
+void orc_step(void *m, double dt, int n) {
+  Model *mm = static_cast<Model *>(m);
+  for (int k = 0; k < n; k++) { mm->step(dt); mm->time += dt; if (mm->swaths) mm->addSwaths(); }
+}
+void orc_set_swaths(void *m, int on) { static_cast<Model *>(m)->swaths = on != 0; }
+void orc_add_swaths(void *m) { Model *mm = static_cast<Model *>(m); if (!mm->uhColumn.empty()) mm->addSwaths(); }
+int orc_uh_column(void *m, float *out) {
+  Model *mm = static_cast<Model *>(m);
+  if (mm->uhColumn.empty()) return -1;
+  std::memcpy(out, mm->uhColumn.data(), mm->uhColumn.size() * 4);
+  return 0;
+}
 // One part of a step, split where the reference's GPU step splits it (src/gpu/step.ts:12-31): 0 beginStep with the
 // sedimentation and the transport (GpuDynamics.step), 1 physicsStep (on the CPU there), 2 mixStep and the projection
 // with the wind clipping (GpuDynamics.mix, endGpuStep), 3 afterStep's lightning and time += dt. Parts 0-3 in order
@@ -1448,7 +1479,7 @@ int orc_step_part(void *m, int part, double dt) {
     case 0: mm->fallPhase(); mm->collectHail(dt); mm->sedimentStep(dt); mm->dynamics(dt); return 0;
     case 1: mm->physicsStep(dt); return 0;
     case 2: mm->mixStep(dt); mm->project(dt, true); return 0;
-    case 3: mm->updateRotation(dt); if (mm->electrified) mm->discharge(dt); mm->time += dt; return 0;
+    case 3: mm->updateRotation(dt); if (mm->electrified) mm->discharge(dt); mm->time += dt; if (mm->swaths) mm->addSwaths(); return 0;
     default: return -1;
   }
 }

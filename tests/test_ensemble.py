@@ -1,0 +1,166 @@
+"""The ensemble and regime front-ends (src/ensemble, src/regime): on the CPU the reference's own tests of addSwaths,
+the probabilities, spread and probability maps (ensemble.test.ts:21-31, 58-74) and regimeOf (grid.test.ts:39-52), on
+the oracle and the host logic; the analytic sounding's withMoisture / withShear; on the GPU a batch of members run
+together (swaths folded in on the device, samples from one launch) against the oracle's runMember, and the device's
+probability maps against the restated probabilityMap."""
+import math
+
+import numpy as np
+import pytest
+
+from meteolab_amd import config as C
+from meteolab_amd import ensemble as E
+from oracle.binding import OracleRun
+
+
+def sample(t, **o):
+    s = {"t": t, "updraft": 0, "uh": 0, "rain": 0, "gust": 0, "hail": 0, "flashes": 0, "rotationHeld": 0, "cores": 0}
+    s.update(o)
+    return s
+
+
+def member(samples, **maps):
+    """A 2 x 2 member with the given highlights and swaths (ensemble.test.ts:8-12)."""
+    f = {k: np.asarray(maps.get(k, [0, 0, 0, 0]), np.float32) for k in ("rainTotal", "hailSize", "gustMax", "uhMax")}
+    return {"seed": 1, "samples": samples, "nx": 2, "ny": 2, "width": 2000, "depth": 2000, **f}
+
+
+def host_probability_map(results, kind):
+    """probabilityMap (combine.ts:70-75) restated on the host, as the check of the device's."""
+    field, threshold = E.MAPS[kind]
+    out = np.zeros(len(results[0]["rainTotal"]), np.float32)
+    for r in results:
+        at = threshold * E.uh_scale(r["width"] / r["nx"]) if kind == "uh" else threshold
+        out += (np.asarray(r[field], np.float64) >= at).astype(np.float32)
+    return (out.astype(np.float64) / len(results)).astype(np.float32)
+
+
+def test_probabilities_spread_and_maps():
+    # ensemble.test.ts:58-74.
+    results = [
+        member([sample(0), sample(600, updraft=30, rain=60, rotationHeld=E.MESO_PERSISTENCE)], rainTotal=[20, 0, 0, 0], gustMax=[25, 0, 0, 0]),
+        member([sample(0), sample(600, updraft=20, rain=5, gust=22)], rainTotal=[12, 3, 0, 0]),
+        member([sample(0), sample(600, updraft=4)]),
+        member([sample(0), sample(600, updraft=15, rain=2, hail=25)], rainTotal=[0, 11, 0, 0]),
+    ]
+    peaks = [E.member_peaks(r) for r in results]
+    p = E.probabilities(peaks)
+    assert peaks[0]["mesocyclone"] == 0 and peaks[1]["mesocyclone"] is None
+    assert [p["storm"], p["mesocyclone"], p["downpour"], p["hail"], p["gust"], p["total"]] == [.75, .25, .25, .25, .25, .25]
+    assert E.spread([x["updraft"] for x in peaks]) == {"min": 4, "median": 17.5, "max": 30}
+    assert list(host_probability_map(results, "rain")) == [.5, .25, 0, 0]
+    assert list(host_probability_map(results, "gust")) == [.25, 0, 0, 0]
+
+
+def test_regime_of_a_runs_course():
+    # grid.test.ts:39-52.
+    def run(updrafts, every=600, **o):
+        return {"seed": 1, "nx": 1, "ny": 1, "width": 1, "depth": 1, "rainTotal": np.zeros(1, np.float32),
+                "samples": [sample(k * every, updraft=w, rain=20 if w > 5 else 0, cores=1, **o) for k, w in enumerate(updrafts)]}
+    assert E.regime_of(run([0, 4, 6, 3])) == "none"
+    assert E.regime_of(run([0, 30, 12, 2, 1])) == "ordinary"
+    assert E.regime_of(run([0, 30, 12, 20, 18])) == "multicell"
+    assert E.regime_of(run([0, 30, 12, 20, 18], rotationHeld=E.MESO_PERSISTENCE)) == "supercell"
+    calm = "2 1 3 4 6 8 10 13 16 21 25 30 37 44 52 60 69 68 62 57 54 55 57 54 56 57 53 52 54 51 46 45 47 41 36 30 27 28 30 30 27 21 15 10 7 6"
+    sheared = "2 1 3 4 5 7 9 11 14 17 21 24 28 33 38 42 49 54 57 57 52 52 51 49 51 48 43 40 42 45 51 50 45 39 33 35 37 41 42 41 38 37 37 39 36 32"
+    assert E.regime_of(run([float(x) for x in calm.split()], every=60)) == "ordinary"
+    assert E.regime_of(run([float(x) for x in sheared.split()], every=60)) == "multicell"
+
+
+def test_moisture_and_shear_of_the_analytic_sounding():
+    # grid.test.ts:11-37 on this sounding: the 0-6 km shear asked is made and the shape (shear depth) kept; a calm
+    # sounding stays calm (the Weisman-Klemp branch); the vapour cap scales and the shear stays.
+    prof = C.profile()
+    for s in (0, 15, 40):
+        assert E.shear06(E.with_shear(prof, s)) == pytest.approx(s, abs=1e-9)
+    assert E.with_shear(prof, 15)["shear_depth"] == prof["shear_depth"]
+    assert E.shear06(E.with_shear(C.profile(shear=1.0), 20)) == 0
+    assert E.with_moisture(prof, .75)["qv_max"] == pytest.approx(.0105, abs=1e-12)
+    assert E.shear06(E.with_moisture(prof, 1.1)) == E.shear06(prof)
+    cells = E.regime_cells(prof, "full")
+    g = E.GRIDS["full"]
+    assert len(cells) == len(g["moisture"]) * len(g["shear"])
+    for c in cells:
+        assert c["shear"] == pytest.approx(g["shear"][c["column"]], abs=1e-9)
+        assert c["profile"]["qv_max"] == pytest.approx(prof["qv_max"] * g["moisture"][c["row"]])
+
+
+def test_oracle_folds_columns_into_the_ground_shifted_by_the_domain_motion():
+    # ensemble.test.ts:21-31 on the small grid: the domain moving 5 m/s east for 480 s has moved 2 columns (2.4 km at
+    # dx = 1200 m): its column 1, with u = 10 m/s over the domain, lies over ground column 3 (15 m/s over the ground);
+    # a column at rest over the domain has the domain's 5 m/s.
+    # Aloft, the rotation tests' cyclonic and anticyclonic updraught patches give helicity columns of both signs.
+    from test_rotation import planted
+    g = C.small_grid()
+    nx, layer = g["nx"], g["nx"] * g["ny"]
+    o = OracleRun(g, C.profile(), dict(C.run(g), frame_u=5.0, frame_v=0.0))
+    u, v, w = planted(g, "pair")
+    u[:layer] = 0
+    v[:layer] = 0
+    u[1] = 10
+    o.set_field("u", u); o.set_field("v", v); o.set_field("w", w)
+    o.update_rotation(0.0, keep=True)
+    o.set_time(480.0)
+    o.add_swaths()
+    gust, uh, col = o.column("gustMax"), o.column("uhMax"), o.uh_column()
+    assert gust[3] == 15 and gust[0] == 5
+    assert np.all(gust[np.arange(g["nx"] * g["ny"]) != 3] == 5)
+    shift = (np.arange(g["nx"] * g["ny"]) % nx + 2) % nx + nx * (np.arange(g["nx"] * g["ny"]) // nx)
+    expect = np.zeros_like(uh)
+    expect[shift] = np.maximum(col, 0)
+    assert col.max() > 0 and col.min() < 0
+    assert np.array_equal(uh, expect)
+
+
+def oracle_member(g, prof, spec, minutes, dt=3.0):
+    """runMember (member.ts:103-117) on the oracle: samples every 60 s and the swaths."""
+    o = OracleRun(g, prof, spec)
+    o.set_swaths(True)
+    samples, flashes = [], 0
+
+    def take():
+        nonlocal flashes
+        d, rot, m = o.model_diagnostics(), o.rotation(), o.diag()
+        total = int(m["flashesIC"] + m["flashesCG"])
+        samples.append({"t": m["time"], "updraft": d["updraft"], "uh": rot["uh"], "rain": d["rainRate"], "gust": d["gust"],
+                        "hail": d["hail"], "flashes": (total - flashes) * 60 / E.SAMPLE_EVERY, "rotationHeld": rot["persisted"],
+                        "cores": d["cores"], "outflow": d["outflow"], "outflowLow": d["outflowLow"]})
+        flashes = total
+
+    take()
+    for _ in range(int(minutes)):
+        o.step(int(round(E.SAMPLE_EVERY / dt)), dt)
+        take()
+    return o, {"samples": samples, "nx": g["nx"], "ny": g["ny"], "width": g["width"], "depth": g["depth"],
+               **{k: o.column(E.COLUMN_OF[k]) for k in ("rainTotal", "hailSize", "gustMax", "uhMax")}}
+
+
+@pytest.mark.gpu
+def test_members_stepped_together_sample_and_swath_as_the_oracle():
+    import meteolab_amd as M
+    g = C.small_grid()
+    prof = C.profile()
+    specs = [dict(C.ensemble_member(g, k), frame_u=4.0, frame_v=-2.0) for k in range(3)]
+    minutes = 12
+    with M.Context(0, 32) as ctx:
+        pool = ctx.pool(g, 3)
+        runs = [pool.run(prof, s) for s in specs]
+        print("members on the device,", minutes, "minutes", flush=True)
+        results = E.run_members(runs, minutes)
+        for k, spec in enumerate(specs):
+            _, want = oracle_member(g, prof, spec, minutes)
+            got = results[k]
+            assert [s["t"] for s in got["samples"]] == [60.0 * j for j in range(minutes + 1)]
+            for a, b in zip(got["samples"], want["samples"]):
+                for key in ("updraft", "uh", "rain", "gust", "outflow"):
+                    assert a[key] == pytest.approx(b[key], rel=1e-3, abs=1e-6), (k, a["t"], key, a[key], b[key])
+                assert a["flashes"] == b["flashes"] and a["rotationHeld"] == b["rotationHeld"], (k, a["t"])
+            for key in ("rainTotal", "gustMax", "uhMax", "hailSize"):
+                x, y = np.asarray(got[key], np.float64), np.asarray(want[key], np.float64)
+                ny_ = np.linalg.norm(y)
+                err = np.linalg.norm(x - y) / ny_ if ny_ > 0 else np.linalg.norm(x)
+                assert err <= 1e-4, (k, key, err)
+        assert max(np.max(r["gustMax"]) for r in results) > 0 and max(np.max(r["uhMax"]) for r in results) > 0
+        for kind in E.MAPS:
+            dev = E.probability_map(runs, kind)
+            assert np.array_equal(dev, host_probability_map(results, kind)), kind

@@ -19,7 +19,7 @@ needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reaso
 ABI_FUNCTIONS = ["ctxCreate", "ctxDestroy", "deviceName", "poolCreate", "poolDestroy", "slotBytes", "runCreate", "runRelease",
                  "upload", "download", "uploadColumn", "downloadColumn", "step", "sync", "diag", "levels", "flashes", "setTime",
                  "dynamics", "physics", "mix", "afterStep", "fallParts", "setTotals",
-                 "rotation", "refreshRotation", "modelDiagnostics"]
+                 "rotation", "refreshRotation", "modelDiagnostics", "setSwaths", "probabilityMap"]
 
 
 def node(code):
@@ -101,3 +101,31 @@ def test_gpu_dynamics_interface_steps_as_the_oracle(tmp_path):
         nb = np.linalg.norm(b)
         err = np.linalg.norm(fields[k] - b) / nb if nb > 0 else np.linalg.norm(fields[k])
         assert err <= 1e-5, (name, err)
+
+
+@pytest.mark.gpu
+@needs_node
+def test_node_run_members_gives_member_results(tmp_path):
+    # runMembers for two runs of a pool over 2 model minutes: MemberResult objects as src/ensemble/combine.ts reads
+    # them, with a sample every 60 s; the swaths end as the Python host's for the same runs, and so do the maps.
+    g = C.small_grid()
+    code = ("const m = require('./node/meteolab'); const dev = m.MeteolabDevice.create();"
+            f"const g = {json.dumps(g)}; const pool = dev.pool(g, 2);"
+            "const runs = [0, 1].map(k => pool.run(m.profile(), m.run(g, k + 1)));"
+            "const res = m.runMembers(pool, runs, 2);"
+            "const map = pool.probabilityMap(runs, 'gustMax', 5);"
+            "console.log(JSON.stringify({t: res[0].samples.map(s => s.t), keys: Object.keys(res[0]).sort(),"
+            " gust: Array.from(res[1].gustMax), uh: Array.from(res[1].uhMax), map: Array.from(map), seed: res[1].seed})); dev.destroy()")
+    res = node(code)
+    assert res.returncode == 0, res.stderr
+    d = json.loads(res.stdout.strip().splitlines()[-1])
+    assert d["t"] == [0, 60, 120] and d["seed"] == 2
+    assert d["keys"] == sorted(["seed", "samples", "nx", "ny", "width", "depth", "rainTotal", "hailSize", "gustMax", "uhMax"])
+    from meteolab_amd import ensemble as E
+    with M.Context(0, 32) as ctx:
+        pool = ctx.pool(g, 2)
+        runs = [pool.run(C.profile(), C.run(g, seed=k + 1)) for k in range(2)]
+        py = E.run_members(runs, 2)
+        assert np.array_equal(np.asarray(d["gust"], np.float32), py[1]["gustMax"])
+        assert np.array_equal(np.asarray(d["uh"], np.float32), py[1]["uhMax"])
+        assert np.array_equal(np.asarray(d["map"], np.float32), M.probability_map(runs, "gustMax", 5))
