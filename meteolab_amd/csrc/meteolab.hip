@@ -489,13 +489,15 @@ int64_t ml_slot_bytes(ml_ctx *c, const ml_grid *grid) {
   return int64_t(sizesOf(g, c->precision == 64 ? 8 : 4).slot());
 }
 
+// In pool creation: a HIP error frees what the pool holds so far.
+#define POOLCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { freePool(p); return fail(c, ML_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); } } while (0)
 int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified, int *pool) {
   if (!c || !pool || capacity < 1 || capacity >= (1 << SLOT_BITS)) return fail(c, ML_E_ARG, "pool: capacity 1 .. 65535");
   int id = int(std::find(c->pools.begin(), c->pools.end(), nullptr) - c->pools.begin());
   if (id >= MAX_POOLS) return fail(c, ML_E_FULL, "too many pools");
   Pool *p = new Pool();
   if (int r = geoOf(c, grid, p->g, p->plan)) { delete p; return r; }
-  HIPCHK(c, hipSetDevice(c->device));
+  POOLCHK(hipSetDevice(c->device));
   const Geo &g = p->g;
   p->cap = capacity; p->electrified = electrified ? 1 : 0; p->rb = c->precision == 64 ? 8 : 4;
   // LDS the kernels need must fit the device's block limit.
@@ -530,25 +532,25 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
     {&p->sym, sym.data(), sym.size()}, {&p->twx, twx.data(), twx.size()}, {&p->twy, twy.data(), twy.size()},
     {(void **)&p->single, single.data(), single.size() * sizeof(int)}};
   for (auto &t : tabs) {
-    HIPCHK(c, hipMalloc(t.ptr, t.bytes));
-    HIPCHK(c, hipMemcpy(*t.ptr, t.src, t.bytes, hipMemcpyHostToDevice));
+    POOLCHK(hipMalloc(t.ptr, t.bytes));
+    POOLCHK(hipMemcpy(*t.ptr, t.src, t.bytes, hipMemcpyHostToDevice));
   }
   // The electric modes' elimination, once for the grid; the discharge's solve takes as many levels a pass as the
   // block's LDS holds beside its own state.
   const int L = g.nz - 2;
-  HIPCHK(c, hipMalloc(&p->efac, size_t(3) * L * g.nm * 8));
+  POOLCHK(hipMalloc(&p->efac, size_t(3) * L * g.nm * 8));
   hipLaunchKernelGGL(k_electric_factors, dim3(blocks(g.nm, 256)), dim3(256), 0, c->stream, g, (const double *)p->sym, (const double *)p->dzs,
                      (const double *)p->hz, (double *)p->efac);
-  HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  POOLCHK(hipGetLastError());
+  POOLCHK(hipStreamSynchronize(c->stream));
   const size_t dischargeStatic = 8192, ldsMax = c->prop.sharedMemPerBlock;
   int chunk = 1;
   while (chunk < L && solveLdsBytes(g, chunk + 1) + dischargeStatic <= ldsMax) chunk++;
   if (solveLdsBytes(g, chunk) + dischargeStatic > ldsMax) { freePool(p); return fail(c, ML_E_UNSUPPORTED, "grid too large for the discharge's level in LDS"); }
   p->chunk = chunk;
   const int dyn = int(solveLdsBytes(g, chunk));
-  if (p->rb == 8) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_discharge<double>), hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
-  else HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_discharge<float>), hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+  if (p->rb == 8) POOLCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_discharge<double>), hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+  else POOLCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_discharge<float>), hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
   // The rotation diagnostics: the first level at or above 2 km to the last at or below 5 km (model.ts:389), the last
   // levels at or below 1 and 3 km (:1612), and the core-vorticity threshold of this spacing (mesoCoreZeta, :129).
   {
@@ -564,8 +566,8 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
     r.meso = 3e-3 * std::max(1.0, 1200.0 / g.dx);
     const int rotLds = int(rotationLdsBytes(g));
     if (size_t(rotLds) > ldsMax) { freePool(p); return fail(c, ML_E_UNSUPPORTED, "grid too large for the rotation diagnostics' columns in LDS"); }
-    if (p->rb == 8) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_rotation<double>), hipFuncAttributeMaxDynamicSharedMemorySize, rotLds));
-    else HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_rotation<float>), hipFuncAttributeMaxDynamicSharedMemorySize, rotLds));
+    if (p->rb == 8) POOLCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_rotation<double>), hipFuncAttributeMaxDynamicSharedMemorySize, rotLds));
+    else POOLCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_rotation<float>), hipFuncAttributeMaxDynamicSharedMemorySize, rotLds));
   }
   // The model diagnostics: cores between the first level at or above 1.9 km and the last at or below 6.6 km, at most
   // nz - 3 (model.ts:1653); the cold pool below 1.5 km (:1680).
@@ -581,23 +583,24 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
     d.coreHi = std::min(g.nz - 3, d.coreHi);
     const int diagLds = int(modelDiagLdsBytes(g));
     if (size_t(diagLds) > ldsMax) { freePool(p); return fail(c, ML_E_UNSUPPORTED, "grid too large for the diagnostics' columns in LDS"); }
-    if (p->rb == 8) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_model_diag<double>), hipFuncAttributeMaxDynamicSharedMemorySize, diagLds));
-    else HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_model_diag<float>), hipFuncAttributeMaxDynamicSharedMemorySize, diagLds));
+    if (p->rb == 8) POOLCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_model_diag<double>), hipFuncAttributeMaxDynamicSharedMemorySize, diagLds));
+    else POOLCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_model_diag<float>), hipFuncAttributeMaxDynamicSharedMemorySize, diagLds));
   }
   // The transport layouts' level planes in LDS (the plane layout only where it fits; the autotuner skips it otherwise).
   const int levelLds = int(stageLevelLdsBytes(g, p->rb)), planeLds = int(stagePlaneLdsBytes(g, p->rb));
   if (p->rb == 8) {
-    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_level<double>), hipFuncAttributeMaxDynamicSharedMemorySize, levelLds));
-    if (planeFits(g, p->rb, ldsMax)) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_plane<double>), hipFuncAttributeMaxDynamicSharedMemorySize, planeLds));
+    POOLCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_level<double>), hipFuncAttributeMaxDynamicSharedMemorySize, levelLds));
+    if (planeFits(g, p->rb, ldsMax)) POOLCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_plane<double>), hipFuncAttributeMaxDynamicSharedMemorySize, planeLds));
   } else {
-    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_level<float>), hipFuncAttributeMaxDynamicSharedMemorySize, levelLds));
-    if (planeFits(g, p->rb, ldsMax)) HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_plane<float>), hipFuncAttributeMaxDynamicSharedMemorySize, planeLds));
+    POOLCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_level<float>), hipFuncAttributeMaxDynamicSharedMemorySize, levelLds));
+    if (planeFits(g, p->rb, ldsMax)) POOLCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_plane<float>), hipFuncAttributeMaxDynamicSharedMemorySize, planeLds));
   }
   p->used.assign(capacity, 0);
   if (id == int(c->pools.size())) c->pools.push_back(p); else c->pools[id] = p;
   *pool = id;
   return 0;
 }
+#undef POOLCHK
 
 int ml_pool_free(ml_ctx *c, int pool) {
   Pool *p = c ? poolOf(c, pool) : nullptr;
