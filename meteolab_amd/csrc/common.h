@@ -122,6 +122,7 @@ template <class R> struct Bufs {
   int ftiles;      // strip tiles of a level (stageTile)
   double *gpart;   // [slot][2][gmax] a launch's block parts of the charge carried to the ground (falls, conduction)
   int gmax;
+  unsigned char *zflag;  // [slot][N_CARRIED][nz] a level of a carried field holds a non-zero node (this step's transport)
 
   __device__ float *f(int slot, int k, const Geo &g) const { return fields + (size_t(slot) * N_FIELDS + k) * g.n; }
   __device__ float *x(int slot, int k, const Geo &g) const { return stx + (size_t(slot) * N_CARRIED + k) * g.n; }
@@ -142,6 +143,7 @@ template <class R> struct Bufs {
   __device__ double *fillPart(int slot, int cf, int z, int tile, const Geo &g) const {
     return fillp + (((size_t(slot) * N_CONSERVED + cf) * g.nz + z) * ftiles + tile) * 3;
   }
+  __device__ unsigned char *zf(int slot, int f, const Geo &g) const { return zflag + (size_t(slot) * N_CARRIED + f) * g.nz; }
   __device__ int *lm(int slot, int k, const Geo &g) const { return lmark + (size_t(slot) * N_LMARK + k) * g.n; }
   __device__ double *ln(int slot, int k, const Geo &g) const { return lnode + (size_t(slot) * N_LNODE + k) * g.n; }
 };

@@ -328,6 +328,32 @@ __device__ inline bool monotoneField(int f) { return !(f == F_U || f == F_V || f
 // The conserved (positivity-fixed) carried fields by their index (model.ts:1319-1327).
 __device__ inline int conservedField(int k) { const int c[N_CONSERVED] = {F_RAIN, F_SNOW, F_GRAUPEL, F_HAIL, F_Q, F_CLOUD, F_ICE}; return c[k]; }
 __device__ inline int conservedIndex(int f) { return f <= F_HAIL ? f : f == F_Q ? 4 : f == F_CLOUD ? 5 : f == F_ICE ? 6 : -1; }
+// Which levels of the monotone fields fbase .. fbase + nf - 1 hold a non-zero node, before their transport: a block of
+// the transport whose field is zero everywhere its stage's result can reach from (3 levels a stage) has a zero
+// tendency, and its result is its start (zero). One block per (level, field) and run; the other fields are flagged
+// non-zero, and with `all` every field (the autotuner times the whole work).
+template <class R> __global__ __launch_bounds__(256) void k_zero_flags(Geo g, Bufs<R> b, int fbase, int all) {
+  const int slot = b.runs[blockIdx.y], f = fbase + int(blockIdx.x) / g.nz, z = int(blockIdx.x) % g.nz;
+  int any = all || !monotoneField(f);
+  if (!any) {
+    const float *q = b.f(slot, f, g) + size_t(z) * g.layer;
+    for (int c = threadIdx.x; c < g.layer && !any; c += blockDim.x) any = q[c] != 0;
+  }
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) b.zf(slot, f, g)[z] = any ? 1 : 0;
+}
+// Whether field f was zero at levels lo .. hi (clamped to the grid) by this step's flags.
+template <class R> __device__ inline bool zeroLevels(const Bufs<R> &b, const Geo &g, int slot, int f, int lo, int hi) {
+  if (!monotoneField(f)) return false;
+  lo = lo < 0 ? 0 : lo; hi = hi > g.nz - 1 ? g.nz - 1 : hi;
+  const unsigned char *fl = b.zf(slot, f, g);
+  for (int z = lo; z <= hi; z++) if (fl[z]) return false;
+  return true;
+}
+// Stage `stage` of levels z0 .. z1 - 1 reads its input at 3 levels around them, and that input is zero where the
+// state was zero within 3 more levels for each stage before it.
+__device__ inline int stageReach(int stage) { return 3 + 3 * stage; }
+
 // The positivity-fix sums (removed, kept, negative; level-weighted) of one level (or strip tile) of conserved field cf,
 // reduced over the block in a fixed order into their part of the run's sums; k_fill_total adds the parts in level
 // order, so the sums do not depend on the order the blocks run in. Every thread of the block calls; fill is cleared.
@@ -353,6 +379,19 @@ template <class R> __global__ void k_ground_total(Geo g, Bufs<R> b, int which, i
   __shared__ double red[16];
   s = blockSum(s, red);
   if (threadIdx.x == 0 && s != 0) b.sc(slot)[S_CHARGE_GROUND] += s;
+}
+// The result of a stage of levels z0 .. z1 - 1 of a field that is zero within the stage's reach: stages 0 and 1 write
+// their zero output (the start is zero); stage 2's output is the state, which keeps its zeros. The positivity-fix parts
+// of a conserved field are zero. Every thread of the block calls (tile: the strip tile of a level, 0 otherwise; the
+// strip layout passes the tile's nodes as z0 = z1 - 1 with its own writes).
+template <class R> __device__ void zeroStage(const Bufs<R> &b, const Geo &g, int slot, int f, int z0, int z1, int stage, int tile, int cf, double (*red)[3]) {
+  if (stage < 2) {
+    float *out = b.set(stage + 1, slot, f, g);
+    for (size_t i = size_t(z0) * g.layer + threadIdx.x; i < size_t(z1) * g.layer; i += blockDim.x) out[i] = 0.f;
+  } else if (cf >= 0) {
+    double fill[3] = {0, 0, 0};
+    for (int z = z0; z < z1; z++) storeFill(b, g, slot, cf, z, tile, fill, red);
+  }
 }
 // The run's positivity-fix sums from the parts the last transport stage stored (tiles: parts per level).
 template <class R> __global__ void k_fill_total(Geo g, Bufs<R> b, int tiles) {
@@ -427,6 +466,12 @@ template <class R> __global__ __launch_bounds__(512) void k_stage(Geo g, Bufs<R>
   int zc[7];
   for (int k = 0; k < 7; k++) { int zz = z - 3 + k; zz = zz < 0 ? 0 : zz > nz - 1 ? nz - 1 : zz; zc[k] = zz * layer; }
   for (int f = f0; f < f1; f++) {
+    if (zeroLevels(b, g, slot, f, z - stageReach(stage), z + stageReach(stage))) {
+      // Zero within the stage's reach: the strip's zero output, or its zero positivity-fix part.
+      if (stage < 2) { if (live) b.set(outSet, slot, f, g)[i] = 0.f; }
+      else if (conservedIndex(f) >= 0) { double zero[3] = {0, 0, 0}; storeFill(b, g, slot, conservedIndex(f), z, blk % T.tiles, zero, red); }
+      continue;
+    }
     const float *q = b.set(inSet, slot, f, g);
     // u, v and theta (linear upwind5) are transported as departures from their level's base profile B(z), plus the
     // profile's own tendency: the scheme is linear, so this is the same tendency, without float32 sums of ~25 m/s or
@@ -520,6 +565,7 @@ template <class R> __global__ __launch_bounds__(TPB) void k_stage_level(Geo g, B
   const double *B = lvl >= 0 ? b.lv(slot, lvl, g) : nullptr;
   const bool mono = monotoneField(f), isW = f == F_W;
   const int cf = conservedIndex(f);
+  if (zeroLevels(b, g, slot, f, z0 - stageReach(stage), z1 - 1 + stageReach(stage))) { zeroStage(b, g, slot, f, z0, z1, stage, 0, cf, red); return; }
   const Div dNx(nx);
   const R idx = recip(R(g.dx)), idy = recip(R(g.dy)), frac = R((stage == 0 ? 1.0 / 3 : stage == 1 ? 1.0 / 2 : 1.0) * dt);
   const double *rho = b.lv(slot, L_RHO, g);
@@ -725,6 +771,7 @@ template <class R> __global__ __launch_bounds__(TPB) void k_stage_plane(Geo g, B
   extern __shared__ char smem[];
   __shared__ double red[TPB / 64][3];
   const int cf = conservedIndex(f);
+  if (zeroLevels(b, g, slot, f, z0 - stageReach(stage), z1 - 1 + stageReach(stage))) { zeroStage(b, g, slot, f, z0, z1, stage, 0, cf, red); return; }
   if (f == F_U || f == F_V || f == F_THETA) stagePlane<R, true, false>(g, b, slot, f, z0, z1, stage, dt, smem, cf, red);
   else if (f == F_W) stagePlane<R, false, false>(g, b, slot, f, z0, z1, stage, dt, smem, cf, red);
   else stagePlane<R, false, true>(g, b, slot, f, z0, z1, stage, dt, smem, cf, red);

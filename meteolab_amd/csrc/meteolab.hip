@@ -59,7 +59,7 @@ struct Pool {
        *spec = nullptr, *mean0 = nullptr, *km = nullptr, *kh = nullptr, *ephi = nullptr, *cols = nullptr, *lev = nullptr,
        *par = nullptr, *scal = nullptr, *zs = nullptr, *dzs = nullptr, *hz = nullptr, *sym = nullptr, *twx = nullptr, *twy = nullptr,
        *lrho = nullptr, *lmark = nullptr, *lnode = nullptr, *chan = nullptr, *cdep = nullptr, *flog = nullptr, *efac = nullptr,
-       *ehat = nullptr, *emean0 = nullptr, *dspec = nullptr, *fillp = nullptr, *gpart = nullptr;
+       *ehat = nullptr, *emean0 = nullptr, *dspec = nullptr, *fillp = nullptr, *gpart = nullptr, *zflag = nullptr;
   int ftiles = 1, gmax = 1;  // strip tiles of a level; blocks of a run in the launches that carry charge to the ground
   int chunk = 1;          // levels per pass of the discharge's own electric solve (as many as the block's LDS holds)
   RotLevels rot{};        // the rotation diagnostics' levels and threshold
@@ -140,6 +140,7 @@ template <class R> Bufs<R> bufs(const Pool &p, const int *runs) {
   b.efac = (const double *)p.efac; b.lrho = (R *)p.lrho; b.lmark = (int *)p.lmark; b.lnode = (double *)p.lnode; b.chan = (int *)p.chan; b.cdep = (double *)p.cdep; b.flog = (double *)p.flog;
   b.ehat = (double *)p.ehat; b.emean0 = (double *)p.emean0; b.dspec = (double *)p.dspec;
   b.fillp = (double *)p.fillp; b.ftiles = p.ftiles; b.gpart = (double *)p.gpart; b.gmax = p.gmax;
+  b.zflag = (unsigned char *)p.zflag;
   return b;
 }
 
@@ -232,6 +233,9 @@ template <class R> void launchPart(const Pool &p, const int *list, int nr, doubl
       hipLaunchKernelGGL(k_ground_total<R>, dim3(nr), dim3(256), 0, s, g, b, 0, int(blocks(g.layer, SED_COLS)));
       mark("k_ground_total");
     }
+    // Which levels of the fields transported next hold anything (a zero neighbourhood's transport is skipped).
+    hipLaunchKernelGGL(k_zero_flags<R>, dim3(g.nz * (split ? int(F_CHARGE_ICE) : nc), nr), dim3(256), 0, s, g, b, 0, 0);
+    mark("k_zero_flags");
     // dynamics: face projection (FluxTransport.setVelocity) and the three RK3 stages with the stage projections.
     hipLaunchKernelGGL(k_faces<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 0);
     mark("k_faces");
@@ -258,6 +262,8 @@ template <class R> void launchPart(const Pool &p, const int *list, int nr, doubl
     mark("k_sediment");
     hipLaunchKernelGGL(k_ground_total<R>, dim3(nr), dim3(256), 0, s, g, b, 0, int(blocks(g.layer, SED_COLS)));
     mark("k_ground_total");
+    hipLaunchKernelGGL(k_zero_flags<R>, dim3(g.nz * (N_CARRIED - F_CHARGE_ICE), nr), dim3(256), 0, s, g, b, int(F_CHARGE_ICE), 0);
+    mark("k_zero_flags");
     for (int stage = 0; stage < 3; stage++) {
       launchStage<R>(g, b, nr, N_CARRIED - F_CHARGE_ICE, F_CHARGE_ICE, stage, dt, s, tu);
       mark(tu.layout == 2 ? "k_stage_plane" : tu.layout == 1 ? "k_stage_level" : "k_stage");
@@ -355,10 +361,10 @@ void fillTables(Pool &p, std::vector<char> &sym, std::vector<char> &twx, std::ve
 }
 
 struct Sizes {
-  size_t fields, stage, mass, node, node8, spec, mean0, cols, lev, par, scal, lmark, lnode, chan, cdep, flog, fillp, gpart;
+  size_t fields, stage, mass, node, node8, spec, mean0, cols, lev, par, scal, lmark, lnode, chan, cdep, flog, fillp, gpart, zflag;
   size_t slot() const {
     return fields + 2 * stage + mass + 3 * node + 4 * node8 + 3 * spec + 2 * mean0 + cols + lev + par + scal + lmark + lnode + chan + cdep + flog +
-           fillp + gpart;
+           fillp + gpart + zflag;
   }
 };
 // Blocks of a run in the launches that carry charge to the ground: the charge falls and the conduction.
@@ -372,6 +378,7 @@ Sizes sizesOf(const Geo &g, size_t rb) {
   s.flog = size_t(FLASH_LOG) * FLASH_FIELDS * 8;
   s.fillp = size_t(N_CONSERVED) * g.nz * stageTile(g).tiles * 3 * 8;
   s.gpart = 2 * size_t(groundBlocks(g)) * 8;
+  s.zflag = (size_t(N_CARRIED) * g.nz + 15) / 16 * 16;
   return s;
 }
 int geoOf(ml_ctx *c, const ml_grid *o, Geo &g, Plan &pl) {
@@ -441,7 +448,7 @@ void freePool(Pool *p) {
   void *all[] = {p->fields, p->stx, p->sty, p->mass, p->phi, p->cells, p->press, p->spec, p->mean0, p->km, p->kh, p->ephi, p->cols,
                  p->lev, p->par, p->scal, p->zs, p->dzs, p->hz, p->sym, p->twx, p->twy, p->single, p->diag, p->dlist, p->lrho,
                  p->lmark, p->lnode, p->chan, p->cdep, p->flog, p->efac, p->ehat, p->emean0, p->dspec, p->fillp,
-                 p->gpart, p->pmap};
+                 p->gpart, p->pmap, p->zflag};
   for (void *a : all) if (a) hipFree(a);
   delete p;
 }
@@ -512,7 +519,7 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
     {&p->kh, s.node * cap}, {&p->ephi, s.node8 * cap}, {&p->cols, s.cols * cap}, {&p->lev, s.lev * cap}, {&p->par, s.par * cap},
     {&p->scal, s.scal * cap}, {(void **)&p->diag, cap * DIAG_ROW * 8}, {(void **)&p->dlist, cap * sizeof(int)}, {&p->lrho, s.node * cap}, {&p->lmark, s.lmark * cap},
     {&p->lnode, s.lnode * cap}, {&p->chan, s.chan * cap}, {&p->cdep, s.cdep * cap}, {&p->flog, s.flog * cap},
-    {&p->fillp, s.fillp * cap}, {&p->gpart, s.gpart * cap}, {&p->pmap, size_t(g.layer) * 4}};
+    {&p->fillp, s.fillp * cap}, {&p->gpart, s.gpart * cap}, {&p->pmap, size_t(g.layer) * 4}, {&p->zflag, s.zflag * cap}};
   p->ftiles = stageTile(g).tiles; p->gmax = groundBlocks(g);
   for (auto &a : allocs) {
     hipError_t e = hipMalloc(a.ptr, a.bytes);
@@ -854,6 +861,7 @@ template <class R> float timeStage(ml_ctx *c, const Pool &p, const Batch &bt, co
   const int nr = int(bt.runs.size()), nc = p.electrified ? N_CARRIED : F_COLD + 1;
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0); (void)hipEventCreate(&e1);
+  hipLaunchKernelGGL(k_zero_flags<R>, dim3(g.nz * nc, nr), dim3(256), 0, c->stream, g, b, 0, 1);
   float best = 1e30f;
   for (int rep = 0; rep < 3; rep++) {
     (void)hipEventRecord(e0, c->stream);
