@@ -96,6 +96,9 @@ def main():
     ap.add_argument("--plain-members", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--table-only", action="store_true",
+                    help="only the per-kernel table and the transport roofline (eager launches, no graph replays: "
+                         "for hardware-counter runs, whose profiler cannot follow graph replays)")
     ap.add_argument("--compare-in-line", action="store_true",
                     help="also time the same runs with each step's lightning in line (ml_ctx_set_overlap off)")
     ap.add_argument("--transport", default="auto",
@@ -119,6 +122,25 @@ def main():
         layout, count, xcd = a.transport.split(":")
         M.set_batch_tuning(runs, layout, groups=int(count) if layout == "strip" else 1, xcd=xcd == "1",
                            levels=int(count) if layout != "strip" else 1)
+    if a.table_only:
+        # The per-kernel table of the members from their initial state, in the shape asked (or the autotuner's).
+        M.set_batch_tuning(runs, **tuning_args(M.batch_tuning(runs)))
+        if a.warmup:
+            M.step(runs, a.warmup, DT)
+        kt = M.kernel_times(M.profile_step(runs, a.steps, DT))
+        stage_kernel = next(k for k in ("k_stage_plane", "k_stage_level", "k_stage") if k in kt)
+        split = "k_charge_courant" in kt
+        launches, stage_ms = kt[stage_kernel]
+        stage_b = stage_bytes_per_step(n, split) * a.runs
+        print(json.dumps({"steps": a.steps, "tuning": {"batched": M.batch_tuning(runs)},
+                          "roofline": {"kernel": stage_kernel, "bytes_per_launch": stage_b / launches,
+                                       "ms_per_launch": stage_ms / launches, "launches_per_step": launches,
+                                       "achieved": stage_b / (stage_ms / 1e3) / 1e9},
+                          "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][1])},
+                          # a copy of known size, whose counter readings calibrate the transport's
+                          "copy_GBps": ctx.copy_bandwidth()}))
+        ctx.close()
+        return
     if a.warmup:
         M.step(runs, a.warmup, DT)
         ctx.sync()
@@ -162,6 +184,7 @@ def main():
     stage_launches, stage_ms = kt[stage_kernel]
     stage_b = stage_bytes_per_step(n, split) * a.runs
     achieved = stage_b / (stage_ms / 1e3)
+    traffic = measured_traffic(stage_kernel, a.runs, grid, M.batch_tuning(runs), split)
     dominant = max(kt.items(), key=lambda kv: kv[1][1])[0]
 
     # Plain: members run one at a time, each through the same steps on its own (one-run launches).
@@ -200,13 +223,19 @@ def main():
         "latency_per_model_minute_ms": {"p50": percentile(minutes, 50), "p99": percentile(minutes, 99),
                                         "runs_per_batch": a.runs, "minutes": len(minutes)},
         "roofline": {"bound": "hbm", "kernel": stage_kernel, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM,
-                     "traffic": measured_traffic(stage_kernel, a.runs, grid, M.batch_tuning(runs), split),
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
                      "bytes_per_launch": stage_b / stage_launches, "ms_per_launch": stage_ms / stage_launches,
-                     "launches_per_step": stage_launches},
+                     "launches_per_step": stage_launches,
+                     # The transport skips the blocks whose field is zero within the stage's reach, so `achieved` (the
+                     # full-field algorithmic bytes over the time) is an effective rate; the HBM rate the kernel draws
+                     # is the measured traffic over the same time.
+                     "measured_GBps": traffic / (stage_ms / stage_launches / 1e3) / 1e9 if traffic else None},
         "step_roofline": {"bytes_per_cell_step": STEP_BYTES_PER_CELL, "achieved_GBps": batched * STEP_BYTES_PER_CELL / 1e9,
                           "frac": batched * STEP_BYTES_PER_CELL / PEAK_HBM},
         "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][1])},
+        # The longest kernel a step in the per-kernel table (in line). The lightning's discharge, serial by the
+        # reference's algorithm (a block per run growing the channel link by link, with a solve per stage), has no
+        # bandwidth roofline to speak of; `roofline` is the transport's, the largest data-parallel kernel.
         "dominant_kernel": dominant,
         "tuning": {"batched": M.batch_tuning(runs), "plain": plain_tuning},
         "copy_GBps": ctx.copy_bandwidth(),
