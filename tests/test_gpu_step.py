@@ -55,9 +55,10 @@ def test_base_state_and_initial_state_match(ctx):
         assert np.array_equal(r.field(name), o.field(name)), name
 
 
-@pytest.mark.parametrize("grid", ["small", "standard"])
+@pytest.mark.parametrize("grid", ["small", "standard", "large"])
 def test_parity_after_one_and_ten_steps(ctx, grid):
-    g = C.small_grid() if grid == "small" else C.STANDARD
+    # The three sizes of the workloads: the small test grid, the standard (40x32x50) and the large (80x64x50) domain.
+    g = C.small_grid() if grid == "small" else C.STANDARD if grid == "standard" else C.LARGE
     prof, run = C.profile(), C.run(g)
     o = OracleRun(g, prof, run)
     r = ctx.pool(g, 1).run(prof, run)
