@@ -45,6 +45,7 @@ enum : int {
 enum : int {
   S_COURANT = 0,         // 7 words: the largest Courant number of each fall (float bits, atomicMax on uint32)
   S_FILL = 8,            // 7 x 3 doubles: removed, kept, negative of the positivity fix of each conserved field
+  S_CHARGED = 29,        // uint64: nonzero when a node of the run holds charge at the step's electric solve
   S_STEP_END = 30,       // [0, S_STEP_END) is cleared at the start of each step
   S_CLIPPED = 32,        // int64: velocity components clipped since the start (model.ts:1586-1595)
   S_NEGATIVE = 33,       // double: negative mass removed by the positivity fix since the start (model.ts:1119)
@@ -176,6 +177,10 @@ template <class T> __device__ inline T blockSum(T v, T *shared) {
   T t = 0;
   for (int w = 0; w < waves; w++) t += shared[w];
   return t;
+}
+// Whether the run held no charge anywhere at this step's electric solve (k_charge_rhs): its potential is zero.
+template <class B> __device__ inline bool unchargedRun(const B &b, int slot) {
+  return reinterpret_cast<const unsigned long long *>(b.sc(slot))[S_CHARGED] == 0;
 }
 template <class T> __device__ inline T blockMax(T v, T *shared) {
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;

@@ -21,25 +21,30 @@ template <class R> __device__ inline double chargeDensity(const Geo &g, const Bu
 // rho at node i into lrho; at the interior levels -rho/eps0 into `cells` (level k at layer k - 1). The cloudiness the
 // lightning reads (cloud + ice + snow, float32 as lightning.ts:320-322 adds them) goes with it into F_CLOUDY, so that
 // the discharge and the conduction of this step see this step's clouds even while the next step's dynamics runs.
-template <class R> __device__ inline void chargeAt(const Geo &g, const Bufs<R> &b, int slot, int i) {
+template <class R> __device__ inline double chargeAt(const Geo &g, const Bufs<R> &b, int slot, int i) {
   int z = i / g.layer;
   double rho = chargeDensity(g, b, slot, i, b.lv(slot, L_RHO, g)[z]);
   b.arr(b.lrho, slot, g)[i] = R(rho);
   if (z > 0 && z < g.nz - 1) b.arr(b.cells, slot, g)[i - g.layer] = -rho / EPSILON0;
   b.f(slot, F_CLOUDY, g)[i] = (b.f(slot, F_CLOUD, g)[i] + b.f(slot, F_ICE, g)[i]) + b.f(slot, F_SNOW, g)[i];
+  return rho;
 }
-// Also clears the run's strongest-field key, which k_strongest then raises after the solve.
+// Also clears the run's strongest-field key, which k_strongest then raises after the solve, and marks the run charged
+// when any of its nodes holds charge (an uncharged run's electric solve is zero, and skipped).
 template <class R> __global__ __launch_bounds__(256) void k_charge_rhs(Geo g, Bufs<R> b) {
   int slot = b.runs[blockIdx.y];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) reinterpret_cast<unsigned long long *>(b.sc(slot))[S_STRONGEST] = 0;
-  if (i < g.n) chargeAt(g, b, slot, i);
+  int charged = 0;
+  if (i < g.n) charged = chargeAt(g, b, slot, i) != 0;
+  if (__syncthreads_or(charged) && threadIdx.x == 0) atomicOr(reinterpret_cast<unsigned long long *>(b.sc(slot)) + S_CHARGED, 1ull);
 }
 
 // The largest (|E| / E_init)^2 over the interior levels, packed with its node (the first node in node order wins a tie,
 // as the reference's strict comparison does) into the run's S_STRONGEST key.
 template <class R> __global__ __launch_bounds__(256) void k_strongest(Geo g, Bufs<R> b) {
   int slot = b.runs[blockIdx.y];
+  if (unchargedRun(b, slot)) return;  // no field: the key stays 0
   int i = g.layer + blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long key = 0;
   if (i < g.n - g.layer) {

@@ -195,14 +195,29 @@ template <class R> __device__ void inverseLevel(const Geo &g, const Bufs<R> &b, 
   __syncthreads();
 }
 
+// The electric solve (keep, kind 2) of a run without charge: its right-hand side's kept spectra and level sums are
+// zero, the modes are skipped, and its potential is zero.
 template <class R> __global__ __launch_bounds__(SPEC_THREADS) void k_spec_fwd(Geo g, Bufs<R> b, Plan pl, int L, int keep) {
   extern __shared__ char smem[];
   __shared__ double red[SPEC_THREADS / 64];
-  forwardLevel(g, b, pl, b.runs[blockIdx.y], blockIdx.x, smem, red, keep != 0);
+  const int slot = b.runs[blockIdx.y], lev = blockIdx.x;
+  if (keep && unchargedRun(b, slot)) {
+    double *kept = b.eh(slot, g) + size_t(lev) * g.nm * 2;
+    for (int t = threadIdx.x; t < 2 * g.nm; t += blockDim.x) kept[t] = 0;
+    if (threadIdx.x == 0) b.em0(slot, g)[lev] = 0;
+    return;
+  }
+  forwardLevel(g, b, pl, slot, lev, smem, red, keep != 0);
 }
 template <class R> __global__ __launch_bounds__(SPEC_THREADS) void k_spec_inv(Geo g, Bufs<R> b, Plan pl, int L, int kind) {
   extern __shared__ char smem[];
-  inverseLevel(g, b, pl, b.runs[blockIdx.y], blockIdx.x, kind, smem);
+  const int slot = b.runs[blockIdx.y], lev = blockIdx.x;
+  if (kind == 2 && unchargedRun(b, slot)) {
+    double *dst = b.arr(b.ephi, slot, g) + size_t(lev + 1) * g.layer;
+    for (int t = threadIdx.x; t < g.layer; t += blockDim.x) dst[t] = 0;
+    return;
+  }
+  inverseLevel(g, b, pl, slot, lev, kind, smem);
 }
 
 // Row k of the tridiagonal system of mode (symbol a, b) of kind 0 (face projection, advection.ts:222-241, L = nz),
@@ -305,7 +320,7 @@ template <class R> __device__ void modeSolve(const Geo &g, const Bufs<R> &b, int
 template <class R> __global__ __launch_bounds__(MODE_THREADS) void k_modes(Geo g, Bufs<R> b, int kind, int L) {
   int slot = b.runs[blockIdx.y];
   int mode = blockIdx.x * MODE_THREADS + threadIdx.x;
-  if (mode >= g.nm) return;
+  if (mode >= g.nm || (kind == 2 && unchargedRun(b, slot))) return;
   extern __shared__ char smem[];
   modeSolve(g, b, slot, mode, kind, L, reinterpret_cast<double *>(smem) + threadIdx.x, MODE_THREADS);
 }

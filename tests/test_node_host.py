@@ -107,15 +107,18 @@ def test_gpu_dynamics_interface_steps_as_the_oracle(tmp_path):
 @needs_node
 def test_node_run_members_gives_member_results(tmp_path):
     # runMembers for two runs of a pool over 2 model minutes: MemberResult objects as src/ensemble/combine.ts reads
-    # them, with a sample every 60 s; the swaths end as the Python host's for the same runs, and so do the maps.
+    # them, with a sample every 60 s; the map counts the Node host's own swaths, and the swaths end as the Python
+    # host's for the same runs. Each process tunes its own transport layout, and the layouts round differently in
+    # float32, so across the two hosts the swaths agree to the step's parity, not bit for bit.
     g = C.small_grid()
     code = ("const m = require('./node/meteolab'); const dev = m.MeteolabDevice.create();"
             f"const g = {json.dumps(g)}; const pool = dev.pool(g, 2);"
             "const runs = [0, 1].map(k => pool.run(m.profile(), m.run(g, k + 1)));"
             "const res = m.runMembers(pool, runs, 2);"
-            "const map = pool.probabilityMap(runs, 'gustMax', 5);"
+            "const map = pool.probabilityMap(runs, 'gustMax', 0.5);"
             "console.log(JSON.stringify({t: res[0].samples.map(s => s.t), keys: Object.keys(res[0]).sort(),"
-            " gust: Array.from(res[1].gustMax), uh: Array.from(res[1].uhMax), map: Array.from(map), seed: res[1].seed})); dev.destroy()")
+            " gust0: Array.from(res[0].gustMax), gust: Array.from(res[1].gustMax), uh: Array.from(res[1].uhMax),"
+            " map: Array.from(map), seed: res[1].seed})); dev.destroy()")
     res = node(code)
     assert res.returncode == 0, res.stderr
     d = json.loads(res.stdout.strip().splitlines()[-1])
@@ -126,6 +129,9 @@ def test_node_run_members_gives_member_results(tmp_path):
         pool = ctx.pool(g, 2)
         runs = [pool.run(C.profile(), C.run(g, seed=k + 1)) for k in range(2)]
         py = E.run_members(runs, 2)
-        assert np.array_equal(np.asarray(d["gust"], np.float32), py[1]["gustMax"])
-        assert np.array_equal(np.asarray(d["uh"], np.float32), py[1]["uhMax"])
-        assert np.array_equal(np.asarray(d["map"], np.float32), M.probability_map(runs, "gustMax", 5))
+    counts = sum((np.asarray(d[k], np.float64) >= 0.5).astype(np.float32) for k in ("gust0", "gust"))
+    assert 0 < counts.sum() < 2 * len(counts)
+    assert np.array_equal(np.asarray(d["map"], np.float32), (counts.astype(np.float64) / 2).astype(np.float32))
+    for key, col in (("gust", "gustMax"), ("uh", "uhMax")):
+        x, y = np.asarray(d[key], np.float64), np.asarray(py[1][col], np.float64)
+        assert np.linalg.norm(x - y) <= 1e-5 * np.linalg.norm(y), key
