@@ -9,8 +9,11 @@ member_peaks, EVENTS, probabilities, spread and regime_of restate it here, and p
 
 The step's sounding is the analytic one (u = shear tanh(z / shear_depth), vapour capped at qv_max), so withMoisture and
 withShear act as on the reference's Weisman-Klemp profile (grid.ts:35-36, 57-62): the vapour cap scaled, the hodograph
-scaled as a whole to the 0-6 km difference asked. The reference's slider and sounding branches, perturbConfig (which
-perturbs the sliders) and the CAPE of environmentOf (a parcel computation) have no counterpart on this sounding.
+scaled as a whole to the 0-6 km difference asked. perturb_run restates perturbConfig (member.ts:45-65) with the same
+draws in the same order: the ground temperature, the low-level humidity and the thermal's strength and place are
+perturbed; the wind errors are drawn but not applied, since this sounding's wind is one tanh profile with no per-level
+winds. The reference's slider and sounding branches and the CAPE of environmentOf (a parcel computation) have no
+counterpart on this sounding.
 """
 import math
 
@@ -169,11 +172,49 @@ def regime_cells(profile, size):
             for i, m in enumerate(g["moisture"]) for j, s in enumerate(g["shear"])]
 
 
-def ensemble(grid, members, minutes, ctx, profile=None):
+# member.ts:33: the starting data's uncertainty (standard deviations): K, % relative humidity, m/s, degrees, the
+# thermal's relative strength, m.
+UNCERTAINTY = {"temperature": .5, "humidity": 4, "wind": 1.5, "direction": 8, "thermal": .1, "offset": 1500}
+
+
+def normal(rng):
+    """member.ts:36: normal deviates by Box-Muller from a generator of uniforms in [0, 1), two uniforms a deviate (the
+    first under the logarithm)."""
+    def draw():
+        a = next(rng)
+        return math.sqrt(-2 * math.log(1 - a)) * math.cos(2 * math.pi * next(rng))
+    return draw
+
+
+def perturb_run(grid, profile, spec):
+    """perturbConfig (member.ts:45-65) on the analytic sounding and a run: new (profile, spec) dicts, the inputs
+    untouched. From the run's own seed (mulberry32(seed ^ 0x5bd1e995)) in the reference's order: the ground temperature
+    error dT (the sounding's theta0 moves by dT, as surfaceTemp does), the humidity error dRh in % (the vapour cap,
+    which sets the low-level vapour of this sounding, times 1 + dRh / 100, as a sounding's vapour at the ground), the
+    wind errors at 0, 3, 6 and 10 km (u then v; drawn, not applied), the thermal's strength (times max(.5, 1 + .1 n))
+    and its place (startOffset: from the thermal's default place, replacing any offset the run had)."""
+    n, U = normal(C.mulberry32(int(spec["seed"]) ^ 0x5bd1e995)), UNCERTAINTY
+    dT, dRh = n() * U["temperature"], n() * U["humidity"]
+    for _ in range(8):  # du, dv at the four wind nodes
+        n()
+    prof = dict(profile, theta0=profile["theta0"] + dT, qv_max=profile["qv_max"] * (1 + dRh / 100))
+    amp = spec["bubble_amp"] * max(.5, 1 + n() * U["thermal"])
+    ox = n() * U["offset"]
+    oy = n() * U["offset"]
+    return prof, dict(spec, bubble_amp=amp, bubble_x=grid["width"] / 3 + ox, bubble_y=grid["depth"] / 2 + oy)
+
+
+def ensemble(grid, members, minutes, ctx, profile=None, data=False):
     """An ensemble of `members` runs (the ensemble configuration's members, config.ensemble_member) stepped together:
-    their MemberResults, the event probabilities and the four probability maps."""
+    their MemberResults, the event probabilities and the four probability maps. With `data`, every member but the
+    first (the control) has its starting data perturbed by perturb_run, as the reference's "seed and data" ensembles
+    (panel.ts:110)."""
     pool = ctx.pool(grid, members)
-    runs = [pool.run(profile or C.profile(), C.ensemble_member(grid, k)) for k in range(members)]
+    prof = profile or C.profile()
+    starts = [(prof, C.ensemble_member(grid, k)) for k in range(members)]
+    if data:
+        starts = [s if k == 0 else perturb_run(grid, *s) for k, s in enumerate(starts)]
+    runs = [pool.run(p, spec) for p, spec in starts]
     results = run_members(runs, minutes)
     peaks = [member_peaks(r) for r in results]
     maps = {kind: probability_map(runs, kind) for kind in MAPS}

@@ -112,6 +112,57 @@ def test_oracle_folds_columns_into_the_ground_shifted_by_the_domain_motion():
     assert np.array_equal(uh, expect)
 
 
+def test_normal_deviates_of_the_members_generator():
+    # member.ts:36 (Box-Muller): mean 0, variance 1, and the normal law's 68.27 % within one deviation and 95.45 %
+    # within two, over 40,000 deviates of one stream.
+    n = E.normal(C.mulberry32(0x5bd1e995 ^ 7))
+    x = np.array([n() for _ in range(40_000)])
+    assert abs(x.mean()) < .02 and abs(x.std() - 1) < .02
+    assert abs(np.mean(np.abs(x) < 1) - .6827) < .01 and abs(np.mean(np.abs(x) < 2) - .9545) < .005
+
+
+def test_perturbed_starting_data_from_the_members_seed():
+    # ensemble.test.ts:32-40: one seed gives one perturbation, the settings given untouched; another seed another; over
+    # 400 seeds the deviations have about the standard deviations asked (the reference's tolerances).
+    g, prof = C.small_grid(), C.profile()
+    base = C.run(g, seed=5)
+    kept = (dict(prof), dict(base))
+    a = E.perturb_run(g, prof, base)
+    assert a == E.perturb_run(g, prof, dict(base)) and (prof, base) == kept
+    assert E.perturb_run(g, prof, C.run(g, seed=6))[0]["theta0"] != a[0]["theta0"]
+    many = [E.perturb_run(g, prof, C.run(g, seed=k + 1)) for k in range(400)]
+
+    def sd(v):
+        return math.sqrt(sum(x * x for x in v) / len(v))
+    U = E.UNCERTAINTY
+    assert abs(sd([p["theta0"] - prof["theta0"] for p, _ in many]) - U["temperature"]) < .05
+    assert abs(sd([100 * (p["qv_max"] / prof["qv_max"] - 1) for p, _ in many]) / U["humidity"] - 1) < .5
+    assert abs(sd([s["bubble_x"] - g["width"] / 3 for _, s in many]) / U["offset"] - 1) < .5
+    assert abs(sd([s["bubble_y"] - g["depth"] / 2 for _, s in many]) / U["offset"] - 1) < .5
+    strength = [s["bubble_amp"] / 2.0 for _, s in many]
+    assert min(strength) >= .5 and abs(sd([x - 1 for x in strength]) / U["thermal"] - 1) < .5
+    assert all(p["shear"] == prof["shear"] and p["dtheta"] == prof["dtheta"] for p, _ in many)
+
+
+def test_the_thermal_moves_with_its_offset():
+    # ensemble.test.ts:46-51: the warmest node of the thermal's level moves by the offset, (4 dx, 2 dy) -> (4, 2).
+    g = C.small_grid()
+    nx, layer = g["nx"], g["nx"] * g["ny"]
+    dx, dy = g["width"] / nx, g["depth"] / g["ny"]
+
+    def warmest(spec):
+        o = OracleRun(g, C.profile(), spec)
+        zs, _ = o.levels()
+        k = int(np.argmin(np.abs(np.asarray(zs) - 1400)))
+        c = int(np.argmax(o.field("theta")[k * layer:(k + 1) * layer]))
+        return c % nx, c // nx
+
+    plain = C.run(g, seed=3, noise_amp=0.0)
+    x0, y0 = warmest(plain)
+    x1, y1 = warmest(dict(plain, bubble_x=plain["bubble_x"] + 4 * dx, bubble_y=plain["bubble_y"] + 2 * dy))
+    assert (x1 - x0, y1 - y0) == (4, 2)
+
+
 def oracle_member(g, prof, spec, minutes, dt=3.0):
     """runMember (member.ts:103-117) on the oracle: samples every 60 s and the swaths."""
     o = OracleRun(g, prof, spec)
@@ -164,3 +215,20 @@ def test_members_stepped_together_sample_and_swath_as_the_oracle():
         for kind in E.MAPS:
             dev = E.probability_map(runs, kind)
             assert np.array_equal(dev, host_probability_map(results, kind)), kind
+
+
+@pytest.mark.gpu
+def test_a_seed_and_data_ensemble_steps_its_perturbed_members_as_the_oracle():
+    # The control keeps its starting data; the others run the perturbed soundings and thermals as the oracle does.
+    import meteolab_amd as M
+    g, prof = C.small_grid(), C.profile()
+    with M.Context(0, 32) as ctx:
+        print("seed and data ensemble, 3 members, 2 minutes", flush=True)
+        out = E.ensemble(g, 3, 2, ctx, data=True)
+    for k in range(3):
+        p, spec = (prof, C.ensemble_member(g, k)) if k == 0 else E.perturb_run(g, prof, C.ensemble_member(g, k))
+        _, want = oracle_member(g, p, spec, 2)
+        for a, b in zip(out["results"][k]["samples"], want["samples"]):
+            for key in ("updraft", "rain", "gust", "outflow"):
+                assert a[key] == pytest.approx(b[key], rel=1e-3, abs=1e-6), (k, a["t"], key)
+    assert out["results"][1]["samples"][-1]["updraft"] != out["results"][0]["samples"][-1]["updraft"]
