@@ -61,8 +61,10 @@ typedef struct { int nx, ny, nz; double width, depth, height, bottom; } ml_grid;
 
 /* Analytic sounding (src/core/profiles.ts:61-73 weismanKlemp): theta = theta0 + dtheta (z/ztr)^1.25 below ztr,
  * isothermal ttr above; RH = 1 - rh_drop (z/ztr)^1.25 (rh_top above), vapour capped at qv_max; wind
- * u = shear tanh(z / shear_depth), v = 0. */
-typedef struct { double theta0, dtheta, ztr, ttr, rh_drop, rh_top, qv_max, shear, shear_depth; } ml_profile;
+ * u = shear tanh(z / shear_depth), v = 0, plus the wind errors wind_u, wind_v at 0, 3, 6 and 10 km, linear between those
+ * heights and held below and above them (the sounding wind of perturbConfig, src/ensemble/member.ts:55-57; zero for
+ * the plain sounding). */
+typedef struct { double theta0, dtheta, ztr, ttr, rh_drop, rh_top, qv_max, shear, shear_depth, wind_u[4], wind_v[4]; } ml_profile;
 
 /* A run's settings (SimConfig values the step reads, config.ts:15-188) and its initial thermal and noise: a cos^2
  * bubble of bubble_amp K (model.ts:94 ELLIPSOID form) and uniform +-noise_amp K theta noise below noise_top from

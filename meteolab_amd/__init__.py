@@ -45,7 +45,15 @@ class Grid(ctypes.Structure):
 
 class Profile(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in
-                ("theta0", "dtheta", "ztr", "ttr", "rh_drop", "rh_top", "qv_max", "shear", "shear_depth")]
+                ("theta0", "dtheta", "ztr", "ttr", "rh_drop", "rh_top", "qv_max", "shear", "shear_depth")] + \
+        [("wind_u", ctypes.c_double * 4), ("wind_v", ctypes.c_double * 4)]
+
+
+def profile_struct(profile):
+    """The sounding dict as ml_profile; the wind errors (wind_u, wind_v at 0, 3, 6 and 10 km) are zero when absent."""
+    scalars = [float(profile[k]) for k, t in Profile._fields_ if t is ctypes.c_double]
+    winds = [(ctypes.c_double * 4)(*(float(x) for x in profile.get(k, (0.0,) * 4))) for k in ("wind_u", "wind_v")]
+    return Profile(*scalars, *winds)
 
 
 class RunConfig(ctypes.Structure):
@@ -222,7 +230,7 @@ class Run:
     def __init__(self, pool, profile, run):
         self.pool, self.ctx = pool, pool.ctx
         self.n, self.layer, self.grid = pool.n, pool.layer, pool.grid
-        p = Profile(*(float(profile[k]) for k, _ in Profile._fields_))
+        p = profile_struct(profile)
         r = RunConfig(*(run[k] for k, _ in RunConfig._fields_))
         h = ctypes.c_int()
         self.ctx.check(lib().ml_run_create(self.ctx.h, pool.id, ctypes.byref(p), ctypes.byref(r), ctypes.byref(h)))

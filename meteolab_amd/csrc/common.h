@@ -39,8 +39,20 @@ enum : int {
   P_BUBBLE_Z, P_BUBBLE_RH, P_BUBBLE_RZ, P_NOISE_SEED, P_NOISE_AMP, P_NOISE_TOP, P_QVMAX, P_SHEAR, P_SHEAR_DEPTH,
   P_THETA0, P_DTHETA, P_ZTR, P_TTR, P_RH_DROP, P_RH_TOP, P_ELECTRIFIED, P_TIME,
   P_SWATHS,  // nonzero: each step folds its surface wind and updraught helicity into the swath columns
-  N_PARAMS = 32
+  P_WIND_U,  // 4 words each: the sounding's wind errors at 0, 3, 6 and 10 km (m/s)
+  P_WIND_V = P_WIND_U + 4,
+  N_PARAMS = 40
 };
+static_assert(P_WIND_V + 4 <= N_PARAMS, "per-run parameters overflow");
+// The sounding's wind error at height z from its errors d at 0, 3, 6 and 10 km: linear between those heights, held
+// below and above them (perturbConfig's `at`, src/ensemble/member.ts:56).
+__host__ __device__ inline double windError(const double *d, double z) {
+  const double at[4] = {0, 3000, 6000, 10000};
+  int k = 0;
+  while (k < 2 && z >= at[k + 1]) k++;
+  const double f = (z - at[k]) / (at[k + 1] - at[k]);
+  return d[k] + (d[k + 1] - d[k]) * (f < 0 ? 0.0 : f > 1 ? 1.0 : f);
+}
 // Per-run scalars (8-byte words). Per step: cleared by k_begin; the rest accumulate over the run.
 enum : int {
   S_COURANT = 0,         // 7 words: the largest Courant number of each fall (float bits, atomicMax on uint32)

@@ -124,7 +124,8 @@ template <class R> __global__ __launch_bounds__(MDIAG_THREADS) void k_model_diag
   double *o = out + size_t(blockIdx.x) * DIAG_ROW;
   o[MD_UPDRAFT] = up; o[MD_DOWNDRAFT] = down; o[MD_RAINRATE] = rainRate; o[MD_RAINTOTAL] = rainTotal; o[MD_CLOUDTOP] = top;
   o[MD_THERMALTOP] = thermalTop; o[MD_MAXCLOUD] = maxCloud; o[MD_COLDMAX] = coldMax; o[MD_CORES] = cores;
-  o[MD_SHEAR06] = fabs(P[P_SHEAR] * tanh(6000 / P[P_SHEAR_DEPTH]) - P[P_SHEAR] * tanh(0 / P[P_SHEAR_DEPTH]));
+  o[MD_SHEAR06] = hypot(P[P_SHEAR] * tanh(6000 / P[P_SHEAR_DEPTH]) + windError(P + P_WIND_U, 6000) - (P[P_SHEAR] * tanh(0 / P[P_SHEAR_DEPTH]) + windError(P + P_WIND_U, 0)),
+                        windError(P + P_WIND_V, 6000) - windError(P + P_WIND_V, 0));
   o[MD_OUTFLOW] = outflow; o[MD_OUTFLOWLOW] = outflowLow; o[MD_GUST] = gust; o[MD_HAIL] = hailMax; o[MD_CLIPPED] = double(clipped);
 }
 

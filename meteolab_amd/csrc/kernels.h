@@ -45,7 +45,8 @@ template <class R> __global__ void k_base_state(Geo g, Bufs<R> b) {
     double alt = b.zs[z], x = (alt > 0 ? alt : 0) / 10; int k = int(floor(x)); if (k > count - 2) k = count - 2;
     double p = exp(lnP[k] + (lnP[k + 1] - lnP[k]) * (x - k)), exner = pow(p / 100000, KAPPA);
     double tEnv = tempAt(P, alt, p), theta = (tEnv + 273.15) * pow(100000 / p, KAPPA), q = mixRatio(P, alt, tEnv, p);
-    double u = P[P_SHEAR] * tanh(alt / P[P_SHEAR_DEPTH]), v = 0, rho = p / (RD * theta * exner * (1 + .61 * q));
+    double u = P[P_SHEAR] * tanh(alt / P[P_SHEAR_DEPTH]) + windError(P + P_WIND_U, alt), v = windError(P + P_WIND_V, alt);
+    double rho = p / (RD * theta * exner * (1 + .61 * q));
     b.lv(slot, L_P, g)[z] = p; b.lv(slot, L_EXNER, g)[z] = exner; b.lv(slot, L_THETA, g)[z] = theta; b.lv(slot, L_Q, g)[z] = q;
     b.lv(slot, L_U, g)[z] = u; b.lv(slot, L_V, g)[z] = v; b.lv(slot, L_RHO, g)[z] = rho;
     b.lv(slot, L_UENV, g)[z] = u - P[P_FRAME_U]; b.lv(slot, L_VENV, g)[z] = v - P[P_FRAME_V];

@@ -633,6 +633,7 @@ int ml_run_create(ml_ctx *c, int pool, const ml_profile *pr, const ml_run_config
   P[P_BUBBLE_RH] = rc->bubble_rh; P[P_BUBBLE_RZ] = rc->bubble_rz; P[P_NOISE_SEED] = rc->noise_seed; P[P_NOISE_AMP] = rc->noise_amp; P[P_NOISE_TOP] = rc->noise_top;
   P[P_QVMAX] = pr->qv_max; P[P_SHEAR] = pr->shear; P[P_SHEAR_DEPTH] = pr->shear_depth; P[P_THETA0] = pr->theta0; P[P_DTHETA] = pr->dtheta;
   P[P_ZTR] = pr->ztr; P[P_TTR] = pr->ttr; P[P_RH_DROP] = pr->rh_drop; P[P_RH_TOP] = pr->rh_top; P[P_ELECTRIFIED] = p->electrified; P[P_TIME] = 0;
+  for (int k = 0; k < 4; k++) { P[P_WIND_U + k] = pr->wind_u[k]; P[P_WIND_V + k] = pr->wind_v[k]; }
   HIPCHK(c, hipMemcpyAsync((double *)p->par + size_t(slot) * N_PARAMS, P, sizeof(P), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemsetAsync((double *)p->scal + size_t(slot) * N_SCALARS, 0, N_SCALARS * 8, c->stream));
   Sizes s = sizesOf(g, p->rb);

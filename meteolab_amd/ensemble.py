@@ -10,10 +10,9 @@ member_peaks, EVENTS, probabilities, spread and regime_of restate it here, and p
 The step's sounding is the analytic one (u = shear tanh(z / shear_depth), vapour capped at qv_max), so withMoisture and
 withShear act as on the reference's Weisman-Klemp profile (grid.ts:35-36, 57-62): the vapour cap scaled, the hodograph
 scaled as a whole to the 0-6 km difference asked. perturb_run restates perturbConfig (member.ts:45-65) with the same
-draws in the same order: the ground temperature, the low-level humidity and the thermal's strength and place are
-perturbed; the wind errors are drawn but not applied, since this sounding's wind is one tanh profile with no per-level
-winds. The reference's slider and sounding branches and the CAPE of environmentOf (a parcel computation) have no
-counterpart on this sounding.
+draws in the same order: the ground temperature, the low-level humidity, the wind (errors at 0, 3, 6 and 10 km, as for a
+sounding) and the thermal's strength and place. The reference's slider and sounding branches and the CAPE of
+environmentOf (a parcel computation) have no counterpart on this sounding.
 """
 import math
 
@@ -147,9 +146,28 @@ GRIDS = {"small": {"moisture": [.9, 1, 1.1], "shear": [0, 20, 40]},
 CALM = 2
 
 
+WIND_NODES = (0.0, 3000.0, 6000.0, 10000.0)
+
+
+def wind_error(d, z):
+    """The wind error at height z from errors d at WIND_NODES (member.ts:56): linear between, held beyond."""
+    k = 0
+    while k < 2 and z >= WIND_NODES[k + 1]:
+        k += 1
+    return d[k] + (d[k + 1] - d[k]) * min(max((z - WIND_NODES[k]) / (WIND_NODES[k + 1] - WIND_NODES[k]), 0.0), 1.0)
+
+
+def wind(profile, z):
+    """The sounding's wind (u, v) at height z: the tanh profile plus its wind errors."""
+    zero = (0.0,) * 4
+    return (profile["shear"] * math.tanh(z / profile["shear_depth"]) + wind_error(profile.get("wind_u", zero), z),
+            wind_error(profile.get("wind_v", zero), z))
+
+
 def shear06(profile):
-    """The sounding's 0-6 km wind difference, m/s (the shear of environmentOf, grid.ts:24-29)."""
-    return abs(profile["shear"] * math.tanh(6000 / profile["shear_depth"]) - profile["shear"] * math.tanh(0.0))
+    """The sounding's 0-6 km wind difference, m/s (the shear of environmentOf, grid.ts:24-29; model.ts:1682)."""
+    (u0, v0), (u6, v6) = wind(profile, 0.0), wind(profile, 6000.0)
+    return math.hypot(u6 - u0, v6 - v0)
 
 
 def with_moisture(profile, factor):
@@ -158,11 +176,15 @@ def with_moisture(profile, factor):
 
 
 def with_shear(profile, shear):
-    """withShear (grid.ts:55-62) on the analytic sounding: the hodograph scaled as a whole to a 0-6 km difference of
-    `shear` m/s (a calm one stays calm, as the reference's Weisman-Klemp hodograph)."""
+    """withShear (grid.ts:55-62) on the analytic sounding: the hodograph (with its wind errors) scaled as a whole to a
+    0-6 km difference of `shear` m/s (a calm one stays calm, as the reference's Weisman-Klemp hodograph)."""
     now = shear06(profile)
     f = shear / now if now > CALM else 0.0
-    return dict(profile, shear=profile["shear"] * f)
+    out = dict(profile, shear=profile["shear"] * f)
+    for k in ("wind_u", "wind_v"):
+        if k in profile:
+            out[k] = [x * f for x in profile[k]]
+    return out
 
 
 def regime_cells(profile, size):
@@ -191,13 +213,17 @@ def perturb_run(grid, profile, spec):
     untouched. From the run's own seed (mulberry32(seed ^ 0x5bd1e995)) in the reference's order: the ground temperature
     error dT (the sounding's theta0 moves by dT, as surfaceTemp does), the humidity error dRh in % (the vapour cap,
     which sets the low-level vapour of this sounding, times 1 + dRh / 100, as a sounding's vapour at the ground), the
-    wind errors at 0, 3, 6 and 10 km (u then v; drawn, not applied), the thermal's strength (times max(.5, 1 + .1 n))
-    and its place (startOffset: from the thermal's default place, replacing any offset the run had)."""
+    wind errors at 0, 3, 6 and 10 km (u then v, added to the sounding's wind errors, as a sounding's are), the thermal's
+    strength (times max(.5, 1 + .1 n)) and its place (startOffset: from the thermal's default place, replacing any
+    offset the run had)."""
     n, U = normal(C.mulberry32(int(spec["seed"]) ^ 0x5bd1e995)), UNCERTAINTY
     dT, dRh = n() * U["temperature"], n() * U["humidity"]
-    for _ in range(8):  # du, dv at the four wind nodes
-        n()
-    prof = dict(profile, theta0=profile["theta0"] + dT, qv_max=profile["qv_max"] * (1 + dRh / 100))
+    du = [n() * U["wind"] for _ in WIND_NODES]
+    dv = [n() * U["wind"] for _ in WIND_NODES]
+    zero = (0.0,) * 4
+    prof = dict(profile, theta0=profile["theta0"] + dT, qv_max=profile["qv_max"] * (1 + dRh / 100),
+                wind_u=[a + b for a, b in zip(profile.get("wind_u", zero), du)],
+                wind_v=[a + b for a, b in zip(profile.get("wind_v", zero), dv)])
     amp = spec["bubble_amp"] * max(.5, 1 + n() * U["thermal"])
     ox = n() * U["offset"]
     oy = n() * U["offset"]

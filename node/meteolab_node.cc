@@ -52,6 +52,17 @@ double prop(napi_env env, napi_value obj, const char *name, double fallback) {
   if (napi_get_named_property(env, obj, name, &v) == napi_ok) napi_get_value_double(env, v, &d);
   return d;
 }
+// Elements 0..n-1 of the array property `name` into out (zeros where it or an element is absent).
+void propArray(napi_env env, napi_value obj, const char *name, double *out, uint32_t n) {
+  for (uint32_t k = 0; k < n; k++) out[k] = 0;
+  bool has = false;
+  if (napi_has_named_property(env, obj, name, &has) != napi_ok || !has) return;
+  napi_value arr; if (napi_get_named_property(env, obj, name, &arr) != napi_ok) return;
+  for (uint32_t k = 0; k < n; k++) {
+    napi_value v; bool el = false;
+    if (napi_has_element(env, arr, k, &el) == napi_ok && el && napi_get_element(env, arr, k, &v) == napi_ok) napi_get_value_double(env, v, &out[k]);
+  }
+}
 
 // A typed array's data of the expected element type and at least `count` elements.
 template <class T> bool typed(napi_env env, napi_value v, napi_typedarray_type type, size_t count, T *&data) {
@@ -139,6 +150,7 @@ napi_value RunCreate(napi_env env, napi_callback_info info) {
   p.theta0 = prop(env, a[2], "theta0", 300); p.dtheta = prop(env, a[2], "dtheta", 43); p.ztr = prop(env, a[2], "ztr", 12000);
   p.ttr = prop(env, a[2], "ttr", 213); p.rh_drop = prop(env, a[2], "rh_drop", .75); p.rh_top = prop(env, a[2], "rh_top", .25);
   p.qv_max = prop(env, a[2], "qv_max", .014); p.shear = prop(env, a[2], "shear", 25); p.shear_depth = prop(env, a[2], "shear_depth", 3000);
+  propArray(env, a[2], "wind_u", p.wind_u, 4); propArray(env, a[2], "wind_v", p.wind_v, 4);
   ml_run_config c{};
   c.seed = int(prop(env, a[3], "seed", 1)); c.latitude = prop(env, a[3], "latitude", 45); c.turbulence = prop(env, a[3], "turbulence", .55);
   c.tropopause_km = prop(env, a[3], "tropopause_km", 12); c.roughness = prop(env, a[3], "roughness", .03);

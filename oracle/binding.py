@@ -37,7 +37,8 @@ class Grid(ctypes.Structure):
 
 class Profile(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in
-                ("theta0", "dtheta", "ztr", "ttr", "rh_drop", "rh_top", "qv_max", "shear", "shear_depth")]
+                ("theta0", "dtheta", "ztr", "ttr", "rh_drop", "rh_top", "qv_max", "shear", "shear_depth")] + \
+        [("wind_u", ctypes.c_double * 4), ("wind_v", ctypes.c_double * 4)]
 
 
 class Run(ctypes.Structure):
@@ -132,7 +133,10 @@ def make_grid(g):
 
 
 def make_profile(p):
-    return Profile(*(float(p[k]) for k, _ in Profile._fields_))
+    """The sounding dict as orc_profile; the wind errors are zero when absent."""
+    scalars = [float(p[k]) for k, t in Profile._fields_ if t is ctypes.c_double]
+    winds = [(ctypes.c_double * 4)(*(float(x) for x in p.get(k, (0.0,) * 4))) for k in ("wind_u", "wind_v")]
+    return Profile(*scalars, *winds)
 
 
 def make_run(r):

@@ -19,9 +19,10 @@ typedef struct { int nx, ny, nz; double width, depth, height, bottom; } orc_grid
 
 /*
  * Analytic sounding (src/core/profiles.ts weismanKlemp): theta = theta0 + dtheta (z/ztr)^1.25 below ztr, isothermal ttr
- * above; RH = 1 - rh_drop (z/ztr)^1.25 (rh_top above); vapour capped at qv_max; wind u = shear tanh(z / shear_depth), v = 0.
+ * above; RH = 1 - rh_drop (z/ztr)^1.25 (rh_top above); vapour capped at qv_max; wind u = shear tanh(z / shear_depth), v = 0,
+ * plus the wind errors wind_u, wind_v at 0, 3, 6 and 10 km, linear between and held beyond (src/ensemble/member.ts:55-57).
  */
-typedef struct { double theta0, dtheta, ztr, ttr, rh_drop, rh_top, qv_max, shear, shear_depth; } orc_profile;
+typedef struct { double theta0, dtheta, ztr, ttr, rh_drop, rh_top, qv_max, shear, shear_depth, wind_u[4], wind_v[4]; } orc_profile;
 
 /*
  * One run: SimConfig values that matter at the baseline (src/core/fixtures.ts SUMMER_DAY with single: true) and the

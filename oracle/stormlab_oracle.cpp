@@ -925,6 +925,15 @@ struct Lightning {
   }
 };
 
+// The sounding's wind error at height z (perturbConfig's `at`, member.ts:56): d at 0, 3, 6 and 10 km, linear between those
+// heights, held below and above them.
+static double windError(const double *d, double z) {
+  static const double at[4] = {0, 3000, 6000, 10000};
+  int k = 0;
+  while (k < 2 && z >= at[k + 1]) k++;
+  return d[k] + (d[k + 1] - d[k]) * std::min(std::max((z - at[k]) / (at[k + 1] - at[k]), 0.0), 1.0);
+}
+
 // ---- environment.ts:104-131 with an analytic profile (profiles.ts:349-361) ------------------------------------------
 struct Environment {
   F64 p, exner, theta, q, u, v, rho, lnP; orc_profile prof;
@@ -948,7 +957,7 @@ struct Environment {
       double tEnv = temperatureAt(alt, pressureAt(alt));
       theta[z] = (tEnv + 273.15) * std::pow(100000 / pressureAt(alt), KAPPA);
       q[z] = mixingRatio(alt, tEnv - 0, pressureAt(alt));
-      u[z] = prof.shear * std::tanh(alt / prof.shear_depth); v[z] = 0;
+      u[z] = prof.shear * std::tanh(alt / prof.shear_depth) + windError(prof.wind_u, alt); v[z] = windError(prof.wind_v, alt);
       rho[z] = p[z] / (RD * theta[z] * exner[z] * (1 + .61 * q[z]));
     }
   }
@@ -1402,7 +1411,8 @@ struct Model {
       if (w[i] > .6) thermalTop = std::max(thermalTop, alt / 1000);
       if (alt < 1500) coldMax = std::max(coldMax, double(cold[i]));
     }
-    const double u6 = env.prof.shear * std::tanh(6000 / env.prof.shear_depth), u0 = env.prof.shear * std::tanh(0 / env.prof.shear_depth);
+    const double u6 = env.prof.shear * std::tanh(6000 / env.prof.shear_depth) + windError(env.prof.wind_u, 6000), u0 = env.prof.shear * std::tanh(0 / env.prof.shear_depth) + windError(env.prof.wind_u, 0);
+    const double v6 = windError(env.prof.wind_v, 6000), v0 = windError(env.prof.wind_v, 0);
     const double rho0 = env.rho[0];
     double rainRate = 0, rainTotal = 0, gust = 0, hail = 0;
     for (int c = 0; c < layer; c++) gust = std::max(gust, std::hypot(double(f[ORC_U][c]) + run.frame_u, double(f[ORC_V][c]) + run.frame_v));
@@ -1417,7 +1427,7 @@ struct Model {
     const double outflow = outflowDeficit(deficit, warm), outflowLow = deficitNear(deficit, int(rotation[ORC_ROT_LOW]));
     out[ORC_MD_UPDRAFT] = up; out[ORC_MD_DOWNDRAFT] = down; out[ORC_MD_RAINRATE] = rainRate; out[ORC_MD_RAINTOTAL] = rainTotal;
     out[ORC_MD_CLOUDTOP] = top; out[ORC_MD_THERMALTOP] = thermalTop; out[ORC_MD_MAXCLOUD] = maxCloud; out[ORC_MD_COLDMAX] = coldMax;
-    out[ORC_MD_CORES] = countCores(); out[ORC_MD_SHEAR06] = std::hypot(u6 - u0, 0.0); out[ORC_MD_OUTFLOW] = outflow;
+    out[ORC_MD_CORES] = countCores(); out[ORC_MD_SHEAR06] = std::hypot(u6 - u0, v6 - v0); out[ORC_MD_OUTFLOW] = outflow;
     out[ORC_MD_OUTFLOWLOW] = outflowLow; out[ORC_MD_GUST] = gust; out[ORC_MD_HAIL] = hail; out[ORC_MD_CLIPPED] = double(clipped);
   }
   void step(double dt) {

@@ -135,6 +135,8 @@ def test_perturbed_starting_data_from_the_members_seed():
     def sd(v):
         return math.sqrt(sum(x * x for x in v) / len(v))
     U = E.UNCERTAINTY
+    assert abs(sd([p["wind_u"][2] for p, _ in many]) / E.UNCERTAINTY["wind"] - 1) < .5
+    assert abs(sd([p["wind_v"][0] for p, _ in many]) / E.UNCERTAINTY["wind"] - 1) < .5
     assert abs(sd([p["theta0"] - prof["theta0"] for p, _ in many]) - U["temperature"]) < .05
     assert abs(sd([100 * (p["qv_max"] / prof["qv_max"] - 1) for p, _ in many]) / U["humidity"] - 1) < .5
     assert abs(sd([s["bubble_x"] - g["width"] / 3 for _, s in many]) / U["offset"] - 1) < .5
@@ -142,6 +144,32 @@ def test_perturbed_starting_data_from_the_members_seed():
     strength = [s["bubble_amp"] / 2.0 for _, s in many]
     assert min(strength) >= .5 and abs(sd([x - 1 for x in strength]) / U["thermal"] - 1) < .5
     assert all(p["shear"] == prof["shear"] and p["dtheta"] == prof["dtheta"] for p, _ in many)
+    assert "wind_u" not in prof and E.perturb_run(g, dict(prof, wind_u=[1, 1, 1, 1]), base)[0]["wind_u"] == [1 + x for x in a[0]["wind_u"]]
+
+
+WINDY = {"wind_u": [1.0, 2.0, -1.0, 3.0], "wind_v": [-2.0, 0.0, 1.0, .5]}
+
+
+def test_wind_errors_reach_the_sounding():
+    # The errors at 0, 3, 6 and 10 km: the environment's wind less the tanh profile lies on the straight line through
+    # them in each interval (fitted on the levels there), is held above 10 km, and the 0-6 km shear is the closed form.
+    g = dict(nx=8, ny=8, nz=50, width=9600.0, depth=9000.0, height=18900.0, bottom=100.0)
+    prof = dict(C.profile(), **WINDY)
+    o = OracleRun(g, prof, C.run(g))
+    bs = o.base_state()
+    zs, _ = o.levels()
+    err = {"wind_u": bs["u"] - 25 * np.tanh(zs / 3000), "wind_v": bs["v"]}
+    for key, d in WINDY.items():
+        for k, (lo, hi) in enumerate(((0, 3000), (3000, 6000), (6000, 10000))):
+            at = (zs >= lo) & (zs <= hi)
+            slope, icpt = np.polyfit(zs[at], err[key][at], 1)
+            assert icpt + slope * lo == pytest.approx(d[k], abs=1e-9) and icpt + slope * hi == pytest.approx(d[k + 1], abs=1e-9)
+        assert np.allclose(err[key][zs >= 10000], d[3], atol=1e-12)
+    want = math.hypot(25 * math.tanh(2.0) - 1.0 - 1.0, 1.0 + 2.0)
+    assert E.shear06(prof) == pytest.approx(want, rel=1e-14)
+    assert o.model_diagnostics()["shear06"] == pytest.approx(want, rel=1e-14)
+    scaled = E.with_shear(prof, 15)
+    assert E.shear06(scaled) == pytest.approx(15, rel=1e-12) and scaled["wind_v"][3] == pytest.approx(.5 * 15 / want)
 
 
 def test_the_thermal_moves_with_its_offset():
@@ -232,3 +260,20 @@ def test_a_seed_and_data_ensemble_steps_its_perturbed_members_as_the_oracle():
             for key in ("updraft", "rain", "gust", "outflow"):
                 assert a[key] == pytest.approx(b[key], rel=1e-3, abs=1e-6), (k, a["t"], key)
     assert out["results"][1]["samples"][-1]["updraft"] != out["results"][0]["samples"][-1]["updraft"]
+
+
+@pytest.mark.gpu
+def test_device_sounding_with_wind_errors_is_the_oracles():
+    import meteolab_amd as M
+    g = C.small_grid()
+    prof = dict(C.profile(), **WINDY)
+    o = OracleRun(g, prof, C.run(g))
+    with M.Context(0, 32) as ctx:
+        print("device sounding with wind errors", flush=True)
+        r = ctx.pool(g, 1).run(prof, C.run(g))
+        lv, md = r.levels(), r.model_diagnostics()
+    bs = o.base_state()
+    for k in ("u", "v"):
+        assert np.allclose(lv[k], bs[k], rtol=1e-12, atol=1e-12), k
+    assert abs(bs["v"]).max() > 0
+    assert md["shear06"] == pytest.approx(o.model_diagnostics()["shear06"], rel=1e-14)
