@@ -176,15 +176,19 @@ def with_moisture(profile, factor):
 
 
 def with_shear(profile, shear):
-    """withShear (grid.ts:55-62) on the analytic sounding: the hodograph (with its wind errors) scaled as a whole to a
-    0-6 km difference of `shear` m/s (a calm one stays calm, as the reference's Weisman-Klemp hodograph)."""
+    """withShear (grid.ts:49-80) on the analytic sounding, to a 0-6 km difference of `shear` m/s with the hodograph's
+    shape kept. Without wind errors it is the Weisman-Klemp hodograph, calm at the ground: scaled as a whole (a calm
+    one stays calm). With wind errors it is a sounding's: the difference from the ground wind scaled, or, if calm, a
+    straight westerly shear over 0-6 km added; both stay a tanh profile plus errors at the wind nodes."""
     now = shear06(profile)
-    f = shear / now if now > CALM else 0.0
-    out = dict(profile, shear=profile["shear"] * f)
-    for k in ("wind_u", "wind_v"):
-        if k in profile:
-            out[k] = [x * f for x in profile[k]]
-    return out
+    if "wind_u" not in profile and "wind_v" not in profile:
+        return dict(profile, shear=profile["shear"] * (shear / now if now > CALM else 0.0))
+    f, added = (shear / now, 0.0) if now > CALM else (1.0, float(shear))
+    zero = (0.0,) * 4
+    eu, ev = profile.get("wind_u", zero), profile.get("wind_v", zero)
+    ramp = [min(z, 6000.0) / 6000 for z in WIND_NODES]
+    return dict(profile, shear=profile["shear"] * f, wind_u=[eu[0] + (e - eu[0]) * f + added * r for e, r in zip(eu, ramp)],
+                wind_v=[ev[0] + (e - ev[0]) * f for e in ev])
 
 
 def regime_cells(profile, size):

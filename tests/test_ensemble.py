@@ -168,8 +168,15 @@ def test_wind_errors_reach_the_sounding():
     want = math.hypot(25 * math.tanh(2.0) - 1.0 - 1.0, 1.0 + 2.0)
     assert E.shear06(prof) == pytest.approx(want, rel=1e-14)
     assert o.model_diagnostics()["shear06"] == pytest.approx(want, rel=1e-14)
-    scaled = E.with_shear(prof, 15)
-    assert E.shear06(scaled) == pytest.approx(15, rel=1e-12) and scaled["wind_v"][3] == pytest.approx(.5 * 15 / want)
+    # withShear (grid.ts:49-80) of a sounding: the difference from the ground wind scaled, the ground wind kept; a calm
+    # one gets a straight westerly 0-6 km shear, a quarter of it per 1.5 km.
+    for s in (0, 15, 40):
+        scaled = E.with_shear(prof, s)
+        assert E.shear06(scaled) == pytest.approx(s, abs=1e-12) and E.wind(scaled, 0) == pytest.approx(E.wind(prof, 0), abs=1e-12)
+    calm = dict(C.profile(shear=0.0), wind_u=[1.0] * 4, wind_v=[-1.0] * 4)
+    made = E.with_shear(calm, 20)
+    for z in (0, 1500, 3000, 6000, 9000):
+        assert E.wind(made, z) == pytest.approx((1 + 20 * min(z, 6000) / 6000, -1), abs=1e-12)
 
 
 def test_the_thermal_moves_with_its_offset():
