@@ -75,6 +75,22 @@ def test_parity_after_one_and_ten_steps(ctx, grid):
     assert d["maxW"] == pytest.approx(od["maxW"], rel=1e-5)
 
 
+def test_large_domain_parity_of_the_float64_kernels():
+    # The same steps of the large domain with the float64 kernels (Context precision 64): w stays within 1e-5 after ten
+    # steps too, since the float32 transport's rounding is what moves it on the large domain.
+    g = C.LARGE
+    prof, run = C.profile(), C.run(g)
+    o = OracleRun(g, prof, run)
+    with M.Context(0, 64) as c64:
+        print("large domain, float64 kernels", flush=True)
+        r = c64.pool(g, 1).run(prof, run)
+        o.step(1); r.step(1)
+        w1 = compare(r, o)
+        o.step(9); r.step(9)
+        w10 = compare(r, o)
+    print("large float64: 1 step", max(w1.values()), "10 steps", max(w10.values()))
+
+
 def mature(g, steps, seed=1):
     """The oracle run `steps` steps into the storm (cloud, rain and ice present), for a restart on the GPU."""
     o = OracleRun(g, C.profile(), C.run(g, seed=seed))
