@@ -335,5 +335,37 @@ function runMembers (pool, runs, minutes, dt = 3) {
     rainTotal: r.column('precipitation'), hailSize: r.column('hailSize'), gustMax: r.column('gustMax'), uhMax: r.column('uhMax') }))
 }
 
+// mulberry32 (the published 32-bit generator the reference uses, src/core/math.ts:7-14): uniforms in [0, 1).
+function mulberry32 (seed) {
+  let s = seed >>> 0
+  return () => {
+    s = (s + 0x6D2B79F5) >>> 0
+    let t = Math.imul(s ^ (s >>> 15), 1 | s)
+    t = (t + Math.imul(t ^ (t >>> 7), 61 | t)) ^ t
+    return ((t ^ (t >>> 14)) >>> 0) / 4294967296
+  }
+}
+
+// The starting data's uncertainty (src/ensemble/member.ts:33): K, % relative humidity, m/s, degrees, the thermal's
+// relative strength, m.
+const UNCERTAINTY = { temperature: 0.5, humidity: 4, wind: 1.5, direction: 8, thermal: 0.1, offset: 1500 }
+
+// perturbConfig (src/ensemble/member.ts:45-65) on the analytic sounding and a run: new [profile, run] objects, the
+// inputs untouched. Normal deviates by Box-Muller from mulberry32(seed ^ 0x5bd1e995), in the reference's order: the
+// ground temperature (theta0), the humidity (the vapour cap times 1 + dRh / 100), the wind errors at 0, 3, 6 and 10 km
+// (u, then v, added to the sounding's wind_u / wind_v), the thermal's strength and its place from the default one.
+function perturbRun (grid, prof, spec) {
+  const rng = mulberry32(spec.seed ^ 0x5bd1e995), U = UNCERTAINTY
+  const n = () => { const a = rng(); return Math.sqrt(-2 * Math.log(1 - a)) * Math.cos(2 * Math.PI * rng()) }
+  const dT = n() * U.temperature, dRh = n() * U.humidity
+  const du = [0, 1, 2, 3].map(() => n() * U.wind), dv = [0, 1, 2, 3].map(() => n() * U.wind)
+  const zero = [0, 0, 0, 0], wu = prof.wind_u || zero, wv = prof.wind_v || zero
+  const p = Object.assign({}, prof, { theta0: prof.theta0 + dT, qv_max: prof.qv_max * (1 + dRh / 100),
+    wind_u: wu.map((x, k) => x + du[k]), wind_v: wv.map((x, k) => x + dv[k]) })
+  const amp = spec.bubble_amp * Math.max(0.5, 1 + n() * U.thermal)
+  const ox = n() * U.offset, oy = n() * U.offset
+  return [p, Object.assign({}, spec, { bubble_amp: amp, bubble_x: grid.width / 3 + ox, bubble_y: grid.depth / 2 + oy })]
+}
+
 module.exports = { MeteolabDevice, runMembers, MeteolabPool, MeteolabRun, GpuDynamics, FIELDS, CARRIED, COLUMNS, DIAG, PARTS, STANDARD, LARGE, profile, run, addon,
-  STATE_FIELDS, STATE_COLUMNS, ROTATION, MODEL_DIAGNOSTICS, saveState, readState, applyState }
+  STATE_FIELDS, STATE_COLUMNS, ROTATION, MODEL_DIAGNOSTICS, saveState, readState, applyState, mulberry32, UNCERTAINTY, perturbRun }

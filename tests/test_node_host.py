@@ -103,6 +103,28 @@ def test_gpu_dynamics_interface_steps_as_the_oracle(tmp_path):
         assert err <= 1e-5, (name, err)
 
 
+@needs_node
+def test_node_perturb_run_is_the_python_hosts():
+    # The Node host's perturbRun against meteolab_amd.ensemble.perturb_run on the same inputs: the same draws in the same
+    # order (V8's and the C library's logarithm and cosine may differ in the last bit), and mulberry32's first uniforms.
+    code = ("const m = require('./node/meteolab'); const g = m.STANDARD, out = [];"
+            "for (const s of [1, 2, 7, 40]) { const p = Object.assign(m.profile(), s === 7 ? {wind_u: [1, 2, 3, 4]} : {}), r = m.run(g, s);"
+            " out.push({p, r, got: m.perturbRun(g, p, r)}) }"
+            "const u = m.mulberry32(42); console.log(JSON.stringify({out, u: [u(), u(), u()]}))")
+    res = node(code)
+    assert res.returncode == 0, res.stderr
+    d = json.loads(res.stdout)
+    want = C.mulberry32(42)
+    assert d["u"] == [next(want) for _ in range(3)]
+    from meteolab_amd import ensemble as E
+    for case in d["out"]:
+        p, r = E.perturb_run(C.STANDARD, case["p"], case["r"])
+        for got, exp in zip(case["got"], (p, r)):
+            assert sorted(got) == sorted(exp)
+            for k, v in exp.items():
+                assert np.allclose(got[k], v, rtol=1e-13, atol=1e-12), (k, got[k], v)
+
+
 @pytest.mark.gpu
 @needs_node
 def test_node_run_members_gives_member_results(tmp_path):
