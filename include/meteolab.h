@@ -114,6 +114,10 @@ int ml_run_upload_column(ml_ctx *ctx, int run, int column, const float *host);
 int ml_run_download_column(ml_ctx *ctx, int run, int column, float *host);
 /* The run's base state, ML_LEVELS x nz float64. */
 int ml_run_levels(ml_ctx *ctx, int run, double *out);
+/* The pressure of the run's last projection (model.ts:358 pressure; the solver's cells), (nz - 1) x nx x ny float64 in
+ * cell order c = x + nx (y + ny k), cell k between levels k and k + 1: what GpuDynamics.mix returns as `pressure`
+ * (gpu/dynamics.ts:356-395) and endGpuStep copies into the model (model.ts:1145-1150). Stream-synchronising. */
+int ml_run_pressure(ml_ctx *ctx, int run, double *out);
 /* The run's diagnostics, ML_D_COUNT float64 (stream-synchronising). */
 int ml_run_diag(ml_ctx *ctx, int run, double *out);
 /* The run's most recent flashes (at most 64 are kept), oldest first: up to `max` records of 8 float64 each: time (s),
@@ -187,10 +191,12 @@ int ml_after_step(ml_ctx *ctx, const int *runs, int count, double dt);
  * (stream-synchronising); `max` below it is ML_E_ARG. */
 int ml_fall_inputs(ml_ctx *ctx, int run, int *parts, int max);
 int ml_sync(ml_ctx *ctx);
-/* The autotuned launch shape of a batch's transport kernel (chosen by timing on the device when the batch is first
- * stepped), shape[4]: shape[0] the layout (0 strips of a level with shape[1] field groups per block, 1 shape[3] whole
- * levels of one field per block), shape[2] 1 if each run's blocks are kept on one XCD; *ms the stage-0 transport time
- * it measured. No counterpart in the reference. */
+/* The launch shape of a batch's transport kernel, shape[4]: shape[0] the layout (0 strips of a level with shape[1]
+ * field groups per block, 1 shape[3] whole levels of one field per block, 2 the same with each level in a padded LDS
+ * plane), shape[2] 1 if each run's blocks are kept on one XCD; *ms the stage-0 transport time it measured. The layout
+ * is fixed by the grid (2 where its planes fit a block's LDS, else 1), since the layouts round differently in float32;
+ * levels per block and the block order, which leave the arithmetic unchanged, are autotuned by timing on the device
+ * when the batch is first stepped. No counterpart in the reference. */
 int ml_batch_tuning(ml_ctx *ctx, const int *runs, int count, int *shape, float *ms);
 /* Sets a batch's transport launch shape (as ml_batch_tuning reports it) in place of the autotuned one; the next
  * ml_step recaptures the step. No counterpart in the reference. */

@@ -84,6 +84,7 @@ SIGNATURES = {
     "ml_run_upload_column": (_i, [_vp, _i, _i, _fp]),
     "ml_run_download_column": (_i, [_vp, _i, _i, _fp]),
     "ml_run_levels": (_i, [_vp, _i, _dp]),
+    "ml_run_pressure": (_i, [_vp, _i, _dp]),
     "ml_run_diag": (_i, [_vp, _i, _dp]),
     "ml_run_set_time": (_i, [_vp, _i, _d]),
     "ml_run_set_totals": (_i, [_vp, _i, _d, _i64, _i64, _d]),
@@ -269,6 +270,13 @@ class Run:
         out = np.empty(len(LEVELS) * nz, np.float64)
         self.ctx.check(lib().ml_run_levels(self.ctx.h, self.h, _dptr(out)))
         return dict(zip(LEVELS, out.reshape(len(LEVELS), nz)))
+
+    def pressure(self):
+        """The last projection's pressure cells, (nz - 1, ny, nx) float64 (model.ts:358)."""
+        g = self.grid
+        out = np.empty((g["nz"] - 1) * g["ny"] * g["nx"], np.float64)
+        self.ctx.check(lib().ml_run_pressure(self.ctx.h, self.h, _dptr(out)))
+        return out.reshape(g["nz"] - 1, g["ny"], g["nx"])
 
     def diagnostics(self):
         out = np.empty(len(DIAG), np.float64)

@@ -88,10 +88,6 @@ struct ml_ctx {
   hipDeviceProp_t prop{};
   std::string err;
   std::vector<Pool *> pools;
-  // The transport layout of each grid (nx, ny, nz, electrified), fixed by its first autotuning: the layouts round
-  // differently in float32, so a run's result must not depend on the size of the batch it is stepped in. Levels per
-  // block, field groups and block order are tuned per batch (they leave the arithmetic unchanged).
-  std::map<std::vector<int>, int> layouts;
 };
 
 namespace {
@@ -710,6 +706,15 @@ int ml_run_levels(ml_ctx *c, int run, double *out) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
+int ml_run_pressure(ml_ctx *c, int run, double *out) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot) || !out) return fail(c, ML_E_ARG, "pressure: bad run");
+  HIPCHK(c, hipSetDevice(c->device));
+  // The cells (nz - 1 levels of nx ny) lead the slot's node-sized pressure array.
+  HIPCHK(c, hipMemcpyAsync(out, (double *)p->press + size_t(slot) * p->g.n, size_t(p->g.nz - 1) * p->g.layer * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
 int ml_run_set_time(ml_ctx *c, int run, double time) {
   Pool *p; int slot;
   if (!c || !decode(c, run, p, slot)) return fail(c, ML_E_ARG, "set time: bad run");
@@ -853,9 +858,9 @@ int ml_run_flashes(ml_ctx *c, int run, double *out, int max) {
 }  // extern "C"
 
 namespace {
-// The autotuner: the transport kernel's layout (k_stage with each field-group split, or k_stage_level; each with and
-// without the XCD-per-run block order), by timing the stage-0 transport of this batch (it reads the state and writes
-// only stage buffer X, which every step rewrites) for each candidate, fastest of 3 launches.
+// The autotuner: the transport kernel's levels per block, with and without the XCD-per-run block order, by timing the
+// stage-0 transport of this batch (it reads the state and writes only stage buffer X, which every step rewrites) for
+// each candidate, fastest of 3 launches.
 template <class R> float timeStage(ml_ctx *c, const Pool &p, const Batch &bt, const Tuning &tu) {
   const Geo &g = p.g;
   Bufs<R> b = bufs<R>(p, bt.dev);
@@ -876,32 +881,20 @@ template <class R> float timeStage(ml_ctx *c, const Pool &p, const Batch &bt, co
   (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
   return best;
 }
+// The transport layout of a grid by a fixed rule: the padded-plane layout where its level planes fit a block's LDS, else
+// whole levels. The layouts round differently in float32, so the layout is a function of the grid alone: a run's result
+// does not depend on the batch it is stepped in, on the process or host that steps it, or on timing noise.
+static int layoutOf(const ml_ctx *c, const Pool *p) { return planeFits(p->g, p->rb, c->prop.sharedMemPerBlock) ? 2 : 1; }
 static int tune(ml_ctx *c, Pool *p, Batch *bt) {
   HIPCHK(c, hipSetDevice(c->device));
-  const int options[] = {1, 2, 3, 4, 6, 9, 17};
-  const int nc = p->electrified ? N_CARRIED : F_COLD + 1, nr = int(bt->runs.size());
+  const int nr = int(bt->runs.size()), layout = layoutOf(c, p);
+  // Only what leaves the arithmetic unchanged is timed: levels per block and the XCD-per-run block order.
   std::vector<Tuning> cands;
-  for (int xcd = 0; xcd < (nr % 8 == 0 ? 2 : 1); xcd++) {
-    for (int layout : {1, 2}) {
-      if (layout == 2 && !planeFits(p->g, p->rb, c->prop.sharedMemPerBlock)) continue;
-      for (int k : {1, 2, 5, 10}) {
-        if (k > p->g.nz) continue;
-        Tuning lv; lv.layout = layout; lv.xcd = xcd; lv.levels = k; cands.push_back(lv);
-      }
+  for (int xcd = 0; xcd < (nr % 8 == 0 ? 2 : 1); xcd++)
+    for (int k : {1, 2, 5, 10}) {
+      if (k > p->g.nz) continue;
+      Tuning lv; lv.layout = layout; lv.xcd = xcd; lv.levels = k; cands.push_back(lv);
     }
-
-    for (int gr : options) {
-      if (gr > nc) continue;
-      Tuning tl; tl.layout = 0; tl.groups = gr; tl.xcd = xcd; cands.push_back(tl);
-    }
-  }
-  const std::vector<int> key = {p->g.nx, p->g.ny, p->g.nz, p->electrified};
-  auto fixed = c->layouts.find(key);
-  if (fixed != c->layouts.end()) {
-    std::vector<Tuning> same;
-    for (const Tuning &t : cands) if (t.layout == fixed->second) same.push_back(t);
-    if (!same.empty()) cands.swap(same);
-  }
   float fastest = 1e30f;
   for (Tuning &t : cands) {
     t.ms = p->rb == 8 ? timeStage<double>(c, *p, *bt, t) : timeStage<float>(c, *p, *bt, t);
@@ -912,7 +905,6 @@ static int tune(ml_ctx *c, Pool *p, Batch *bt) {
   // way on every run of the same batch.
   Tuning best = cands[0];
   for (const Tuning &t : cands) if (t.ms <= fastest * 1.03f) { best = t; break; }
-  if (fixed == c->layouts.end()) c->layouts[key] = best.layout;
   bt->tune = best;
   bt->tuned = true;
   return 0;

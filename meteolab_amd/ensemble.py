@@ -212,18 +212,28 @@ def normal(rng):
     return draw
 
 
-def perturb_run(grid, profile, spec):
+# The wind slider nodes of perturbConfig's loop (member.ts:50) in its order, and those SUMMER_DAY-based configurations
+# carry (src/core/fixtures.ts:9: wind0, 3, 6, 10 with their directions).
+SLIDER_KEYS = ("0", "05", "1", "3", "6", "10")
+SLIDER_NODES = ("0", "3", "6", "10")
+
+
+def perturb_run(grid, profile, spec, sliders=SLIDER_NODES):
     """perturbConfig (member.ts:45-65) on the analytic sounding and a run: new (profile, spec) dicts, the inputs
     untouched. From the run's own seed (mulberry32(seed ^ 0x5bd1e995)) in the reference's order: the ground temperature
     error dT (the sounding's theta0 moves by dT, as surfaceTemp does), the humidity error dRh in % (the vapour cap,
     which sets the low-level vapour of this sounding, times 1 + dRh / 100, as a sounding's vapour at the ground), the
-    wind errors at 0, 3, 6 and 10 km (u then v, added to the sounding's wind errors, as a sounding's are), the thermal's
-    strength (times max(.5, 1 + .1 n)) and its place (startOffset: from the thermal's default place, replacing any
-    offset the run had)."""
+    wind errors at 0, 3, 6 and 10 km (u then v, added to the sounding's wind errors, as a sounding's are), a speed and a
+    direction error for each wind slider node the emulated configuration carries (`sliders`; drawn and not applied, as
+    the analytic sounding has no sliders), the thermal's strength (times max(.5, 1 + .1 n)) and its place (startOffset:
+    from the thermal's default place, replacing any offset the run had)."""
     n, U = normal(C.mulberry32(int(spec["seed"]) ^ 0x5bd1e995)), UNCERTAINTY
     dT, dRh = n() * U["temperature"], n() * U["humidity"]
     du = [n() * U["wind"] for _ in WIND_NODES]
     dv = [n() * U["wind"] for _ in WIND_NODES]
+    for key in SLIDER_KEYS:
+        if key in sliders:
+            n(), n()
     zero = (0.0,) * 4
     prof = dict(profile, theta0=profile["theta0"] + dT, qv_max=profile["qv_max"] * (1 + dRh / 100),
                 wind_u=[a + b for a, b in zip(profile.get("wind_u", zero), du)],

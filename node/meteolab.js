@@ -135,6 +135,12 @@ class MeteolabRun {
     return Object.fromEntries(DIAG.map((k, i) => [k, out[i]]))
   }
   /** The base state per level (environment.ts:124-130). */
+  /** The last projection's pressure cells, (nz - 1) nx ny float64 (model.ts:358 pressure). */
+  pressure () {
+    const g = this.pool.grid, out = new Float64Array((g.nz - 1) * g.nx * g.ny)
+    this.device.call(() => addon.pressure(this.device.ctx, this.handle, out))
+    return out
+  }
   levels () {
     const nz = this.pool.grid.nz, out = new Float64Array(LEVELS.length * nz)
     this.device.call(() => addon.levels(this.device.ctx, this.handle, out))
@@ -196,6 +202,10 @@ class GpuDynamics {
   }
   async step (fields, env, dt, timed = false, falls = []) {
     try {
+      // The baseline's DynamicsEnv only: the device steps the base state it was created with; zones and nudging
+      // (gpu/dynamics.ts:237-244) are not part of this step, and are refused rather than dropped.
+      if (env && (env.zoneTheta || env.zoneVapour || env.nudge)) throw new Error('GpuDynamics.step: DynamicsEnv zones and nudging are not supported')
+      if (env && env.shape && Array.from(env.shape).some(s => s !== 0)) throw new Error('GpuDynamics.step: DynamicsEnv zone shape is not supported')
       const t0 = Date.now(), r = this.run, layer = this.pool.layer
       this.load(fields)
       r.setColumn('precipitation', new Float32Array(layer))
@@ -219,8 +229,8 @@ class GpuDynamics {
       for (const { k } of mix.mixed) r.setField(CARRIED[k], fields[k].q)
       r.setField('condensate', mix.condensate)
       this.pool.part([r], 'mix', dt)
-      // The projection's pressure stays on the device; endGpuStep takes null as "no pressure".
-      return { fields: mix.mixed.map(({ k }) => r.field(CARRIED[k])), pressure: null, gpuMs: Date.now() - t0 }
+      // The projection's pressure cells, as the model's pressure array (endGpuStep copies them, model.ts:1145-1150).
+      return { fields: mix.mixed.map(({ k }) => r.field(CARRIED[k])), pressure: r.pressure(), gpuMs: Date.now() - t0 }
     } catch (e) { this.errors.push(String(e && e.message ? e.message : e)); throw e }
   }
   /** Whole steps on the device from the resident state (or `fields` first): pool.step. */
@@ -353,12 +363,19 @@ const UNCERTAINTY = { temperature: 0.5, humidity: 4, wind: 1.5, direction: 8, th
 // perturbConfig (src/ensemble/member.ts:45-65) on the analytic sounding and a run: new [profile, run] objects, the
 // inputs untouched. Normal deviates by Box-Muller from mulberry32(seed ^ 0x5bd1e995), in the reference's order: the
 // ground temperature (theta0), the humidity (the vapour cap times 1 + dRh / 100), the wind errors at 0, 3, 6 and 10 km
-// (u, then v, added to the sounding's wind_u / wind_v), the thermal's strength and its place from the default one.
-function perturbRun (grid, prof, spec) {
+// (u, then v, added to the sounding's wind_u / wind_v), a speed and a direction error for each wind slider node in
+// `sliders` (drawn, not applied), the thermal's strength and its place from the default one.
+// The wind slider nodes SUMMER_DAY-based configurations carry (src/core/fixtures.ts:9: wind0, 3, 6, 10 and their
+// directions).
+const SLIDER_NODES = ['0', '3', '6', '10']
+function perturbRun (grid, prof, spec, sliders = SLIDER_NODES) {
   const rng = mulberry32(spec.seed ^ 0x5bd1e995), U = UNCERTAINTY
   const n = () => { const a = rng(); return Math.sqrt(-2 * Math.log(1 - a)) * Math.cos(2 * Math.PI * rng()) }
   const dT = n() * U.temperature, dRh = n() * U.humidity
   const du = [0, 1, 2, 3].map(() => n() * U.wind), dv = [0, 1, 2, 3].map(() => n() * U.wind)
+  // The slider wind nodes' speed and direction errors (member.ts:50-54), drawn and not applied: the analytic sounding
+  // has no sliders, but the draws keep the thermal's in the reference's places of the stream.
+  for (const k of ['0', '05', '1', '3', '6', '10']) if (sliders.includes(k)) { n(); n() }
   const zero = [0, 0, 0, 0], wu = prof.wind_u || zero, wv = prof.wind_v || zero
   const p = Object.assign({}, prof, { theta0: prof.theta0 + dT, qv_max: prof.qv_max * (1 + dRh / 100),
     wind_u: wu.map((x, k) => x + du[k]), wind_v: wv.map((x, k) => x + dv[k]) })
@@ -368,4 +385,4 @@ function perturbRun (grid, prof, spec) {
 }
 
 module.exports = { MeteolabDevice, runMembers, MeteolabPool, MeteolabRun, GpuDynamics, FIELDS, CARRIED, COLUMNS, DIAG, PARTS, STANDARD, LARGE, profile, run, addon,
-  STATE_FIELDS, STATE_COLUMNS, ROTATION, MODEL_DIAGNOSTICS, saveState, readState, applyState, mulberry32, UNCERTAINTY, perturbRun }
+  STATE_FIELDS, STATE_COLUMNS, ROTATION, MODEL_DIAGNOSTICS, saveState, readState, applyState, mulberry32, UNCERTAINTY, SLIDER_NODES, perturbRun }

@@ -252,7 +252,8 @@ napi_value Sync(napi_env env, napi_callback_info info) {
   return status(env, ctx, ml_sync(ctx));
 }
 
-// diag(ctx, run, Float64Array(ML_D_COUNT)), levels(ctx, run, Float64Array(7 nz)).
+// diag(ctx, run, Float64Array(ML_D_COUNT)), levels(ctx, run, Float64Array(7 nz)), pressure(ctx, run,
+// Float64Array((nz - 1) nx ny)).
 template <int (*F)(ml_ctx *, int, double *)> napi_value Doubles(napi_env env, napi_callback_info info) {
   napi_value a[3]; if (!args(env, info, 3, a)) return nullptr;
   ml_ctx *ctx = ctxOf(env, a[0]); if (!ctx) return nullptr;
@@ -293,7 +294,7 @@ napi_value Init(napi_env env, napi_value exports) {
     {"poolDestroy", PoolDestroy}, {"slotBytes", SlotBytes}, {"runCreate", RunCreate}, {"runRelease", RunOp<ml_run_release>},
     {"upload", Transfer<true, false>}, {"download", Transfer<false, false>}, {"uploadColumn", Transfer<true, true>},
     {"downloadColumn", Transfer<false, true>}, {"step", Step}, {"sync", Sync}, {"diag", Doubles<ml_run_diag>},
-    {"levels", Doubles<ml_run_levels>}, {"flashes", Flashes}, {"setTime", SetTime}, {"dynamics", Part<ml_dynamics>},
+    {"levels", Doubles<ml_run_levels>}, {"pressure", Doubles<ml_run_pressure>}, {"flashes", Flashes}, {"setTime", SetTime}, {"dynamics", Part<ml_dynamics>},
     {"physics", Part<ml_physics>}, {"mix", Part<ml_mix>}, {"afterStep", Part<ml_after_step>}, {"fallParts", FallParts},
     {"setTotals", SetTotals}, {"rotation", Doubles<ml_run_rotation>}, {"refreshRotation", RunOp<ml_run_refresh_rotation>},
     {"modelDiagnostics", ModelDiagnostics}, {"setSwaths", SetSwaths}, {"probabilityMap", ProbabilityMap},

@@ -1,9 +1,9 @@
 'use strict'
 // Steps one run of a grid through the Node host and writes its carried fields (float32, carried order) to a file, and
 // its diagnostics as one JSON line on stdout: the Node side of the boundary's parity test (tests/test_node_host.py).
-//   node node/step_fields.js <grid json> <steps> <out.bin>
+//   node tests/node/step_fields.js <grid json> <steps> <out.bin>
 const fs = require('fs')
-const { MeteolabDevice, FIELDS, profile, run } = require('./meteolab')
+const { MeteolabDevice, FIELDS, profile, run } = require('../../node/meteolab')
 
 const grid = JSON.parse(process.argv[2]), steps = Number(process.argv[3]), out = process.argv[4]
 const dev = MeteolabDevice.create()

@@ -147,7 +147,30 @@ def test_perturbed_starting_data_from_the_members_seed():
     assert "wind_u" not in prof and E.perturb_run(g, dict(prof, wind_u=[1, 1, 1, 1]), base)[0]["wind_u"] == [1 + x for x in a[0]["wind_u"]]
 
 
-WINDY = {"wind_u": [1.0, 2.0, -1.0, 3.0], "wind_v": [-2.0, 0.0, 1.0, .5]}
+def test_perturbed_thermal_takes_its_draws_after_the_slider_winds():
+    # perturbConfig (member.ts:45-65) draws, from mulberry32(seed ^ 0x5bd1e995), dT and dRh, the 4 + 4 wind errors, then
+    # a speed and a direction for each wind slider the configuration has (SUMMER_DAY: wind0, 3, 6, 10 and windDir0, 3, 6,
+    # 10; fixtures.ts:9), then the thermal's strength and the two offsets: deviates 18, 19 and 20 of the stream. Without
+    # sliders they are deviates 10, 11 and 12.
+    g, prof = C.small_grid(), C.profile()
+    for seed in (1, 7, 123456):
+        spec = C.run(g, seed=seed)
+        draw = E.normal(C.mulberry32(seed ^ 0x5bd1e995))
+        z = [draw() for _ in range(21)]
+        U = E.UNCERTAINTY
+        for sliders, at in ((E.SLIDER_NODES, 18), ((), 10), (("0", "05", "1", "3", "6", "10"), 22)):
+            if at + 2 >= len(z):
+                z += [draw() for _ in range(at + 3 - len(z))]
+            p, s = E.perturb_run(g, prof, spec, sliders=sliders)
+            assert p["theta0"] == prof["theta0"] + z[0] * U["temperature"]
+            assert p["wind_v"][3] == z[9] * U["wind"]
+            assert s["bubble_amp"] == spec["bubble_amp"] * max(.5, 1 + z[at] * U["thermal"])
+            assert s["bubble_x"] == g["width"] / 3 + z[at + 1] * U["offset"]
+            assert s["bubble_y"] == g["depth"] / 2 + z[at + 2] * U["offset"]
+    assert E.perturb_run(g, prof, C.run(g, seed=3)) == E.perturb_run(g, prof, C.run(g, seed=3), sliders=("0", "3", "6", "10"))
+
+
+WINDY ={"wind_u": [1.0, 2.0, -1.0, 3.0], "wind_v": [-2.0, 0.0, 1.0, .5]}
 
 
 def test_wind_errors_reach_the_sounding():
