@@ -158,6 +158,14 @@ int ml_probability_map(ml_ctx *ctx, const int *runs, int count, int column, doub
  * the strongest ground wind (m/s), the largest hailstones at the ground (mm), and the wind components clipped since the
  * start. One launch for the runs, the ensemble's sampling of every member at once. */
 int ml_model_diagnostics(ml_ctx *ctx, const int *runs, int count, double *out);
+/* ml_model_diagnostics without allocating: the same kernel and the same results, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in
+ * the reference. Its scratch (the diagnostics rows and the run list) is a static array in the library's device memory,
+ * one per device, held until the process ends; it calls no hipMalloc, hipFree or other allocating routine. At most
+ * ML_NOALLOC_RUNS runs a call (more is ML_E_ARG). Every call on one device shares that device's array: calls that
+ * overlap on one device (from any context or thread) race. Stream-synchronising. */
+enum { ML_NOALLOC_RUNS = 1024 };
+int ml_model_diagnostics_noalloc(ml_ctx *ctx, const int *runs, int count, double *out);
 
 /* Advances `count` runs of one pool by `steps` model steps of dt s each: AtmosphereModel.step(dt) then time += dt
  * (model.ts:873-877 and its callers, e.g. tests/bench/threads.ts:38-42). One launch per kernel covers all the runs;

@@ -93,6 +93,7 @@ SIGNATURES = {
     "ml_run_set_swaths": (_i, [_vp, _i, _i]),
     "ml_probability_map": (_i, [_vp, _ip, _i, _i, _d, _fp]),
     "ml_model_diagnostics": (_i, [_vp, _ip, _i, _dp]),
+    "ml_model_diagnostics_noalloc": (_i, [_vp, _ip, _i, _dp]),
     "ml_run_flashes": (_i, [_vp, _i, _dp, _i]),
     "ml_step": (_i, [_vp, _ip, _i, _i, _d]),
     "ml_ctx_set_overlap": (_i, [_vp, _i]),
@@ -346,6 +347,19 @@ def model_diagnostics(runs):
     out = np.empty((len(runs), len(MODEL_DIAGNOSTICS)), np.float64)
     ctx.check(lib().ml_model_diagnostics(ctx.h, ids, len(runs), _dptr(out)))
     return [dict(zip(MODEL_DIAGNOSTICS, row)) for row in out]
+
+
+def model_diagnostics_noalloc(runs):
+    """model_diagnostics through ml_model_diagnostics_noalloc: the same results, the scratch in the library's static
+    device array (at most ML_NOALLOC_RUNS runs a call)."""
+    ctx = runs[0].ctx
+    ids = (ctypes.c_int * len(runs))(*[r.h for r in runs])
+    out = np.empty((len(runs), len(MODEL_DIAGNOSTICS)), np.float64)
+    ctx.check(lib().ml_model_diagnostics_noalloc(ctx.h, ids, len(runs), _dptr(out)))
+    return [dict(zip(MODEL_DIAGNOSTICS, row)) for row in out]
+
+
+NOALLOC_RUNS = 1024  # ML_NOALLOC_RUNS
 
 
 def probability_map(runs, column, threshold):

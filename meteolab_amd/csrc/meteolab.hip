@@ -26,6 +26,14 @@
 
 using namespace ml;
 
+// The scratch of ml_model_diagnostics_noalloc, in the library's own device memory (each device holds its own copy until
+// the process ends): the diagnostics rows of up to ML_NOALLOC_RUNS runs, then their run list. In host code `scratch`
+// names a host-side stand-in; the device address of each device's copy comes from hipGetSymbolAddress at context
+// creation (ml_ctx::scratch), and only that address is handed to kernels and copies.
+constexpr size_t NOALLOC_ROWS_BYTES = size_t(ML_NOALLOC_RUNS) * DIAG_ROW * sizeof(double);
+constexpr size_t NOALLOC_BYTES = NOALLOC_ROWS_BYTES + size_t(ML_NOALLOC_RUNS) * sizeof(int);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
 namespace {
 
 constexpr int MAX_POOLS = 4096, SLOT_BITS = 16;
@@ -87,6 +95,7 @@ struct ml_ctx {
   int overlap = 1;  // a step's discharge beside the next step's dynamics (ml_ctx_set_overlap)
   hipDeviceProp_t prop{};
   std::string err;
+  void *scratch = nullptr;  // this device's copy of the static scratch array (hipGetSymbolAddress)
   std::vector<Pool *> pools;
 };
 
@@ -423,6 +432,9 @@ int ml_ctx_create(int device, int precision, ml_ctx **out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
+  // The device address of this device's scratch array, with the device current.
+  if (e == hipSuccess) e = hipGetSymbolAddress(&c->scratch, HIP_SYMBOL(scratch));
+  if (e == hipSuccess && !c->scratch) e = hipErrorInvalidSymbol;
   if (e != hipSuccess) {
     if (c->join) hipEventDestroy(c->join);
     if (c->fork) hipEventDestroy(c->fork);
@@ -431,6 +443,8 @@ int ml_ctx_create(int device, int precision, ml_ctx **out) {
     delete c;
     return ML_E_HIP;
   }
+  // Room for any message, so that setting one later (ml_model_diagnostics_noalloc's errors) allocates nothing.
+  c->err.reserve(1024);
   *out = c;
   return 0;
 }
@@ -835,6 +849,39 @@ int ml_model_diagnostics(ml_ctx *c, const int *runs, int count, double *out) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   static_assert(N_MD == ML_MD_COUNT, "the ABI's diagnostics record is the kernel's row");
   for (int k = 0; k < count; k++) std::memcpy(out + size_t(k) * ML_MD_COUNT, rows.data() + size_t(k) * DIAG_ROW, ML_MD_COUNT * 8);
+  return 0;
+}
+
+// An error of ml_model_diagnostics_noalloc: the message goes into the context's reserved string without a temporary.
+static int failFixed(ml_ctx *c, int code, const char *msg) { if (c) c->err.assign(msg); return code; }
+#define HIPCHK_FIXED(c, x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return failFixed(c, ML_E_HIP, hipGetErrorString(e_)); } while (0)
+
+int ml_model_diagnostics_noalloc(ml_ctx *c, const int *runs, int count, double *out) {
+  if (!c || !runs || count < 1 || !out) return failFixed(c, ML_E_ARG, "model diagnostics (no allocation): no runs");
+  if (count > ML_NOALLOC_RUNS) return failFixed(c, ML_E_ARG, "model diagnostics (no allocation): more than ML_NOALLOC_RUNS (1024) runs a call");
+  int slots[ML_NOALLOC_RUNS];
+  Pool *p = nullptr;
+  for (int k = 0; k < count; k++) {
+    Pool *q; int slot;
+    if (!decode(c, runs[k], q, slot)) return failFixed(c, ML_E_ARG, "model diagnostics (no allocation): bad run");
+    if (p && q != p) return failFixed(c, ML_E_ARG, "model diagnostics (no allocation): runs of one pool per call");
+    p = q; slots[k] = slot;
+  }
+  if (count > p->cap) return failFixed(c, ML_E_ARG, "model diagnostics (no allocation): more runs than the pool holds");
+  HIPCHK_FIXED(c, hipSetDevice(c->device));
+  // This device's scratch: the rows first, then the run list (the kernel's Bufs::runs).
+  double *rows = static_cast<double *>(c->scratch);
+  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(c->scratch) + NOALLOC_ROWS_BYTES);
+  HIPCHK_FIXED(c, hipMemcpyAsync(list, slots, count * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  const size_t lds = modelDiagLdsBytes(p->g);
+  if (p->rb == 8) hipLaunchKernelGGL(k_model_diag<double>, dim3(count), dim3(MDIAG_THREADS), lds, c->stream, p->g, bufs<double>(*p, list), p->dlev, rows);
+  else hipLaunchKernelGGL(k_model_diag<float>, dim3(count), dim3(MDIAG_THREADS), lds, c->stream, p->g, bufs<float>(*p, list), p->dlev, rows);
+  HIPCHK_FIXED(c, hipGetLastError());
+  // The rows' first ML_MD_COUNT doubles, straight into the caller's array.
+  HIPCHK_FIXED(c, hipMemcpy2DAsync(out, ML_MD_COUNT * sizeof(double), rows, DIAG_ROW * sizeof(double), ML_MD_COUNT * sizeof(double), count,
+                                   hipMemcpyDeviceToHost, c->stream));
+  // The run list was copied from this function's stack: the stream is drained before it returns.
+  HIPCHK_FIXED(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 
