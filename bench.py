@@ -1,16 +1,27 @@
 """Benchmark of the StormLab step on one MI355X: cell-steps/s of batched runs against the plain one-run step, p50/p99
-latency per model-minute, the dominant kernel's roofline and the oracle's CPU rate on a bounded sample.
+latency per model-minute, the transport kernel's roofline and the oracle's CPU rate on a bounded sample.
 
-Default workload (the ensemble configuration): 64 standard-domain runs (40x32x50, dt 3 s), each with its own seeded
-thermal (amplitude 1.5-3 K, centre offset +-5 km) and theta noise, advanced together for 600 steps (30 model minutes),
-one launch per kernel for all runs, one captured hipGraph per step. The plain rate is that of members run one at a time,
-each on its own through the same steps. A cell-step is one grid node advanced by one model step.
+Workload (the ensemble configuration): 64 standard-domain runs (40x32x50, dt 3 s), each with its own seeded thermal
+(amplitude 1.5-3 K, centre offset +-5 km) and theta noise, stepped together with one launch per kernel for all runs and
+one captured hipGraph per step. A cell-step is one grid node advanced by one model step.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--runs R] [--grid standard|large]
+* The timed window: W untimed warm-up steps, then exactly K timed steps (the line's `value`, `ms_per_step`, `roofline`).
+  The window's model time is named in `config.workload`.
+* `full_run`: the same 64 members from their initial state through the whole 30-minute run (600 steps) in the same
+  process: ms/step, p50/p99 per model-minute, the per-kernel table and the transport roofline over the whole run.
+* `plain`: members run one at a time, each on its own through the same steps.
+* `cpu_baseline`: the oracle (the C++ restatement of the reference's step) on the host's cores, one thread and all cores
+  (one process per core), on a bounded sample: the first steps from the initial state and steps of the mature storm
+  (restarted from the GPU state at 22.5 model minutes).
 
-Prints one JSON line.
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--runs R] [--grid standard|large] [--full-steps S]
+
+Prints one JSON line. `--table-only` prints the per-kernel table of the window (eager launches only: the mode the
+profiler runs are taken in); `--summarize` turns a profiler run of that mode into the summaries under profiles/.
 """
 import argparse
+import csv
+import glob
 import json
 import os
 import sys
@@ -27,28 +38,17 @@ from meteolab_amd import config as C  # noqa: E402
 PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
 DT = 3.0
 METRIC = "cell-steps/s on 1 MI355X, batched vs plain; p50/p99 latency per model-minute"
+STAGE_KERNELS = ("k_stage_plane", "k_stage_level", "k_stage")
 
-
-def stage_bytes(n, stage, carried=17, buoyancy=True):
-    """Algorithmic HBM bytes of one k_stage launch (one RK3 stage of `carried` fields) per run of n nodes: each field's
-    stage input read, its start read (stage 0: the same array), its output written (float32), the face mass fluxes and
-    their divergence read once (4 x float32) and, with w among the fields, the 8 fields of the buoyancy read once."""
-    per_field = 8 if stage == 0 else 12
-    return n * (carried * per_field + 16 + (8 * 4 if buoyancy else 0))
-
-
-def stage_bytes_per_step(n, split):
-    """The transport's algorithmic bytes of a step per run: three launches of the 17 carried fields, or, with the step
-    split around its lightning, three of the 13 fields before the charges and three of the 4 charges."""
-    if split:
-        return sum(stage_bytes(n, s, 13) + stage_bytes(n, s, 4, buoyancy=False) for s in range(3))
-    return sum(stage_bytes(n, s) for s in range(3))
-
-
+# Algorithmic bytes of the RK3 transport per cell-step: each of the 17 carried fields read as the stage input and as the
+# stage start and written (float32; stage 0 reads one array for both: 8 + 12 + 12 B over the three stages), and the
+# face mass fluxes with their divergence read once a stage (4 x float32 x 3 stages).
+TRANSPORT_BYTES_PER_CELL = 17 * (8 + 12 + 12) + 3 * 16  # 592
 # Compulsory float32 traffic of a whole step per cell (the step's fields, each phase fused, no reuse across the step's
 # global syncs): fall speeds 44, sedimentation 84, face projection 96, RK3 transport 592, stage projections 160,
 # positivity fix 92, column physics 148, turbulence and mixing 104, final projection 80, lightning 88.
 STEP_BYTES_PER_CELL = 1488
+SNAPSHOT_STEP = 450  # 22.5 model minutes: the mature storms the CPU baseline's storm sample restarts from
 
 
 def tuning_args(t):
@@ -56,55 +56,276 @@ def tuning_args(t):
     return {"layout": t["transport_layout"], "groups": t["field_groups"], "xcd": t["xcd_per_run"], "levels": t["levels_per_block"]}
 
 
-def measured_traffic(kernel, runs, grid, tuning, split):
-    """HBM bytes per launch of the transport kernel (the mean over a step's launches) from the committed counter summary
-    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size) when it was taken on this
-    workload, launch shape and step split, else None."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_ensemble{runs}_{grid['nx']}x{grid['ny']}x{grid['nz']}_hbm.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        rec = json.load(f)
-    shape = {k: tuning[k] for k in ("transport_layout", "field_groups", "levels_per_block", "xcd_per_run")}
-    return rec["bytes_per_launch"] if rec.get("shape") == shape and rec.get("split", False) == split else None
+def shape_key(t):
+    return {k: t[k] for k in ("transport_layout", "field_groups", "levels_per_block", "xcd_per_run")}
+
+
+def workload_tag(runs, grid):
+    return f"ensemble{runs}_{grid['nx']}x{grid['ny']}x{grid['nz']}"
+
+
+def window_tag(steps, warmup):
+    return f"steps{warmup + 1}-{warmup + steps}"
+
+
+def model_minutes(steps, warmup):
+    return f"model minutes {warmup * DT / 60:g}-{(warmup + steps) * DT / 60:g} (steps {warmup + 1}-{warmup + steps})"
 
 
 def percentile(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
 
 
-def cpu_baseline(grid, steps):
-    """The oracle (single-threaded C++ restatement of the reference's step) on a bounded sample of the workload."""
-    from oracle.binding import OracleRun
-    o = OracleRun(grid, C.profile(), C.ensemble_member(grid, 0))
-    t0 = time.perf_counter()
-    o.step(steps)
-    dt = time.perf_counter() - t0
-    n = C.nodes(grid)
-    return {"value": n * steps / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"1 run of {grid['nx']}x{grid['ny']}x{grid['nz']} (ensemble member 0), steps 1-{steps} from the "
-                      f"initial state, 1 thread, {dt:.1f} s"}
+def per_minute(ms):
+    """Device time of each consecutive 20 steps (one model minute at dt 3 s)."""
+    k = int(round(60 / DT))
+    return [float(ms[i:i + k].sum()) for i in range(0, len(ms) - k + 1, k)]
+
+
+def transport_roofline(kt, runs, n):
+    """The transport kernel's roofline from a per-kernel table: algorithmic bytes per launch (592 B a cell-step over
+    the step's launches of the kernel) over the table's mean launch time."""
+    kernel = next(k for k in STAGE_KERNELS if k in kt)
+    launches, ms = kt[kernel]
+    per_launch = TRANSPORT_BYTES_PER_CELL * n * runs / launches
+    achieved = per_launch / (ms / launches / 1e3)
+    return kernel, {"bound": "hbm", "kernel": kernel, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                    "frac": achieved / PEAK_HBM, "bytes_per_launch": per_launch, "ms_per_launch": ms / launches,
+                    "launches_per_step": launches, "bytes_per_cell_step": TRANSPORT_BYTES_PER_CELL}
+
+
+def measured_traffic(kernel, runs, grid, tuning, steps, warmup):
+    """HBM bytes per launch of the transport kernel over the same window and launch shape, from the committed counter
+    summary of that window (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, scaled by a copy kernel of known size), else
+    None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload_tag(runs, grid)}_{window_tag(steps, warmup)}_hbm.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    return rec["bytes_per_launch"] if rec.get("shape") == shape_key(tuning) else None
+
+
+def members(pool, grid, count):
+    prof = C.profile()
+    return [pool.run(prof, C.ensemble_member(grid, k)) for k in range(count)]
+
+
+def snapshot(run):
+    """A run's state as the oracle restarts from it: the carried fields, fieldZ, the ground columns and the time."""
+    return {"fields": {name: run.field(name) for name in M.CARRIED + ["fieldZ"]},
+            "columns": {name: run.column(name) for name in M.COLUMNS}, "time": float(run.diagnostics()["time"])}
+
+
+def cpu_baseline(grid, states, pre_steps, storm_steps, cores):
+    """The oracle on the host: one thread, then `cores` single-threaded processes at once, each on a sample of the
+    workload: `pre_steps` steps of ensemble members from their initial state and `storm_steps` steps of the same
+    members restarted from the GPU's state at the snapshot step (the mature storms)."""
+    from oracle.timing import cpu_model, host_cores, time_jobs
+    prof, n = C.profile(), C.nodes(grid)
+    cores = max(1, min(cores, len(states)))
+
+    def sample(count):
+        pre = [(grid, prof, C.ensemble_member(grid, k), None, pre_steps) for k in range(count)]
+        storm = [(grid, prof, C.ensemble_member(grid, k), states[k], storm_steps) for k in range(count)]
+        t_pre, _ = time_jobs(pre)
+        t_storm, _ = time_jobs(storm)
+        cells_pre, cells_storm = count * n * pre_steps, count * n * storm_steps
+        return {"value": (cells_pre + cells_storm) / (t_pre + t_storm), "unit": "cell-steps/s", "cores": count,
+                "pre_storm": {"value": cells_pre / t_pre, "steps": pre_steps, "wall_s": round(t_pre, 2)},
+                "storm": {"value": cells_storm / t_storm, "steps": storm_steps, "wall_s": round(t_storm, 2)}}
+
+    one = sample(1)
+    every = sample(cores) if cores > 1 else one
+    snap_min = SNAPSHOT_STEP * DT / 60
+    return {"value": every["value"], "unit": "cell-steps/s", "cores": every["cores"], "kind": "port",
+            "sample": f"the oracle (C++ restatement, float64 arithmetic), one single-threaded process per core: "
+                      f"{every['cores']} ensemble members of {grid['nx']}x{grid['ny']}x{grid['nz']} for steps 1-{pre_steps} "
+                      f"from their initial state, then the same members restarted from the GPU state at {snap_min:g} model "
+                      f"minutes (mature storms) for {storm_steps} steps; value = all cell-steps of both over both wall times",
+            "one_thread": one, "all_cores": every, "cpu_model": cpu_model(), "host_cores": host_cores()}
+
+
+def table_only(a, grid, n):
+    """The per-kernel table of the window (eager launches, warm-up included), as the profiler runs take it."""
+    ctx = M.Context(0, 32)
+    pool = ctx.pool(grid, a.runs)
+    runs = members(pool, grid, a.runs)
+    apply_transport(a, runs)
+    if a.warmup:
+        M.profile_step(runs, a.warmup, DT)
+    kt = M.kernel_times(M.profile_step(runs, a.steps, DT))
+    kernel, roof = transport_roofline(kt, a.runs, n)
+    print(json.dumps({"steps": a.steps, "warmup": a.warmup, "runs": a.runs, "grid": f"{grid['nx']}x{grid['ny']}x{grid['nz']}",
+                      "window": model_minutes(a.steps, a.warmup), "tuning": M.batch_tuning(runs), "roofline": roof,
+                      "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][1])},
+                      # a copy of known size, whose counter readings calibrate the transport's
+                      "copy_GBps": ctx.copy_bandwidth()}))
+    ctx.close()
+
+
+def apply_transport(a, runs):
+    if a.transport != "auto":
+        layout, count, xcd = a.transport.split(":")
+        M.set_batch_tuning(runs, layout, xcd=xcd == "1", levels=int(count))
+
+
+# ---- summaries of profiler runs of --table-only (rocprofv3 -f csv) ---------------------------------------------------
+
+def _csv_rows(directory, suffix):
+    paths = sorted(glob.glob(os.path.join(directory, "**", f"*{suffix}"), recursive=True))
+    if not paths:
+        raise SystemExit(f"no *{suffix} under {directory}")
+    rows = []
+    for p in paths:
+        with open(p, newline="") as f:
+            rows += list(csv.DictReader(f))
+    return rows
+
+
+def _grid(row):
+    for key in ("Grid_Size_X", "Grid_Size"):
+        if key in row and row[key] not in (None, ""):
+            return int(row[key])
+    return 0
+
+
+def _short(name):
+    return name.split("(")[0].replace("void ", "").strip()
+
+
+def _window_dispatches(rows, kernel, steps):
+    """The transport's dispatches of the measured window: the last 3 * steps at each grid size of the step's last launches
+    (the stage launches of the fields before the charges and of the charges), in dispatch order."""
+    mine = sorted((r for r in rows if _short(r["Kernel_Name"]).startswith(kernel + "<")), key=lambda r: int(r["Dispatch_Id"]))
+    sizes = sorted({_grid(r) for r in mine[-6:]})
+    chosen = []
+    for s in sizes:
+        chosen += [r for r in mine if _grid(r) == s][-3 * steps:]
+    return sorted(chosen, key=lambda r: int(r["Dispatch_Id"])), sizes
+
+
+def _counter(rows, name, dispatches):
+    by = {}
+    for r in rows:
+        if r["Counter_Name"] == name:
+            by[int(r["Dispatch_Id"])] = by.get(int(r["Dispatch_Id"]), 0.0) + float(r["Counter_Value"])
+    return [by[d] for d in dispatches if d in by]
+
+
+def summarize(a):
+    """Writes the kernel-trace summary, the HBM traffic record and the VALU record of a --table-only window from the
+    rocprofv3 CSV directories given, beside the bench JSON of the traced command."""
+    bench = json.load(open(a.bench_json))
+    grid = C.GRIDS[a.grid]
+    n, steps, warmup, runs = C.nodes(grid), bench["steps"], bench["warmup"], bench["runs"]
+    kernel = bench["roofline"]["kernel"]
+    base = f"{kernel}_{workload_tag(runs, grid)}_{window_tag(steps, warmup)}"
+    cmd = f"python3 bench.py --table-only --no-cpu --steps {steps} --warmup {warmup} --runs {runs} --grid {a.grid}"
+    out = {}
+    if a.trace:
+        rows = _csv_rows(a.trace, "kernel_trace.csv")
+        chosen, sizes = _window_dispatches(rows, kernel, steps)
+        us = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in chosen]
+        groups = {}
+        for r in rows:
+            key = (_short(r["Kernel_Name"]), _grid(r))
+            d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+            c, t = groups.get(key, (0, 0.0))
+            groups[key] = (c + 1, t + d)
+        total = sum(t for _, t in groups.values())
+        lines = [f"# {kernel}: {runs} {a.grid}-domain runs, {model_minutes(steps, warmup)} — kernel trace", "",
+                 f"Command (repository root, one MI355X): `rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/trace -- {cmd}`", "",
+                 "Every dispatch of the command, grouped by kernel and grid: the autotuner's stage-0 timing launches, the "
+                 f"{warmup} warm-up steps and the {steps} measured steps, all launched eagerly, and the copy probe.", "",
+                 "| kernel | grid x | dispatches | total ms | average us | % |", "|---|---|---|---|---|---|"]
+        for (name, gx), (c, t) in sorted(groups.items(), key=lambda kv: -kv[1][1]):
+            if t / total < 0.002:
+                continue
+            lines.append(f"| {name} | {gx} | {c} | {t / 1e3:.1f} | {t / c:.1f} | {100 * t / total:.1f} |")
+        lines += ["", f"The measured window's transport dispatches: the last {3 * steps} at each of grids {sizes} "
+                      f"({len(chosen)} in all) average **{np.mean(us):.1f} us** a launch; the bench JSON of the same "
+                      f"command reports {bench['roofline']['ms_per_launch'] * 1e3:.1f} us (HIP events on the library stream), "
+                      f"achieved {bench['roofline']['achieved']:.0f} GB/s at {TRANSPORT_BYTES_PER_CELL} B a cell-step "
+                      f"(frac {bench['roofline']['frac']:.3f}).", f"Launch shape: {bench['tuning']}.", ""]
+        with open(os.path.join(ROOT, "profiles", base + "_trace.md"), "w") as f:
+            f.write("\n".join(lines))
+        out["trace_us"] = float(np.mean(us))
+        ids = [int(r["Dispatch_Id"]) for r in chosen]
+    if a.fetch and a.write:
+        ids_f = [int(r["Dispatch_Id"]) for r in _window_dispatches(_csv_rows(a.fetch, "counter_collection.csv"), kernel, steps)[0]]
+        ids_w = [int(r["Dispatch_Id"]) for r in _window_dispatches(_csv_rows(a.write, "counter_collection.csv"), kernel, steps)[0]]
+        frows, wrows = _csv_rows(a.fetch, "counter_collection.csv"), _csv_rows(a.write, "counter_collection.csv")
+        copy_f = [float(r["Counter_Value"]) for r in frows if _short(r["Kernel_Name"]).startswith("k_copy") and r["Counter_Name"] == "FETCH_SIZE"]
+        copy_w = [float(r["Counter_Value"]) for r in wrows if _short(r["Kernel_Name"]).startswith("k_copy") and r["Counter_Name"] == "WRITE_SIZE"]
+        # The copy kernel reads and writes 1 GiB a dispatch: the scale from each counter's KiB to bytes.
+        f_scale, w_scale = (1 << 30) / (np.mean(copy_f) * 1024), (1 << 30) / (np.mean(copy_w) * 1024)
+        fetch = np.mean(_counter(frows, "FETCH_SIZE", ids_f)) * 1024 * f_scale
+        write = np.mean(_counter(wrows, "WRITE_SIZE", ids_w)) * 1024 * w_scale
+        algorithmic = bench["roofline"]["bytes_per_launch"]
+        rec = {"kernel": kernel, "workload": f"{runs} {a.grid}-domain runs, ensemble members 0-{runs - 1}",
+               "window": model_minutes(steps, warmup), "shape": shape_key(bench["tuning"]),
+               "bytes_per_launch": fetch + write, "read_bytes_per_launch": fetch, "written_bytes_per_launch": write,
+               "algorithmic_bytes_per_launch": algorithmic, "traffic_over_algorithmic": (fetch + write) / algorithmic,
+               "fetch_scale": f_scale, "write_scale": w_scale, "dispatches": len(ids_f),
+               "calibration": "k_copy of the copy probe (1 GiB read + 1 GiB written a dispatch): scale = 1 GiB / counter bytes",
+               "commands": [f"rocprofv3 --pmc FETCH_SIZE -f csv -d gpurun_out/pmc_fetch -- {cmd}",
+                            f"rocprofv3 --pmc WRITE_SIZE -f csv -d gpurun_out/pmc_write -- {cmd}"]}
+        with open(os.path.join(ROOT, "profiles", base + "_hbm.json"), "w") as f:
+            json.dump(rec, f, indent=1)
+        out["traffic"] = rec["bytes_per_launch"]
+    if a.valu:
+        vrows = _csv_rows(a.valu, "counter_collection.csv")
+        chosen, sizes = _window_dispatches(vrows, kernel, steps)
+        names = sorted({r["Counter_Name"] for r in vrows})
+        per = {}
+        for s in sizes:
+            ids_s = [int(r["Dispatch_Id"]) for r in chosen if _grid(r) == s]
+            per[s] = {c: float(np.mean(_counter(vrows, c, ids_s))) for c in names}
+        # Lane instructions per node and field of a stage launch: wave instructions x 64 lanes / (runs x n x fields).
+        fields = {max(sizes): 13, min(sizes): 4} if len(sizes) == 2 else {sizes[0]: 17}
+        rec = {"kernel": kernel, "window": model_minutes(steps, warmup), "shape": shape_key(bench["tuning"]),
+               "counters_per_launch": {str(s): v for s, v in per.items()},
+               "valu_lane_instructions_per_node_field": {str(s): per[s]["SQ_INSTS_VALU"] * 64 / (runs * n * fields[s]) for s in sizes
+                                                         if "SQ_INSTS_VALU" in per[s]},
+               "command": f"rocprofv3 --pmc {' '.join(names)} -f csv -d gpurun_out/pmc_valu -- {cmd}"}
+        with open(os.path.join(ROOT, "profiles", base + "_valu.json"), "w") as f:
+            json.dump(rec, f, indent=1)
+        out["valu"] = rec["valu_lane_instructions_per_node_field"]
+    print(json.dumps(out))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=600)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--runs", type=int, default=64)
     ap.add_argument("--grid", default="standard", choices=["standard", "large"])
+    ap.add_argument("--full-steps", type=int, default=600, help="steps of the full run (0: none)")
     ap.add_argument("--plain-members", type=int, default=4)
-    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--cpu-pre-steps", type=int, default=60)
+    ap.add_argument("--cpu-storm-steps", type=int, default=20)
+    ap.add_argument("--cpu-cores", type=int, default=16, help="most processes of the all-core CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--table-only", action="store_true",
-                    help="only the per-kernel table and the transport roofline (eager launches, no graph replays: "
-                         "for hardware-counter runs, whose profiler cannot follow graph replays)")
+                    help="only the per-kernel table of the window and the transport roofline (eager launches, no graph "
+                         "replays: for the profiler's runs, which cannot follow graph replays)")
     ap.add_argument("--compare-in-line", action="store_true",
-                    help="also time the same runs with each step's lightning in line (ml_ctx_set_overlap off)")
+                    help="also time the window with each step's lightning in line (ml_ctx_set_overlap off)")
     ap.add_argument("--transport", default="auto",
-                    help="the transport kernel's launch shape: auto (the autotuner's), or layout:n:xcd with layout strip "
-                         "(n field groups) or level / plane (n levels a block) and xcd 0 or 1")
+                    help="the transport kernel's launch shape: auto (the grid's layout, tuned levels and block order), or "
+                         "layout:n:xcd with layout level or plane, n levels a block and xcd 0 or 1")
+    ap.add_argument("--summarize", action="store_true", help="write profiles/ summaries of --table-only profiler runs")
+    ap.add_argument("--bench-json", help="--summarize: the --table-only JSON line of the profiled command")
+    ap.add_argument("--trace", help="--summarize: the rocprofv3 --kernel-trace -f csv directory")
+    ap.add_argument("--fetch", help="--summarize: the rocprofv3 --pmc FETCH_SIZE -f csv directory")
+    ap.add_argument("--write", help="--summarize: the rocprofv3 --pmc WRITE_SIZE -f csv directory")
+    ap.add_argument("--valu", help="--summarize: the rocprofv3 --pmc SQ_... -f csv directory")
     a = ap.parse_args()
+    if a.summarize:
+        summarize(a)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {a.gpus} was asked): the batched runs are independent and a "
               "multi-GPU run would shard them by run with no exchange; no multi-GPU figure is measured.", file=sys.stderr)
@@ -112,54 +333,43 @@ def main():
 
     grid = C.GRIDS[a.grid]
     n = C.nodes(grid)
-    prof = C.profile()
+    if a.table_only:
+        table_only(a, grid, n)
+        return
     ctx = M.Context(0, 32)
 
-    # Batched: R runs in one pool, one launch per kernel for all of them.
+    # The timed window: R runs in one pool, one launch per kernel for all of them, a graph replay per step.
     pool = ctx.pool(grid, a.runs)
-    runs = [pool.run(prof, C.ensemble_member(grid, k)) for k in range(a.runs)]
-    if a.transport != "auto":
-        layout, count, xcd = a.transport.split(":")
-        M.set_batch_tuning(runs, layout, groups=int(count) if layout == "strip" else 1, xcd=xcd == "1",
-                           levels=int(count) if layout != "strip" else 1)
-    if a.table_only:
-        # The per-kernel table of the members from their initial state, in the shape asked (or the autotuner's).
-        M.set_batch_tuning(runs, **tuning_args(M.batch_tuning(runs)))
-        if a.warmup:
-            M.step(runs, a.warmup, DT)
-        kt = M.kernel_times(M.profile_step(runs, a.steps, DT))
-        stage_kernel = next(k for k in ("k_stage_plane", "k_stage_level", "k_stage") if k in kt)
-        split = "k_charge_courant" in kt
-        launches, stage_ms = kt[stage_kernel]
-        stage_b = stage_bytes_per_step(n, split) * a.runs
-        print(json.dumps({"steps": a.steps, "tuning": {"batched": M.batch_tuning(runs)},
-                          "roofline": {"kernel": stage_kernel, "bytes_per_launch": stage_b / launches,
-                                       "ms_per_launch": stage_ms / launches, "launches_per_step": launches,
-                                       "achieved": stage_b / (stage_ms / 1e3) / 1e9},
-                          "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][1])},
-                          # a copy of known size, whose counter readings calibrate the transport's
-                          "copy_GBps": ctx.copy_bandwidth()}))
-        ctx.close()
-        return
+    runs = members(pool, grid, a.runs)
+    apply_transport(a, runs)
     if a.warmup:
         M.step(runs, a.warmup, DT)
         ctx.sync()
     ms = M.step_timed(runs, a.steps, DT)
-    total_s = ms.sum() / 1e3
-    batched = a.runs * n * a.steps / total_s
-    # Latency per model-minute: the device time of each consecutive 20 steps (60 s of model time) of the batch.
-    per_minute = int(round(60 / DT))
-    minutes = [ms[k:k + per_minute].sum() for k in range(0, len(ms) - per_minute + 1, per_minute)]
+    batched = a.runs * n * a.steps / (ms.sum() / 1e3)
+    minutes = per_minute(ms)
+    tuning = M.batch_tuning(runs)
+    pool.destroy()
 
-    # Optionally the same members through the same steps with each step's lightning in line (ml_ctx_set_overlap off):
-    # what running a step's discharge beside the next step's dynamics gains. (Off by default: its whole-field transport
-    # launches would mix into a kernel trace of this command.)
+    # The window's kernel times: the same members from their initial state through the same warm-up and the same steps,
+    # launched eagerly with HIP events between the kernels on the library's stream (a graph replay cannot be split).
+    epool = ctx.pool(grid, a.runs)
+    eruns = members(epool, grid, a.runs)
+    M.set_batch_tuning(eruns, **tuning_args(tuning))
+    if a.warmup:
+        M.step(eruns, a.warmup, DT)
+    kt = M.kernel_times(M.profile_step(eruns, a.steps, DT))
+    epool.destroy()
+    kernel, roofline = transport_roofline(kt, a.runs, n)
+    roofline["traffic"] = measured_traffic(kernel, a.runs, grid, tuning, a.steps, a.warmup)
+    roofline["window"] = model_minutes(a.steps, a.warmup)
+
     in_line = None
     if a.compare_in_line:
         ctx.set_overlap(False)
         lpool = ctx.pool(grid, a.runs)
-        lruns = [lpool.run(prof, C.ensemble_member(grid, k)) for k in range(a.runs)]
-        M.set_batch_tuning(lruns, **tuning_args(M.batch_tuning(runs)))
+        lruns = members(lpool, grid, a.runs)
+        M.set_batch_tuning(lruns, **tuning_args(tuning))
         if a.warmup:
             M.step(lruns, a.warmup, DT)
         lms = M.step_timed(lruns, a.steps, DT)
@@ -168,37 +378,60 @@ def main():
         in_line = {"ms_per_step": float(lms.mean()), "value": a.runs * n * a.steps / (lms.sum() / 1e3),
                    "overlap_speedup": float(lms.sum() / ms.sum())}
 
-    # Kernel times over the same workload: the same members from their initial state through the same steps, launched
-    # eagerly with HIP events between the kernels on the library's stream (the graph replays above cannot be split by
-    # kernel). The lightning kernels cost nothing until the storms electrify, so the table is over the whole run.
-    epool = ctx.pool(grid, a.runs)
-    eruns = [epool.run(prof, C.ensemble_member(grid, k)) for k in range(a.runs)]
-    M.set_batch_tuning(eruns, **tuning_args(M.batch_tuning(runs)))
-    if a.warmup:
-        M.step(eruns, a.warmup, DT)
-    kt = M.kernel_times(M.profile_step(eruns, a.steps, DT))
-    epool.destroy()
-    stage_kernel = next(k for k in ("k_stage_plane", "k_stage_level", "k_stage") if k in kt)
-    # Per step: the transport's launches (3, or 6 with the step split around its lightning), their time and bytes.
-    split = "k_charge_courant" in kt
-    stage_launches, stage_ms = kt[stage_kernel]
-    stage_b = stage_bytes_per_step(n, split) * a.runs
-    achieved = stage_b / (stage_ms / 1e3)
-    traffic = measured_traffic(stage_kernel, a.runs, grid, M.batch_tuning(runs), split)
-    dominant = max(kt.items(), key=lambda kv: kv[1][1])[0]
-
-    # Plain: members run one at a time, each through the same steps on its own (one-run launches).
-    members = [k * a.runs // a.plain_members for k in range(a.plain_members)]
-    ppool = ctx.pool(grid, len(members))
-    plain_ms = []
-    for k in members:
-        r = ppool.run(prof, C.ensemble_member(grid, k))
+    # Plain: members run one at a time, each alone through the window and through the full run.
+    picks = [k * a.runs // a.plain_members for k in range(a.plain_members)]
+    ppool = ctx.pool(grid, 1)
+    plain_ms, plain_full = [], []
+    for k in picks:
+        r = ppool.run(C.profile(), C.ensemble_member(grid, k))
         if a.warmup:
             M.step([r], a.warmup, DT)
         plain_ms.append(M.step_timed([r], a.steps, DT))
+        r.release()
+        if a.full_steps:
+            r = ppool.run(C.profile(), C.ensemble_member(grid, k))
+            plain_full.append(M.step_timed([r], a.full_steps, DT))
+            r.release()
+    plain_tuning = M.batch_tuning([ppool.run(C.profile(), C.ensemble_member(grid, 0))])
     pms = np.concatenate(plain_ms)
     plain = n * pms.size / (pms.sum() / 1e3)
-    plain_tuning = M.batch_tuning([r])
+
+    # The full run: the same members from their initial state through the 30 model minutes, a graph replay per step;
+    # at the snapshot step the states the CPU baseline's storm sample restarts from are read (outside the timed steps).
+    full, states = None, []
+    if a.full_steps:
+        fpool = ctx.pool(grid, a.runs)
+        fruns = members(fpool, grid, a.runs)
+        M.set_batch_tuning(fruns, **tuning_args(tuning))
+        first = min(SNAPSHOT_STEP, a.full_steps)
+        fms = [M.step_timed(fruns, first, DT)]
+        if not a.no_cpu:
+            states = [snapshot(r) for r in fruns[:min(a.cpu_cores, a.runs)]]
+        if a.full_steps > first:
+            fms.append(M.step_timed(fruns, a.full_steps - first, DT))
+        fms = np.concatenate(fms)
+        fpool.destroy()
+        tpool = ctx.pool(grid, a.runs)
+        truns = members(tpool, grid, a.runs)
+        M.set_batch_tuning(truns, **tuning_args(tuning))
+        fkt = M.kernel_times(M.profile_step(truns, a.full_steps, DT))
+        tpool.destroy()
+        _, froof = transport_roofline(fkt, a.runs, n)
+        froof["traffic"] = measured_traffic(kernel, a.runs, grid, tuning, a.full_steps, 0)
+        fvalue = a.runs * n * a.full_steps / (fms.sum() / 1e3)
+        pf = np.concatenate(plain_full)
+        fmin = per_minute(fms)
+        full = {"steps": a.full_steps, "window": model_minutes(a.full_steps, 0), "value": fvalue,
+                "ms_per_step": float(fms.mean()),
+                "latency_per_model_minute_ms": {"p50": percentile(fmin, 50), "p99": percentile(fmin, 99),
+                                                "max": max(fmin), "minutes": len(fmin)},
+                "plain": {"value": n * pf.size / (pf.sum() / 1e3), "ms_per_step": float(pf.mean()), "members": picks},
+                "batched_over_plain": fvalue / (n * pf.size / (pf.sum() / 1e3)),
+                "roofline": froof,
+                "step_roofline": {"bytes_per_cell_step": STEP_BYTES_PER_CELL, "achieved_GBps": fvalue * STEP_BYTES_PER_CELL / 1e9,
+                                  "frac": fvalue * STEP_BYTES_PER_CELL / PEAK_HBM},
+                "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(fkt.items(), key=lambda kv: -kv[1][1])},
+                "dominant_kernel": max(fkt.items(), key=lambda kv: kv[1][1])[0]}
 
     out = {
         "metric": METRIC,
@@ -213,39 +446,31 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"ensemble of {a.runs} {a.grid}-domain runs ({grid['nx']}x{grid['ny']}x{grid['nz']}, "
-                               f"dt 3 s) batched in one launch per kernel, {a.steps} steps after {a.warmup} warm-up steps",
+        "config": {"workload": f"ensemble of {a.runs} {a.grid}-domain runs ({grid['nx']}x{grid['ny']}x{grid['nz']}, dt 3 s) "
+                               f"batched in one launch per kernel: {model_minutes(a.steps, a.warmup)} of the 30-minute run "
+                               f"after {a.warmup} warm-up steps (full_run: all 600 steps)",
                    "runs": a.runs, "grid": f"{grid['nx']}x{grid['ny']}x{grid['nz']}", "dt": DT, "parallelism": "1 GPU"},
-        "plain": {"value": plain, "unit": "cell-steps/s", "ms_per_step": float(pms.mean()), "steps": a.steps,
-                  "members": members},
+        "plain": {"value": plain, "unit": "cell-steps/s", "ms_per_step": float(pms.mean()), "steps": a.steps, "members": picks},
         "batched_over_plain": batched / plain,
-
         "latency_per_model_minute_ms": {"p50": percentile(minutes, 50), "p99": percentile(minutes, 99),
                                         "runs_per_batch": a.runs, "minutes": len(minutes)},
-        "roofline": {"bound": "hbm", "kernel": stage_kernel, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
-                     "bytes_per_launch": stage_b / stage_launches, "ms_per_launch": stage_ms / stage_launches,
-                     "launches_per_step": stage_launches,
-                     # The transport skips the blocks whose field is zero within the stage's reach, so `achieved` (the
-                     # full-field algorithmic bytes over the time) is an effective rate; the HBM rate the kernel draws
-                     # is the measured traffic over the same time.
-                     "measured_GBps": traffic / (stage_ms / stage_launches / 1e3) / 1e9 if traffic else None},
-        "step_roofline": {"bytes_per_cell_step": STEP_BYTES_PER_CELL, "achieved_GBps": batched * STEP_BYTES_PER_CELL / 1e9,
-                          "frac": batched * STEP_BYTES_PER_CELL / PEAK_HBM},
+        "roofline": roofline,
         "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][1])},
-        # The longest kernel a step in the per-kernel table (in line). The lightning's discharge, serial by the
-        # reference's algorithm (a block per run growing the channel link by link, with a solve per stage), has no
-        # bandwidth roofline to speak of; `roofline` is the transport's, the largest data-parallel kernel.
-        "dominant_kernel": dominant,
-        "tuning": {"batched": M.batch_tuning(runs), "plain": plain_tuning},
+        "dominant_kernel": max(kt.items(), key=lambda kv: kv[1][1])[0],
+        "tuning": {"batched": tuning, "plain": plain_tuning},
         "copy_GBps": ctx.copy_bandwidth(),
         "device": ctx.adapter,
     }
+    if full:
+        out["full_run"] = full
     if in_line:
         out["lightning_in_line"] = in_line
-    if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(grid, a.cpu_steps)
     ctx.close()
+    if not a.no_cpu:
+        from oracle.timing import host_cores
+        if not states:
+            raise SystemExit("the CPU baseline's storm sample needs --full-steps of at least the snapshot step")
+        out["cpu_baseline"] = cpu_baseline(grid, states, a.cpu_pre_steps, a.cpu_storm_steps, min(a.cpu_cores, host_cores()))
     print(json.dumps(out))
 
 

@@ -433,9 +433,9 @@ def batch_tuning(runs):
 
 
 def set_batch_tuning(runs, layout, groups=1, xcd=False, levels=1):
-    """Forces this batch's transport launch shape (ml_batch_set_tuning): layout "strip" (k_stage, `groups` field
-    groups per block), "level" (k_stage_level, `levels` levels per block) or "plane" (k_stage_plane, the same with
-    each level in a padded LDS plane)."""
+    """Forces this batch's transport launch shape (ml_batch_set_tuning): layout "level" (k_stage_level, `levels`
+    levels per block) or "plane" (k_stage_plane, the same with each level in a padded LDS plane); the "strip" layout
+    is not built and is refused."""
     ctx = runs[0].ctx
     ids = (ctypes.c_int * len(runs))(*[r.h for r in runs])
     shape = (ctypes.c_int * 4)(LAYOUTS.index(layout), int(groups), 1 if xcd else 0, int(levels))

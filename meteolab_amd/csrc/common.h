@@ -100,7 +100,7 @@ template <class R> struct Bufs {
   float *fields;   // [slot][N_FIELDS][n]
   float *stx;      // [slot][N_CARRIED][n] RK3 stage buffer X (stage 0 output) / mixed fields
   float *sty;      // [slot][N_CARRIED][n] RK3 stage buffer Y (stage 1 output)
-  R *mass;         // [slot][4][n]  face mass fluxes mx, my, mz and their divergence
+  double *mass;    // [slot][3][n]  face mass fluxes mx, my, mz (float64 in both variants)
   double *phi;     // [slot][n]     potential of the face projection (float64, the spectral solves' output)
   double *cells;   // [slot][n]     right-hand side of a spectral solve (cells or interior layers) (float64)
   double *press;   // [slot][n]     pressure of the projection (model.ts:358), cells (float64)
@@ -142,7 +142,7 @@ template <class R> struct Bufs {
   __device__ float *y(int slot, int k, const Geo &g) const { return sty + (size_t(slot) * N_CARRIED + k) * g.n; }
   // Buffer set: 0 the state, 1 stage buffer X, 2 stage buffer Y.
   __device__ float *set(int which, int slot, int k, const Geo &g) const { return which == 0 ? f(slot, k, g) : which == 1 ? x(slot, k, g) : y(slot, k, g); }
-  __device__ R *m(int slot, int k, const Geo &g) const { return mass + (size_t(slot) * 4 + k) * g.n; }
+  __device__ double *m(int slot, int k, const Geo &g) const { return mass + (size_t(slot) * 3 + k) * g.n; }
   template <class T> __device__ T *arr(T *base, int slot, const Geo &g) const { return base + size_t(slot) * g.n; }
   __device__ double *sp(int slot, const Geo &g) const { return spec + size_t(slot) * g.nz * g.nm * 2; }
   __device__ double *eh(int slot, const Geo &g) const { return ehat + size_t(slot) * g.nz * g.nm * 2; }

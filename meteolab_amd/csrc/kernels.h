@@ -218,26 +218,28 @@ template <class R> __global__ __launch_bounds__(SED_COLS) void k_sediment(Geo g,
 }
 
 // ---- face mass fluxes and their divergence (advection.ts:190-211 faces + divergence) --------------------------------
-template <class R> __device__ inline void faceFluxes(const Geo &g, const double *rho, const float *u, const float *v, const float *w, int x, int y, int z, R &mx, R &my, R &mz) {
+// float64 from the float32 winds.
+__device__ inline void faceFluxes(const Geo &g, const double *rho, const float *u, const float *v, const float *w, int x, int y, int z, double &mx, double &my, double &mz) {
   int i = x + g.nx * (y + g.ny * z), l = z * g.layer;
   int xp = x + 1 == g.nx ? 0 : x + 1, yp = y + 1 == g.ny ? 0 : y + 1;
-  R r = R(rho[z]);
-  mx = r * (R(u[i]) + R(u[xp + y * g.nx + l])) / 2;
-  my = r * (R(v[i]) + R(v[x + yp * g.nx + l])) / 2;
-  mz = z < g.nz - 1 ? R((rho[z] + rho[z + 1]) / 2) * (R(w[i]) + R(w[i + g.layer])) / 2 : R(0);
+  double r = rho[z];
+  mx = r * (double(u[i]) + double(u[xp + y * g.nx + l])) / 2;
+  my = r * (double(v[i]) + double(v[x + yp * g.nx + l])) / 2;
+  mz = z < g.nz - 1 ? (rho[z] + rho[z + 1]) / 2 * (double(w[i]) + double(w[i + g.layer])) / 2 : 0.0;
 }
 // Face potential correction (advection.ts:244-255 correctFaces) of the faces of node (x, y, z).
-template <class R> __device__ inline void correctedFaces(const Geo &g, const Bufs<R> &b, const double *rho, const float *u, const float *v, const float *w, const double *phi, int x, int y, int z, R &mx, R &my, R &mz) {
+template <class R> __device__ inline void correctedFaces(const Geo &g, const Bufs<R> &b, const double *rho, const float *u, const float *v, const float *w, const double *phi, int x, int y, int z, double &mx, double &my, double &mz) {
   faceFluxes(g, rho, u, v, w, x, y, z, mx, my, mz);
   int i = x + g.nx * (y + g.ny * z), l = z * g.layer;
   int xp = x + 1 == g.nx ? 0 : x + 1, yp = y + 1 == g.ny ? 0 : y + 1;
-  R r = R(rho[z]);
-  mx -= r * R(phi[xp + y * g.nx + l] - phi[i]) / R(g.dx);
-  my -= r * R(phi[x + yp * g.nx + l] - phi[i]) / R(g.dy);
-  if (z < g.nz - 1) mz -= R((rho[z] + rho[z + 1]) / 2) * R(phi[i + g.layer] - phi[i]) / R(b.dzs[z]);
+  double r = rho[z];
+  mx -= r * (phi[xp + y * g.nx + l] - phi[i]) / g.dx;
+  my -= r * (phi[x + yp * g.nx + l] - phi[i]) / g.dy;
+  if (z < g.nz - 1) mz -= (rho[z] + rho[z + 1]) / 2 * (phi[i + g.layer] - phi[i]) / b.dzs[z];
 }
 // corrected = 0: faces and divergence of the node velocities (the right-hand side of the face projection);
-// corrected = 1: the projected faces (faces minus the potential's gradient) and their divergence.
+// corrected = 1: the projected faces (faces minus the potential's gradient), whose divergence the transport's
+// node-relative flux form takes as zero (it is the divergence the projection removed).
 template <class R> __global__ __launch_bounds__(TPB) void k_faces(Geo g, Bufs<R> b, int corrected) {
   int slot = b.runs[blockIdx.y];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -248,31 +250,20 @@ template <class R> __global__ __launch_bounds__(TPB) void k_faces(Geo g, Bufs<R>
   const double *phi = b.arr(b.phi, slot, g);
   int xm = x == 0 ? g.nx - 1 : x - 1, ym = y == 0 ? g.ny - 1 : y - 1, xp = x + 1 == g.nx ? 0 : x + 1, yp = y + 1 == g.ny ? 0 : y + 1;
   int l = z * g.layer;
-  R mx, my, mz;
+  double mx, my, mz;
   if (corrected) correctedFaces(g, b, rho, u, v, w, phi, x, y, z, mx, my, mz);
   else faceFluxes(g, rho, u, v, w, x, y, z, mx, my, mz);
   b.m(slot, 0, g)[i] = mx; b.m(slot, 1, g)[i] = my; b.m(slot, 2, g)[i] = mz;
-  // The divergence of the faces in difference form: mx_i - mx_{i-1} = rho (u_{i+1} - u_{i-1}) / 2, so no float sum of
-  // two ~25 m/s winds is ever differenced; float64, as the right-hand side of the projection (whose low modes amplify
-  // its rounding).
+  if (corrected) return;
+  // The right-hand side of the face projection: the divergence of the faces in difference form,
+  // mx_i - mx_{i-1} = rho (u_{i+1} - u_{i-1}) / 2, so no sum of two ~25 m/s winds is ever differenced (the
+  // projection's low modes amplify its rounding); modes multiplies it by h.
   double rzd = rho[z];
   double dv = rzd * (double(u[xp + y * g.nx + l]) - double(u[xm + y * g.nx + l])) / (2 * g.dx) + rzd * (double(v[x + yp * g.nx + l]) - double(v[x + ym * g.nx + l])) / (2 * g.dy);
   double dup = z < g.nz - 1 ? (rho[z] + rho[z + 1]) / 2 * (double(w[i]) + double(w[i + g.layer])) : 0.0;
   double ddn = z > 0 ? (rho[z - 1] + rho[z]) / 2 * (double(w[i - g.layer]) + double(w[i])) : 0.0;
   dv += (dup - ddn) / (2 * b.hz[z]);
-  R rz = R(rzd), div = R(dv);
-  if (corrected) {
-    // Minus the divergence of rho grad(phi) through the same faces (differences of phi first).
-    double pc = phi[i];
-    R lx = R((phi[xp + y * g.nx + l] - pc) - (pc - phi[xm + y * g.nx + l])), ly = R((phi[x + yp * g.nx + l] - pc) - (pc - phi[x + ym * g.nx + l]));
-    R gu = z < g.nz - 1 ? R((rho[z] + rho[z + 1]) / 2) * R(phi[i + g.layer] - pc) / R(b.dzs[z]) : R(0);
-    R gd = z > 0 ? R((rho[z - 1] + rho[z]) / 2) * R(pc - phi[i - g.layer]) / R(b.dzs[z - 1]) : R(0);
-    div -= rz * lx / R(g.dx * g.dx) + rz * ly / R(g.dy * g.dy) + (gu - gd) / R(b.hz[z]);
-    b.m(slot, 3, g)[i] = div;
-  } else {
-    // The right-hand side of the face projection is the divergence itself (modes multiplies by h).
-    b.arr(b.cells, slot, g)[i] = dv;
-  }
+  b.arr(b.cells, slot, g)[i] = dv;
 }
 
 // ---- transport: flux-form WENO5 / linear upwind tendency and the RK3 stage update --------------------------------
@@ -287,22 +278,29 @@ template <class R> __global__ __launch_bounds__(TPB) void k_faces(Geo g, Bufs<R>
 __device__ inline float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline double recip(double x) { return 1.0 / x; }
 
-// The candidate values are kept times six and the 1/6 joins the final normalisation, and the float reciprocals are the
-// hardware's: each face value moves by an ulp or two from the reference's order of operations.
-template <class R> __device__ inline R wenoFace(R a, R b_, R c, R d, R e) {
+__device__ inline double upwind5d(double a, double b_, double c, double d, double e) { return (2 * a - 13 * b_ + 47 * c + 27 * d - 3 * e) / 60; }
+// The face values relative to the stencil's centre c (the upwind node), from the differences of the other nodes to it:
+// face - c, in the reference's formulas rewritten on da = a - c, db = b - c, dd = d - c, de = e - c (the weights of
+// the five coefficients sum to one). Differences of nearby float values are exact, so the result's rounding is relative
+// to the local variation instead of to the field's magnitude: a float32 flux of m (face - c) keeps the digits a float32
+// flux of m face loses to the cancellation of neighbouring faces.
+template <class R> __device__ inline R upwindRel(R da, R db, R dd, R de) { return (2 * da - 13 * db + 27 * dd - 3 * de) * R(1.0 / 60); }
+// WENO5 (advection.ts:388-399) relative to c: the weights from the smoothness indicators of the differences (the
+// indicators are differences already) and the epsilon of the values' magnitudes (a, ..., e); the candidates as
+// differences from c. The candidates are kept times six and the 1/6 joins the final normalisation, and the float
+// reciprocals are the hardware's.
+template <class R> __device__ inline R wenoRel(R a, R b_, R c, R d, R e) {
   R s = mmax(mmax(mmax(mabs(a), mabs(b_)), mmax(mabs(c), mabs(d))), mabs(e));
   if (s == 0) return 0;
-  if (s < R(1e-25)) {
-    // The smoothness indicators (~s^2) vanish beside the reference's 1e-40 in epsilon: the linear weights, exactly the
-    // limit of its weights (and 1/s would overflow a float for a denormal s).
-    return (R(.1) * (2 * a - 7 * b_ + 11 * c) + R(.6) * (-b_ + 5 * c + 2 * d) + R(.3) * (2 * c + 5 * d - e)) / 6;
-  }
+  R da = a - c, db = b_ - c, dd = d - c, de = e - c;
+  if (s < R(1e-25)) return (R(.1) * (2 * da - 7 * db) + R(.6) * (-db + 2 * dd) + R(.3) * (5 * dd - de)) / 6;
   R is = recip(s);
-  a *= is; b_ *= is; c *= is; d *= is; e *= is;
-  R p0 = 2 * a - 7 * b_ + 11 * c, p1 = -b_ + 5 * c + 2 * d, p2 = 2 * c + 5 * d - e;
-  R s0 = a - 2 * b_ + c, t0 = a - 4 * b_ + 3 * c, s1 = b_ - 2 * c + d, t1 = b_ - d, s2 = c - 2 * d + e, t2 = 3 * c - 4 * d + e;
-  R tiny = R(1e-20) * is;  // <= 1e5
-  R eps = R(2e-7) * (a * a + b_ * b_ + c * c + d * d + e * e) + tiny * tiny;
+  R an = a * is, bn = b_ * is, cn = c * is, dn = d * is, en = e * is;
+  da *= is; db *= is; dd *= is; de *= is;
+  R p0 = 2 * da - 7 * db, p1 = -db + 2 * dd, p2 = 5 * dd - de;
+  R s0 = da - 2 * db, t0 = da - 4 * db, s1 = db + dd, t1 = db - dd, s2 = de - 2 * dd, t2 = de - 4 * dd;
+  R tiny = R(1e-20) * is;
+  R eps = R(2e-7) * (an * an + bn * bn + cn * cn + dn * dn + en * en) + tiny * tiny;
   R b0 = eps + R(13.0 / 12) * s0 * s0 + R(.25) * t0 * t0, b1 = eps + R(13.0 / 12) * s1 * s1 + R(.25) * t1 * t1, b2 = eps + R(13.0 / 12) * s2 * s2 + R(.25) * t2 * t2;
   R w0, w1, w2;
   if (tiny > 1) {
@@ -314,15 +312,12 @@ template <class R> __device__ inline R wenoFace(R a, R b_, R c, R d, R e) {
   }
   return s * ((w0 * p0 + w1 * p1 + w2 * p2) * recip(6 * (w0 + w1 + w2)));
 }
-template <class R> __device__ inline R upwindFace(R a, R b_, R c, R d, R e) { return (2 * a - 13 * b_ + 47 * c + 27 * d - 3 * e) * R(1.0 / 60); }
-__device__ inline double upwind5d(double a, double b_, double c, double d, double e) { return (2 * a - 13 * b_ + 47 * c + 27 * d - 3 * e) / 60; }
-template <class R> __device__ inline R faceValue(bool monotone, R a, R b_, R c, R d, R e) { return monotone ? wenoFace(a, b_, c, d, e) : upwindFace(a, b_, c, d, e); }
-// Flux through the face between node values q0 (this node) and q1 (next), with upstream neighbours.
-// Flow towards +: stencil (m2, m1, q0, p1, p2); towards -: (p3, p2, p1, q0, m1).
-// The stencil is chosen first and the face value taken once (lanes of both flow directions share one evaluation).
-template <class R> __device__ inline R faceFlux(bool monotone, R m, R qm2, R qm1, R q0, R qp1, R qp2, R qp3) {
+// The flux m (face - q0) through the face between q0 and qp1, relative to q0 (the face's western, southern or lower
+// node), WENO5 from the stencil chosen by the flow's direction.
+template <class R> __device__ inline R relWeno(R m, R qm2, R qm1, R q0, R qp1, R qp2, R qp3) {
   const bool up = m >= 0;
-  return m * faceValue(monotone, up ? qm2 : qp3, up ? qm1 : qp2, up ? q0 : qp1, up ? qp1 : q0, up ? qp2 : qm1);
+  const R a = up ? qm2 : qp3, b_ = up ? qm1 : qp2, c = up ? q0 : qp1, d = up ? qp1 : q0, e = up ? qp2 : qm1;
+  return m * (wenoRel(a, b_, c, d, e) + (c - q0));
 }
 __device__ inline bool monotoneField(int f) { return !(f == F_U || f == F_V || f == F_W || f == F_THETA); }
 // Index of a carried field among the conserved ones (rain, snow, graupel, hail, q, cloud, ice), -1 if it is not one.
@@ -405,25 +400,9 @@ template <class R> __global__ void k_fill_total(Geo g, Bufs<R> b, int tiles) {
 }
 constexpr double TINY_WATER = 1e-10;
 
-// Transport blocks: a strip of TY rows of one level of one run, for all carried fields in turn, so the face mass fluxes
-// are read once per node. Each field's rows of the strip with three halo rows on either side (periodic in y; the rows
-// are whole, so periodic in x) are staged in LDS; the x and y face fluxes of the strip go to LDS; the vertical stencil
-// comes from memory (the blocks of a run are kept on one XCD, see blockRun, so its levels share that XCD's L2).
-// stage 0: state -> X, stage 1: X -> Y, stage 2: Y -> state (each node reads its own start value first); stage 2 also
-// sums the positivity fix of the conserved fields (model.ts:606-617 fillNegative) for the physics kernel.
-struct StageTile { int ty, tiles, threads; };
-__host__ __device__ inline StageTile stageTile(const Geo &g) {
-  // Strips of about 256-512 nodes that divide the level.
-  int ty = 1;
-  for (int t = 1; t <= g.ny; t++) if (g.ny % t == 0 && g.nx * t <= 512) ty = t;
-  StageTile s; s.ty = ty; s.tiles = g.ny / ty; s.threads = ((g.nx * ty + 63) / 64) * 64;
-  return s;
-}
-__host__ __device__ inline size_t stageLdsBytes(const Geo &g, size_t rb) {
-  StageTile t = stageTile(g);
-  size_t q = size_t(t.ty + 6) * g.nx * 4, fx = size_t(t.ty) * g.nx * rb, fy = size_t(t.ty + 1) * g.nx * rb;
-  return ((q + 15) / 16) * 16 + fx + fy;
-}
+// Transport blocks: K levels of one field of one run. stage 0: state -> X, stage 1: X -> Y, stage 2: Y -> state (each
+// node reads its own start value first); stage 2 also sums the positivity fix of the conserved fields (model.ts:606-617
+// fillNegative) for the physics kernel.
 // Block -> (run index in the launch list, block within the run). Workgroups go to the eight XCDs in turn; with `xcd`
 // set and a batch that divides into eight, each run's blocks are all given to one XCD, in order.
 __device__ inline void blockRun(int nr, int perRun, int xcd, int &runIdx, int &b) {
@@ -432,120 +411,45 @@ __device__ inline void blockRun(int nr, int perRun, int xcd, int &runIdx, int &b
   else { runIdx = p / perRun; b = p % perRun; }
 }
 
-// The launch takes fields fbase .. fbase + nc - 1 (all carried fields, or a part of a step split around the lightning).
-// `groups` splits them among blocks (group k takes fields [k nc / groups, (k + 1) nc / groups) of them): more blocks in
-// flight against the mass fluxes read once per group; the autotuner picks it per batch.
-template <class R> __global__ __launch_bounds__(512) void k_stage(Geo g, Bufs<R> b, int stage, double dt, int nr, int nc, int fbase, int groups, int xcd) {
-  const StageTile T = stageTile(g);
-  int runIdx, blk;
-  blockRun(nr, T.tiles * g.nz * groups, xcd, runIdx, blk);
-  const int grp = blk % groups;
-  blk /= groups;
-  const int slot = b.runs[runIdx], z = blk / T.tiles, y0 = (blk % T.tiles) * T.ty;
-  const int f0 = fbase + grp * nc / groups, f1 = fbase + (grp + 1) * nc / groups;
-  const int inSet = stage, outSet = stage == 2 ? 0 : stage + 1;
-  const int layer = g.layer, nx = g.nx, ny = g.ny, nz = g.nz, ty = T.ty, tn = nx * ty;
-  extern __shared__ char smem[];
-  float *ql = reinterpret_cast<float *>(smem);                        // rows y0-3 .. y0+ty+2
-  R *fxl = reinterpret_cast<R *>(smem + ((size_t(ty + 6) * nx * 4 + 15) / 16) * 16);  // ty rows
-  R *fyl = fxl + tn;                                                   // faces of rows y0-1 .. y0+ty-1
-  __shared__ double red[16][3];
-  const int t = threadIdx.x;
-  const bool live = t < tn;
-  const int y = y0 + t / nx, x = t % nx, c = y * nx + x, i = z * layer + c;
-  const double *rho = b.lv(slot, L_RHO, g);
-  const R ihz = R(1.0 / b.hz[z]), irz = R(1.0 / rho[z]), ihzrz = R(1.0 / (b.hz[z] * rho[z])), idx = recip(R(g.dx)), idy = recip(R(g.dy)), frac = R((stage == 0 ? 1.0 / 3 : stage == 1 ? 1.0 / 2 : 1.0) * dt);
-  // The node's face mass fluxes and divergence, once for all fields.
-  R mx = 0, my = 0, myd = 0, mzt = 0, mzb = 0, dv = 0;
-  if (live) {
-    mx = b.m(slot, 0, g)[i]; my = b.m(slot, 1, g)[i]; dv = b.m(slot, 3, g)[i];
-    mzt = z == nz - 1 ? R(0) : b.m(slot, 2, g)[i]; mzb = z == 0 ? R(0) : b.m(slot, 2, g)[i - layer];
-  }
-  // The y face below the strip's first row (row y0 - 1) belongs to threads t < nx.
-  const int yb = y0 == 0 ? ny - 1 : y0 - 1;
-  if (t < nx) myd = b.m(slot, 1, g)[z * layer + yb * nx + t];
-  int zc[7];
-  for (int k = 0; k < 7; k++) { int zz = z - 3 + k; zz = zz < 0 ? 0 : zz > nz - 1 ? nz - 1 : zz; zc[k] = zz * layer; }
-  for (int f = f0; f < f1; f++) {
-    if (zeroLevels(b, g, slot, f, z - stageReach(stage), z + stageReach(stage))) {
-      // Zero within the stage's reach: the strip's zero output, or its zero positivity-fix part.
-      if (stage < 2) { if (live) b.set(outSet, slot, f, g)[i] = 0.f; }
-      else if (conservedIndex(f) >= 0) { double zero[3] = {0, 0, 0}; storeFill(b, g, slot, conservedIndex(f), z, blk % T.tiles, zero, red); }
-      continue;
-    }
-    const float *q = b.set(inSet, slot, f, g);
-    // u, v and theta (linear upwind5) are transported as departures from their level's base profile B(z), plus the
-    // profile's own tendency: the scheme is linear, so this is the same tendency, without float32 sums of ~25 m/s or
-    // ~300 K values (the reference's WebGPU step carries theta the same way, gpu/dynamics.ts:78-97).
-    const int lvl = f == F_U ? L_UENV : f == F_V ? L_VENV : f == F_THETA ? L_THETA : -1;
-    const double *B = lvl >= 0 ? b.lv(slot, lvl, g) : nullptr;
-    const double Bz = B ? B[z] : 0;
-    const float *qz = q + size_t(z) * layer;
-    for (int k = t; k < (ty + 6) * nx; k += blockDim.x) {
-      int r = k / nx, yy = y0 - 3 + r; yy += yy < 0 ? ny : 0; yy -= yy >= ny ? ny : 0;
-      float v = qz[yy * nx + (k - r * nx)];
-      ql[k] = B ? float(double(v) - Bz) : v;
-    }
-    __syncthreads();
-    const bool mono = monotoneField(f);
-    auto Q = [&](int row, int xx) { xx += xx < 0 ? nx : 0; xx -= xx >= nx ? nx : 0; return R(ql[row * nx + xx]); };
-    if (live) {
-      int r = t / nx + 3;  // the node's row in ql
-      fxl[t] = faceFlux(mono, mx, Q(r, x - 2), Q(r, x - 1), Q(r, x), Q(r, x + 1), Q(r, x + 2), Q(r, x + 3));
-      fyl[nx + t] = faceFlux(mono, my, Q(r - 2, x), Q(r - 1, x), Q(r, x), Q(r + 1, x), Q(r + 2, x), Q(r + 3, x));
-    }
-    if (t < nx) fyl[t] = faceFlux(mono, myd, Q(0, t), Q(1, t), Q(2, t), Q(3, t), Q(4, t), Q(5, t));
-    __syncthreads();
-    double fill[3] = {0, 0, 0};
-    if (live) {
-      double Bc[7];
-      for (int k = 0; k < 7; k++) Bc[k] = B ? B[zc[k] / layer] : 0;
-      R col[7];
-      const R qn = R(ql[(t / nx + 3) * nx + x]);
-      for (int k = 0; k < 7; k++) col[k] = k == 3 ? qn : B ? R(double(q[zc[k] + c]) - Bc[k]) : R(q[zc[k] + c]);
-      // Top face of this node (none through the model top) and of the node below (none into the ground).
-      R fzt = z == nz - 1 ? R(0) : faceFlux(mono, mzt, col[1], col[2], col[3], col[4], col[5], col[6]);
-      R fzb = z == 0 ? R(0) : faceFlux(mono, mzb, col[0], col[1], col[2], col[3], col[4], col[5]);
-      const int xm = x == 0 ? nx - 1 : x - 1;
-      R flux = (fxl[t] - fxl[t - x + xm]) * idx + (fyl[nx + t] - fyl[t]) * idy + (fzt - fzb) * ihz;
-      R tend = -(flux - qn * dv) * irz;
-      if (B) {
-        // The base profile's face values at the top and bottom faces for the flow's direction (float64), minus B(z):
-        // its tendency is the vertical flux of these, the horizontal fluxes cancelling against -B div.
-        R pt = R((mzt >= 0 ? upwind5d(Bc[1], Bc[2], Bc[3], Bc[4], Bc[5]) : upwind5d(Bc[6], Bc[5], Bc[4], Bc[3], Bc[2])) - Bz);
-        R pb = R((mzb >= 0 ? upwind5d(Bc[0], Bc[1], Bc[2], Bc[3], Bc[4]) : upwind5d(Bc[5], Bc[4], Bc[3], Bc[2], Bc[1])) - Bz);
-        tend -= (mzt * pt - mzb * pb) * ihzrz;
-      }
-      if (f == F_W) {
-        // g B with the fields of this stage; the departure of theta_v from the environment in float64 (it is a small
-        // difference of numbers near 300 K).
-        const double te = b.lv(slot, L_THETA, g)[z], qe = b.lv(slot, L_Q, g)[z], thvEnv = te * (1 + .61 * qe);
-        const float th = b.set(inSet, slot, F_THETA, g)[i], qv = b.set(inSet, slot, F_Q, g)[i];
-        R loaded = R(b.set(inSet, slot, F_CLOUD, g)[i]) + R(b.set(inSet, slot, F_ICE, g)[i])
-          + (R(b.set(inSet, slot, F_RAIN, g)[i]) + R(b.set(inSet, slot, F_SNOW, g)[i]) + R(b.set(inSet, slot, F_GRAUPEL, g)[i]) + R(b.set(inSet, slot, F_HAIL, g)[i]));
-        double dthv = (double(th) * (1 + .61 * double(qv)) - thvEnv) * (1.0 / thvEnv);
-        tend += R(G) * (R(dthv) - loaded);
-      }
-      float val = float(R(b.f(slot, f, g)[i]) + frac * tend);
-      if (f == F_W && stage < 2 && (z == 0 || z == nz - 1)) val = 0;
-      b.set(outSet, slot, f, g)[i] = val;
-      if (stage == 2) {
-        double xv = val, wz = b.lv(slot, L_LW, g)[z];
-        if (xv < TINY_WATER) { if (xv != 0) { fill[0] = wz * xv; if (xv < 0) fill[2] = -wz * xv; } }
-        else fill[1] = wz * xv;
-      }
-    }
-    const int cf = conservedIndex(f);
-    // The strip's removed, kept and negative mass (level weights): its part of the run's positivity-fix sums.
-    if (stage == 2 && cf >= 0) storeFill(b, g, slot, cf, z, blk % T.tiles, fill, red);
-    __syncthreads();
-  }
+// ---- the node-relative flux form of the tendency ---------------------------------------------------------------------
+// tend rho0 = -sum_faces (m (q_face - q)) / spacing for a node of value q: the reference's flux form
+// -(sum (m q_face) / spacing - q div) (advection.ts:294-378) with div the divergence of the same face mass fluxes
+// (advection.ts:255), rewritten on each face value's departure from the node's value, which the float rounding of
+// the neighbouring faces' fluxes then no longer cancels out of.
+// Linear upwind5 (u, v, w, theta) along one axis, in float64: D[k] = q(k - 3) - q(0), k = 0..6 (D[3] = 0); mp, mm the
+// face mass fluxes at +1/2 and -1/2. Returns mp (face+ - q) - mm (face- - q).
+__device__ inline double linearAxis(double mp, double mm, const double *D) {
+  double fp, fm;
+  if (mp >= 0) fp = upwindRel<double>(D[1], D[2], D[4], D[5]);                                      // stencil -2 .. 2, centre 0
+  else { const double c = D[4]; fp = upwindRel<double>(D[6] - c, D[5] - c, -c, D[2] - c) + c; }     // stencil 3 .. -1, centre 1
+  if (mm >= 0) { const double c = D[2]; fm = upwindRel<double>(D[0] - c, D[1] - c, -c, D[4] - c) + c; }  // -3 .. 1, centre -1
+  else fm = upwindRel<double>(D[5], D[4], D[2], D[1]);                                              // stencil 2 .. -2, centre 0
+  return mp * fp - mm * fm;
 }
+// The base profile's own tendency of a departure-transported field (u, v, theta), exact for the linear scheme: its
+// vertical face values relative to the level's value, times the vertical face fluxes.
+__device__ inline double baseTendency(const double *Bc, double mzt, double mzb, double ihzrz) {
+  const double Bz = Bc[3];
+  const double pt = mzt >= 0 ? upwind5d(Bc[1], Bc[2], Bc[3], Bc[4], Bc[5]) - Bz : upwind5d(Bc[6], Bc[5], Bc[4], Bc[3], Bc[2]) - Bz;
+  const double pb = mzb >= 0 ? upwind5d(Bc[0], Bc[1], Bc[2], Bc[3], Bc[4]) - Bz : upwind5d(Bc[5], Bc[4], Bc[3], Bc[2], Bc[1]) - Bz;
+  return (mzt * pt - mzb * pb) * ihzrz;
+}
+// The buoyancy of w's tendency (model.ts:1204-1214) at node i of the stage's input set, in float64.
+template <class R> __device__ inline double buoyancy(const Bufs<R> &b, const Geo &g, int set, int slot, size_t i, double thvEnv, double ithv) {
+  const float th = b.set(set, slot, F_THETA, g)[i], qv = b.set(set, slot, F_Q, g)[i];
+  const double loaded = double(b.set(set, slot, F_CLOUD, g)[i]) + double(b.set(set, slot, F_ICE, g)[i])
+    + (double(b.set(set, slot, F_RAIN, g)[i]) + double(b.set(set, slot, F_SNOW, g)[i]) + double(b.set(set, slot, F_GRAUPEL, g)[i]) + double(b.set(set, slot, F_HAIL, g)[i]));
+  const double dthv = (double(th) * (1 + .61 * double(qv)) - thvEnv) * ithv;
+  return G * (dthv - loaded);
+}
+// The level's (base) values a vertical stencil reads, z - 3 .. z + 3 clamped to the column.
+__device__ inline void columnLevels(const Geo &g, int z, int *zc) { for (int k = 0; k < 7; k++) { int zz = z - 3 + k; zc[k] = zz < 0 ? 0 : zz > g.nz - 1 ? g.nz - 1 : zz; } }
 
-// The other transport layout: one block of 256 threads per (K levels, field, run), each level in turn whole in LDS
-// (periodic in x and y, so the level is its own halo), each thread a few nodes. The flux through the face between two
-// levels of a block is taken once: the top face of a level is the bottom face of the next (kept per node in LDS, each
-// node by the thread that owns it). The divisions of the tendency are products with per-level reciprocals.
+// The level transport layout: one block of 256 threads per (K levels, field, run), each level in turn whole in LDS
+// (periodic in x and y, so the level is its own halo), each thread a few nodes. A WENO field's face fluxes are taken
+// once a face, relative to the face's western / southern / lower node, and kept in LDS (the top face of a level is the
+// bottom face of the next); a linear field's node takes its six faces itself, in float64. The stage update is float64
+// from the float32 start, rounded once.
 __host__ __device__ inline size_t stageLevelLdsBytes(const Geo &g, size_t rb) { return ((g.layer * 4 + 15) / 16) * 16 + 3 * size_t(g.layer) * rb; }
 template <class R> __global__ __launch_bounds__(TPB) void k_stage_level(Geo g, Bufs<R> b, int stage, double dt, int nr, int nc, int fbase, int xcd, int K) {
   const int zBlocks = (g.nz + K - 1) / K;
@@ -561,64 +465,72 @@ template <class R> __global__ __launch_bounds__(TPB) void k_stage_level(Geo g, B
   float *ql = reinterpret_cast<float *>(smem);
   R *fxl = reinterpret_cast<R *>(smem + ((layer * sizeof(float) + 15) / 16) * 16), *fyl = fxl + layer, *fzl = fyl + layer;
   __shared__ double red[TPB / 64][3];
-  // Departures from the level's base profile for u, v and theta, as in k_stage.
+  // u, v and theta: departures from the level's base profile (their vertical differences; the profile's own tendency
+  // added exactly).
   const int lvl = f == F_U ? L_UENV : f == F_V ? L_VENV : f == F_THETA ? L_THETA : -1;
   const double *B = lvl >= 0 ? b.lv(slot, lvl, g) : nullptr;
   const bool mono = monotoneField(f), isW = f == F_W;
   const int cf = conservedIndex(f);
   if (zeroLevels(b, g, slot, f, z0 - stageReach(stage), z1 - 1 + stageReach(stage))) { zeroStage(b, g, slot, f, z0, z1, stage, 0, cf, red); return; }
   const Div dNx(nx);
-  const R idx = recip(R(g.dx)), idy = recip(R(g.dy)), frac = R((stage == 0 ? 1.0 / 3 : stage == 1 ? 1.0 / 2 : 1.0) * dt);
+  const double fracd = (stage == 0 ? 1.0 / 3 : stage == 1 ? 1.0 / 2 : 1.0) * dt;
   const double *rho = b.lv(slot, L_RHO, g);
-  const R *mz = b.m(slot, 2, g), *div = b.m(slot, 3, g);
+  const double *mz = b.m(slot, 2, g);
   double fill[3] = {0, 0, 0};
   for (int z = z0; z < z1; z++) {
-    const double Bz = B ? B[z] : 0;
     const float *qz = q + size_t(z) * layer;
-    for (int c = threadIdx.x; c < layer; c += TPB) ql[c] = B ? float(double(qz[c]) - Bz) : qz[c];
+    for (int c = threadIdx.x; c < layer; c += TPB) ql[c] = qz[c];
     __syncthreads();
-    const R *mx = b.m(slot, 0, g) + size_t(z) * layer, *my = b.m(slot, 1, g) + size_t(z) * layer;
-    for (int c = threadIdx.x; c < layer; c += TPB) {
-      const int y = dNx.q(c), x = c - y * nx, row = y * nx;
-      auto X = [&](int o) { int xx = x + o; xx += xx < 0 ? nx : 0; xx -= xx >= nx ? nx : 0; return R(ql[row + xx]); };
-      auto Y = [&](int o) { int yy = y + o; yy += yy < 0 ? ny : 0; yy -= yy >= ny ? ny : 0; return R(ql[yy * nx + x]); };
-      fxl[c] = faceFlux(mono, mx[c], X(-2), X(-1), X(0), X(1), X(2), X(3));
-      fyl[c] = faceFlux(mono, my[c], Y(-2), Y(-1), Y(0), Y(1), Y(2), Y(3));
+    const double *mx = b.m(slot, 0, g) + size_t(z) * layer, *my = b.m(slot, 1, g) + size_t(z) * layer;
+    if (mono) {
+      for (int c = threadIdx.x; c < layer; c += TPB) {
+        const int y = dNx.q(c), x = c - y * nx, row = y * nx;
+        auto X = [&](int o) { int xx = x + o; xx += xx < 0 ? nx : 0; xx -= xx >= nx ? nx : 0; return R(ql[row + xx]); };
+        auto Y = [&](int o) { int yy = y + o; yy += yy < 0 ? ny : 0; yy -= yy >= ny ? ny : 0; return R(ql[yy * nx + x]); };
+        fxl[c] = relWeno(R(mx[c]), X(-2), X(-1), X(0), X(1), X(2), X(3));
+        fyl[c] = relWeno(R(my[c]), Y(-2), Y(-1), Y(0), Y(1), Y(2), Y(3));
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    const R ihz = R(1.0 / b.hz[z]), irz = R(1.0 / rho[z]), ihzrz = R(1.0 / (b.hz[z] * rho[z]));
+    const double ihzd = 1.0 / b.hz[z], irzd = 1.0 / rho[z], ihzrz = 1.0 / (b.hz[z] * rho[z]);
+    const R idx = recip(R(g.dx)), idy = recip(R(g.dy)), ihz = R(ihzd), irz = R(irzd);
     double thvEnv = 0, ithv = 0;
     if (isW) { const double te = b.lv(slot, L_THETA, g)[z], qe = b.lv(slot, L_Q, g)[z]; thvEnv = te * (1 + .61 * qe); ithv = 1.0 / thvEnv; }
     int zc[7];
-    double Bc[7];
-    for (int k = 0; k < 7; k++) { int zz = z - 3 + k; zz = zz < 0 ? 0 : zz > nz - 1 ? nz - 1 : zz; zc[k] = zz * layer; Bc[k] = B ? B[zz] : 0; }
-    R pt[2] = {0, 0}, pb[2] = {0, 0};
-    if (B) {
-      pt[0] = R(upwind5d(Bc[1], Bc[2], Bc[3], Bc[4], Bc[5]) - Bz); pt[1] = R(upwind5d(Bc[6], Bc[5], Bc[4], Bc[3], Bc[2]) - Bz);
-      pb[0] = R(upwind5d(Bc[0], Bc[1], Bc[2], Bc[3], Bc[4]) - Bz); pb[1] = R(upwind5d(Bc[5], Bc[4], Bc[3], Bc[2], Bc[1]) - Bz);
-    }
+    columnLevels(g, z, zc);
+    double Bc[7], Bd[7];
+    for (int k = 0; k < 7; k++) { Bc[k] = B ? B[zc[k]] : 0; Bd[k] = B ? B[zc[k]] - B[z] : 0; }
     const double wz = b.lv(slot, L_LW, g)[z];
     for (int c = threadIdx.x; c < layer; c += TPB) {
-      const int y = dNx.q(c), x = c - y * nx;
-      const int i = z * layer + c;
-      R col[7];
-      for (int k = 0; k < 7; k++) col[k] = k == 3 ? R(ql[c]) : B ? R(double(q[zc[k] + c]) - Bc[k]) : R(q[zc[k] + c]);
-      const R mzt = z == nz - 1 ? R(0) : mz[i], mzb = z == 0 ? R(0) : mz[i - layer];
-      const R fzt = z == nz - 1 ? R(0) : faceFlux(mono, mzt, col[1], col[2], col[3], col[4], col[5], col[6]);
-      const R fzb = z == 0 ? R(0) : z > z0 ? fzl[c] : faceFlux(mono, mzb, col[0], col[1], col[2], col[3], col[4], col[5]);
-      fzl[c] = fzt;
-      const int xm = x == 0 ? nx - 1 : x - 1, ym = y == 0 ? ny - 1 : y - 1;
-      const R flux = (fxl[c] - fxl[y * nx + xm]) * idx + (fyl[c] - fyl[ym * nx + x]) * idy + (fzt - fzb) * ihz;
-      R tend = -(flux - R(ql[c]) * div[i]) * irz;
-      if (B) tend -= (mzt * pt[mzt >= 0 ? 0 : 1] - mzb * pb[mzb >= 0 ? 0 : 1]) * ihzrz;
-      if (isW) {
-        const float th = b.set(inSet, slot, F_THETA, g)[i], qv = b.set(inSet, slot, F_Q, g)[i];
-        const R loaded = R(b.set(inSet, slot, F_CLOUD, g)[i]) + R(b.set(inSet, slot, F_ICE, g)[i])
-          + (R(b.set(inSet, slot, F_RAIN, g)[i]) + R(b.set(inSet, slot, F_SNOW, g)[i]) + R(b.set(inSet, slot, F_GRAUPEL, g)[i]) + R(b.set(inSet, slot, F_HAIL, g)[i]));
-        const double dthv = (double(th) * (1 + .61 * double(qv)) - thvEnv) * ithv;
-        tend += R(G) * (R(dthv) - loaded);
+      const int y = dNx.q(c), x = c - y * nx, row = y * nx;
+      const size_t i = size_t(z) * layer + c;
+      const int cw = row + (x == 0 ? nx - 1 : x - 1), cs = (y == 0 ? ny - 1 : y - 1) * nx + x;
+      const double mzt = z == nz - 1 ? 0.0 : mz[i], mzb = z == 0 ? 0.0 : mz[i - layer];
+      float v[7];
+      for (int k = 0; k < 7; k++) v[k] = k == 3 ? ql[c] : q[size_t(zc[k]) * layer + c];
+      double tendd;
+      if (mono) {
+        const R fzt = z == nz - 1 ? R(0) : relWeno(R(mzt), R(v[1]), R(v[2]), R(v[3]), R(v[4]), R(v[5]), R(v[6]));
+        const R fzb = z == 0 ? R(0) : z > z0 ? fzl[c] : relWeno(R(mzb), R(v[0]), R(v[1]), R(v[2]), R(v[3]), R(v[4]), R(v[5]));
+        fzl[c] = fzt;
+        const R q0 = v[3];
+        const R fluxX = fxl[c] - (fxl[cw] - R(mx[cw]) * (q0 - R(ql[cw])));
+        const R fluxY = fyl[c] - (fyl[cs] - R(my[cs]) * (q0 - R(ql[cs])));
+        const R fluxZ = fzt - (fzb - R(mzb) * (q0 - R(v[2])));
+        tendd = double(-(fluxX * idx + fluxY * idy + fluxZ * ihz) * irz);
+      } else {
+        double Dx[7], Dy[7], Dz[7];
+        const double q0 = v[3];
+        for (int k = 0; k < 7; k++) {
+          int xx = x + k - 3; xx += xx < 0 ? nx : 0; xx -= xx >= nx ? nx : 0;
+          int yy = y + k - 3; yy += yy < 0 ? ny : 0; yy -= yy >= ny ? ny : 0;
+          Dx[k] = double(ql[row + xx]) - q0; Dy[k] = double(ql[yy * nx + x]) - q0; Dz[k] = (double(v[k]) - q0) - Bd[k];
+        }
+        tendd = -(linearAxis(mx[c], mx[cw], Dx) / g.dx + linearAxis(my[c], my[cs], Dy) / g.dy + linearAxis(mzt, mzb, Dz) * ihzd) * irzd;
+        if (B) tendd -= baseTendency(Bc, mzt, mzb, ihzrz);
+        if (isW) tendd += buoyancy(b, g, inSet, slot, i, thvEnv, ithv);
       }
-      float val = float(R(start[i]) + frac * tend);
+      float val = float(double(start[i]) + fracd * tendd);
       if (isW && stage < 2 && (z == 0 || z == nz - 1)) val = 0;
       out[i] = val;
       if (stage == 2) {
@@ -650,11 +562,6 @@ __host__ __device__ inline bool planeFits(const Geo &g, size_t rb, size_t ldsMax
 // Element i of an array whose base is uniform over the block: a 32-bit offset from a base in scalar registers.
 template <class T> __device__ inline T at(const T *base, int i) { return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + unsigned(i) * unsigned(sizeof(T))); }
 template <class T> __device__ inline void put(T *base, int i, T v) { *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + unsigned(i) * unsigned(sizeof(T))) = v; }
-template <class R, bool MONO> __device__ inline R faceFluxT(R m, R qm2, R qm1, R q0, R qp1, R qp2, R qp3) {
-  const bool up = m >= 0;
-  const R a = up ? qm2 : qp3, b_ = up ? qm1 : qp2, c = up ? q0 : qp1, d = up ? qp1 : q0, e = up ? qp2 : qm1;
-  return m * (MONO ? wenoFace(a, b_, c, d, e) : upwindFace(a, b_, c, d, e));
-}
 // Levels z0 .. z1-1 of field f of a run. BASE: u, v or theta, transported as departures from the base profile.
 // cf: the conserved index of the field (-1: none); each level's positivity-fix sums go to their part of the run's.
 template <class R, bool BASE, bool MONO> __device__ void stagePlane(const Geo &g, const Bufs<R> &b, int slot, int f, int z0, int z1, int stage, double dt, char *smem,
@@ -670,7 +577,7 @@ template <class R, bool BASE, bool MONO> __device__ void stagePlane(const Geo &g
   R *fzl = reinterpret_cast<R *>(reinterpret_cast<char *>(fyl) + align16i(int(nx * (ny + 1) * sizeof(R))));
   const double *B = BASE ? b.lv(slot, f == F_U ? L_UENV : f == F_V ? L_VENV : L_THETA, g) : nullptr;
   const bool isW = !BASE && !MONO && f == F_W;
-  const R idx = recip(R(g.dx)), idy = recip(R(g.dy)), frac = R((stage == 0 ? 1.0 / 3 : stage == 1 ? 1.0 / 2 : 1.0) * dt);
+  const double fracd = (stage == 0 ? 1.0 / 3 : stage == 1 ? 1.0 / 2 : 1.0) * dt;
   const double *rho = b.lv(slot, L_RHO, g);
   // This thread's first node and place in the plane, and the (x, y) steps between its nodes and places.
   const Div dNx(nx), dW(W);
@@ -679,8 +586,8 @@ template <class R, bool BASE, bool MONO> __device__ void stagePlane(const Geo &g
   for (int z = z0; z < z1; z++) {
     const size_t lz = size_t(z) * layer;
     const float *qz = q + lz;
-    const double Bz = BASE ? B[z] : 0;
-    // Place (px, py) of the plane holds node ((px - 3) mod nx, (py - 3) mod ny).
+    // Place (px, py) of the plane holds node ((px - 3) mod nx, (py - 3) mod ny): the values themselves (a base
+    // profile is constant over a level, so the horizontal differences of departures are those of the values).
     {
       int px = px00, py = py00;
 #pragma unroll 4
@@ -688,19 +595,19 @@ template <class R, bool BASE, bool MONO> __device__ void stagePlane(const Geo &g
         int xs = px - 3, ys = py - 3;
         xs += xs < 0 ? nx : 0; xs -= xs >= nx ? nx : 0;
         ys += ys < 0 ? ny : 0; ys -= ys >= ny ? ny : 0;
-        const float v = at(qz, ys * nx + xs);
-        ql[p] = BASE ? float(double(v) - Bz) : v;
+        ql[p] = at(qz, ys * nx + xs);
         px += spx; py += spy; if (px >= W) { px -= W; py++; }
       }
     }
     __syncthreads();
-    {
-      const R *mxz = b.m(slot, 0, g) + lz, *myz = b.m(slot, 1, g) + lz;
+    const double *mxz = b.m(slot, 0, g) + lz, *myz = b.m(slot, 1, g) + lz;
+    if (MONO) {
+      // Each thread's node's eastern and northern faces, relative to the face's western / southern node.
       int x = x00, y = y00;
       for (int c = threadIdx.x; c < layer; c += T) {
         const float *s = ql + (y + 3) * W + x + 3;
-        const R fx = faceFluxT<R, MONO>(at(mxz, c), R(s[-2]), R(s[-1]), R(s[0]), R(s[1]), R(s[2]), R(s[3]));
-        const R fy = faceFluxT<R, MONO>(at(myz, c), R(s[-2 * W]), R(s[-W]), R(s[0]), R(s[W]), R(s[2 * W]), R(s[3 * W]));
+        const R fx = relWeno(R(at(mxz, c)), R(s[-2]), R(s[-1]), R(s[0]), R(s[1]), R(s[2]), R(s[3]));
+        const R fy = relWeno(R(at(myz, c)), R(s[-2 * W]), R(s[-W]), R(s[0]), R(s[W]), R(s[2 * W]), R(s[3 * W]));
         R *ex = fxl + y * X1 + x + 1;  // face x + 1/2; the row's first slot is the face west of x = 0
         ex[0] = fx;
         if (x == nx - 1) ex[-nx] = fx;
@@ -708,49 +615,58 @@ template <class R, bool BASE, bool MONO> __device__ void stagePlane(const Geo &g
         if (y == ny - 1) fyl[x] = fy;
         x += sx; y += sy; if (x >= nx) { x -= nx; y++; }
       }
+      __syncthreads();
     }
-    __syncthreads();
-    const R ihz = R(1.0 / b.hz[z]), irz = R(1.0 / rho[z]), ihzrz = R(1.0 / (b.hz[z] * rho[z]));
+    const double ihzd = 1.0 / b.hz[z], irzd = 1.0 / rho[z], ihzrz = 1.0 / (b.hz[z] * rho[z]);
+    const R idx = recip(R(g.dx)), idy = recip(R(g.dy)), ihz = R(ihzd), irz = R(irzd);
     double thvEnv = 0, ithv = 0;
     if (isW) { const double te = b.lv(slot, L_THETA, g)[z], qe = b.lv(slot, L_Q, g)[z]; thvEnv = te * (1 + .61 * qe); ithv = 1.0 / thvEnv; }
+    int zc[7];
+    columnLevels(g, z, zc);
     const float *qc[7];
-    double Bc[7];
-    for (int k = 0; k < 7; k++) { int zz = z - 3 + k; zz = zz < 0 ? 0 : zz > nz - 1 ? nz - 1 : zz; qc[k] = q + size_t(zz) * layer; Bc[k] = BASE ? B[zz] : 0; }
-    R pt[2] = {0, 0}, pb[2] = {0, 0};
-    if (BASE) {
-      pt[0] = R(upwind5d(Bc[1], Bc[2], Bc[3], Bc[4], Bc[5]) - Bz); pt[1] = R(upwind5d(Bc[6], Bc[5], Bc[4], Bc[3], Bc[2]) - Bz);
-      pb[0] = R(upwind5d(Bc[0], Bc[1], Bc[2], Bc[3], Bc[4]) - Bz); pb[1] = R(upwind5d(Bc[5], Bc[4], Bc[3], Bc[2], Bc[1]) - Bz);
-    }
+    double Bc[7], Bd[7];
+    for (int k = 0; k < 7; k++) { qc[k] = q + size_t(zc[k]) * layer; Bc[k] = BASE ? B[zc[k]] : 0; Bd[k] = BASE ? B[zc[k]] - B[z] : 0; }
     const double wz = b.lv(slot, L_LW, g)[z];
-    const R *mzz = b.m(slot, 2, g) + lz, *divz = b.m(slot, 3, g) + lz;
-    const R *mzb_ = z > 0 ? mzz - layer : mzz;
+    const double *mzz = b.m(slot, 2, g) + lz;
+    const double *mzb_ = z > 0 ? mzz - layer : mzz;
     const float *startz = start + lz;
     float *outz = out + lz;
-    auto colAt = [&](int k, int c) { const float v = at(qc[k], c); return BASE ? R(double(v) - Bc[k]) : R(v); };
     int x = x00, y = y00;
     for (int c = threadIdx.x; c < layer; c += T) {
-      const R q0 = R(ql[(y + 3) * W + x + 3]);
-      const R c1 = colAt(1, c), c2 = colAt(2, c), c4 = colAt(4, c), c5 = colAt(5, c), c6 = colAt(6, c);
-      const R mzt = z == nz - 1 ? R(0) : at(mzz, c), mzb = z == 0 ? R(0) : at(mzb_, c);
-      const R fzt = z == nz - 1 ? R(0) : faceFluxT<R, MONO>(mzt, c1, c2, q0, c4, c5, c6);
-      R fzb;
-      if (z == 0) fzb = 0;
-      else if (z > z0) fzb = fzl[c];
-      else fzb = faceFluxT<R, MONO>(mzb, colAt(0, c), c1, c2, q0, c4, c5);
-      fzl[c] = fzt;
-      const R *fxw = fxl + y * X1 + x, *fys = fyl + y * nx + x;
-      const R flux = (fxw[1] - fxw[0]) * idx + (fys[nx] - fys[0]) * idy + (fzt - fzb) * ihz;
-      R tend = -(flux - q0 * at(divz, c)) * irz;
-      if (BASE) tend -= (mzt * pt[mzt >= 0 ? 0 : 1] - mzb * pb[mzb >= 0 ? 0 : 1]) * ihzrz;
-      if (isW) {
-        const int i = int(lz) + c;
-        const float th = b.set(inSet, slot, F_THETA, g)[i], qv = b.set(inSet, slot, F_Q, g)[i];
-        const R loaded = R(b.set(inSet, slot, F_CLOUD, g)[i]) + R(b.set(inSet, slot, F_ICE, g)[i])
-          + (R(b.set(inSet, slot, F_RAIN, g)[i]) + R(b.set(inSet, slot, F_SNOW, g)[i]) + R(b.set(inSet, slot, F_GRAUPEL, g)[i]) + R(b.set(inSet, slot, F_HAIL, g)[i]));
-        const double dthv = (double(th) * (1 + .61 * double(qv)) - thvEnv) * ithv;
-        tend += R(G) * (R(dthv) - loaded);
+      const float *s = ql + (y + 3) * W + x + 3;
+      float v[7];
+      for (int k = 0; k < 7; k++) v[k] = k == 3 ? s[0] : at(qc[k], c);
+      const double mzt = z == nz - 1 ? 0.0 : at(mzz, c), mzb = z == 0 ? 0.0 : at(mzb_, c);
+      const int cw = x == 0 ? c + nx - 1 : c - 1, cs = y == 0 ? c + layer - nx : c - nx;
+      double tendd;
+      if (MONO) {
+        const R fzt = z == nz - 1 ? R(0) : relWeno(R(mzt), R(v[1]), R(v[2]), R(v[3]), R(v[4]), R(v[5]), R(v[6]));
+        R fzb;  // the bottom face relative to the node below
+        if (z == 0) fzb = 0;
+        else if (z > z0) fzb = fzl[c];
+        else fzb = relWeno(R(mzb), R(v[0]), R(v[1]), R(v[2]), R(v[3]), R(v[4]), R(v[5]));
+        fzl[c] = fzt;
+        // The western, southern and bottom faces relative to this node: minus m (q - q_neighbour).
+        const R q0 = v[3];
+        const R *fxw = fxl + y * X1 + x, *fys = fyl + y * nx + x;
+        const R fluxX = fxw[1] - (fxw[0] - R(at(mxz, cw)) * (q0 - R(s[-1])));
+        const R fluxY = fys[nx] - (fys[0] - R(at(myz, cs)) * (q0 - R(s[-W])));
+        const R fluxZ = fzt - (fzb - R(mzb) * (q0 - R(v[2])));
+        tendd = double(-(fluxX * idx + fluxY * idy + fluxZ * ihz) * irz);
+      } else {
+        // A linear field's node takes its six faces itself, in float64, relative to its own value.
+        double Dx[7], Dy[7], Dz[7];
+        const double q0 = v[3];
+        for (int k = 0; k < 7; k++) {
+          Dx[k] = double(s[k - 3]) - q0; Dy[k] = double(s[(k - 3) * W]) - q0;
+          Dz[k] = BASE ? (double(v[k]) - q0) - Bd[k] : double(v[k]) - q0;
+        }
+        tendd = -(linearAxis(at(mxz, c), at(mxz, cw), Dx) / g.dx + linearAxis(at(myz, c), at(myz, cs), Dy) / g.dy + linearAxis(mzt, mzb, Dz) * ihzd) * irzd;
+        if (BASE) tendd -= baseTendency(Bc, mzt, mzb, ihzrz);
+        if (isW) tendd += buoyancy(b, g, inSet, slot, lz + c, thvEnv, ithv);
       }
-      float val = float(R(at(startz, c)) + frac * tend);
+      // The stage update in float64 from the float32 start, rounded once.
+      float val = float(double(at(startz, c)) + fracd * tendd);
       if (isW && stage < 2 && (z == 0 || z == nz - 1)) val = 0;
       put(outz, c, val);
       if (MONO && stage == 2) {
@@ -809,18 +725,20 @@ template <class R> __global__ __launch_bounds__(TPB) void k_pcorrect(Geo g, Bufs
     const double *p = b.arr(b.press, slot, g);
     int cl = g.layer, nz = g.nz;
     int lo = (k > 0 ? k - 1 : 0) * cl, hi = (k < nz - 2 ? k : nz - 2) * cl;
-    R wLo = k > 0 ? R(b.dzs[k - 1] / b.hz[k]) : R(0), wHi = k < nz - 1 ? R(b.dzs[k] / b.hz[k]) : R(0);
+    // The gradient and the correction in float64 from the float32 winds, rounded once (the correction of w is about as
+    // large as w itself, so its float32 product would add an ulp a projection).
+    double wLo = k > 0 ? b.dzs[k - 1] / b.hz[k] : 0.0, wHi = k < nz - 1 ? b.dzs[k] / b.hz[k] : 0.0;
     int jm = (j == 0 ? g.ny - 1 : j - 1) * g.nx, j0 = j * g.nx, im = i == 0 ? g.nx - 1 : i - 1;
     auto P = [&](int ii, int jj, int base) { return p[base + ii + jj]; };  // float64: differences before rounding
     // Differences of neighbouring pressures first (as for the divergence).
-    R gx = (wLo * R((P(i, jm, lo) - P(im, jm, lo)) + (P(i, j0, lo) - P(im, j0, lo))) + wHi * R((P(i, jm, hi) - P(im, jm, hi)) + (P(i, j0, hi) - P(im, j0, hi)))) / (4 * R(g.dx));
-    R gy = (wLo * R((P(im, j0, lo) - P(im, jm, lo)) + (P(i, j0, lo) - P(i, jm, lo))) + wHi * R((P(im, j0, hi) - P(im, jm, hi)) + (P(i, j0, hi) - P(i, jm, hi)))) / (4 * R(g.dy));
+    double gx = (wLo * ((P(i, jm, lo) - P(im, jm, lo)) + (P(i, j0, lo) - P(im, j0, lo))) + wHi * ((P(i, jm, hi) - P(im, jm, hi)) + (P(i, j0, hi) - P(im, j0, hi)))) / (4 * g.dx);
+    double gy = (wLo * ((P(im, j0, lo) - P(im, jm, lo)) + (P(i, j0, lo) - P(i, jm, lo))) + wHi * ((P(im, j0, hi) - P(im, jm, hi)) + (P(i, j0, hi) - P(i, jm, hi)))) / (4 * g.dy);
     float *u = b.set(set, slot, F_U, g), *v = b.set(set, slot, F_V, g), *w = b.set(set, slot, F_W, g);
-    float un = float(R(u[node]) - R(dt) * gx), vn = float(R(v[node]) - R(dt) * gy), wn = w[node];
+    float un = float(double(u[node]) - dt * gx), vn = float(double(v[node]) - dt * gy), wn = w[node];
     if (k > 0 && k < nz - 1) {
       int below = (k - 1) * cl, above = k * cl;
-      R gz = R(((P(im, jm, above) - P(im, jm, below)) + (P(i, jm, above) - P(i, jm, below))) + ((P(im, j0, above) - P(im, j0, below)) + (P(i, j0, above) - P(i, j0, below)))) / (4 * R(b.hz[k]));
-      wn = float(R(wn) - R(dt) * gz);
+      double gz = (((P(im, jm, above) - P(im, jm, below)) + (P(i, jm, above) - P(i, jm, below))) + ((P(im, j0, above) - P(im, j0, below)) + (P(i, j0, above) - P(i, j0, below)))) / (4 * b.hz[k]);
+      wn = float(double(wn) - dt * gz);
     }
     if (final) {
       if (un < -120.f || un > 120.f) { un = un < -120.f ? -120.f : 120.f; clipped++; }
@@ -941,24 +859,26 @@ template <class R> __global__ __launch_bounds__(TPB) void k_viscosity(Geo g, Buf
   double local = z == 0 ? b.dzs[0] : z == nz - 1 ? b.dzs[nz - 2] : b.hz[z], len2 = cs * cbrt(g.dx * g.dy * local); len2 *= len2;
   double mn = fmin(fmin(g.dx, g.dy), fmin(z > 0 ? b.dzs[z - 1] : 1e300, z < nz - 1 ? b.dzs[z] : 1e300)), cap = .05 * mn * mn / dt;
   int up = z < nz - 1 ? layer : 0, down = z > 0 ? -layer : 0;
-  R span = R(b.zs[z + (up ? 1 : 0)] - b.zs[z - (down ? 1 : 0)]);
+  // float64 arithmetic from the float32 fields (the node's K is a small difference of large terms where the air is
+  // near neutral), rounded to the pool's type once.
+  const double span = b.zs[z + (up ? 1 : 0)] - b.zs[z - (down ? 1 : 0)];
   int row = y * nx + z * layer;
   int e = (x + 1 == nx ? 0 : x + 1) + row, wst = (x == 0 ? nx - 1 : x - 1) + row;
   int n = x + (y + 1 == ny ? 0 : y + 1) * nx + z * layer, s = x + (y == 0 ? ny - 1 : y - 1) * nx + z * layer;
-  R hx = R(2 * g.dx), hy = R(2 * g.dy);
-  R dudx = (R(u[e]) - R(u[wst])) / hx, dvdy = (R(v[n]) - R(v[s])) / hy, dwdz = (R(w[i + up]) - R(w[i + down])) / span;
-  R dudy = (R(u[n]) - R(u[s])) / hy, dvdx = (R(v[e]) - R(v[wst])) / hx;
-  R dudz = (R(u[i + up]) - R(u[i + down])) / span, dwdx = (R(w[e]) - R(w[wst])) / hx;
-  R dvdz = (R(v[i + up]) - R(v[i + down])) / span, dwdy = (R(w[n]) - R(w[s])) / hy;
-  R s2 = 2 * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + (dudy + dvdx) * (dudy + dvdx) + (dudz + dwdx) * (dudz + dwdx) + (dvdz + dwdy) * (dvdz + dwdy);
-  R n2 = R(G / b.lv(slot, L_THETA, g)[z]) * (R(theta[i + up]) - R(theta[i + down])) / span;
+  const double hx = 2 * g.dx, hy = 2 * g.dy;
+  double dudx = (double(u[e]) - double(u[wst])) / hx, dvdy = (double(v[n]) - double(v[s])) / hy, dwdz = (double(w[i + up]) - double(w[i + down])) / span;
+  double dudy = (double(u[n]) - double(u[s])) / hy, dvdx = (double(v[e]) - double(v[wst])) / hx;
+  double dudz = (double(u[i + up]) - double(u[i + down])) / span, dwdx = (double(w[e]) - double(w[wst])) / hx;
+  double dvdz = (double(v[i + up]) - double(v[i + down])) / span, dwdy = (double(w[n]) - double(w[s])) / hy;
+  double s2 = 2 * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + (dudy + dvdx) * (dudy + dvdx) + (dudz + dwdx) * (dudz + dwdx) + (dvdz + dwdy) * (dvdz + dwdy);
+  double n2 = G / b.lv(slot, L_THETA, g)[z] * (double(theta[i + up]) - double(theta[i + down])) / span;
   if (b.f(slot, F_CONDENSATE, g)[i] > 1e-5f) {
-    auto te = [&](int k, int level) { return R(theta[k]) * mexp(R(LV) * R(q[k]) / (R(CP) * R(theta[k]) * R(exner[level]))); };
-    n2 = R(G) / te(i, z) * (te(i + up, z + (up ? 1 : 0)) - te(i + down, z - (down ? 1 : 0))) / span;
+    auto te = [&](int k, int level) { return double(theta[k]) * exp(LV * double(q[k]) / (CP * double(theta[k]) * exner[level])); };
+    n2 = G / te(i, z) * (te(i + up, z + (up ? 1 : 0)) - te(i + down, z - (down ? 1 : 0))) / span;
   }
-  R rate = msqrt(mmax(R(0), s2 - n2 * 3));
-  b.arr(b.km, slot, g)[i] = mmin(R(cap), R(len2) * rate);
-  b.arr(b.kh, slot, g)[i] = mmin(R(capH), R(lenH2) * rate);
+  const double rate = sqrt(fmax(0.0, s2 - n2 / (1.0 / 3)));  // turbulent Prandtl number 1/3
+  b.arr(b.km, slot, g)[i] = R(fmin(cap, len2 * rate));
+  b.arr(b.kh, slot, g)[i] = R(fmin(capH, lenH2 * rate));
 }
 
 // ---- subgrid mixing of the 8 mixed fields, surface drag and rigid walls (turbulence.ts:143-185 mix, model.ts:1161-1195)
@@ -975,14 +895,15 @@ template <class R> __global__ __launch_bounds__(TPB) void k_mix(Geo g, Bufs<R> b
   int n = x + (y + 1 == ny ? 0 : y + 1) * nx + z * layer, s = x + (y == 0 ? ny - 1 : y - 1) * nx + z * layer;
   const R *km = b.arr(b.km, slot, g), *kh = b.arr(b.kh, slot, g);
   const double *rho = b.lv(slot, L_RHO, g);
-  const R dx = R(g.dx), dy = R(g.dy);
-  R khi = kh[i], kmi = km[i];
-  R kx0 = (khi + kh[e]) / 2, kx1 = (kh[wst] + khi) / 2, ky0 = (khi + kh[n]) / 2, ky1 = (kh[s] + khi) / 2;
-  R kzu = 0, kzd = 0, cu = 0, cd = 0;
-  if (z < nz - 1) { kzu = (kmi + km[i + layer]) / 2 * R((rho[z] + rho[z + 1]) / 2) / R(b.dzs[z]); cu = R(1 / (rho[z] * b.hz[z])); }
-  if (z > 0) { kzd = (km[i - layer] + kmi) / 2 * R((rho[z - 1] + rho[z]) / 2) / R(b.dzs[z - 1]); cd = R(1 / (rho[z] * b.hz[z])); }
+  // float64 arithmetic from the float32 fields and the pool's K, each new value rounded once.
+  const double dx = g.dx, dy = g.dy;
+  const double khi = kh[i], kmi = km[i];
+  const double kx0 = (khi + double(kh[e])) / 2, kx1 = (double(kh[wst]) + khi) / 2, ky0 = (khi + double(kh[n])) / 2, ky1 = (double(kh[s]) + khi) / 2;
+  double kzu = 0, kzd = 0, cu = 0, cd = 0;
+  if (z < nz - 1) { kzu = (kmi + double(km[i + layer])) / 2 * ((rho[z] + rho[z + 1]) / 2) / b.dzs[z]; cu = 1 / (rho[z] * b.hz[z]); }
+  if (z > 0) { kzd = (double(km[i - layer]) + kmi) / 2 * ((rho[z - 1] + rho[z]) / 2) / b.dzs[z - 1]; cd = 1 / (rho[z] * b.hz[z]); }
   const int mixed[8] = {F_U, F_V, F_W, F_THETA, F_Q, F_CLOUD, F_ICE, F_ICENUMBER};
-  const R factors[8] = {1, 1, 1, 3, 3, 3, 3, 3};
+  const double factors[8] = {1, 1, 1, 3, 3, 3, 3, 3};
   float outv[8];
   for (int k = 0; k < 8; k++) {
     const float *a = b.f(slot, mixed[k], g);
@@ -990,12 +911,12 @@ template <class R> __global__ __launch_bounds__(TPB) void k_mix(Geo g, Bufs<R> b
     double base = 0, bu = 0, bd = 0;
     int lvl = mixed[k] == F_U ? L_UENV : mixed[k] == F_V ? L_VENV : mixed[k] == F_THETA ? L_THETA : mixed[k] == F_Q ? L_Q : -1;
     if (lvl >= 0) { const double *B = b.lv(slot, lvl, g); base = B[z]; bu = z < nz - 1 ? B[z + 1] : 0; bd = z > 0 ? B[z - 1] : 0; }
-    R here = R(double(a[i]) - base);
-    R t = (kx0 * (R(double(a[e]) - base) - here) - kx1 * (here - R(double(a[wst]) - base))) / (dx * dx)
-        + (ky0 * (R(double(a[n]) - base) - here) - ky1 * (here - R(double(a[s]) - base))) / (dy * dy);
-    if (z < nz - 1) t += kzu * (R(double(a[i + layer]) - bu) - here) * cu;
-    if (z > 0) t -= kzd * (here - R(double(a[i - layer]) - bd)) * cd;
-    outv[k] = float(R(a[i]) + R(dt) * factors[k] * t);
+    const double here = double(a[i]) - base;
+    double t = (kx0 * ((double(a[e]) - base) - here) - kx1 * (here - (double(a[wst]) - base))) / (dx * dx)
+             + (ky0 * ((double(a[n]) - base) - here) - ky1 * (here - (double(a[s]) - base))) / (dy * dy);
+    if (z < nz - 1) t += kzu * ((double(a[i + layer]) - bu) - here) * cu;
+    if (z > 0) t -= kzd * (here - (double(a[i - layer]) - bd)) * cd;
+    outv[k] = float(double(a[i]) + dt * factors[k] * t);
   }
   if (z == 0) {
     // Surface drag on the ground node: dV/dt = -C_D (|V| V - |V_env| V_env) / h0, on the wind over the ground.

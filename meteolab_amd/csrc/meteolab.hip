@@ -38,10 +38,10 @@ namespace {
 
 constexpr int MAX_POOLS = 4096, SLOT_BITS = 16;
 
-// The transport kernel's launch: layout 0 is k_stage (strips of a level, `groups` field groups per strip), layout 1 is
-// k_stage_level (`levels` whole levels of one field per block), layout 2 is k_stage_plane (as layout 1, each level in a
-// padded LDS plane); `xcd` keeps each run's blocks on one XCD.
-struct Tuning { int layout = 0, groups = 1, xcd = 0, levels = 1; float ms = 0; };
+// The transport kernel's launch: layout 1 is k_stage_level (`levels` whole levels of one field per block), layout 2 is
+// k_stage_plane (as layout 1, each level in a padded LDS plane); `xcd` keeps each run's blocks on one XCD. (Layout 0,
+// strips of a level, is not built; `groups` is kept for the reported shape and is 1.)
+struct Tuning { int layout = 1, groups = 1, xcd = 0, levels = 1; float ms = 0; };
 
 struct Batch {
   std::vector<int> runs;
@@ -137,7 +137,7 @@ bool radices(int n, int *out, int &count) {
 
 template <class R> Bufs<R> bufs(const Pool &p, const int *runs) {
   Bufs<R> b;
-  b.fields = (float *)p.fields; b.stx = (float *)p.stx; b.sty = (float *)p.sty; b.mass = (R *)p.mass; b.phi = (double *)p.phi;
+  b.fields = (float *)p.fields; b.stx = (float *)p.stx; b.sty = (float *)p.sty; b.mass = (double *)p.mass; b.phi = (double *)p.phi;
   b.cells = (double *)p.cells; b.press = (double *)p.press; b.spec = (double *)p.spec; b.mean0 = (double *)p.mean0; b.km = (R *)p.km; b.kh = (R *)p.kh;
   b.ephi = (double *)p.ephi; b.cols = (float *)p.cols; b.lev = (double *)p.lev; b.par = (double *)p.par; b.scal = (double *)p.scal;
   b.zs = (const double *)p.zs; b.dzs = (const double *)p.dzs; b.hz = (const double *)p.hz; b.sym = (const double *)p.sym;
@@ -179,12 +179,9 @@ template <class R> void launchStage(const Geo &g, const Bufs<R> &b, int nr, int 
   if (tu.layout == 1) {
     const unsigned zBlocks = unsigned((g.nz + tu.levels - 1) / tu.levels);
     hipLaunchKernelGGL(k_stage_level<R>, dim3(zBlocks * nc * nr), dim3(TPB), stageLevelLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, fbase, tu.xcd, tu.levels);
-  } else if (tu.layout == 2) {
+  } else {
     const unsigned zBlocks = unsigned((g.nz + tu.levels - 1) / tu.levels);
     hipLaunchKernelGGL(k_stage_plane<R>, dim3(zBlocks * nc * nr), dim3(TPB), stagePlaneLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, fbase, tu.xcd, tu.levels);
-  } else {
-    const StageTile st = stageTile(g);
-    hipLaunchKernelGGL(k_stage<R>, dim3(unsigned(st.tiles) * g.nz * nr * tu.groups), dim3(st.threads), stageLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, fbase, tu.groups, tu.xcd);
   }
 }
 
@@ -249,7 +246,7 @@ template <class R> void launchPart(const Pool &p, const int *list, int nr, doubl
     mark("k_faces");
     for (int stage = 0; stage < 3; stage++) {
       launchStage<R>(g, b, nr, split ? int(F_CHARGE_ICE) : nc, 0, stage, dt, s, tu);
-      mark(tu.layout == 2 ? "k_stage_plane" : tu.layout == 1 ? "k_stage_level" : "k_stage");
+      mark(tu.layout == 2 ? "k_stage_plane" : "k_stage_level");
       if (stage < 2) {
         hipLaunchKernelGGL(k_pdiv<R>, dim3(cb, nr), dim3(TPB), 0, s, g, b, stage + 1, dt);
         mark("k_pdiv");
@@ -271,13 +268,13 @@ template <class R> void launchPart(const Pool &p, const int *list, int nr, doubl
     mark("k_zero_flags");
     for (int stage = 0; stage < 3; stage++) {
       launchStage<R>(g, b, nr, N_CARRIED - F_CHARGE_ICE, F_CHARGE_ICE, stage, dt, s, tu);
-      mark(tu.layout == 2 ? "k_stage_plane" : tu.layout == 1 ? "k_stage_level" : "k_stage");
+      mark(tu.layout == 2 ? "k_stage_plane" : "k_stage_level");
     }
   }
   // endStep: positivity fix + physics (physicsStep), turbulence + drag (mixStep), projection (model.ts:1103-1113).
   if (part == PART_PHYSICS) {
     // The positivity-fix sums from the last transport stage's parts, in level order.
-    hipLaunchKernelGGL(k_fill_total<R>, dim3(nr), dim3(64), 0, s, g, b, tu.layout == 0 ? stageTile(g).tiles : 1);
+    hipLaunchKernelGGL(k_fill_total<R>, dim3(nr), dim3(64), 0, s, g, b, 1);
     mark("k_fill_total");
     hipLaunchKernelGGL((k_physics<R, double>), dim3(nb, nr), dim3(TPB), 0, s, g, b, dt, el);
     mark("k_physics");
@@ -376,12 +373,12 @@ struct Sizes {
 int groundBlocks(const Geo &g) { return std::max(int(blocks(g.layer, SED_COLS)), int(blocks(size_t(g.layer) * (g.nz - 2), 256))); }
 Sizes sizesOf(const Geo &g, size_t rb) {
   Sizes s;
-  s.fields = size_t(N_FIELDS) * g.n * 4; s.stage = size_t(N_CARRIED) * g.n * 4; s.mass = 4 * size_t(g.n) * rb; s.node = size_t(g.n) * rb; s.node8 = size_t(g.n) * 8;
+  s.fields = size_t(N_FIELDS) * g.n * 4; s.stage = size_t(N_CARRIED) * g.n * 4; s.mass = 3 * size_t(g.n) * 8; s.node = size_t(g.n) * rb; s.node8 = size_t(g.n) * 8;
   s.spec = size_t(g.nz) * g.nm * 2 * 8; s.mean0 = size_t(g.nz) * 8; s.cols = size_t(N_COLUMNS) * g.layer * 4;
   s.lev = size_t(N_LEVELS) * g.nz * 8; s.par = N_PARAMS * 8; s.scal = N_SCALARS * 8;
   s.lmark = size_t(N_LMARK) * g.n * 4; s.lnode = size_t(N_LNODE) * g.n * 8; s.chan = 2 * size_t(MAX_CHANNEL) * 4; s.cdep = size_t(MAX_CHANNEL) * 8;
   s.flog = size_t(FLASH_LOG) * FLASH_FIELDS * 8;
-  s.fillp = size_t(N_CONSERVED) * g.nz * stageTile(g).tiles * 3 * 8;
+  s.fillp = size_t(N_CONSERVED) * g.nz * 3 * 8;
   s.gpart = 2 * size_t(groundBlocks(g)) * 8;
   s.zflag = (size_t(N_CARRIED) * g.nz + 15) / 16 * 16;
   return s;
@@ -518,8 +515,7 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
   const Geo &g = p->g;
   p->cap = capacity; p->electrified = electrified ? 1 : 0; p->rb = c->precision == 64 ? 8 : 4;
   // LDS the kernels need must fit the device's block limit.
-  size_t stageLds = stageLdsBytes(g, p->rb);
-  if (stageLds > c->prop.sharedMemPerBlock || stageLevelLdsBytes(g, p->rb) > c->prop.sharedMemPerBlock ||
+  if (stageLevelLdsBytes(g, p->rb) > c->prop.sharedMemPerBlock ||
       specLdsBytes(g, 8) + 4096 > c->prop.sharedMemPerBlock) { delete p; return fail(c, ML_E_UNSUPPORTED, "grid too large for a level in LDS"); }
   Sizes s = sizesOf(g, p->rb);
   size_t cap = capacity;
@@ -530,7 +526,7 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
     {&p->scal, s.scal * cap}, {(void **)&p->diag, cap * DIAG_ROW * 8}, {(void **)&p->dlist, cap * sizeof(int)}, {&p->lrho, s.node * cap}, {&p->lmark, s.lmark * cap},
     {&p->lnode, s.lnode * cap}, {&p->chan, s.chan * cap}, {&p->cdep, s.cdep * cap}, {&p->flog, s.flog * cap},
     {&p->fillp, s.fillp * cap}, {&p->gpart, s.gpart * cap}, {&p->pmap, size_t(g.layer) * 4}, {&p->zflag, s.zflag * cap}};
-  p->ftiles = stageTile(g).tiles; p->gmax = groundBlocks(g);
+  p->ftiles = 1; p->gmax = groundBlocks(g);
   for (auto &a : allocs) {
     hipError_t e = hipMalloc(a.ptr, a.bytes);
     if (e == hipSuccess) e = hipMemset(*a.ptr, 0, a.bytes);
@@ -1098,10 +1094,11 @@ int ml_batch_set_tuning(ml_ctx *c, const int *runs, int count, const int *shape)
   if (int r = batchOf(c, runs, count, p, bt)) return r;
   const int nc = p->electrified ? N_CARRIED : F_COLD + 1;
   if (!shape || shape[0] < 0 || shape[0] > 2 || shape[1] < 1 || shape[1] > nc || shape[2] < 0 || shape[2] > 1 || shape[3] < 1 || shape[3] > p->g.nz)
-    return fail(c, ML_E_ARG, "set tuning: layout 0, 1 or 2, 1 <= field groups <= carried fields, xcd 0 or 1, 1 <= levels <= nz");
+    return fail(c, ML_E_ARG, "set tuning: layout 1 or 2, 1 <= field groups <= carried fields, xcd 0 or 1, 1 <= levels <= nz");
+  if (shape[0] == 0) return fail(c, ML_E_UNSUPPORTED, "set tuning: the strip layout (0) is not built; layouts 1 (levels) and 2 (planes)");
   if (shape[0] == 2 && !planeFits(p->g, p->rb, c->prop.sharedMemPerBlock)) return fail(c, ML_E_UNSUPPORTED, "set tuning: the plane layout needs nx, ny >= 6 and its LDS within a block's");
-  bt->tune.layout = shape[0]; bt->tune.groups = shape[0] == 0 ? shape[1] : 1; bt->tune.xcd = shape[2];
-  bt->tune.levels = shape[0] == 0 ? 1 : shape[3]; bt->tune.ms = 0;
+  bt->tune.layout = shape[0]; bt->tune.groups = 1; bt->tune.xcd = shape[2];
+  bt->tune.levels = shape[3]; bt->tune.ms = 0;
   // The captured steps hold the old launch shape: captured again at the next step.
   bt->dt = 0;
   return 0;

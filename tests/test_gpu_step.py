@@ -13,6 +13,12 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 
+def margin(worst, tol=TOL):
+    """The worst per-field relative L2, its field and the margin to the bound, printed by each parity test."""
+    k = max(worst, key=worst.get)
+    return f"worst {worst[k]:.2e} ({k}), bound {tol:g}, margin {tol / worst[k]:.1f}x"
+
+
 @pytest.fixture(scope="module")
 def ctx():
     c = M.Context(0, 32)
@@ -61,15 +67,14 @@ def test_parity_after_one_and_ten_steps(ctx, grid):
     # the large (80x64x50) one. The float32 rounding of the transport adds ~1e-6 a step to w's relative difference
     # while w is still ~0.2 m/s; on the large domain it reaches 1.25e-5 after ten steps (8e-6 on the standard).
     g = C.small_grid() if grid == "small" else C.STANDARD if grid == "standard" else C.LARGE
-    tol10 = 2e-5 if grid == "large" else TOL
     prof, run = C.profile(), C.run(g)
     o = OracleRun(g, prof, run)
     r = ctx.pool(g, 1).run(prof, run)
     o.step(1); r.step(1)
     w1 = compare(r, o)
     o.step(9); r.step(9)
-    w10 = compare(r, o, tol=tol10)
-    print(grid, "1 step", max(w1.values()), "10 steps", max(w10.values()))
+    w10 = compare(r, o)
+    print(grid, "after 1 step:", margin(w1), "| after 10 steps:", margin(w10))
     d, od = r.diagnostics(), o.diag()
     assert d["time"] == od["time"] == 30.0
     assert d["maxW"] == pytest.approx(od["maxW"], rel=1e-5)
@@ -119,8 +124,8 @@ def test_parity_in_a_mature_storm(ctx):
     o.step(1); r.step(1)
     w1 = compare(r, o)
     o.step(9); r.step(9)
-    w10 = compare(r, o, tol=1e-4)
-    print("mature 1 step", w1, "\n10 steps", w10)
+    w10 = compare(r, o)
+    print("mature storm after 1 step:", margin(w1), "| after 10 steps:", margin(w10))
     assert r.column("precipitation").max() == pytest.approx(o.column("precipitation").max(), rel=1e-4)
 
 
@@ -171,7 +176,7 @@ def mature_small():
 
 MARCH_OFF = pytest.mark.skip(reason="the column-march transport layout was taken out after a whole-step run with it faulted "
                                    "the GPU, cause not found on the CPU; set_batch_tuning rejects it")
-TRANSPORT_SHAPES = [("strip", 1, False, 1), ("strip", 4, True, 1), ("strip", 17, True, 1), ("level", 1, False, 1),
+TRANSPORT_SHAPES = [("level", 1, False, 1),
                     ("level", 1, True, 1), ("level", 1, True, 5), ("level", 1, False, 10), ("plane", 1, False, 1),
                     ("plane", 1, True, 2), ("plane", 1, True, 5), ("plane", 1, False, 10),
                     pytest.param("march", 1, True, 5, marks=MARCH_OFF), pytest.param("march", 1, False, 24, marks=MARCH_OFF),
@@ -236,5 +241,5 @@ def test_lightning_matches_the_oracle_flash_for_flash(ctx, mature_small):
     assert r.diagnostics()["firstFlash"] == ref.diag()["firstFlash"] == 1968.0
     log = r.flashes()
     assert len(log) == sum(counts[-1])
-    w = compare(r, ref, tol=1e-4)
-    print("after 50 steps and", counts[-1], "flashes:", {k: v for k, v in w.items() if k.startswith("charge")})
+    w = compare(r, ref)
+    print("after 50 steps and", counts[-1], "flashes:", margin(w))
