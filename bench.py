@@ -173,14 +173,12 @@ def apply_transport(a, runs):
 # ---- summaries of profiler runs of --table-only (rocprofv3 -f csv) ---------------------------------------------------
 
 def _csv_rows(directory, suffix):
-    paths = sorted(glob.glob(os.path.join(directory, "**", f"*{suffix}"), recursive=True))
+    """The rows of the newest *suffix file under a profiler output directory (one profiled process)."""
+    paths = glob.glob(os.path.join(directory, "**", f"*{suffix}"), recursive=True)
     if not paths:
         raise SystemExit(f"no *{suffix} under {directory}")
-    rows = []
-    for p in paths:
-        with open(p, newline="") as f:
-            rows += list(csv.DictReader(f))
-    return rows
+    with open(max(paths, key=os.path.getmtime), newline="") as f:
+        return list(csv.DictReader(f))
 
 
 def _grid(row):
@@ -191,13 +189,16 @@ def _grid(row):
 
 
 def _short(name):
-    return name.split("(")[0].replace("void ", "").strip()
+    """A kernel's name without its return type, namespaces and arguments (k_stage_plane<float>)."""
+    name = name.replace("(anonymous namespace)::", "").replace("void ", "").replace("ml::", "")
+    return name.split("(")[0].strip()
 
 
 def _window_dispatches(rows, kernel, steps):
     """The transport's dispatches of the measured window: the last 3 * steps at each grid size of the step's last launches
     (the stage launches of the fields before the charges and of the charges), in dispatch order."""
-    mine = sorted((r for r in rows if _short(r["Kernel_Name"]).startswith(kernel + "<")), key=lambda r: int(r["Dispatch_Id"]))
+    unique = {int(r["Dispatch_Id"]): r for r in rows if _short(r["Kernel_Name"]).startswith(kernel + "<")}  # a counter file has a row a counter
+    mine = [unique[d] for d in sorted(unique)]
     sizes = sorted({_grid(r) for r in mine[-6:]})
     chosen = []
     for s in sizes:

@@ -50,16 +50,21 @@ class Run(ctypes.Structure):
                 ("noise_top", ctypes.c_double)]
 
 
-_lib = None
+_libs = {}
 
 
 def lib():
     """Loads liboracle.so (built by ``make -C oracle``)."""
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
-        L = ctypes.CDLL(LIB_PATH)
+    return load(LIB_PATH)
+
+
+def load(path):
+    """An oracle library at `path` with its C ABI's signatures (the checks of tests/test_oracle_physics.py run on a
+    library built with a planted mistake this way)."""
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
+        L = ctypes.CDLL(path)
         vp, d, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
         fp, dp = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)
         L.orc_create.restype = vp
@@ -116,8 +121,45 @@ def lib():
         L.orc_mulberry32.restype = ctypes.c_uint32
         L.orc_mulberry32.argtypes = [ctypes.POINTER(ctypes.c_uint32)]
         L.orc_electric_solve.argtypes = [ctypes.POINTER(Grid), dp, dp]
-        _lib = L
-    return _lib
+        _physics_signatures(L)
+        _libs[path] = L
+    return _libs[path]
+
+
+def _physics_signatures(L):
+    """The column physics, turbulence, sedimentation and lightning pieces of oracle.h."""
+    vp, d, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+    fp, dp, ip = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    G = ctypes.POINTER(Grid)
+    sig = {
+        "orc_sat_adjust_mixed": (d, [fp, fp, fp, fp, d, d, fp]),
+        "orc_freeze_rain": (None, [fp, fp, fp, d]),
+        "orc_ice_processes": (d, [fp, fp, fp, fp, fp, d, d, d, d]),
+        "orc_settle_crystals": (None, [fp, fp]),
+        "orc_snow_processes": (d, [fp, fp, fp, fp, fp, fp, d, d, d, d, d]),
+        "orc_graupel_processes": (d, [fp, fp, fp, fp, fp, fp, fp, d, d, d, d]),
+        "orc_hail_processes": (d, [fp, fp, fp, fp, fp, fp, fp, fp, d, d, d, d]),
+        "orc_hail_max_diameter": (d, [d, d]),
+        "orc_wet_growing": (i, [i, d, d, d, d, d, d, d, d, d]),
+        "orc_cooper_crystals": (d, [d]), "orc_splintering": (d, [d]), "orc_fragments_per_kg": (d, [d]),
+        "orc_koenig_growth": (d, [d, d]), "orc_crystal_size": (d, [d, d, d]), "orc_constant": (d, [i]),
+        "orc_charging_rate": (d, [d, d, d, d, d, d, d, d, i, i, d, i]),
+        "orc_inductive_rate": (d, [d, d, d, d, d, d, d, i, i]),
+        "orc_transfer_charge": (None, [dp, dp, dp]),
+        "orc_crystal_number": (d, [d, d, d]), "orc_crystal_diameter": (d, [d, d, d]),
+        "orc_viscosity": (None, [G, fp, fp, fp, fp, dp, d, d, fp, fp, dp, dp, dp, i]),
+        "orc_mix": (None, [G, dp, dp, dp, fp, dp, d, d]),
+        "orc_fall_parts": (i, [vp, i, i, d]), "orc_sediment": (i, [vp, i, i, d, i]), "orc_charge_ground": (d, [vp]),
+        "orc_lightning_create": (vp, [G, dp, i]), "orc_lightning_destroy": (None, [vp]),
+        "orc_lightning_solve": (None, [vp, fp, fp, fp, fp]), "orc_lightning_strongest": (d, [vp, ip]),
+        "orc_lightning_discharge": (i, [vp, fp, fp, fp, fp, d, i, dp, i]),
+        "orc_lightning_channel": (i, [vp, i, ip, ip, ip, i]), "orc_lightning_conduct": (d, [vp, fp, d]),
+        "orc_lightning_set_cloudy": (None, [vp, fp]), "orc_lightning_set_corona": (None, [vp, i]),
+        "orc_lightning_phi": (None, [vp, dp]), "orc_lightning_totals": (None, [vp, dp, dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype, f.argtypes = res, args
 
 
 def fptr(a):

@@ -129,6 +129,60 @@ uint32_t orc_mulberry32(uint32_t *state);
 /* Electric potential of a node charge density (ElectricSolver.solve): rho, phi are n float64. */
 void orc_electric_solve(const orc_grid *grid, const double *rho, double *phi);
 
+/* The column physics' pieces on one node, float32 fields in/out (ice.ts, graupel.ts, hail.ts, charge.ts). */
+double orc_sat_adjust_mixed(float *theta, float *q, float *cloud, float *ice, double exner, double p, float *number /* may be null */);
+void orc_freeze_rain(float *theta, float *rain, float *graupel, double exner);
+double orc_ice_processes(float *theta, float *q, float *ice, float *snow, float *number, double exner, double p, double rho, double dt);
+void orc_settle_crystals(float *ice, float *number);
+double orc_snow_processes(float *theta, float *q, float *cloud, float *ice, float *snow, float *rain, double exner, double p, double rho, double rho_ground, double dt);
+double orc_graupel_processes(float *theta, float *q, float *cloud, float *ice, float *snow, float *rain, float *graupel, double exner, double p, double rho, double dt);
+double orc_hail_processes(float *theta, float *q, float *cloud, float *ice, float *snow, float *rain, float *graupel, float *hail, double exner, double p, double rho, double dt);
+double orc_hail_max_diameter(double h, double rho);
+/* wetGrowing (hail.ts:55-60) of graupel (rimer 0) or hail (1). */
+int orc_wet_growing(int rimer, double q, double tc, double qv, double p, double rho, double cloud, double rain, double ice, double snow);
+double orc_cooper_crystals(double tc);
+double orc_splintering(double tc);
+double orc_fragments_per_kg(double lambda);
+double orc_koenig_growth(double tc, double m);
+double orc_crystal_size(double qi, double n, double rho);
+/* Constants the tests use: 0 LF, 1 LS, 2 CP, 3 KAPPA, 4 SPLINTERS, 5 CRYSTAL_MASS, 6 LV. */
+double orc_constant(int which);
+/* chargingRate (charge.ts:138-158, Takahashi scheme); have_crystals 0: the reference's `crystals` undefined. */
+double orc_charging_rate(double tc, double rho, double cloud, double ice, double snow, double graupel, double hail, double snow_fall, int wet_g, int wet_h, double crystals, int have_crystals);
+double orc_inductive_rate(double tc, double rho, double cloud, double graupel, double hail, double ez, double rimer_charge, int wet_g, int wet_h);
+/* transferCharge (charge.ts:215-227): charge[4] in place from the masses before[4] and after[4]. */
+void orc_transfer_charge(double *charge, const double *before, const double *after);
+double orc_crystal_number(double tc, double qi, double rho);
+double orc_crystal_diameter(double qi, double rho, double tc);
+
+/* Turbulence.viscosity (explicit K; anisotropic 1 the baseline's, 0 the class default kh = km; condensate null: dry)
+ * and Turbulence.mix on a grid. */
+void orc_viscosity(const orc_grid *grid, const float *u, const float *v, const float *w, const float *theta, const double *theta_env, double cs, double dt,
+                   const float *q, const float *condensate, const double *exner, double *km, double *kh, int anisotropic);
+void orc_mix(const orc_grid *grid, const double *rho, const double *km, const double *kh, float *a, const double *base, double factor, double dt);
+/* fallParts and sediment (model.ts:565-598) of a model's field with the speeds in field `fall`; deposit 1 lands it in
+ * the precipitation column (under the moving ground at the model's time), 0 counts it as charge to the ground. */
+int orc_fall_parts(void *model, int field, int fall, double dt);
+int orc_sediment(void *model, int field, int fall, double dt, int deposit);
+double orc_charge_ground(void *model);
+
+/* Lightning (lightning.ts) on its own: a grid, the air density per level and the seed. */
+void *orc_lightning_create(const orc_grid *grid, const double *air, int seed);
+void orc_lightning_destroy(void *lightning);
+void orc_lightning_solve(void *lightning, float *ice, float *graupel, float *rain, float *air);
+/* strongest(): the largest |E| / E_init and its node. */
+double orc_lightning_strongest(void *lightning, int *node);
+/* discharge(carriers, time, limit): the flashes, 8 float64 each: time, CG, start node, channel nodes, potential, charge,
+ * charge moment, ground node (-1 for an IC); returns their number. orc_lightning_channel: flash k's channel nodes, their
+ * parents and polarities. */
+int orc_lightning_discharge(void *lightning, float *ice, float *graupel, float *rain, float *air, double time, int limit, double *records, int max);
+int orc_lightning_channel(void *lightning, int flash, int *nodes, int *parent, int *polarity, int max);
+double orc_lightning_conduct(void *lightning, float *air_charge, double dt);
+void orc_lightning_set_cloudy(void *lightning, const float *cloudy);
+void orc_lightning_set_corona(void *lightning, int on);
+void orc_lightning_phi(void *lightning, double *out);
+void orc_lightning_totals(void *lightning, double *from_ground, double *conducted);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,16 @@
 // stated as parity-unpinned where the tests document it.
 #include "oracle.h"
 
+// ORC_MUTANT (0 or undefined in the oracle itself) builds the oracle with one planted mistake, for the test that shows
+// the checks of tests/test_oracle_physics.py catch each: 1 the Koenig growth table's -15 degC peak halved, 2 the charge
+// that evaporates lost instead of left to the ions, 3 wet growth freezing all it collects, 4 sublimation growing the
+// ice, 5 the stability's sign in the eddy viscosity, 6 no screening of the conduction in cloud, 7 the Takahashi table's
+// sign, 8 the latent heat of vaporisation heating the deposition below -38 degC, 9 the sedimentation's inflow density from the
+// level itself instead of the one above.
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -151,7 +161,11 @@ double saturationAdjustMixed(float &theta, float &q, float &cloud, float &ice, d
     double d = (q - qs) / (1 + LS * LS * qs / (CP * RV * tk * tk));
     if (d < 0) d = std::max(d, -double(ice));
     if (d == 0) break;
+#if ORC_MUTANT == 8
+    q = float(q - d); ice = float(ice + d); theta = float(theta + LV / (CP * exner) * d); total += d;
+#else
     q = float(q - d); ice = float(ice + d); theta = float(theta + LS / (CP * exner) * d); total += d;
+#endif
   }
   return total;
 }
@@ -167,7 +181,12 @@ const double KOENIG_A2[31] = {.4006, .4831, .5320, .5307, .5319, .5249, .4888, .
   .5655, .5478, .5203, .4906, .4447, .4126, .3960, .4149, .4320, .4506, .4483, .4460, .4433, .4413, .4382, .4361};
 double koenigGrowth(double tc, double m) {
   double x = clamp01(-tc, 1, 31) - 1; int k = std::min(29, int(std::floor(x))); double f = x - k, g = m * 1000;
-  return 1e-3 * ((1 - f) * KOENIG_A1[k] * std::pow(g, KOENIG_A2[k]) + f * KOENIG_A1[k + 1] * std::pow(g, KOENIG_A2[k + 1]));
+  double a1 = KOENIG_A1[k], a2 = KOENIG_A1[k + 1];
+#if ORC_MUTANT == 1
+  if (k == 14) a1 /= 2;
+  if (k + 1 == 14) a2 /= 2;
+#endif
+  return 1e-3 * ((1 - f) * a1 * std::pow(g, KOENIG_A2[k]) + f * a2 * std::pow(g, KOENIG_A2[k + 1]));
 }
 double iceProcesses(float &theta, float &q, float &ice, float &snow, float &number, double exner, double p, double rho, double dt) {
   double tk = theta * exner, tc = tk - T0;
@@ -181,7 +200,12 @@ double iceProcesses(float &theta, float &q, float &ice, float &snow, float &numb
     else {
       double a = LS * LS / (KA * RV * tk * tk), b = 1 / (rho * qsi * PSI);
       double lost = std::min(qi, n * 2 * PI * crystalSize(qi, n * rho, rho) * (1 - q / qsi) / (a + b) * dt);
-      change -= lost; number = float(number * (1 - lost / qi));
+#if ORC_MUTANT == 4
+      change += lost;
+#else
+      change -= lost;
+#endif
+      number = float(number * (1 - lost / qi));
     }
   }
   ice = float(ice + change); q = float(q - change); theta = float(theta + heat * change);
@@ -304,7 +328,11 @@ double hailProcesses(float &theta, float &q, float &cloud, float &ice, float &sn
     double wet = wetGrowth(lambda, a, N0H, tc, q, p, rho, k * (double(ice) + snow)) * dt;
     if (liquid + haci + hacs > wet) {
       haci = std::min(double(ice), k * ice * dt); hacs = std::min(double(snow), k * snow * dt);
+#if ORC_MUTANT == 3
+      frozen = liquid;
+#else
       frozen = std::min(liquid, std::max(0.0, wet - haci - hacs));
+#endif
     }
     cloud = float(cloud - hacw); rain = float(rain + (liquid - frozen - hacr)); ice = float(ice - haci); snow = float(snow - hacs); hail = float(hail + (frozen + haci + hacs));
     theta = float(theta + LF * heat * frozen);
@@ -335,7 +363,11 @@ double takahashiCharge(double tc, double lwc) {
   auto at = [&](const double *r) { return r[col] + f * (r[col + 1] - r[col]); };
   double a = at(TAKAHASHI_TABLE[row]), b = at(TAKAHASHI_TABLE[row + 1]), g = std::min(1.0, (w - TAKAHASHI_LWC[row]) / (TAKAHASHI_LWC[row + 1] - TAKAHASHI_LWC[row]));
   double cold = tc < -30 ? 1 - std::pow((tc + 30) / 10, 2) : 1;
+#if ORC_MUTANT == 7
+  return -cold * (a + g * (b - a));
+#else
   return cold * (a + g * (b - a));
+#endif
 }
 double takahashiScaled(double table, double d, double dv) {
   double q = table * std::min(5 * std::pow(d / 100e-6, 2) * std::fabs(dv) / 8, 10.0) * 1e-15;
@@ -365,13 +397,14 @@ double rimerRate(const Moments &m, double rebound, double ni, double di, double 
   if (ns > 0) { double v = std::fabs(fall - vs); rate += pair(ns * k * swept(ds) * v * takahashiScaled(table, ds, v)); }
   return rate;
 }
-// chargingRate with the Takahashi scheme (charge.ts:138-158) and the model's own crystals (`crystals`, per m3).
-double chargingRate(double tc, double rho, double cloud, double ice, double snow, double graupel, double hail, double snowFall, bool wetG, bool wetH, double crystals) {
+// chargingRate with the Takahashi scheme (charge.ts:138-158) and the model's own crystals (`crystals`, per m3); without
+// them (`haveCrystals` false, the reference's `crystals` undefined) the number and size estimated from the mass.
+double chargingRate(double tc, double rho, double cloud, double ice, double snow, double graupel, double hail, double snowFall, bool wetG, bool wetH, double crystals, bool haveCrystals = true) {
   if (tc >= 0 || tc < -40 || cloud <= 0 || ice + snow <= 0 || graupel + hail <= 0) return 0;
   double rebound = .5 * (1 - .1 * std::exp(.1 * tc)), water = rho * cloud * 1000;
-  double ni = ice <= 0 ? 0 : crystals;
+  double ni = ice <= 0 ? 0 : haveCrystals ? crystals : crystalNumber(tc, ice, rho);
   auto size = [](double m) { return .1871 * std::pow(m, .3429); };
-  double di = ice <= 0 ? 0 : crystals != 0 ? std::min(CRYSTAL_MAX, std::max(CRYSTAL_MIN, size(rho * ice / crystals))) : crystalDiameter(ice, rho, tc);
+  double di = ice <= 0 ? 0 : haveCrystals && crystals != 0 ? std::min(CRYSTAL_MAX, std::max(CRYSTAL_MIN, size(rho * ice / crystals))) : crystalDiameter(ice, rho, tc);
   double ss = snow > 0 ? std::pow(PI * RHO_S * N0S / (rho * snow), .25) : 0, ns = snow > 0 ? N0S / ss : 0, ds = snow > 0 ? 1.8171 / ss : 0;
   double rate = 0, table = takahashiCharge(tc, water);
   if (graupel > 0 && !wetG) rate += rimerRate(exponentialMoments(N0G, RHO_G, graupel, rho), rebound, ni, di, ns, ds, snowFall, table);
@@ -402,7 +435,70 @@ void transferCharge(double *charge, const double *before, const double *after) {
   if (pool == 0) return;
   double moved = gained > 0 ? std::min(1.0, lost > 0 ? gained / lost : 1) : 0;
   if (moved > 0) for (int k = 0; k < 4; k++) if (after[k] > before[k]) charge[k] += pool * moved * (after[k] - before[k]) / gained;
+#if ORC_MUTANT != 2
   charge[3] += pool * (1 - moved);
+#endif
+}
+
+// ---- turbulence.ts:87-185 ------------------------------------------------------------------------------------------
+// Turbulence.viscosity (turbulence.ts:87-137) as the baseline configures it (model.ts:778-792): anisotropic (or, with
+// `anisotropic` false, the class's default: kh = km), the cube vertical length, explicit vertical mixing, the theta_e
+// stability of cloudy nodes (condensate above 1e-5; a null condensate: dry air everywhere). km, kh: n float64.
+void viscosityOf(const Grid &g, const float *u, const float *v, const float *w, const float *theta, const double *thetaEnv, double cs, double dt,
+                 const float *q, const float *condensate, const double *exner, double *km, double *kh, bool anisotropic = true) {
+  int nx = g.nx, ny = g.ny, nz = g.nz, layer = g.layer;
+  double lenH2 = std::pow(cs * std::sqrt(g.dx * g.dy), 2), capH = .05 * std::pow(std::min(g.dx, g.dy), 2) / dt;
+  const double inf = std::numeric_limits<double>::infinity();
+  for (int z = 0, i = 0; z < nz; z++) {
+    double local = z == 0 ? g.dzs[0] : z == nz - 1 ? g.dzs[nz - 2] : g.hz[z], len2 = std::pow(cs * std::cbrt(g.dx * g.dy * local), 2);
+    double cap = .05 * std::pow(std::min(std::min(g.dx, g.dy), std::min(z > 0 ? g.dzs[z - 1] : inf, z < nz - 1 ? g.dzs[z] : inf)), 2) / dt;
+    int up = z < nz - 1 ? layer : 0, below = z > 0 ? -layer : 0; double span = g.zs[z + (up ? 1 : 0)] - g.zs[z - (below ? 1 : 0)];
+    for (int y = 0; y < ny; y++) {
+      int row = y * nx + z * layer;
+      for (int x = 0; x < nx; x++, i++) {
+        int e = g.xp[x] + row, wst = g.xm[x] + row, n = x + g.yp[y] + z * layer, s = x + g.ym[y] + z * layer; double hx = 2 * g.dx, hy = 2 * g.dy; int down = below;
+        double dudx = (double(u[e]) - u[wst]) / hx, dvdy = (double(v[n]) - v[s]) / hy, dwdz = (double(w[i + up]) - w[i + down]) / span;
+        double dudy = (double(u[n]) - u[s]) / hy, dvdx = (double(v[e]) - v[wst]) / hx;
+        double dudz = (double(u[i + up]) - u[i + down]) / span, dwdx = (double(w[e]) - w[wst]) / hx;
+        double dvdz = (double(v[i + up]) - v[i + down]) / span, dwdy = (double(w[n]) - w[s]) / hy;
+        double s2 = 2 * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + std::pow(dudy + dvdx, 2) + std::pow(dudz + dwdx, 2) + std::pow(dvdz + dwdy, 2);
+        double n2 = G / thetaEnv[z] * (double(theta[i + up]) - theta[i + down]) / span;
+        if (condensate && condensate[i] > 1e-5) {
+          auto te = [&](int k, int level) { return theta[k] * std::exp(LV * q[k] / (CP * theta[k] * exner[level])); };
+          n2 = G / te(i, z) * (te(i + up, z + (up ? 1 : 0)) - te(i + down, z - (down ? 1 : 0))) / span;
+        }
+#if ORC_MUTANT == 5
+        double rate = std::sqrt(std::max(0.0, s2 + n2 / (1.0 / 3)));
+#else
+        double rate = std::sqrt(std::max(0.0, s2 - n2 / (1.0 / 3)));
+#endif
+        km[i] = std::min(cap, len2 * rate); kh[i] = anisotropic ? std::min(capH, lenH2 * rate) : km[i];
+      }
+    }
+  }
+}
+// Turbulence.mix (turbulence.ts:143-185), explicit, flat ground: the departure of `a` from `base` (per level; null: none)
+// diffused with K times `factor`; t: n float32 scratch.
+void mixOf(const Grid &g, const double *rho, const double *km, const double *kh, float *a, const double *base, double factor, double dt, float *t) {
+  int nx = g.nx, ny = g.ny, nz = g.nz, layer = g.layer; const F64 &h = g.hz;
+  std::fill(t, t + g.n, 0.0f);
+  auto dev = [&](int i, int z) { return a[i] - (base ? base[z] : 0); };
+  for (int z = 0, i = 0; z < nz; z++) {
+    double rUp = z < nz - 1 ? (rho[z] + rho[z + 1]) / 2 : 0;
+    for (int y = 0; y < ny; y++) {
+      int row = y * nx + z * layer;
+      for (int x = 0; x < nx; x++, i++) {
+        double here = dev(i, z); int e = g.xp[x] + row, n = x + g.yp[y] + z * layer;
+        { double fx = factor * (kh[i] + kh[e]) / 2 * (dev(e, z) - here) / g.dx; t[i] = float(t[i] + fx / g.dx); t[e] = float(t[e] - fx / g.dx); }
+        { double fy = factor * (kh[i] + kh[n]) / 2 * (dev(n, z) - here) / g.dy; t[i] = float(t[i] + fy / g.dy); t[n] = float(t[n] - fy / g.dy); }
+        if (z < nz - 1) {
+          double fz = factor * rUp * (km[i] + km[i + layer]) / 2 * (dev(i + layer, z + 1) - here) / g.dzs[z];
+          t[i] = float(t[i] + fz / (rho[z] * h[z])); t[i + layer] = float(t[i + layer] - fz / (rho[z + 1] * h[z + 1]));
+        }
+      }
+    }
+  }
+  for (int i = 0; i < g.n; i++) a[i] = float(a[i] + dt * t[i]);
 }
 
 // ---- spectral.ts:8-151 ---------------------------------------------------------------------------------------------
@@ -744,10 +840,12 @@ struct ElectricSolver {
 // ---- lightning.ts:14-420 --------------------------------------------------------------------------------------------
 constexpr double E_INIT = 2.84e5, RHO_STANDARD = 1.225, F_CRIT = .43, RHO_MIN = 1e-10, CHANNEL_RADIUS = 1, SHIELDING = .25;
 constexpr int MAX_CHANNEL = 3000, STROKES = 4, MAX_FLASHES = 10;
-constexpr double FLASH_WINDOW = 3, SIGMA_GROUND = 2e-14, SIGMA_SCALE = 6000, CLOUD_SIGMA = .1, CLOUDY = 1e-5, CORONA_FIELD = 5e3, CORONA_RATE = 5e-13;
+constexpr double FLASH_WINDOW = 3, SIGMA_GROUND = 2e-14, SIGMA_SCALE = 6000, CLOUD_SIGMA = ORC_MUTANT == 6 ? 1 : .1, CLOUDY = 1e-5, CORONA_FIELD = 5e3, CORONA_RATE = 5e-13;
 int flashLimit(double dt) { return std::max(1, int(std::floor(MAX_FLASHES * dt / FLASH_WINDOW + .5))); }
 
-struct Flash { double time; bool cg; int start; int nodes; double potential, charge, moment; };
+// A flash (lightning.ts Flash, :57-72): the channel's nodes in growth order, each one's parent (-1 for the first) and
+// polarity (the sign of the charge its deposit takes), and the ground node a CG reached (-1 for an IC).
+struct Flash { double time; bool cg; int start; int nodes; double potential, charge, moment; int ground; std::vector<int> channel, parent, polarity; };
 
 struct Lightning {
   const Grid &g; ElectricSolver solver; F64 rho, phi, change, sigma, air, eInit, capacitance; F32 cloudy; bool hasCloud = false;
@@ -816,6 +914,8 @@ struct Lightning {
     for (int i = 0; i < g.n; i++) cloudy[i] = float(cloudy[i] + ice[i]);
     for (int i = 0; i < g.n; i++) cloudy[i] = float(cloudy[i] + snow[i]);
   }
+  // cloud([cloudy]) (lightning.ts:179-186) with one array: the cloudiness itself.
+  void setCloudy(const float *c) { cloudy.assign(c, c + g.n); hasCloud = true; }
   bool clear(int i) const { return std::fabs(rho[i]) < RHO_MIN || (hasCloud && cloudy[i] <= CLOUDY); }
   std::vector<Flash> discharge(float *const *carriers, double time, int limit) {
     std::vector<Flash> flashes; int stamp = ++flashId;
@@ -872,6 +972,7 @@ struct Lightning {
       if (ground < 0) { sum = 0; for (int i : nodes) sum += phi[i]; potential = sum / nodes.size(); }
     }
     Flash f; f.time = time; f.cg = ground >= 0; f.start = start; f.nodes = int(nodes.size()); f.potential = ground >= 0 ? before : potential;
+    f.ground = ground; f.channel = nodes; f.parent = parent; f.polarity = polarity;
     f.charge = ground >= 0 ? lowered : neutralized; f.moment = ground >= 0 ? moment : 0;
     return f;
   }
@@ -1037,15 +1138,40 @@ struct Model {
       }
     }
   }
-  // collectHail (model.ts:697-706) with the ground under the domain fixed (frame offset zero: identity weights).
+  // updateGround (model.ts:644-657): the four ground columns under each domain column, with bilinear weights, from the
+  // distance the domain has moved over the ground (frameOffset, model.ts:637). Without frame motion the identity.
+  std::vector<int> groundCell; F64 groundWeight;
+  void updateGround() {
+    const int nx = g.nx, ny = g.ny;
+    groundCell.resize(size_t(g.layer) * 4); groundWeight.resize(size_t(g.layer) * 4);
+    auto mod = [](double x, double n) { return std::fmod(std::fmod(x, n) + n, n); };
+    const double ox = run.frame_u * time, oy = run.frame_v * time, sx = mod(ox / g.dx, nx), sy = mod(oy / g.dy, ny);
+    const int x0 = int(std::floor(sx)), y0 = int(std::floor(sy));
+    const double fx = sx - x0, fy = sy - y0;
+    for (int y = 0; y < ny; y++) for (int x = 0; x < nx; x++) {
+      const int i = x + nx * y, ax = (x + x0) % nx, bx = (ax + 1) % nx, ay = (y + y0) % ny * nx, by = (ay / nx + 1) % ny * nx, k = i * 4;
+      groundCell[k] = ax + ay; groundCell[k + 1] = bx + ay; groundCell[k + 2] = ax + by; groundCell[k + 3] = bx + by;
+      groundWeight[k] = (1 - fx) * (1 - fy); groundWeight[k + 1] = fx * (1 - fy); groundWeight[k + 2] = (1 - fx) * fy; groundWeight[k + 3] = fx * fy;
+    }
+  }
+  // deposit (model.ts:691-694): mm of water falling out of domain column i onto the ground columns under it.
+  void deposit(int i, double mm, F32 &target) {
+    const int k = i * 4;
+    for (int j = 0; j < 4; j++) if (groundWeight[k + j] > 0) target[groundCell[k + j]] = float(target[groundCell[k + j]] + groundWeight[k + j] * mm);
+  }
+  // collectHail (model.ts:697-706).
   void collectHail(double dt) {
     const F64 &rho = env.rho;
     for (int b = 0; b < g.layer; b++) {
       double h = f[ORC_HAIL][b];
       if (!(h > 1e-9)) continue;
-      col[ORC_HAIL_GROUND][b] = float(col[ORC_HAIL_GROUND][b] + 1 * (rho[0] * h * f[ORC_FALL_HAIL][b] * dt));
+      deposit(b, rho[0] * h * f[ORC_FALL_HAIL][b] * dt, col[ORC_HAIL_GROUND]);
       double size = hailMaxDiameter(h, rho[0]) * 1000;
-      col[ORC_HAIL_SIZE][b] = float(std::max(double(col[ORC_HAIL_SIZE][b]), size));
+      for (int j = 0; j < 4; j++) {
+        if (!(groundWeight[b * 4 + j] > 0)) continue;
+        float &hs = col[ORC_HAIL_SIZE][groundCell[b * 4 + j]];
+        hs = float(std::max(double(hs), size));
+      }
     }
   }
   // fallParts, sedimentPart, sediment (model.ts:565-598).
@@ -1056,7 +1182,7 @@ struct Model {
   }
   void sedimentPart(float *rain, const float *vt, double dt, bool deposit) {
     int nz = g.nz, layer = g.layer; const F64 &rho = env.rho, &lw = levelWeight;
-    if (deposit) for (int c = 0; c < layer; c++) col[ORC_PRECIPITATION][c] = float(col[ORC_PRECIPITATION][c] + 1 * (rho[0] * rain[c] * vt[c] * dt));
+    if (deposit) for (int c = 0; c < layer; c++) this->deposit(c, rho[0] * rain[c] * vt[c] * dt, col[ORC_PRECIPITATION]);
     else {
       double out = 0;
       for (int i = 0; i < layer; i++) out += double(rain[i]) * vt[i];
@@ -1064,7 +1190,7 @@ struct Model {
       chargeGround += out * dt * g.dx * g.dy;
     }
     for (int z = 0; z < nz; z++) {
-      int l = z * layer; double inflow = z < nz - 1 ? rho[z + 1] : 0;
+      int l = z * layer; double inflow = z < nz - 1 ? rho[ORC_MUTANT == 9 ? z : z + 1] : 0;
       for (int i = l; i < l + layer; i++)
         rain[i] = float(rain[i] + dt * (inflow * (z < nz - 1 ? double(rain[i + layer]) * vt[i + layer] : 0) - rho[z] * rain[i] * vt[i]) / (lw[z] * g.dz * 1));
     }
@@ -1182,58 +1308,10 @@ struct Model {
     }
     for (int i = z * layer, e = i + layer; i < e; i++) condensate[i] = float(double(cloud[i]) + ice[i]);
   }
-  // Turbulence.viscosity (turbulence.ts:87-137): anisotropic, cube vertical length, explicit, theta_e moist stability.
   void viscosity(double dt) {
-    int nx = g.nx, ny = g.ny, nz = g.nz, layer = g.layer;
-    const float *u = F(ORC_U), *v = F(ORC_V), *w = F(ORC_W), *theta = F(ORC_THETA), *q = F(ORC_Q);
-    double lenH2 = std::pow(cs * std::sqrt(g.dx * g.dy), 2), capH = .05 * std::pow(std::min(g.dx, g.dy), 2) / dt;
-    const double inf = std::numeric_limits<double>::infinity();
-    for (int z = 0, i = 0; z < nz; z++) {
-      double local = z == 0 ? g.dzs[0] : z == nz - 1 ? g.dzs[nz - 2] : g.hz[z], len2 = std::pow(cs * std::cbrt(g.dx * g.dy * local), 2);
-      double cap = .05 * std::pow(std::min(std::min(g.dx, g.dy), std::min(z > 0 ? g.dzs[z - 1] : inf, z < nz - 1 ? g.dzs[z] : inf)), 2) / dt;
-      int up = z < nz - 1 ? layer : 0, below = z > 0 ? -layer : 0; double span = g.zs[z + (up ? 1 : 0)] - g.zs[z - (below ? 1 : 0)];
-      for (int y = 0; y < ny; y++) {
-        int row = y * nx + z * layer;
-        for (int x = 0; x < nx; x++, i++) {
-          int e = g.xp[x] + row, wst = g.xm[x] + row, n = x + g.yp[y] + z * layer, s = x + g.ym[y] + z * layer; double hx = 2 * g.dx, hy = 2 * g.dy; int down = below;
-          double dudx = (double(u[e]) - u[wst]) / hx, dvdy = (double(v[n]) - v[s]) / hy, dwdz = (double(w[i + up]) - w[i + down]) / span;
-          double dudy = (double(u[n]) - u[s]) / hy, dvdx = (double(v[e]) - v[wst]) / hx;
-          double dudz = (double(u[i + up]) - u[i + down]) / span, dwdx = (double(w[e]) - w[wst]) / hx;
-          double dvdz = (double(v[i + up]) - v[i + down]) / span, dwdy = (double(w[n]) - w[s]) / hy;
-          double s2 = 2 * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + std::pow(dudy + dvdx, 2) + std::pow(dudz + dwdx, 2) + std::pow(dvdz + dwdy, 2);
-          double n2 = G / env.theta[z] * (double(theta[i + up]) - theta[i + down]) / span;
-          if (condensate[i] > 1e-5) {
-            auto te = [&](int k, int level) { return theta[k] * std::exp(LV * q[k] / (CP * theta[k] * env.exner[level])); };
-            n2 = G / te(i, z) * (te(i + up, z + (up ? 1 : 0)) - te(i + down, z - (down ? 1 : 0))) / span;
-          }
-          double rate = std::sqrt(std::max(0.0, s2 - n2 / (1.0 / 3)));
-          km[i] = std::min(cap, len2 * rate); kh[i] = std::min(capH, lenH2 * rate);
-        }
-      }
-    }
+    viscosityOf(g, F(ORC_U), F(ORC_V), F(ORC_W), F(ORC_THETA), env.theta.data(), cs, dt, F(ORC_Q), condensate.data(), env.exner.data(), km.data(), kh.data());
   }
-  // Turbulence.mix (turbulence.ts:143-185), explicit, flat ground.
-  void mix(float *a, const double *base, double factor, double dt) {
-    int nx = g.nx, ny = g.ny, nz = g.nz, layer = g.layer; float *t = tmp.data(); const F64 &rho = env.rho, &h = g.hz;
-    std::fill(tmp.begin(), tmp.end(), 0.0f);
-    auto dev = [&](int i, int z) { return a[i] - (base ? base[z] : 0); };
-    for (int z = 0, i = 0; z < nz; z++) {
-      double rUp = z < nz - 1 ? (rho[z] + rho[z + 1]) / 2 : 0;
-      for (int y = 0; y < ny; y++) {
-        int row = y * nx + z * layer;
-        for (int x = 0; x < nx; x++, i++) {
-          double here = dev(i, z); int e = g.xp[x] + row, n = x + g.yp[y] + z * layer;
-          { double fx = factor * (kh[i] + kh[e]) / 2 * (dev(e, z) - here) / g.dx; t[i] = float(t[i] + fx / g.dx); t[e] = float(t[e] - fx / g.dx); }
-          { double fy = factor * (kh[i] + kh[n]) / 2 * (dev(n, z) - here) / g.dy; t[i] = float(t[i] + fy / g.dy); t[n] = float(t[n] - fy / g.dy); }
-          if (z < nz - 1) {
-            double fz = factor * rUp * (km[i] + km[i + layer]) / 2 * (dev(i + layer, z + 1) - here) / g.dzs[z];
-            t[i] = float(t[i] + fz / (rho[z] * h[z])); t[i + layer] = float(t[i + layer] - fz / (rho[z + 1] * h[z + 1]));
-          }
-        }
-      }
-    }
-    for (int i = 0; i < g.n; i++) a[i] = float(a[i] + dt * t[i]);
-  }
+  void mix(float *a, const double *base, double factor, double dt) { mixOf(g, env.rho.data(), km.data(), kh.data(), a, base, factor, dt, tmp.data()); }
   // mixStep (model.ts:1161-1195), flat ground.
   void mixStep(double dt) {
     viscosity(dt);
@@ -1431,6 +1509,7 @@ struct Model {
     out[ORC_MD_OUTFLOWLOW] = outflowLow; out[ORC_MD_GUST] = gust; out[ORC_MD_HAIL] = hail; out[ORC_MD_CLIPPED] = double(clipped);
   }
   void step(double dt) {
+    updateGround();
     fallPhase();
     collectHail(dt);
     sedimentStep(dt);
@@ -1486,7 +1565,7 @@ int orc_uh_column(void *m, float *out) {
 int orc_step_part(void *m, int part, double dt) {
   Model *mm = static_cast<Model *>(m);
   switch (part) {
-    case 0: mm->fallPhase(); mm->collectHail(dt); mm->sedimentStep(dt); mm->dynamics(dt); return 0;
+    case 0: mm->updateGround(); mm->fallPhase(); mm->collectHail(dt); mm->sedimentStep(dt); mm->dynamics(dt); return 0;
     case 1: mm->physicsStep(dt); return 0;
     case 2: mm->mixStep(dt); mm->project(dt, true); return 0;
     case 3: mm->updateRotation(dt); if (mm->electrified) mm->discharge(dt); mm->time += dt; if (mm->swaths) mm->addSwaths(); return 0;
@@ -1580,6 +1659,107 @@ void orc_advect(const orc_grid *grid, const double *rho, const float *u, const f
     for (int k = 0; k < 3; k++) { t.tendency(q, tend.data(), monotone != 0); for (int i = 0; i < g.n; i++) q[i] = float(st[i] + RK3[k] * dt * tend[i]); }
   }
 }
+// ---- the column physics' pieces on one node (ice.ts, graupel.ts, hail.ts, charge.ts) ---------------------------------
+double orc_sat_adjust_mixed(float *theta, float *q, float *cloud, float *ice, double exner, double p, float *number) {
+  float spare = 0;
+  return saturationAdjustMixed(*theta, *q, *cloud, *ice, exner, p, number ? *number : spare);
+}
+void orc_freeze_rain(float *theta, float *rain, float *graupel, double exner) { freezeRain(*theta, *rain, *graupel, exner); }
+double orc_ice_processes(float *theta, float *q, float *ice, float *snow, float *number, double exner, double p, double rho, double dt) {
+  return iceProcesses(*theta, *q, *ice, *snow, *number, exner, p, rho, dt);
+}
+void orc_settle_crystals(float *ice, float *number) { settleCrystals(*ice, *number); }
+double orc_snow_processes(float *theta, float *q, float *cloud, float *ice, float *snow, float *rain, double exner, double p, double rho, double rho_ground, double dt) {
+  return snowProcesses(*theta, *q, *cloud, *ice, *snow, *rain, exner, p, rho, rho_ground, dt);
+}
+double orc_graupel_processes(float *theta, float *q, float *cloud, float *ice, float *snow, float *rain, float *graupel, double exner, double p, double rho, double dt) {
+  return graupelProcesses(*theta, *q, *cloud, *ice, *snow, *rain, *graupel, exner, p, rho, dt);
+}
+double orc_hail_processes(float *theta, float *q, float *cloud, float *ice, float *snow, float *rain, float *graupel, float *hail, double exner, double p, double rho, double dt) {
+  return hailProcesses(*theta, *q, *cloud, *ice, *snow, *rain, *graupel, *hail, exner, p, rho, dt);
+}
+double orc_hail_max_diameter(double h, double rho) { return hailMaxDiameter(h, rho); }
+int orc_wet_growing(int rimer, double q, double tc, double qv, double p, double rho, double cloud, double rain, double ice, double snow) {
+  return wetGrowing(q, rimer ? N0H : N0G, rimer ? RHO_H : RHO_G, tc, qv, p, rho, cloud, rain, ice, snow) ? 1 : 0;
+}
+double orc_cooper_crystals(double tc) { return cooperCrystals(tc); }
+double orc_splintering(double tc) { return splintering(tc); }
+double orc_fragments_per_kg(double lambda) { return fragmentsPerKg(lambda); }
+double orc_koenig_growth(double tc, double m) { return koenigGrowth(tc, m); }
+double orc_crystal_size(double qi, double n, double rho) { return crystalSize(qi, n, rho); }
+double orc_constant(int which) { const double c[] = {LF, LS, CP, KAPPA, SPLINTERS, CRYSTAL_MASS, LV}; return which >= 0 && which < 7 ? c[which] : 0; }
+double orc_charging_rate(double tc, double rho, double cloud, double ice, double snow, double graupel, double hail, double snow_fall, int wet_g, int wet_h, double crystals, int have_crystals) {
+  return chargingRate(tc, rho, cloud, ice, snow, graupel, hail, snow_fall, wet_g != 0, wet_h != 0, crystals, have_crystals != 0);
+}
+double orc_inductive_rate(double tc, double rho, double cloud, double graupel, double hail, double ez, double rimer_charge, int wet_g, int wet_h) {
+  return inductiveRate(tc, rho, cloud, graupel, hail, ez, rimer_charge, wet_g != 0, wet_h != 0);
+}
+void orc_transfer_charge(double *charge, const double *before, const double *after) { transferCharge(charge, before, after); }
+double orc_crystal_number(double tc, double qi, double rho) { return crystalNumber(tc, qi, rho); }
+double orc_crystal_diameter(double qi, double rho, double tc) { return crystalDiameter(qi, rho, tc); }
+
+// ---- turbulence and sedimentation on their own ------------------------------------------------------------------------
+void orc_viscosity(const orc_grid *grid, const float *u, const float *v, const float *w, const float *theta, const double *theta_env, double cs, double dt,
+                   const float *q, const float *condensate, const double *exner, double *km, double *kh, int anisotropic) {
+  Grid g(*grid); viscosityOf(g, u, v, w, theta, theta_env, cs, dt, q, condensate, exner, km, kh, anisotropic != 0);
+}
+void orc_mix(const orc_grid *grid, const double *rho, const double *km, const double *kh, float *a, const double *base, double factor, double dt) {
+  Grid g(*grid); F32 t(g.n); mixOf(g, rho, km, kh, a, base, factor, dt, t.data());
+}
+int orc_fall_parts(void *m, int field, int fall, double dt) {
+  Model *mm = static_cast<Model *>(m);
+  if (field < 0 || field >= ORC_FIELDS || fall < 0 || fall >= ORC_FIELDS) return -1;
+  return mm->fallParts(mm->F(field), mm->F(fall), dt);
+}
+int orc_sediment(void *m, int field, int fall, double dt, int deposit) {
+  Model *mm = static_cast<Model *>(m);
+  if (field < 0 || field >= ORC_FIELDS || fall < 0 || fall >= ORC_FIELDS) return -1;
+  mm->updateGround();
+  mm->sediment(field, fall, dt, deposit != 0);
+  return 0;
+}
+double orc_charge_ground(void *m) { return static_cast<Model *>(m)->chargeGround; }
+
+// ---- lightning on its own (lightning.ts Lightning) ---------------------------------------------------------------------
+struct OrcLightning { Grid g; F64 air; Lightning l; std::vector<Flash> last; OrcLightning(const orc_grid &o, const double *a, int seed) : g(o), air(a, a + g.nz), l(g, air, seed) {} };
+void *orc_lightning_create(const orc_grid *grid, const double *air, int seed) { return new OrcLightning(*grid, air, seed); }
+void orc_lightning_destroy(void *l) { delete static_cast<OrcLightning *>(l); }
+void orc_lightning_solve(void *l, float *ice, float *graupel, float *rain, float *air) {
+  float *c[4] = {ice, graupel, rain, air}; static_cast<OrcLightning *>(l)->l.solve(c);
+}
+double orc_lightning_strongest(void *l, int *node) {
+  auto r = static_cast<OrcLightning *>(l)->l.strongest(-1);
+  if (node) *node = r.first;
+  return r.second;
+}
+int orc_lightning_discharge(void *l, float *ice, float *graupel, float *rain, float *air, double time, int limit, double *records, int max) {
+  OrcLightning *o = static_cast<OrcLightning *>(l);
+  float *c[4] = {ice, graupel, rain, air};
+  o->last = o->l.discharge(c, time, limit);
+  int n = int(o->last.size());
+  for (int k = 0; k < n && k < max; k++) {
+    const Flash &f = o->last[k]; double *r = records + size_t(k) * 8;
+    r[0] = f.time; r[1] = f.cg ? 1 : 0; r[2] = f.start; r[3] = f.nodes; r[4] = f.potential; r[5] = f.charge; r[6] = f.moment; r[7] = f.ground;
+  }
+  return n;
+}
+int orc_lightning_channel(void *l, int flash, int *nodes, int *parent, int *polarity, int max) {
+  OrcLightning *o = static_cast<OrcLightning *>(l);
+  if (flash < 0 || flash >= int(o->last.size())) return -1;
+  const Flash &f = o->last[flash]; int n = int(f.channel.size());
+  for (int k = 0; k < n && k < max; k++) { nodes[k] = f.channel[k]; parent[k] = f.parent[k]; polarity[k] = k < int(f.polarity.size()) ? f.polarity[k] : 0; }
+  return n;
+}
+double orc_lightning_conduct(void *l, float *air_charge, double dt) { return static_cast<OrcLightning *>(l)->l.conduct(air_charge, dt); }
+void orc_lightning_set_cloudy(void *l, const float *cloudy) { static_cast<OrcLightning *>(l)->l.setCloudy(cloudy); }
+void orc_lightning_set_corona(void *l, int on) { static_cast<OrcLightning *>(l)->l.corona = on != 0; }
+void orc_lightning_phi(void *l, double *out) { OrcLightning *o = static_cast<OrcLightning *>(l); std::memcpy(out, o->l.phi.data(), sizeof(double) * o->g.n); }
+void orc_lightning_totals(void *l, double *from_ground, double *conducted) {
+  OrcLightning *o = static_cast<OrcLightning *>(l);
+  if (from_ground) *from_ground = o->l.fromGround;
+  if (conducted) *conducted = o->l.conducted;
+}
+
 uint32_t orc_mulberry32(uint32_t *state) { double v = mulberry(*state); return uint32_t(v * 4294967296.0); }
 void orc_electric_solve(const orc_grid *grid, const double *rho, double *phi) { Grid g(*grid); ElectricSolver s(g); s.solve(rho, phi); }
 
