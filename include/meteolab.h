@@ -42,6 +42,9 @@ enum {
   ML_D_TIME = 0, ML_D_MAXW, ML_D_MINW, ML_D_MAXTHETAP, ML_D_RAINTOTAL, ML_D_WATER, ML_D_CHARGE, ML_D_CLIPPED,
   ML_D_NEGFILLED, ML_D_CHARGEGROUND, ML_D_FLASHES_IC, ML_D_FLASHES_CG, ML_D_FIRSTFLASH, ML_D_ERROR, ML_D_COUNT
 };
+/* ML_D_ERROR: bits of conditions a step met and does not handle, 0 when none: 1, a lightning front with more prunes
+ * than the step keeps; 2, a water fall needing more than 16 sedimentation sub-steps in one step (its rain beyond the
+ * 16th does not reach the gauge). */
 
 /* AtmosphereModel.diagnostics() of a run (ml_model_diagnostics; ModelDiagnostics, model.ts:161-180), float64. */
 enum {

@@ -85,6 +85,11 @@ enum : int { FL_TIME = 0, FL_CG, FL_START, FL_NODES, FL_CHARGE, FL_POTENTIAL, FL
 // The falls of a step in order (model.ts:928-932, sedimentStep): water into the gauge, then charge with its carrier's
 // speed.
 constexpr int N_FALLS = 7;
+// Sedimentation sub-steps of a water fall whose ground-node values a step keeps for the rain gauge (fallParts: large
+// hail through the 100 m ground levels takes 2 at dt = 3 s). A fall that needs more flags S_ERROR (ERR_LAND_PARTS).
+constexpr int LAND_PARTS = 16;
+// S_ERROR bits.
+enum : long long { ERR_PRUNE = 1, ERR_LAND_PARTS = 2 };
 // The conserved (positivity-fixed) carried fields (model.ts:1319-1327): rain, snow, graupel, hail, q, cloud, ice.
 constexpr int N_CONSERVED = 7;
 
@@ -136,6 +141,7 @@ template <class R> struct Bufs {
   double *gpart;   // [slot][2][gmax] a launch's block parts of the charge carried to the ground (falls, conduction)
   int gmax;
   unsigned char *zflag;  // [slot][N_CARRIED][nz] a level of a carried field holds a non-zero node (this step's transport)
+  float *landed;   // [slot][4][LAND_PARTS][layer] each water fall's ground-node values at the start of its sub-steps
 
   __device__ float *f(int slot, int k, const Geo &g) const { return fields + (size_t(slot) * N_FIELDS + k) * g.n; }
   __device__ float *x(int slot, int k, const Geo &g) const { return stx + (size_t(slot) * N_CARRIED + k) * g.n; }
@@ -157,6 +163,7 @@ template <class R> struct Bufs {
     return fillp + (((size_t(slot) * N_CONSERVED + cf) * g.nz + z) * ftiles + tile) * 3;
   }
   __device__ unsigned char *zf(int slot, int f, const Geo &g) const { return zflag + (size_t(slot) * N_CARRIED + f) * g.nz; }
+  __device__ float *land(int slot, int k, int part, const Geo &g) const { return landed + ((size_t(slot) * 4 + k) * LAND_PARTS + part) * g.layer; }
   __device__ int *lm(int slot, int k, const Geo &g) const { return lmark + (size_t(slot) * N_LMARK + k) * g.n; }
   __device__ double *ln(int slot, int k, const Geo &g) const { return lnode + (size_t(slot) * N_LNODE + k) * g.n; }
 };

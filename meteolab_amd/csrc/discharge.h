@@ -414,7 +414,7 @@ template <class R> struct Discharge {
     for (int f = f0; f < f1; f++) if (flink[f] == from) { if (at < PRUNE_MAX) prunes[at] = f; at++; }
     __syncthreads();
     if (threadIdx.x == 0) {
-      if (total > PRUNE_MAX) { long long e = 1; memcpy(b.sc(slot) + S_ERROR, &e, 8); }
+      if (total > PRUNE_MAX) atomicOr(reinterpret_cast<unsigned long long *>(b.sc(slot) + S_ERROR), (unsigned long long)ERR_PRUNE);
       for (int k = min(total, PRUNE_MAX) - 1; k >= 0; k--) removeAt(prunes[k], fnode[prunes[k]]);
     }
     __syncthreads();

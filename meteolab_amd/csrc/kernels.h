@@ -97,7 +97,7 @@ template <class R> __global__ void k_init(Geo g, Bufs<R> b) {
   if (i < g.layer) for (int k = 0; k < N_COLUMNS; k++) b.col(slot, k, g)[i] = 0;
 }
 
-// ---- fall speeds, Courant numbers and the hail swath (model.ts:1220-1235 'fall', 575-580 fallParts, 697-706) --------
+// ---- fall speeds and Courant numbers (model.ts:1220-1235 'fall', 575-580 fallParts) ---------------------------------
 // chargeCourant: also the charge falls' Courant numbers (0 when k_charge_courant takes them after the lightning of the
 // step before has changed the charges).
 template <class R> __global__ __launch_bounds__(TPB) void k_fall(Geo g, Bufs<R> b, double dt, int electrified, int chargeCourant) {
@@ -126,13 +126,6 @@ template <class R> __global__ __launch_bounds__(TPB) void k_fall(Geo g, Bufs<R> 
         cour[5] = b.f(slot, F_CHARGE_GRAUPEL, g)[i] != 0 ? c * vcg : 0;
         cour[6] = b.f(slot, F_CHARGE_RAIN, g)[i] != 0 ? c * vr : 0;
       }
-    }
-    // collectHail: the swath at the ground node before the sedimentation moves the hail.
-    if (z == 0 && hail > 1e-9f) {
-      float *hg = b.col(slot, C_HAIL_GROUND, g), *hs = b.col(slot, C_HAIL_SIZE, g);
-      hg[i] = float(hg[i] + R(r0 * R(hail) * R(vh) * R(dt)));
-      R size = hailMaxDiameter(R(hail), r0) * 1000;
-      hs[i] = float(mmax(R(hs[i]), size));
     }
   }
   __shared__ double part[N_FALLS][TPB / 64];
@@ -193,12 +186,14 @@ template <class R> __global__ __launch_bounds__(SED_COLS) void k_sediment(Geo g,
     R pdt = R(dt / parts);
     float *a = b.f(slot, fields[k], g) + col; const float *vt = b.f(slot, falls[k], g) + col;
     if (live) for (int z = 0; z < g.nz; z++) { qa[z * SED_COLS + threadIdx.x] = a[size_t(z) * g.layer]; va[z * SED_COLS + threadIdx.x] = vt[size_t(z) * g.layer]; }
+    // A water fall keeps its ground-node values of each sub-step; k_ground puts them into the gauge.
+    const bool deposit = k < 4;
+    if (deposit && parts > LAND_PARTS && live && col == 0)
+      atomicOr(reinterpret_cast<unsigned long long *>(b.sc(slot) + S_ERROR), (unsigned long long)ERR_LAND_PARTS);
     if (live) {
-      bool deposit = k < 4;
-      float *prec = b.col(slot, C_PRECIPITATION, g);
       for (int p = 0; p < parts; p++) {
         float a0 = qa[threadIdx.x], v0 = va[threadIdx.x];
-        if (deposit) prec[col] = float(prec[col] + R(rho[0]) * R(a0) * R(v0) * pdt);
+        if (deposit) { if (p < LAND_PARTS) b.land(slot, k, p, g)[col] = a0; }
         else charge += double(a0) * v0 * rho[0] * double(pdt) * g.dx * g.dy;
         float here = a0, hv = v0;
         for (int z = 0; z < g.nz; z++) {
@@ -215,6 +210,83 @@ template <class R> __global__ __launch_bounds__(SED_COLS) void k_sediment(Geo g,
     __shared__ double red[SED_COLS / 64];
     storePart(b, slot, 0, blockSum(charge, red));
   }
+}
+
+// ---- the ground under the moving domain (model.ts:637-657 frameOffset, updateGround; 691-706 deposit, collectHail) ----
+// The domain moves over the ground at the frame velocity, so at time t domain column (x, y) lies over the four ground
+// columns from (x + x0, y + y0) with bilinear weights, (x0 + fx, y0 + fy) being the distance moved in columns, periodic
+// (the identity without frame motion). Here each ground column gathers from the domain columns over it (up to four),
+// in the order the domain columns deposit theirs (by column, then by corner), so every ground value is the same chain
+// of float32 sums as when each domain column adds to its four ground columns in turn.
+struct GroundSources {
+  int n;
+  int col[4];    // the domain columns, in deposit order
+  double w[4];   // the weight of this ground column in each one's footprint (> 0)
+};
+__device__ inline GroundSources groundSources(const Geo &g, const double *P, int ground) {
+  const double nx = g.nx, ny = g.ny, ox = P[P_FRAME_U] * P[P_TIME], oy = P[P_FRAME_V] * P[P_TIME];
+  const double sx = fmod(fmod(ox / g.dx, nx) + nx, nx), sy = fmod(fmod(oy / g.dy, ny) + ny, ny);
+  const int x0 = int(floor(sx)), y0 = int(floor(sy));
+  const double fx = sx - x0, fy = sy - y0;
+  const double w[4] = {(1 - fx) * (1 - fy), fx * (1 - fy), (1 - fx) * fy, fx * fy};
+  const int gx = ground % g.nx, gy = ground / g.nx;
+  GroundSources s;
+  s.n = 0;
+  int key[4];
+  for (int j = 0; j < 4; j++) {
+    if (!(w[j] > 0)) continue;
+    // Corner j of domain column (x, y) is ground column (x + x0 + (j & 1), y + y0 + (j >> 1)); 0 <= x0 < nx.
+    const int x = (gx - x0 - (j & 1) + g.nx) % g.nx, y = (gy - y0 - (j >> 1) + g.ny) % g.ny;
+    int k = s.n++;
+    const int kk = (x + g.nx * y) * 4 + j;
+    while (k > 0 && key[k - 1] > kk) { key[k] = key[k - 1]; s.w[k] = s.w[k - 1]; k--; }
+    key[k] = kk; s.w[k] = w[j];
+  }
+  for (int k = 0; k < s.n; k++) s.col[k] = key[k] >> 2;
+  return s;
+}
+// One thread per ground column. what = 0: the hail swath (collectHail), between the fall speeds and the sedimentation
+// that moves the hail; what = 1: the water falls' sub-steps into the rain gauge (sedimentPart's deposit, water falls in
+// order, each sub-step over the columns in order), from the ground-node values k_sediment kept.
+template <class R> __global__ __launch_bounds__(TPB) void k_ground(Geo g, Bufs<R> b, double dt, int what) {
+  const int slot = b.runs[blockIdx.y];
+  const int ground = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ground >= g.layer) return;
+  const GroundSources s = groundSources(g, b.pr(slot), ground);
+  const double *rho = b.lv(slot, L_RHO, g);
+  const R r0 = R(rho[0]);
+  if (what == 0) {
+    const float *hail = b.f(slot, F_HAIL, g), *vh = b.f(slot, F_FALL_HAIL, g);
+    float *hg = b.col(slot, C_HAIL_GROUND, g), *hs = b.col(slot, C_HAIL_SIZE, g);
+    float total = hg[ground], largest = hs[ground];
+    for (int k = 0; k < s.n; k++) {
+      const float h = hail[s.col[k]];
+      if (!(double(h) > 1e-9)) continue;
+      const R mm = R(r0 * R(h) * R(vh[s.col[k]]) * R(dt));
+      total = float(double(total) + __dmul_rn(s.w[k], double(mm)));
+      largest = float(mmax(R(largest), hailMaxDiameter(R(h), r0) * 1000));
+    }
+    hg[ground] = total; hs[ground] = largest;
+    return;
+  }
+  const int falls[4] = {F_FALL_RAIN, F_FALL_SNOW, F_FALL_GRAUPEL, F_FALL_HAIL};
+  float *prec = b.col(slot, C_PRECIPITATION, g);
+  float total = prec[ground];
+  for (int k = 0; k < 4; k++) {
+    const double courant = __longlong_as_double((long long)reinterpret_cast<const unsigned long long *>(b.sc(slot))[S_COURANT + k]);
+    if (!(courant > 0)) continue;  // as k_sediment: nothing of this fall moves
+    int parts = int(ceil(courant / .8)); if (parts < 1) parts = 1;
+    const R pdt = R(dt / parts);
+    const float *vt = b.f(slot, falls[k], g);
+    for (int p = 0; p < parts && p < LAND_PARTS; p++) {
+      const float *a = b.land(slot, k, p, g);
+      for (int q = 0; q < s.n; q++) {
+        const R mm = r0 * R(a[s.col[q]]) * R(vt[s.col[q]]) * pdt;
+        total = float(double(total) + __dmul_rn(s.w[q], double(mm)));
+      }
+    }
+  }
+  prec[ground] = total;
 }
 
 // ---- face mass fluxes and their divergence (advection.ts:190-211 faces + divergence) --------------------------------

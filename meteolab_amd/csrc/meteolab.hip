@@ -67,7 +67,7 @@ struct Pool {
        *spec = nullptr, *mean0 = nullptr, *km = nullptr, *kh = nullptr, *ephi = nullptr, *cols = nullptr, *lev = nullptr,
        *par = nullptr, *scal = nullptr, *zs = nullptr, *dzs = nullptr, *hz = nullptr, *sym = nullptr, *twx = nullptr, *twy = nullptr,
        *lrho = nullptr, *lmark = nullptr, *lnode = nullptr, *chan = nullptr, *cdep = nullptr, *flog = nullptr, *efac = nullptr,
-       *ehat = nullptr, *emean0 = nullptr, *dspec = nullptr, *fillp = nullptr, *gpart = nullptr, *zflag = nullptr;
+       *ehat = nullptr, *emean0 = nullptr, *dspec = nullptr, *fillp = nullptr, *gpart = nullptr, *zflag = nullptr, *landed = nullptr;
   int ftiles = 1, gmax = 1;  // strip tiles of a level; blocks of a run in the launches that carry charge to the ground
   int chunk = 1;          // levels per pass of the discharge's own electric solve (as many as the block's LDS holds)
   RotLevels rot{};        // the rotation diagnostics' levels and threshold
@@ -145,7 +145,7 @@ template <class R> Bufs<R> bufs(const Pool &p, const int *runs) {
   b.efac = (const double *)p.efac; b.lrho = (R *)p.lrho; b.lmark = (int *)p.lmark; b.lnode = (double *)p.lnode; b.chan = (int *)p.chan; b.cdep = (double *)p.cdep; b.flog = (double *)p.flog;
   b.ehat = (double *)p.ehat; b.emean0 = (double *)p.emean0; b.dspec = (double *)p.dspec;
   b.fillp = (double *)p.fillp; b.ftiles = p.ftiles; b.gpart = (double *)p.gpart; b.gmax = p.gmax;
-  b.zflag = (unsigned char *)p.zflag;
+  b.zflag = (unsigned char *)p.zflag; b.landed = (float *)p.landed;
   return b;
 }
 
@@ -229,8 +229,12 @@ template <class R> void launchPart(const Pool &p, const int *list, int nr, doubl
     mark("k_begin");
     hipLaunchKernelGGL(k_fall<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, dt, el, split ? 0 : el);
     mark("k_fall");
+    hipLaunchKernelGGL(k_ground<R>, dim3(blocks(g.layer, TPB), nr), dim3(TPB), 0, s, g, b, dt, 0);
+    mark("k_ground");
     hipLaunchKernelGGL(k_sediment<R>, dim3(blocks(g.layer, SED_COLS), nr), dim3(SED_COLS), size_t(2) * SED_COLS * g.nz * sizeof(float), s, g, b, dt, 0, el && !split ? N_FALLS : 4);
     mark("k_sediment");
+    hipLaunchKernelGGL(k_ground<R>, dim3(blocks(g.layer, TPB), nr), dim3(TPB), 0, s, g, b, dt, 1);
+    mark("k_ground");
     if (el && !split) {
       hipLaunchKernelGGL(k_ground_total<R>, dim3(nr), dim3(256), 0, s, g, b, 0, int(blocks(g.layer, SED_COLS)));
       mark("k_ground_total");
@@ -363,10 +367,10 @@ void fillTables(Pool &p, std::vector<char> &sym, std::vector<char> &twx, std::ve
 }
 
 struct Sizes {
-  size_t fields, stage, mass, node, node8, spec, mean0, cols, lev, par, scal, lmark, lnode, chan, cdep, flog, fillp, gpart, zflag;
+  size_t fields, stage, mass, node, node8, spec, mean0, cols, lev, par, scal, lmark, lnode, chan, cdep, flog, fillp, gpart, zflag, landed;
   size_t slot() const {
     return fields + 2 * stage + mass + 3 * node + 4 * node8 + 3 * spec + 2 * mean0 + cols + lev + par + scal + lmark + lnode + chan + cdep + flog +
-           fillp + gpart + zflag;
+           fillp + gpart + zflag + landed;
   }
 };
 // Blocks of a run in the launches that carry charge to the ground: the charge falls and the conduction.
@@ -381,6 +385,7 @@ Sizes sizesOf(const Geo &g, size_t rb) {
   s.fillp = size_t(N_CONSERVED) * g.nz * 3 * 8;
   s.gpart = 2 * size_t(groundBlocks(g)) * 8;
   s.zflag = (size_t(N_CARRIED) * g.nz + 15) / 16 * 16;
+  s.landed = size_t(4) * LAND_PARTS * g.layer * 4;
   return s;
 }
 int geoOf(ml_ctx *c, const ml_grid *o, Geo &g, Plan &pl) {
@@ -455,7 +460,7 @@ void freePool(Pool *p) {
   void *all[] = {p->fields, p->stx, p->sty, p->mass, p->phi, p->cells, p->press, p->spec, p->mean0, p->km, p->kh, p->ephi, p->cols,
                  p->lev, p->par, p->scal, p->zs, p->dzs, p->hz, p->sym, p->twx, p->twy, p->single, p->diag, p->dlist, p->lrho,
                  p->lmark, p->lnode, p->chan, p->cdep, p->flog, p->efac, p->ehat, p->emean0, p->dspec, p->fillp,
-                 p->gpart, p->pmap, p->zflag};
+                 p->gpart, p->pmap, p->zflag, p->landed};
   for (void *a : all) if (a) hipFree(a);
   delete p;
 }
@@ -525,7 +530,8 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
     {&p->kh, s.node * cap}, {&p->ephi, s.node8 * cap}, {&p->cols, s.cols * cap}, {&p->lev, s.lev * cap}, {&p->par, s.par * cap},
     {&p->scal, s.scal * cap}, {(void **)&p->diag, cap * DIAG_ROW * 8}, {(void **)&p->dlist, cap * sizeof(int)}, {&p->lrho, s.node * cap}, {&p->lmark, s.lmark * cap},
     {&p->lnode, s.lnode * cap}, {&p->chan, s.chan * cap}, {&p->cdep, s.cdep * cap}, {&p->flog, s.flog * cap},
-    {&p->fillp, s.fillp * cap}, {&p->gpart, s.gpart * cap}, {&p->pmap, size_t(g.layer) * 4}, {&p->zflag, s.zflag * cap}};
+    {&p->fillp, s.fillp * cap}, {&p->gpart, s.gpart * cap}, {&p->pmap, size_t(g.layer) * 4}, {&p->zflag, s.zflag * cap},
+    {&p->landed, s.landed * cap}};
   p->ftiles = 1; p->gmax = groundBlocks(g);
   for (auto &a : allocs) {
     hipError_t e = hipMalloc(a.ptr, a.bytes);
