@@ -134,6 +134,19 @@ int ml_run_set_time(ml_ctx *ctx, int run, double time);
  * held (s, header.persisted, rotation.persisted). The flash log and the first-flash time are left as they are (a new
  * run has none: applyState forgets the flashes after the state's time). */
 int ml_run_set_totals(ml_ctx *ctx, int run, double charge_ground, int64_t flashes_ic, int64_t flashes_cg, double persisted);
+/* Interventions (InterventionKind, src/core/intervene.ts:13). */
+enum { ML_INT_THERMAL = 0, ML_INT_WARM, ML_INT_COOL, ML_INT_MOISTEN, ML_INT_DRY, ML_INT_COLDPOOL, ML_INT_VORTEX, ML_INTERVENTIONS };
+/* intervene(m, {kind, x, y, strength}) (src/core/intervene.ts:38-68; the worker's 'intervene' message,
+ * src/simulation.worker.ts:219, and injectBubble, model.ts:482): the kind's patch (INTERVENTIONS, intervene.ts:22-30)
+ * centred at (x, y) m from the domain's corner, `strength` times its standard size, added to the run's fields as they
+ * are. thermal_radius: the thermal's e-folding radius in km (SimConfig.bubbleRadius; <= 0: its default 4.2). out
+ * (may be NULL): the heat (J) and vapour (kg) put in, also added to the run's counts (ml_run_added). Between steps
+ * only; stream-synchronising. */
+int ml_run_intervene(ml_ctx *ctx, int run, int kind, double x, double y, double strength, double thermal_radius, double *out);
+/* AtmosphereModel.added (model.ts:227-228): heat (J), vapour (kg) put in by interventions, and how many, 3 float64. */
+int ml_run_added(ml_ctx *ctx, int run, double *out);
+/* Sets those counts (applyState's header.added, src/core/state.ts:78). */
+int ml_run_set_added(ml_ctx *ctx, int run, double heat, double vapour, double count);
 /* The rotation diagnostics of the run's last step (updateRotation, model.ts:1609-1648; RotationState :142-158),
  * ML_ROT_COUNT float64: the largest updraught helicity 2-5 km (m2/s2) and its column (x, y), the largest
  * anticyclonic one, how long the mesocyclone criterion has held (s), the largest 0-1 km and 0-3 km helicity and the

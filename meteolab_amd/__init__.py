@@ -21,6 +21,8 @@ FIELDS = CARRIED + ["fieldZ", "fallRain", "fallSnow", "fallGraupel", "fallHail",
                     "fallChargeGraupel", "condensate"]
 FIELD_ID = {name: k for k, name in enumerate(FIELDS)}
 COLUMNS = ["precipitation", "hailGround", "hailSize", "gustMax", "uhMax"]
+# InterventionKind (src/core/intervene.ts:13) in ml_run_intervene's order.
+INTERVENTIONS = ["thermal", "warm", "cool", "moisten", "dry", "coldPool", "vortex"]
 LEVELS = ["p", "exner", "theta", "q", "u", "v", "rho"]
 DIAG = ["time", "maxW", "minW", "maxThetaP", "rainTotal", "water", "charge", "clipped", "negativeFilled",
         "chargeGround", "flashesIC", "flashesCG", "firstFlash", "error"]
@@ -89,6 +91,9 @@ SIGNATURES = {
     "ml_run_set_time": (_i, [_vp, _i, _d]),
     "ml_run_set_totals": (_i, [_vp, _i, _d, _i64, _i64, _d]),
     "ml_run_rotation": (_i, [_vp, _i, _dp]),
+    "ml_run_intervene": (_i, [_vp, _i, _i, _d, _d, _d, _d, _dp]),
+    "ml_run_added": (_i, [_vp, _i, _dp]),
+    "ml_run_set_added": (_i, [_vp, _i, _d, _d, _d]),
     "ml_run_refresh_rotation": (_i, [_vp, _i]),
     "ml_run_set_swaths": (_i, [_vp, _i, _i]),
     "ml_probability_map": (_i, [_vp, _ip, _i, _i, _d, _fp]),
@@ -303,6 +308,26 @@ class Run:
     def set_totals(self, charge_ground, flashes_ic, flashes_cg, persisted=0.0):
         """The totals of a saved state's header (ml_run_set_totals)."""
         self.ctx.check(lib().ml_run_set_totals(self.ctx.h, self.h, float(charge_ground), int(flashes_ic), int(flashes_cg), float(persisted)))
+
+    def intervene(self, kind, x, y, strength=1.0, thermal_radius=0.0):
+        """intervene (src/core/intervene.ts:38-68) on the run's fields (ml_run_intervene): `kind` one of INTERVENTIONS at
+        (x, y) m from the domain's corner; returns {"heat": J, "vapour": kg} put in."""
+        if kind not in INTERVENTIONS:
+            raise ValueError(f"intervention kind {kind!r}: one of {INTERVENTIONS}")
+        out = np.empty(2, np.float64)
+        self.ctx.check(lib().ml_run_intervene(self.ctx.h, self.h, INTERVENTIONS.index(kind), float(x), float(y), float(strength),
+                                              float(thermal_radius), _dptr(out)))
+        return {"heat": float(out[0]), "vapour": float(out[1])}
+
+    def added(self):
+        """AtmosphereModel.added (ml_run_added): the heat (J) and vapour (kg) put in by interventions, and their count."""
+        out = np.empty(3, np.float64)
+        self.ctx.check(lib().ml_run_added(self.ctx.h, self.h, _dptr(out)))
+        return {"heat": float(out[0]), "vapour": float(out[1]), "count": int(out[2])}
+
+    def set_added(self, heat, vapour, count):
+        """Sets those counts (a saved state's header.added; ml_run_set_added)."""
+        self.ctx.check(lib().ml_run_set_added(self.ctx.h, self.h, float(heat), float(vapour), float(count)))
 
     def rotation(self):
         """The rotation diagnostics of the last step (ml_run_rotation), by the names of ROTATION."""

@@ -72,9 +72,12 @@ enum : int {
                          // solve; cleared with its right-hand side, read by the discharge (which may run during the
                          // next step's dynamics)
   S_ROT = 48,            // the rotation diagnostics (model.ts:142-158 RotationState), ROT_* from here
+  S_ADDED = 59,          // 3 doubles: heat (J), vapour (kg) put in by interventions and their count (model.ts:227-228)
+  S_LAST_ADDED = 62,     // 2 doubles: the heat and vapour of the run's last intervention
   N_SCALARS = 64
 };
 enum : int { ROT_UH = 0, ROT_X, ROT_Y, ROT_ANTI, ROT_PERSISTED, ROT_UH01, ROT_UH03, ROT_LOW, ROT_CORE_ZETA, ROT_CORE_UH, ROT_CORE_ANTI, N_ROT };
+static_assert(S_ROT + N_ROT <= S_ADDED, "the rotation diagnostics overlap the intervention counts");
 // Lightning channel limits (lightning.ts:32-37) and the per-run flash log.
 constexpr int MAX_CHANNEL = 3000, STROKES = 4, FLASH_LOG = 64, FLASH_FIELDS = 8;
 // The lightning's per-node int arrays and per-front-entry arrays of a slot.

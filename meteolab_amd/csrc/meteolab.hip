@@ -23,6 +23,7 @@
 #include "spectral.h"
 #include "rotation.h"
 #include "diagnostics.h"
+#include "intervene.h"
 
 using namespace ml;
 
@@ -60,6 +61,7 @@ struct Batch {
 
 struct Pool {
   Geo g{};
+  double width = 0, depth = 0;  // the domain's extent (m), as the grid gives it
   Plan plan{};
   int cap = 0, electrified = 1;
   size_t rb = 4;  // bytes of the arithmetic type
@@ -516,6 +518,7 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
   if (id >= MAX_POOLS) return fail(c, ML_E_FULL, "too many pools");
   Pool *p = new Pool();
   if (int r = geoOf(c, grid, p->g, p->plan)) { delete p; return r; }
+  p->width = grid->width; p->depth = grid->depth;
   POOLCHK(hipSetDevice(c->device));
   const Geo &g = p->g;
   p->cap = capacity; p->electrified = electrified ? 1 : 0; p->rb = c->precision == 64 ? 8 : 4;
@@ -749,6 +752,56 @@ int ml_run_set_totals(ml_ctx *c, int run, double charge_ground, int64_t flashes_
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipMemcpyAsync(sc + S_CHARGE_GROUND, v, sizeof(v), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(sc + S_ROT + ROT_PERSISTED, &persisted, 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ml_run_intervene(ml_ctx *c, int run, int kind, double x, double y, double strength, double thermal_radius, double *out) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot)) return fail(c, ML_E_ARG, "intervene: bad run");
+  if (kind < 0 || kind >= ML_INTERVENTIONS || !std::isfinite(x) || !std::isfinite(y) || !std::isfinite(strength) || !std::isfinite(thermal_radius))
+    return fail(c, ML_E_ARG, "intervene: a kind 0-6 and finite position, strength and radius");
+  // INTERVENTIONS (intervene.ts:22-30): radius (m, < 0: the thermal's own), depth (m), top (m, < 0: REACH = 3 depths),
+  // theta (K), vapour (kg/kg), w (m/s), spin (m/s) per unit strength.
+  static const double table[ML_INTERVENTIONS][7] = {
+      {-1, 1800, 2700, 3.2, 0, 1.1, 0}, {5000, 1000, -1, 2, 0, 0, 0}, {5000, 1000, -1, -2, 0, 0, 0},
+      {5000, 1200, -1, 0, .002, 0, 0}, {5000, 1200, -1, 0, -.002, 0, 0}, {4000, 700, -1, -6, 0, 0, 0},
+      {3000, 3000, -1, 0, 0, 0, 10}};
+  const double *t = table[kind];
+  Patch k;
+  k.radius = t[0] > 0 ? t[0] : (thermal_radius > 0 ? thermal_radius : 4.2) * 1000;
+  k.depth = t[1]; k.theta = t[3]; k.vapour = t[4]; k.w = t[5]; k.spin = t[6];
+  k.a0 = strength; k.x = x; k.y = y; k.width = p->width; k.depth_y = p->depth;
+  const double top = t[2] > 0 ? t[2] : t[1] * 3;
+  k.levels = 0;
+  while (k.levels < p->g.nz && p->zsH[k.levels] < top) k.levels++;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (p->rb == 8) hipLaunchKernelGGL(k_intervene<double>, dim3(1), dim3(INTERVENE_THREADS), 0, c->stream, p->g, bufs<double>(*p, p->single), slot, k);
+  else hipLaunchKernelGGL(k_intervene<float>, dim3(1), dim3(INTERVENE_THREADS), 0, c->stream, p->g, bufs<float>(*p, p->single), slot, k);
+  HIPCHK(c, hipGetLastError());
+  double last[2];
+  HIPCHK(c, hipMemcpyAsync(last, (double *)p->scal + size_t(slot) * N_SCALARS + S_LAST_ADDED, sizeof(last), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (out) { out[0] = last[0]; out[1] = last[1]; }
+  return 0;
+}
+
+int ml_run_added(ml_ctx *c, int run, double *out) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot) || !out) return fail(c, ML_E_ARG, "added: bad run");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(out, (double *)p->scal + size_t(slot) * N_SCALARS + S_ADDED, 3 * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ml_run_set_added(ml_ctx *c, int run, double heat, double vapour, double count) {
+  Pool *p; int slot;
+  if (!c || !decode(c, run, p, slot) || !std::isfinite(heat) || !std::isfinite(vapour) || !(count >= 0) || count != std::floor(count))
+    return fail(c, ML_E_ARG, "set added: bad run or value");
+  const double v[3] = {heat, vapour, count};
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync((double *)p->scal + size_t(slot) * N_SCALARS + S_ADDED, v, sizeof(v), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }

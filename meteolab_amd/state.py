@@ -6,10 +6,11 @@ run can be continued later or passed on, in the reference's layout:
     STATE_COLUMNS order, node i = x + nx (y + ny z).
 
 save_state writes version 5 (state.ts:18): the two-moment numbers and volumes, which this step does not carry, are
-zeros, and the water and charge budgets, which it does not keep, are zeros. read_state takes versions 1-5 and refuses
+zeros, and the water and charge budgets, which it does not keep, are zeros; the heat and vapour put in by
+interventions (header.added) are the run's. read_state takes versions 1-5 and refuses
 what the reference refuses, with its messages (state.ts:53-68). apply_state puts a state into a run of the same grid
-(state.ts:74-110): the fields the file holds, zeros for those it does not, the columns, the model time and the
-lightning totals and how long the rotation has held; the flash log starts empty. Everything else a step needs it recomputes from these, except the
+(state.ts:74-110): the fields the file holds, zeros for those it does not, the columns, the model time, the
+lightning totals, how long the rotation has held and the interventions' counts; the flash log starts empty. Everything else a step needs it recomputes from these, except the
 vertical field of the last discharge, which a step computes at its end: a restored run's first step charges without
 it, as the reference's applyState leaves it.
 """
@@ -52,7 +53,7 @@ def save_state(run, config=None):
               "chargeGround": _number(d["chargeGround"]),
               "flashCount": {"ic": int(d["flashesIC"]), "cg": int(d["flashesCG"])},
               "water": {"condensed": 0, "evaporated": 0, "rainEvaporated": 0, "surface": 0, "start": 0},
-              "added": {"heat": 0, "vapour": 0, "count": 0},
+              "added": {k: _number(v) if k != "count" else int(v) for k, v in run.added().items()},
               "charge": {"separated": 0, "ic": 0, "cgNegative": 0, "cgPositive": 0, "precipitation": 0, "conduction": 0}}
     text = json.dumps(header, separators=(",", ":")).encode()
     start = math.ceil((8 + len(text)) / 4) * 4
@@ -113,5 +114,7 @@ def apply_state(run, state):
     run.set_time(header["time"])
     count = header.get("flashCount") or {}
     run.set_totals(header.get("chargeGround") or 0.0, count.get("ic") or 0, count.get("cg") or 0, header["persisted"])
+    added = header.get("added") or {}
+    run.set_added(added.get("heat") or 0.0, added.get("vapour") or 0.0, added.get("count") or 0)
     # The rotation diagnostics are not in the file: recomputed from its fields (state.ts:73-74).
     run.refresh_rotation()

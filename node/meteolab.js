@@ -19,6 +19,8 @@ const FIELDS = ['rain', 'snow', 'graupel', 'hail', 'u', 'v', 'w', 'theta', 'q', 
   'chargeIce', 'chargeGraupel', 'chargeRain', 'chargeAir', 'fieldZ', 'fallRain', 'fallSnow', 'fallGraupel', 'fallHail',
   'fallChargeIce', 'fallChargeGraupel', 'condensate']
 const COLUMNS = ['precipitation', 'hailGround', 'hailSize', 'gustMax', 'uhMax']
+// InterventionKind (src/core/intervene.ts:13) in ml_run_intervene's order.
+const INTERVENTIONS = ['thermal', 'warm', 'cool', 'moisten', 'dry', 'coldPool', 'vortex']
 const DIAG = ['time', 'maxW', 'minW', 'maxThetaP', 'rainTotal', 'water', 'charge', 'clipped', 'negativeFilled',
   'chargeGround', 'flashesIC', 'flashesCG', 'firstFlash', 'error']
 const LEVELS = ['p', 'exner', 'theta', 'q', 'u', 'v', 'rho']
@@ -170,6 +172,26 @@ class MeteolabRun {
     this.device.call(() => addon.rotation(this.device.ctx, this.handle, out))
     return Object.fromEntries(ROTATION.map((k, i) => [k, out[i]]))
   }
+  /**
+   * intervene (src/core/intervene.ts:38-68; the worker's 'intervene' message, src/simulation.worker.ts:219) on the
+   * run's fields: { kind, x, y, strength } as the reference's Intervention; thermalRadius is SimConfig.bubbleRadius (km,
+   * absent: 4.2). Returns { heat, vapour } put in (J, kg), also added to added().
+   */
+  intervene (it, thermalRadius = 0) {
+    const kind = INTERVENTIONS.indexOf(it.kind)
+    if (kind < 0) throw new Error(`intervention kind ${it.kind}: one of ${INTERVENTIONS.join(', ')}`)
+    const out = new Float64Array(2)
+    const strength = it.strength === undefined ? 1 : it.strength
+    this.device.call(() => addon.intervene(this.device.ctx, this.handle, kind, it.x, it.y, strength, thermalRadius || 0, out))
+    return { heat: out[0], vapour: out[1] }
+  }
+  /** AtmosphereModel.added: { heat, vapour, count } put in by interventions. */
+  added () {
+    const out = new Float64Array(3)
+    this.device.call(() => addon.added(this.device.ctx, this.handle, out))
+    return { heat: out[0], vapour: out[1], count: out[2] }
+  }
+  setAdded (a) { this.device.call(() => addon.setAdded(this.device.ctx, this.handle, a.heat || 0, a.vapour || 0, a.count || 0)) }
   /** addSwaths after every step (src/ensemble/member.ts:85-100) into the gustMax and uhMax columns. */
   setSwaths (on) { this.device.call(() => addon.setSwaths(this.device.ctx, this.handle, on ? 1 : 0)) }
   /** The rotation diagnostics of the fields as they are, keeping how long the rotation has held (refreshRotation). */
@@ -256,7 +278,8 @@ class GpuDynamics {
 
 // The .meteolab state file (src/core/state.ts:1-110): 'MLAB', the header's length (uint32 LE), the JSON header,
 // padding to 4 bytes, float32 fields in STATE_FIELDS order, then the STATE_COLUMNS. Version 5 is written; the
-// two-moment numbers and volumes (not carried by this step) and the water and charge budgets (not kept) are zeros.
+// two-moment numbers and volumes (not carried by this step) and the water and charge budgets (not kept) are zeros; the
+// interventions' counts (header.added) are the run's.
 const STATE_FIELDS = ['u', 'v', 'w', 'theta', 'q', 'cloud', 'rain', 'cold', 'ice', 'snow', 'graupel', 'hail', 'chargeIce',
   'chargeGraupel', 'chargeRain', 'chargeAir', 'iceNumber', 'rainNumber', 'graupelNumber', 'graupelVolume', 'hailNumber', 'hailVolume']
 const STATE_COLUMNS = ['precipitation', 'hailGround', 'hailSize']
@@ -269,7 +292,7 @@ function saveState (r, config = null) {
   const g = r.pool.grid, n = r.pool.n, layer = r.pool.layer, d = r.diagnostics()
   const header = { format: 'meteolab-state', version: STATE_VERSION, config: config === null ? r.spec : config, time: d.time, persisted: r.rotation().persisted,
     nx: g.nx, ny: g.ny, nz: g.nz, chargeGround: d.chargeGround, flashCount: { ic: d.flashesIC, cg: d.flashesCG },
-    water: { condensed: 0, evaporated: 0, rainEvaporated: 0, surface: 0, start: 0 }, added: { heat: 0, vapour: 0, count: 0 },
+    water: { condensed: 0, evaporated: 0, rainEvaporated: 0, surface: 0, start: 0 }, added: r.added(),
     charge: { separated: 0, ic: 0, cgNegative: 0, cgPositive: 0, precipitation: 0, conduction: 0 } }
   const json = Buffer.from(JSON.stringify(header)), start = Math.ceil((8 + json.length) / 4) * 4
   const buffer = Buffer.alloc(start + (STATE_FIELDS.length * n + STATE_COLUMNS.length * layer) * 4)
@@ -309,6 +332,7 @@ function applyState (r, state) {
   r.time = h.time
   const c = h.flashCount || {}
   r.setTotals(h.chargeGround || 0, c.ic || 0, c.cg || 0, h.persisted)
+  r.setAdded(h.added || {})
   // The rotation diagnostics are not in the file: recomputed from its fields (state.ts:73-74).
   r.refreshRotation()
 }
@@ -384,5 +408,5 @@ function perturbRun (grid, prof, spec, sliders = SLIDER_NODES) {
   return [p, Object.assign({}, spec, { bubble_amp: amp, bubble_x: grid.width / 3 + ox, bubble_y: grid.depth / 2 + oy })]
 }
 
-module.exports = { MeteolabDevice, runMembers, MeteolabPool, MeteolabRun, GpuDynamics, FIELDS, CARRIED, COLUMNS, DIAG, PARTS, STANDARD, LARGE, profile, run, addon,
+module.exports = { MeteolabDevice, runMembers, MeteolabPool, MeteolabRun, GpuDynamics, FIELDS, CARRIED, COLUMNS, INTERVENTIONS, DIAG, PARTS, STANDARD, LARGE, profile, run, addon,
   STATE_FIELDS, STATE_COLUMNS, ROTATION, MODEL_DIAGNOSTICS, saveState, readState, applyState, mulberry32, UNCERTAINTY, SLIDER_NODES, perturbRun }

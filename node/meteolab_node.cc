@@ -288,6 +288,26 @@ napi_value SetTotals(napi_env env, napi_callback_info info) {
   return status(env, ctx, ml_run_set_totals(ctx, run, q, int64_t(ic), int64_t(cg), held));
 }
 
+// intervene(ctx, run, kind, x, y, strength, thermalRadius, Float64Array out[2]): ml_run_intervene.
+napi_value Intervene(napi_env env, napi_callback_info info) {
+  napi_value a[8]; if (!args(env, info, 8, a)) return nullptr;
+  ml_ctx *ctx = ctxOf(env, a[0]); if (!ctx) return nullptr;
+  int32_t run, kind; double x, y, strength, radius;
+  if (!intOf(env, a[1], run) || !intOf(env, a[2], kind) || !numberOf(env, a[3], x) || !numberOf(env, a[4], y) ||
+      !numberOf(env, a[5], strength) || !numberOf(env, a[6], radius)) return nullptr;
+  double *out; if (!typed(env, a[7], napi_float64_array, 2, out)) return nullptr;
+  return status(env, ctx, ml_run_intervene(ctx, run, kind, x, y, strength, radius, out));
+}
+
+// setAdded(ctx, run, heat, vapour, count): a saved state's header.added (ml_run_set_added).
+napi_value SetAdded(napi_env env, napi_callback_info info) {
+  napi_value a[5]; if (!args(env, info, 5, a)) return nullptr;
+  ml_ctx *ctx = ctxOf(env, a[0]); if (!ctx) return nullptr;
+  int32_t run; double heat, vapour, count;
+  if (!intOf(env, a[1], run) || !numberOf(env, a[2], heat) || !numberOf(env, a[3], vapour) || !numberOf(env, a[4], count)) return nullptr;
+  return status(env, ctx, ml_run_set_added(ctx, run, heat, vapour, count));
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct { const char *name; napi_callback fn; } fns[] = {
     {"ctxCreate", CtxCreate}, {"ctxDestroy", CtxDestroy}, {"deviceName", DeviceName}, {"poolCreate", PoolCreate},
@@ -298,6 +318,7 @@ napi_value Init(napi_env env, napi_value exports) {
     {"physics", Part<ml_physics>}, {"mix", Part<ml_mix>}, {"afterStep", Part<ml_after_step>}, {"fallParts", FallParts},
     {"setTotals", SetTotals}, {"rotation", Doubles<ml_run_rotation>}, {"refreshRotation", RunOp<ml_run_refresh_rotation>},
     {"modelDiagnostics", ModelDiagnostics}, {"setSwaths", SetSwaths}, {"probabilityMap", ProbabilityMap},
+    {"intervene", Intervene}, {"added", Doubles<ml_run_added>}, {"setAdded", SetAdded},
   };
   for (auto &f : fns) {
     napi_value v;

@@ -21,6 +21,7 @@ EXTRA = ["fieldZ", "fallRain", "fallSnow", "fallGraupel", "fallHail", "fallCharg
 FIELDS = CARRIED + EXTRA
 FIELD_ID = {name: k for k, name in enumerate(FIELDS)}
 COLUMNS = ["precipitation", "hailGround", "hailSize", "gustMax", "uhMax"]
+INTERVENTIONS = ["thermal", "warm", "cool", "moisten", "dry", "coldPool", "vortex"]
 # ModelDiagnostics (model.ts:161-180) in the order orc_model_diagnostics writes it.
 MODEL_DIAGNOSTICS = ["updraft", "downdraft", "rainRate", "rainTotal", "cloudTop", "thermalTop", "maxCloud", "coldMax",
                      "cores", "shear06", "outflow", "outflowLow", "gust", "hail", "clipped"]
@@ -85,6 +86,9 @@ def load(path):
         L.orc_diag.argtypes = [vp, dp]
         L.orc_pressure.argtypes = [vp, dp]
         L.orc_set_time.argtypes = [vp, d]
+        L.orc_intervene.argtypes = [vp, i, d, d, d, d, dp]
+        L.orc_intervene.restype = i
+        L.orc_added.argtypes = [vp, dp]
         L.orc_rotation.argtypes = [vp, dp]
         L.orc_update_rotation.argtypes = [vp, d, i]
         L.orc_model_diagnostics.argtypes = [vp, dp]
@@ -257,6 +261,17 @@ class OracleRun:
 
     def set_time(self, t):
         lib().orc_set_time(self.h, t)
+
+    def intervene(self, kind, x, y, strength=1.0, thermal_radius=0.0):
+        """intervene (intervene.ts:38-68): returns {"heat": J, "vapour": kg} put in."""
+        out = np.empty(2, np.float64)
+        assert lib().orc_intervene(self.h, INTERVENTIONS.index(kind), float(x), float(y), float(strength), float(thermal_radius), dptr(out)) == 0
+        return {"heat": float(out[0]), "vapour": float(out[1])}
+
+    def added(self):
+        out = np.empty(3, np.float64)
+        lib().orc_added(self.h, dptr(out))
+        return {"heat": float(out[0]), "vapour": float(out[1]), "count": int(out[2])}
 
     def rotation(self):
         """The rotation diagnostics of the last step (updateRotation), by the names of ROTATION."""

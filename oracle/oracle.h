@@ -73,6 +73,12 @@ void orc_diag(void *model, double *out);
 /* The model's projection pressure (cells) after the last step, (nz-1) * layer float64. */
 void orc_pressure(void *model, double *out);
 void orc_set_time(void *model, double t);
+/* intervene (intervene.ts:38-68): kind 0-6 = thermal, warm, cool, moisten, dry, coldPool, vortex at (x, y) m from the
+ * domain's corner; thermal_radius (km, <= 0: 4.2) is the thermal's e-folding radius (SimConfig.bubbleRadius). out[2]:
+ * the heat (J) and vapour (kg) put in. 0, or -1 for an unknown kind. */
+int orc_intervene(void *model, int kind, double x, double y, double strength, double thermal_radius, double *out);
+/* AtmosphereModel.added: heat (J), vapour (kg), count. */
+void orc_added(void *model, double *out);
 /* Rotation diagnostics of the last step (updateRotation, model.ts:1609-1648), ORC_ROT float64 in this order. */
 enum {
   ORC_ROT_UH = 0, ORC_ROT_X, ORC_ROT_Y, ORC_ROT_ANTI, ORC_ROT_PERSISTED, ORC_ROT_UH01, ORC_ROT_UH03, ORC_ROT_LOW,

@@ -1072,6 +1072,7 @@ struct Model {
   FluxTransport flux; PressureSolver solver; std::unique_ptr<Lightning> lightning;
   F64 pressure, divergence, km, kh, levelWeight, uEnv, vEnv;
   double time = 0, negativeFilled = 0, chargeGround = 0, firstFlash = -1; long clipped = 0; int flashesIC = 0, flashesCG = 0;
+  double addedHeat = 0, addedVapour = 0, addedCount = 0;  // AtmosphereModel.added (model.ts:227-228)
   bool electrified = true;
   double cs, dragCd, f_coriolis;
   // RotationState (model.ts:142-158) in the order of ORC_ROT_*, and the levels updateRotation sums over.
@@ -1597,6 +1598,58 @@ void orc_grid_levels(void *m, double *zs, double *hz) { Model *mm = static_cast<
 void orc_diag(void *m, double *out) { static_cast<Model *>(m)->diag(out); }
 void orc_pressure(void *m, double *out) { Model *mm = static_cast<Model *>(m); std::memcpy(out, mm->pressure.data(), sizeof(double) * mm->pressure.size()); }
 void orc_set_time(void *m, double t) { static_cast<Model *>(m)->time = t; }
+// intervene (intervene.ts:38-68) with the patch table INTERVENTIONS (intervene.ts:22-30): a smooth patch round on the
+// map (e-folding radius, the thermal's own bubbleRadius, default 4.2 km) fading with height, added to the fields as
+// they are; the heat (J) and vapour (kg) put in are counted into AtmosphereModel.added. Flat ground (no solid nodes).
+int orc_intervene(void *m, int kind, double x, double y, double strength, double thermal_radius, double *out) {
+  struct Patch { double radius, depth, top, theta, vapour, w, spin; };
+  static const Patch table[7] = {
+      {-1, 1800, 2700, 3.2, 0, 1.1, 0},  // thermal
+      {5000, 1000, -1, 2, 0, 0, 0},      // warm
+      {5000, 1000, -1, -2, 0, 0, 0},     // cool
+      {5000, 1200, -1, 0, .002, 0, 0},   // moisten
+      {5000, 1200, -1, 0, -.002, 0, 0},  // dry
+      {4000, 700, -1, -6, 0, 0, 0},      // coldPool
+      {3000, 3000, -1, 0, 0, 0, 10},     // vortex
+  };
+  if (kind < 0 || kind > 6) return -1;
+  Model *mm = static_cast<Model *>(m);
+  const Grid &g = mm->g; const Environment &e = mm->env; const Patch &k = table[kind];
+  const double MOIST_CAP = .95, REACH = 3;
+  const double radius = k.radius > 0 ? k.radius : (thermal_radius > 0 ? thermal_radius : 4.2) * 1000, a0 = strength;
+  const double top = k.top > 0 ? k.top : k.depth * REACH, W = g.width, D = g.depth;
+  double heat = 0, vapour = 0;
+  for (int z = 0; z < g.nz && g.zs[z] < top; z++) {
+    const double mass = e.rho[z] * g.dx * g.dy * g.hz[z], exner = e.exner[z], p = e.p[z];
+    for (int yy = 0; yy < g.ny; yy++) for (int xx = 0; xx < g.nx; xx++) {
+      double ddx = xx * g.dx - x, ddy = yy * g.dy - y;
+      if (ddx > W / 2) ddx -= W;
+      if (ddx < -W / 2) ddx += W;
+      if (ddy > D / 2) ddy -= D;
+      if (ddy < -D / 2) ddy += D;
+      const int i = xx + g.nx * (yy + g.ny * z);
+      const double r2 = (ddx / radius) * (ddx / radius) + (ddy / radius) * (ddy / radius);
+      const double a = std::exp(-r2 - (g.zs[z] / k.depth) * (g.zs[z] / k.depth)) * a0;
+      float &theta = mm->f[ORC_THETA][i], &q = mm->f[ORC_Q][i];
+      if (k.theta != 0) { theta = float(theta + k.theta * a); heat += CP * exner * k.theta * a * mass; }
+      if (k.w != 0) mm->f[ORC_W][i] = float(mm->f[ORC_W][i] + k.w * a);
+      if (k.vapour != 0) {
+        const double before = q, cap = MOIST_CAP * qsatP(theta * exner - 273.15, p);
+        q = float(k.vapour > 0 ? std::max(before, std::min(cap, before + k.vapour * a)) : std::max(0.0, before + k.vapour * a));
+        vapour += (q - before) * mass;
+      }
+      if (k.spin != 0) {
+        const double r = std::sqrt(r2), zd = g.zs[z] / k.depth;
+        const double speed = k.spin * a0 * r * std::exp((1 - r2) / 2) * std::exp(-(zd * zd)), d = std::hypot(ddx, ddy);
+        if (d > 0) { mm->f[ORC_U][i] = float(mm->f[ORC_U][i] - speed * ddy / d); mm->f[ORC_V][i] = float(mm->f[ORC_V][i] + speed * ddx / d); }
+      }
+    }
+  }
+  mm->addedHeat += heat; mm->addedVapour += vapour; mm->addedCount += 1;
+  if (out) { out[0] = heat; out[1] = vapour; }
+  return 0;
+}
+void orc_added(void *m, double *out) { Model *mm = static_cast<Model *>(m); out[0] = mm->addedHeat; out[1] = mm->addedVapour; out[2] = mm->addedCount; }
 void orc_update_rotation(void *m, double dt, int keep) {
   Model *mm = static_cast<Model *>(m);
   const double held = mm->rotation[ORC_ROT_PERSISTED];

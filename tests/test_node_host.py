@@ -19,7 +19,7 @@ needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reaso
 ABI_FUNCTIONS = ["ctxCreate", "ctxDestroy", "deviceName", "poolCreate", "poolDestroy", "slotBytes", "runCreate", "runRelease",
                  "upload", "download", "uploadColumn", "downloadColumn", "step", "sync", "diag", "levels", "pressure", "flashes", "setTime",
                  "dynamics", "physics", "mix", "afterStep", "fallParts", "setTotals",
-                 "rotation", "refreshRotation", "modelDiagnostics", "setSwaths", "probabilityMap"]
+                 "rotation", "refreshRotation", "modelDiagnostics", "setSwaths", "probabilityMap", "intervene", "added", "setAdded"]
 
 
 def node(code):

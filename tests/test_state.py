@@ -36,6 +36,7 @@ class FakeRun:
         self.fields = {k: rng.standard_normal(n).astype(np.float32) for k in STATE_FIELDS}
         self.columns = {k: rng.standard_normal(layer).astype(np.float32) for k in STATE_COLUMNS}
         self.time, self.counts = 612.0, (1.5, 3, 1, 90.0)
+        self.added_counts = {"heat": 2.5e13, "vapour": -1.25e6, "count": 3}
 
     def diagnostics(self):
         return {"time": self.time, "chargeGround": self.counts[0], "flashesIC": self.counts[1], "flashesCG": self.counts[2]}
@@ -65,6 +66,12 @@ class FakeRun:
     def refresh_rotation(self):
         self.refreshed = True
 
+    def added(self):
+        return dict(self.added_counts)
+
+    def set_added(self, heat, vapour, count):
+        self.added_counts = {"heat": heat, "vapour": vapour, "count": count}
+
 
 def test_saved_layout_is_the_references():
     g = C.small_grid()
@@ -76,6 +83,7 @@ def test_saved_layout_is_the_references():
     assert header["format"] == "meteolab-state" and header["version"] == 5 and header["config"] == {"seed": 7}
     assert (header["nx"], header["ny"], header["nz"], header["time"]) == (g["nx"], g["ny"], g["nz"], 612.0)
     assert header["flashCount"] == {"ic": 3, "cg": 1} and header["chargeGround"] == 1.5 and header["persisted"] == 90
+    assert header["added"] == {"heat": 2.5e13, "vapour": -1.25e6, "count": 3}
     start = math.ceil((8 + length) / 4) * 4
     n, layer = g["nx"] * g["ny"] * g["nz"], g["nx"] * g["ny"]
     assert len(buf) == start + (22 * n + 3 * layer) * 4
@@ -97,6 +105,7 @@ def test_apply_puts_fields_columns_time_and_counts_into_the_run():
     for k in STATE_COLUMNS:
         assert np.array_equal(dst.columns[k], src.columns[k]), k
     assert dst.time == 612.0 and dst.counts == (1.5, 3, 1, 90) and dst.refreshed
+    assert dst.added_counts == {"heat": 2.5e13, "vapour": -1.25e6, "count": 3}
 
 
 def test_opens_a_version_1_file_without_charge():
@@ -110,6 +119,7 @@ def test_opens_a_version_1_file_without_charge():
     apply_state(run, read_state(buf))
     assert run.time == 60 and run.fields["hail"][0] == 1 and not run.fields["chargeIce"].any()
     assert run.counts == (0.0, 0, 0, 0)
+    assert run.added_counts == {"heat": 0.0, "vapour": 0.0, "count": 0}
     assert FIELD_COUNT[1] == 12
 
 
@@ -147,6 +157,7 @@ const g = s.header, n = g.nx * g.ny * g.nz, layer = g.nx * g.ny
 const fake = {{ pool: {{ grid: {{ nx: g.nx, ny: g.ny, nz: g.nz }}, n, layer }}, spec: {{ seed: 3 }},
   diagnostics: () => ({{ time: g.time, chargeGround: g.chargeGround, flashesIC: g.flashCount.ic, flashesCG: g.flashCount.cg }}),
   rotation: () => ({{ persisted: g.persisted }}),
+  added: () => g.added,
   field: k => s.data.slice(m.STATE_FIELDS.indexOf(k) * n, (m.STATE_FIELDS.indexOf(k) + 1) * n),
   column: k => s.data.slice(22 * n + m.STATE_COLUMNS.indexOf(k) * layer, 22 * n + (m.STATE_COLUMNS.indexOf(k) + 1) * layer) }}
 fs.writeFileSync('{tmp_path / "node.meteolab"}', m.saveState(fake, {{ seed: 3 }}))
