@@ -165,6 +165,11 @@ int ml_run_set_swaths(ml_ctx *ctx, int run, int on);
  * ML_GUST_MAX, ML_UH_MAX) reached `threshold` there (for helicity the caller scales the threshold to the grid,
  * uhScale). */
 int ml_probability_map(ml_ctx *ctx, const int *runs, int count, int column, double threshold, float *out);
+/* A numerics check with no counterpart in the reference: the transport kernels' WENO5 face value (weno,
+ * src/core/advection.ts:388-399, minus c) of each of n stencils (a, b, c, d, e; 5n float32, flow from a towards e),
+ * computed on the device in the context's arithmetic (float32 or float64), into out (n float64). Allocates its own
+ * device buffers; stream-synchronising. */
+int ml_check_weno(ml_ctx *ctx, const float *stencils, int n, double *out);
 /* AtmosphereModel.diagnostics() (model.ts:1672-1696) of `count` runs of one pool from their fields as they are,
  * ML_MD_COUNT float64 a run into out (count * ML_MD_COUNT): the largest up- and downdraught (m/s), the heaviest
  * precipitation reaching the ground (mm/h, surfaceflux.ts:22-37 at the ground nodes) and the largest rain total (mm),

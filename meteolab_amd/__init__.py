@@ -97,6 +97,7 @@ SIGNATURES = {
     "ml_run_refresh_rotation": (_i, [_vp, _i]),
     "ml_run_set_swaths": (_i, [_vp, _i, _i]),
     "ml_probability_map": (_i, [_vp, _ip, _i, _i, _d, _fp]),
+    "ml_check_weno": (_i, [_vp, _fp, _i, _dp]),
     "ml_model_diagnostics": (_i, [_vp, _ip, _i, _dp]),
     "ml_model_diagnostics_noalloc": (_i, [_vp, _ip, _i, _dp]),
     "ml_run_flashes": (_i, [_vp, _i, _dp, _i]),
@@ -168,6 +169,15 @@ class Context:
 
     def slot_bytes(self, grid):
         return self.check(lib().ml_slot_bytes(self.h, ctypes.byref(make_grid(grid))))
+
+    def check_weno(self, stencils):
+        """The transport kernels' WENO5 face values minus c of stencils [n, 5] (a, b, c, d, e), on the device
+        (ml_check_weno)."""
+        st = np.ascontiguousarray(stencils, np.float32)
+        assert st.ndim == 2 and st.shape[1] == 5
+        out = np.empty(len(st), np.float64)
+        self.check(lib().ml_check_weno(self.h, _fptr(st), len(st), _dptr(out)))
+        return out
 
     def sync(self):
         self.check(lib().ml_sync(self.h))

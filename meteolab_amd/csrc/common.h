@@ -177,8 +177,10 @@ template <class R> struct Bufs {
 struct Div {
   int d; float inv;
   __device__ explicit Div(int d_) : d(d_), inv(1.0f / float(d_)) {}
-  __device__ int q(int a) const { int v = int(float(a) * inv), r = a - v * d; return v + (r >= d) - (r < 0); }
+  __device__ int q(int a) const { int v = int(float(a) * inv), r = a - int(__umul24(unsigned(v), unsigned(d))); return v + (r >= d) - (r < 0); }
 };
+// Product of two non-negative ints below 2^24 (a full-rate 24-bit multiply; a 32-bit one is a quarter-rate instruction).
+__device__ inline int mul24(int a, int b) { return int(__umul24(unsigned(a), unsigned(b))); }
 
 // ---- wave64 / block reductions --------------------------------------------------------------------------------
 template <class T> __device__ inline T waveSum(T v) {

@@ -342,13 +342,17 @@ template <class R> __global__ __launch_bounds__(TPB) void k_faces(Geo g, Bufs<R>
 // (advection.ts:294-399 tendencyWeno/tendencyLinear/weno/upwind5, model.ts:1237-1251 the stage update, 1204-1214 the
 // buoyancy in the w tendency, 1045 the rigid walls of the stages)
 //
-// WENO5 weights without underflow in float: the stencil is divided by its largest magnitude s (the weights are
-// scale-free but for the 1e-40 in the reference's epsilon, which becomes 1e-40 / s^2 and is kept). The normalised
-// indicators then lie in [2e-7, ~100], so the reference's one-division form (both sums times (b0 b1 b2)^2) stays
-// within float range; a stencil below 1e-20, where 1e-40 / s^2 exceeds 1, takes the weights one by one relative to eps.
+// WENO5 weights without underflow in float: the weights are scale-free but for the reference's 1e-40 in epsilon, so
+// the stencil is divided by its scale s (the sum of its magnitudes, at least 1e-19, so that 1 / s^2 stays a float),
+// where that term is (1e-20 / s)^2 <= 0.01. Relative to s, epsilon lies in [4e-8 (or 0.01 at the floor), ~1] and the
+// indicators in [epsilon, ~30], so the reference's one-division form (both sums times (b0 b1 b2)^2) stays within float
+// range: a stencil below 1e-20 takes the linear weights, as the reference's 1e-40 makes them. Over stencils of every scale from 1e-30 to 1e8 (zeros, near-constant and mixed signs among them) the face
+// value is within 2e-7 of the stencil's variation of the reference's float64 value.
 // Reciprocal: the hardware's (1 ulp) in float, a division in double.
 __device__ inline float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline double recip(double x) { return 1.0 / x; }
+__device__ inline float fmx(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ inline double fmx(double a, double b) { return __builtin_fmax(a, b); }
 
 __device__ inline double upwind5d(double a, double b_, double c, double d, double e) { return (2 * a - 13 * b_ + 47 * c + 27 * d - 3 * e) / 60; }
 // The face values relative to the stencil's centre c (the upwind node), from the differences of the other nodes to it:
@@ -357,32 +361,24 @@ __device__ inline double upwind5d(double a, double b_, double c, double d, doubl
 // to the local variation instead of to the field's magnitude: a float32 flux of m (face - c) keeps the digits a float32
 // flux of m face loses to the cancellation of neighbouring faces.
 template <class R> __device__ inline R upwindRel(R da, R db, R dd, R de) { return (2 * da - 13 * db + 27 * dd - 3 * de) * R(1.0 / 60); }
-// WENO5 (advection.ts:388-399) relative to c: the weights from the smoothness indicators of the differences (the
-// indicators are differences already) and the epsilon of the values' magnitudes (a, ..., e); the candidates as
-// differences from c. The candidates are kept times six and the 1/6 joins the final normalisation, and the float
-// reciprocals are the hardware's.
+// WENO5 (advection.ts:388-399) relative to c: the candidates (times six, the 1/6 joining the final normalisation) and
+// the indicators from the differences to c, relative to s. Every indicator and epsilon is taken times 4 / s^2 and every
+// weight times 10 / (b0 b1 b2)^2 of the reference's (a factor common to all three, which the normalisation removes).
 template <class R> __device__ inline R wenoRel(R a, R b_, R c, R d, R e) {
-  R s = mmax(mmax(mmax(mabs(a), mabs(b_)), mmax(mabs(c), mabs(d))), mabs(e));
-  if (s == 0) return 0;
-  R da = a - c, db = b_ - c, dd = d - c, de = e - c;
-  if (s < R(1e-25)) return (R(.1) * (2 * da - 7 * db) + R(.6) * (-db + 2 * dd) + R(.3) * (5 * dd - de)) / 6;
-  R is = recip(s);
-  R an = a * is, bn = b_ * is, cn = c * is, dn = d * is, en = e * is;
-  da *= is; db *= is; dd *= is; de *= is;
-  R p0 = 2 * da - 7 * db, p1 = -db + 2 * dd, p2 = 5 * dd - de;
-  R s0 = da - 2 * db, t0 = da - 4 * db, s1 = db + dd, t1 = db - dd, s2 = de - 2 * dd, t2 = de - 4 * dd;
-  R tiny = R(1e-20) * is;
-  R eps = R(2e-7) * (an * an + bn * bn + cn * cn + dn * dn + en * en) + tiny * tiny;
-  R b0 = eps + R(13.0 / 12) * s0 * s0 + R(.25) * t0 * t0, b1 = eps + R(13.0 / 12) * s1 * s1 + R(.25) * t1 * t1, b2 = eps + R(13.0 / 12) * s2 * s2 + R(.25) * t2 * t2;
-  R w0, w1, w2;
-  if (tiny > 1) {
-    R r0 = eps / b0, r1 = eps / b1, r2 = eps / b2;
-    w0 = R(.1) * r0 * r0; w1 = R(.6) * r1 * r1; w2 = R(.3) * r2 * r2;
-  } else {
-    R c0 = b0 * b0, c1 = b1 * b1, c2 = b2 * b2;
-    w0 = R(.1) * c1 * c2; w1 = R(.6) * c0 * c2; w2 = R(.3) * c0 * c1;
-  }
-  return s * ((w0 * p0 + w1 * p1 + w2 * p2) * recip(6 * (w0 + w1 + w2)));
+  // No branch for the reference's all-zero shortcut (advection.ts:390): a zero stencil gives zero through the same
+  // arithmetic, and the branch costs more than the weights it skips (measured).
+  const R s = fmx(mabs(a) + mabs(b_) + mabs(c) + mabs(d) + mabs(e), R(1e-19));
+  const R is = recip(s), tiny = R(2e-20) * is;
+  const R da = (a - c) * is, db = (b_ - c) * is, dd = (d - c) * is, de = (e - c) * is;
+  const R p0 = 2 * da - 7 * db, p1 = -db + 2 * dd, p2 = 5 * dd - de;
+  const R s0 = da - 2 * db, t0 = da - 4 * db, s1 = db + dd, t1 = db - dd, s2 = de - 2 * dd, t2 = de - 4 * dd;
+  const R eps = R(8e-7) * (is * is) * (a * a + b_ * b_ + c * c + d * d + e * e) + tiny * tiny;
+  const R b0 = eps + (R(13.0 / 3) * s0 * s0 + t0 * t0), b1 = eps + (R(13.0 / 3) * s1 * s1 + t1 * t1), b2 = eps + (R(13.0 / 3) * s2 * s2 + t2 * t2);
+  const R c0 = R(3) * (b0 * b0), c1 = b1 * b1, c2 = b2 * b2;
+  // w0 = c1 c2, w1 = 6 c0 c2, w2 = 3 c0 c1 (with c0 here 3 b0^2).
+  const R w0 = c1 * c2, c2x2 = c2 + c2;
+  const R num = w0 * p0 + c0 * (c2x2 * p1 + c1 * p2), den = w0 + c0 * (c2x2 + c1);
+  return s * (num * recip(6 * den));
 }
 // The flux m (face - q0) through the face between q0 and qp1, relative to q0 (the face's western, southern or lower
 // node), WENO5 from the stencil chosen by the flow's direction.
@@ -390,6 +386,14 @@ template <class R> __device__ inline R relWeno(R m, R qm2, R qm1, R q0, R qp1, R
   const bool up = m >= 0;
   const R a = up ? qm2 : qp3, b_ = up ? qm1 : qp2, c = up ? q0 : qp1, d = up ? qp1 : q0, e = up ? qp2 : qm1;
   return m * (wenoRel(a, b_, c, d, e) + (c - q0));
+}
+// The WENO5 face value of each of n stencils (a, b, c, d, e, float32, flow from a towards e), relative to c, in the
+// arithmetic type R: the transport kernels' own reconstruction, for its numerics check (ml_check_weno).
+template <class R> __global__ void k_check_weno(const float *st, int n, double *out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float *v = st + size_t(k) * 5;
+  out[k] = double(wenoRel(R(v[0]), R(v[1]), R(v[2]), R(v[3]), R(v[4])));
 }
 __device__ inline bool monotoneField(int f) { return !(f == F_U || f == F_V || f == F_W || f == F_THETA); }
 // Index of a carried field among the conserved ones (rain, snow, graupel, hail, q, cloud, ice), -1 if it is not one.
@@ -499,11 +503,16 @@ __device__ inline double linearAxis(double mp, double mm, const double *D) {
   return mp * fp - mm * fm;
 }
 // The base profile's own tendency of a departure-transported field (u, v, theta), exact for the linear scheme: its
-// vertical face values relative to the level's value, times the vertical face fluxes.
-__device__ inline double baseTendency(const double *Bc, double mzt, double mzb, double ihzrz) {
+// vertical face values relative to the level's value, times the vertical face fluxes. The face values of a level are
+// the same at every node of it: BaseFaces holds them for either flow direction.
+struct BaseFaces { double topUp, topDown, bottomUp, bottomDown; };
+__device__ inline BaseFaces baseFaces(const double *Bc) {
   const double Bz = Bc[3];
-  const double pt = mzt >= 0 ? upwind5d(Bc[1], Bc[2], Bc[3], Bc[4], Bc[5]) - Bz : upwind5d(Bc[6], Bc[5], Bc[4], Bc[3], Bc[2]) - Bz;
-  const double pb = mzb >= 0 ? upwind5d(Bc[0], Bc[1], Bc[2], Bc[3], Bc[4]) - Bz : upwind5d(Bc[5], Bc[4], Bc[3], Bc[2], Bc[1]) - Bz;
+  return {upwind5d(Bc[1], Bc[2], Bc[3], Bc[4], Bc[5]) - Bz, upwind5d(Bc[6], Bc[5], Bc[4], Bc[3], Bc[2]) - Bz,
+          upwind5d(Bc[0], Bc[1], Bc[2], Bc[3], Bc[4]) - Bz, upwind5d(Bc[5], Bc[4], Bc[3], Bc[2], Bc[1]) - Bz};
+}
+__device__ inline double baseTendency(const BaseFaces &f, double mzt, double mzb, double ihzrz) {
+  const double pt = mzt >= 0 ? f.topUp : f.topDown, pb = mzb >= 0 ? f.bottomUp : f.bottomDown;
   return (mzt * pt - mzb * pb) * ihzrz;
 }
 // The buoyancy of w's tendency (model.ts:1204-1214) at node i of the stage's input set, in float64.
@@ -598,8 +607,10 @@ template <class R> __global__ __launch_bounds__(TPB) void k_stage_level(Geo g, B
           int yy = y + k - 3; yy += yy < 0 ? ny : 0; yy -= yy >= ny ? ny : 0;
           Dx[k] = double(ql[row + xx]) - q0; Dy[k] = double(ql[yy * nx + x]) - q0; Dz[k] = (double(v[k]) - q0) - Bd[k];
         }
+        // Per node, as written: computing the base faces once a level, or 1 / dx once a launch, gives wrong u, v and w
+        // in this kernel (measured; the plane kernel below does both and keeps parity).
         tendd = -(linearAxis(mx[c], mx[cw], Dx) / g.dx + linearAxis(my[c], my[cs], Dy) / g.dy + linearAxis(mzt, mzb, Dz) * ihzd) * irzd;
-        if (B) tendd -= baseTendency(Bc, mzt, mzb, ihzrz);
+        if (B) tendd -= baseTendency(baseFaces(Bc), mzt, mzb, ihzrz);
         if (isW) tendd += buoyancy(b, g, inSet, slot, i, thvEnv, ithv);
       }
       float val = float(double(start[i]) + fracd * tendd);
@@ -651,41 +662,53 @@ template <class R, bool BASE, bool MONO> __device__ void stagePlane(const Geo &g
   const bool isW = !BASE && !MONO && f == F_W;
   const double fracd = (stage == 0 ? 1.0 / 3 : stage == 1 ? 1.0 / 2 : 1.0) * dt;
   const double *rho = b.lv(slot, L_RHO, g);
-  // This thread's first node and place in the plane, and the (x, y) steps between its nodes and places.
-  const Div dNx(nx), dW(W);
+  // This thread's first node and place in the plane, and the (x, y) steps between its nodes and places. A node's place
+  // in the plane (pq) and its western face's place in the x-face plane (pf) step with it: by dq, df per step, and by
+  // W - nx = 6 and X1 - nx = 1 more when the step wraps to the next row.
+  const Div dNx(nx), dW(W), d6(6);
   const int y00 = dNx.q(threadIdx.x), x00 = threadIdx.x - y00 * nx, sy = T / nx, sx = T - sy * nx;
-  const int py00 = dW.q(threadIdx.x), px00 = threadIdx.x - py00 * W, spy = T / W, spx = T - spy * W;
+  const int pq00 = (y00 + 3) * W + x00 + 3, pf00 = y00 * X1 + x00, dq = sy * W + sx, df = sy * X1 + sx;
   for (int z = z0; z < z1; z++) {
     const size_t lz = size_t(z) * layer;
     const float *qz = q + lz;
     // Place (px, py) of the plane holds node ((px - 3) mod nx, (py - 3) mod ny): the values themselves (a base
-    // profile is constant over a level, so the horizontal differences of departures are those of the values).
+    // profile is constant over a level, so the horizontal differences of departures are those of the values). The
+    // level's nodes go to the interior, each at its own place pq; then the rim's periodic images: its top and bottom
+    // three rows (W places each), then the three places at each end of the interior's rows.
     {
-      int px = px00, py = py00;
-#pragma unroll 4
-      for (int p = threadIdx.x; p < planeN; p += T) {
-        int xs = px - 3, ys = py - 3;
+      int x = x00, pq = pq00;
+      for (int c = threadIdx.x; c < layer; c += T) {
+        ql[pq] = at(qz, c);
+        x += sx; pq += dq;
+        if (x >= nx) { x -= nx; pq += W - nx; }
+      }
+      for (int k = threadIdx.x; k < 6 * W; k += T) {
+        const int r = dW.q(k), px = k - mul24(r, W), py = r < 3 ? r : ny + r, ys = r < 3 ? ny - 3 + r : r - 3;
+        int xs = px - 3;
         xs += xs < 0 ? nx : 0; xs -= xs >= nx ? nx : 0;
-        ys += ys < 0 ? ny : 0; ys -= ys >= ny ? ny : 0;
-        ql[p] = at(qz, ys * nx + xs);
-        px += spx; py += spy; if (px >= W) { px -= W; py++; }
+        ql[mul24(py, W) + px] = at(qz, mul24(ys, nx) + xs);
+      }
+      for (int k = threadIdx.x; k < 6 * ny; k += T) {
+        const int ys = d6.q(k), j = k - mul24(ys, 6), px = j < 3 ? j : nx + j, xs = j < 3 ? nx - 3 + j : j - 3;
+        ql[mul24(ys + 3, W) + px] = at(qz, mul24(ys, nx) + xs);
       }
     }
     __syncthreads();
     const double *mxz = b.m(slot, 0, g) + lz, *myz = b.m(slot, 1, g) + lz;
     if (MONO) {
       // Each thread's node's eastern and northern faces, relative to the face's western / southern node.
-      int x = x00, y = y00;
+      int x = x00, y = y00, pq = pq00, pf = pf00;
       for (int c = threadIdx.x; c < layer; c += T) {
-        const float *s = ql + (y + 3) * W + x + 3;
+        const float *s = ql + pq;
         const R fx = relWeno(R(at(mxz, c)), R(s[-2]), R(s[-1]), R(s[0]), R(s[1]), R(s[2]), R(s[3]));
         const R fy = relWeno(R(at(myz, c)), R(s[-2 * W]), R(s[-W]), R(s[0]), R(s[W]), R(s[2 * W]), R(s[3 * W]));
-        R *ex = fxl + y * X1 + x + 1;  // face x + 1/2; the row's first slot is the face west of x = 0
+        R *ex = fxl + pf + 1;          // face x + 1/2; the row's first slot is the face west of x = 0
         ex[0] = fx;
         if (x == nx - 1) ex[-nx] = fx;
-        fyl[(y + 1) * nx + x] = fy;    // face y + 1/2; the first row is the face south of y = 0
+        fyl[c + nx] = fy;              // face y + 1/2; the first row is the face south of y = 0
         if (y == ny - 1) fyl[x] = fy;
-        x += sx; y += sy; if (x >= nx) { x -= nx; y++; }
+        x += sx; y += sy; pq += dq; pf += df;
+        if (x >= nx) { x -= nx; y++; pq += W - nx; pf += X1 - nx; }
       }
       __syncthreads();
     }
@@ -698,14 +721,16 @@ template <class R, bool BASE, bool MONO> __device__ void stagePlane(const Geo &g
     const float *qc[7];
     double Bc[7], Bd[7];
     for (int k = 0; k < 7; k++) { qc[k] = q + size_t(zc[k]) * layer; Bc[k] = BASE ? B[zc[k]] : 0; Bd[k] = BASE ? B[zc[k]] - B[z] : 0; }
+    const BaseFaces bf = baseFaces(Bc);
+    const double idxd = 1.0 / g.dx, idyd = 1.0 / g.dy;
     const double wz = b.lv(slot, L_LW, g)[z];
     const double *mzz = b.m(slot, 2, g) + lz;
     const double *mzb_ = z > 0 ? mzz - layer : mzz;
     const float *startz = start + lz;
     float *outz = out + lz;
-    int x = x00, y = y00;
+    int x = x00, y = y00, pq = pq00, pf = pf00;
     for (int c = threadIdx.x; c < layer; c += T) {
-      const float *s = ql + (y + 3) * W + x + 3;
+      const float *s = ql + pq;
       float v[7];
       for (int k = 0; k < 7; k++) v[k] = k == 3 ? s[0] : at(qc[k], c);
       const double mzt = z == nz - 1 ? 0.0 : at(mzz, c), mzb = z == 0 ? 0.0 : at(mzb_, c);
@@ -720,7 +745,7 @@ template <class R, bool BASE, bool MONO> __device__ void stagePlane(const Geo &g
         fzl[c] = fzt;
         // The western, southern and bottom faces relative to this node: minus m (q - q_neighbour).
         const R q0 = v[3];
-        const R *fxw = fxl + y * X1 + x, *fys = fyl + y * nx + x;
+        const R *fxw = fxl + pf, *fys = fyl + c;
         const R fluxX = fxw[1] - (fxw[0] - R(at(mxz, cw)) * (q0 - R(s[-1])));
         const R fluxY = fys[nx] - (fys[0] - R(at(myz, cs)) * (q0 - R(s[-W])));
         const R fluxZ = fzt - (fzb - R(mzb) * (q0 - R(v[2])));
@@ -733,8 +758,8 @@ template <class R, bool BASE, bool MONO> __device__ void stagePlane(const Geo &g
           Dx[k] = double(s[k - 3]) - q0; Dy[k] = double(s[(k - 3) * W]) - q0;
           Dz[k] = BASE ? (double(v[k]) - q0) - Bd[k] : double(v[k]) - q0;
         }
-        tendd = -(linearAxis(at(mxz, c), at(mxz, cw), Dx) / g.dx + linearAxis(at(myz, c), at(myz, cs), Dy) / g.dy + linearAxis(mzt, mzb, Dz) * ihzd) * irzd;
-        if (BASE) tendd -= baseTendency(Bc, mzt, mzb, ihzrz);
+        tendd = -(linearAxis(at(mxz, c), at(mxz, cw), Dx) * idxd + linearAxis(at(myz, c), at(myz, cs), Dy) * idyd + linearAxis(mzt, mzb, Dz) * ihzd) * irzd;
+        if (BASE) tendd -= baseTendency(bf, mzt, mzb, ihzrz);
         if (isW) tendd += buoyancy(b, g, inSet, slot, lz + c, thvEnv, ithv);
       }
       // The stage update in float64 from the float32 start, rounded once.
@@ -746,7 +771,8 @@ template <class R, bool BASE, bool MONO> __device__ void stagePlane(const Geo &g
         if (xv < TINY_WATER) { if (xv != 0) { fill[0] += wz * xv; if (xv < 0) fill[2] -= wz * xv; } }
         else fill[1] += wz * xv;
       }
-      x += sx; y += sy; if (x >= nx) { x -= nx; y++; }
+      x += sx; y += sy; pq += dq; pf += df;
+      if (x >= nx) { x -= nx; y++; pq += W - nx; pf += X1 - nx; }
     }
     if (MONO && stage == 2 && cf >= 0) storeFill(b, g, slot, cf, z, 0, fill, red);
     __syncthreads();
@@ -761,6 +787,8 @@ template <class R> __global__ __launch_bounds__(TPB) void k_stage_plane(Geo g, B
   __shared__ double red[TPB / 64][3];
   const int cf = conservedIndex(f);
   if (zeroLevels(b, g, slot, f, z0 - stageReach(stage), z1 - 1 + stageReach(stage))) { zeroStage(b, g, slot, f, z0, z1, stage, 0, cf, red); return; }
+  // One kernel for both kinds of field: a launch runs the linear fields' blocks (float64 faces, memory latency) beside
+  // the WENO fields' (arithmetic), which keeps the CUs busier than a kernel of each kind in turn (measured).
   if (f == F_U || f == F_V || f == F_THETA) stagePlane<R, true, false>(g, b, slot, f, z0, z1, stage, dt, smem, cf, red);
   else if (f == F_W) stagePlane<R, false, false>(g, b, slot, f, z0, z1, stage, dt, smem, cf, red);
   else stagePlane<R, false, true>(g, b, slot, f, z0, z1, stage, dt, smem, cf, red);

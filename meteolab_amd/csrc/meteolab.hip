@@ -177,13 +177,14 @@ template <class R> void spectralSolve(const Pool &p, const Bufs<R> &b, int nr, h
 
 // One RK3 stage of the transport of every carried field, in the batch's tuned layout.
 // Fields fbase .. fbase + nc - 1.
-template <class R> void launchStage(const Geo &g, const Bufs<R> &b, int nr, int nc, int fbase, int stage, double dt, hipStream_t s, const Tuning &tu) {
+template <class R> void launchStage(const Geo &g, const Bufs<R> &b, int nr, int nc, int fbase, int stage, double dt, hipStream_t s, const Tuning &tu, Timer *t = nullptr) {
+  const unsigned zBlocks = unsigned((g.nz + tu.levels - 1) / tu.levels);
   if (tu.layout == 1) {
-    const unsigned zBlocks = unsigned((g.nz + tu.levels - 1) / tu.levels);
     hipLaunchKernelGGL(k_stage_level<R>, dim3(zBlocks * nc * nr), dim3(TPB), stageLevelLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, fbase, tu.xcd, tu.levels);
+    if (t) t->mark("k_stage_level");
   } else {
-    const unsigned zBlocks = unsigned((g.nz + tu.levels - 1) / tu.levels);
     hipLaunchKernelGGL(k_stage_plane<R>, dim3(zBlocks * nc * nr), dim3(TPB), stagePlaneLdsBytes(g, sizeof(R)), s, g, b, stage, dt, nr, nc, fbase, tu.xcd, tu.levels);
+    if (t) t->mark("k_stage_plane");
   }
 }
 
@@ -251,8 +252,7 @@ template <class R> void launchPart(const Pool &p, const int *list, int nr, doubl
     hipLaunchKernelGGL(k_faces<R>, dim3(nb, nr), dim3(TPB), 0, s, g, b, 1);
     mark("k_faces");
     for (int stage = 0; stage < 3; stage++) {
-      launchStage<R>(g, b, nr, split ? int(F_CHARGE_ICE) : nc, 0, stage, dt, s, tu);
-      mark(tu.layout == 2 ? "k_stage_plane" : "k_stage_level");
+      launchStage<R>(g, b, nr, split ? int(F_CHARGE_ICE) : nc, 0, stage, dt, s, tu, t);
       if (stage < 2) {
         hipLaunchKernelGGL(k_pdiv<R>, dim3(cb, nr), dim3(TPB), 0, s, g, b, stage + 1, dt);
         mark("k_pdiv");
@@ -273,8 +273,7 @@ template <class R> void launchPart(const Pool &p, const int *list, int nr, doubl
     hipLaunchKernelGGL(k_zero_flags<R>, dim3(g.nz * (N_CARRIED - F_CHARGE_ICE), nr), dim3(256), 0, s, g, b, int(F_CHARGE_ICE), 0);
     mark("k_zero_flags");
     for (int stage = 0; stage < 3; stage++) {
-      launchStage<R>(g, b, nr, N_CARRIED - F_CHARGE_ICE, F_CHARGE_ICE, stage, dt, s, tu);
-      mark(tu.layout == 2 ? "k_stage_plane" : "k_stage_level");
+      launchStage<R>(g, b, nr, N_CARRIED - F_CHARGE_ICE, F_CHARGE_ICE, stage, dt, s, tu, t);
     }
   }
   // endStep: positivity fix + physics (physicsStep), turbulence + drag (mixStep), projection (model.ts:1103-1113).
@@ -823,6 +822,25 @@ int ml_run_set_swaths(ml_ctx *c, int run, int on) {
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipMemcpyAsync((double *)p->par + size_t(slot) * N_PARAMS + P_SWATHS, &v, 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ml_check_weno(ml_ctx *c, const float *stencils, int n, double *out) {
+  if (!c || !stencils || !out || n < 1 || n > (1 << 24)) return fail(c, ML_E_ARG, "check weno: 1 .. 2^24 stencils and an output");
+  HIPCHK(c, hipSetDevice(c->device));
+  float *dst = nullptr; double *dout = nullptr;
+  HIPCHK(c, hipMalloc(&dst, size_t(n) * 5 * sizeof(float)));
+  hipError_t e = hipMalloc(&dout, size_t(n) * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpyAsync(dst, stencils, size_t(n) * 5 * sizeof(float), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    if (c->precision == 64) hipLaunchKernelGGL(k_check_weno<double>, dim3(blocks(n, 256)), dim3(256), 0, c->stream, dst, n, dout);
+    else hipLaunchKernelGGL(k_check_weno<float>, dim3(blocks(n, 256)), dim3(256), 0, c->stream, dst, n, dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(dst); if (dout) hipFree(dout);
+  if (e != hipSuccess) return fail(c, ML_E_HIP, std::string("check weno: ") + hipGetErrorString(e));
   return 0;
 }
 
