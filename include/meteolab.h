@@ -106,6 +106,23 @@ int ml_pool_destroy(ml_ctx *ctx, int pool);
  * its initial state (model.ts:399-451 single-storm start with the config's thermal and noise). *run receives a handle.
  * Replaces new AtmosphereModel(config, grid, profile) for the step's state. */
 int ml_run_create(ml_ctx *ctx, int pool, const ml_profile *profile, const ml_run_config *config, int *run);
+/* `count` runs at once, as ml_run_create of profiles[k] and configs[k] (runs[k] receives run k's handle): one launch of
+ * each start-up kernel for all of them and one wait. Replaces the new AtmosphereModel of each run a batch front-end
+ * starts (the ensemble's runPool, src/ensemble/pool.ts:22-47, one per worker); ML_E_FULL, and no run made, when the
+ * pool has fewer free slots. */
+int ml_runs_create(ml_ctx *ctx, int pool, int count, const ml_profile *profiles, const ml_run_config *configs, int *runs);
+/* Grows a pool to `capacity` slots (at least its own): its runs keep their slots, handles and state, the new slots are
+ * free. Waits for the context's work; the next step of each batch is captured again. On an error the pool is left as it
+ * was. No counterpart in the reference (its runs are garbage-collected objects); the serving loop's pools grow with
+ * the requests admitted. */
+int ml_pool_grow(ml_ctx *ctx, int pool, int capacity);
+/* Free and total device memory in bytes (hipMemGetInfo): what the serving loop admits runs against, at
+ * ml_slot_bytes a run. No counterpart in the reference. */
+int ml_mem_info(ml_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
+/* 1 when all the work given to the context so far has finished, 0 while some is still running, without waiting: the
+ * promise of GpuDynamics.step settling (gpu/dynamics.ts:419, 494) as a poll, so that one host thread keeps several
+ * contexts' steps in flight. */
+int ml_ctx_query(ml_ctx *ctx);
 /* Frees the run's slot. */
 int ml_run_release(ml_ctx *ctx, int run);
 

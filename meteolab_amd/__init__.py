@@ -80,6 +80,10 @@ SIGNATURES = {
     "ml_pool_free": (_i, [_vp, _i]),
     "ml_pool_destroy": (_i, [_vp, _i]),
     "ml_run_create": (_i, [_vp, _i, ctypes.POINTER(Profile), ctypes.POINTER(RunConfig), _ip]),
+    "ml_runs_create": (_i, [_vp, _i, _i, ctypes.POINTER(Profile), ctypes.POINTER(RunConfig), _ip]),
+    "ml_pool_grow": (_i, [_vp, _i, _i]),
+    "ml_mem_info": (_i, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "ml_ctx_query": (_i, [_vp]),
     "ml_run_release": (_i, [_vp, _i]),
     "ml_run_upload": (_i, [_vp, _i, _i, _fp]),
     "ml_run_download": (_i, [_vp, _i, _i, _fp]),
@@ -182,6 +186,16 @@ class Context:
     def sync(self):
         self.check(lib().ml_sync(self.h))
 
+    def idle(self):
+        """True when all the work given to the context has finished, without waiting (ml_ctx_query)."""
+        return self.check(lib().ml_ctx_query(self.h)) == 1
+
+    def mem_info(self):
+        """(free, total) device memory in bytes (ml_mem_info)."""
+        free, total = _i64(), _i64()
+        self.check(lib().ml_mem_info(self.h, ctypes.byref(free), ctypes.byref(total)))
+        return free.value, total.value
+
     def set_overlap(self, on):
         """A step's lightning beside the next step's dynamics (True, the default) or in line (ml_ctx_set_overlap)."""
         self.check(lib().ml_ctx_set_overlap(self.h, 1 if on else 0))
@@ -240,18 +254,36 @@ class Pool:
     def run(self, profile, run):
         return Run(self, profile, run)
 
+    def runs(self, starts):
+        """Runs of (profile, run) pairs made at once (ml_runs_create: one launch of each start-up kernel, one wait)."""
+        starts = list(starts)
+        if not starts:
+            return []
+        profs = (Profile * len(starts))(*(profile_struct(p) for p, _ in starts))
+        confs = (RunConfig * len(starts))(*(RunConfig(*(r[k] for k, _ in RunConfig._fields_)) for _, r in starts))
+        ids = (ctypes.c_int * len(starts))()
+        self.ctx.check(lib().ml_runs_create(self.ctx.h, self.id, len(starts), profs, confs, ids))
+        return [Run(self, p, r, handle=h) for (p, r), h in zip(starts, ids)]
+
+    def grow(self, capacity):
+        """Grows the pool to `capacity` slots, its runs kept (ml_pool_grow)."""
+        self.ctx.check(lib().ml_pool_grow(self.ctx.h, self.id, int(capacity)))
+        self.capacity = int(capacity)
+
 
 class Run:
     """One model run in a pool slot: its base state and initial state are built on the device (ml_run_create)."""
 
-    def __init__(self, pool, profile, run):
+    def __init__(self, pool, profile, run, handle=None):
         self.pool, self.ctx = pool, pool.ctx
         self.n, self.layer, self.grid = pool.n, pool.layer, pool.grid
-        p = profile_struct(profile)
-        r = RunConfig(*(run[k] for k, _ in RunConfig._fields_))
-        h = ctypes.c_int()
-        self.ctx.check(lib().ml_run_create(self.ctx.h, pool.id, ctypes.byref(p), ctypes.byref(r), ctypes.byref(h)))
-        self.h = h.value
+        if handle is None:
+            p = profile_struct(profile)
+            r = RunConfig(*(run[k] for k, _ in RunConfig._fields_))
+            h = ctypes.c_int()
+            self.ctx.check(lib().ml_run_create(self.ctx.h, pool.id, ctypes.byref(p), ctypes.byref(r), ctypes.byref(h)))
+            handle = h.value
+        self.h = handle
         self.spec = {"grid": dict(pool.grid), "profile": dict(profile), "run": dict(run)}
 
     def release(self):

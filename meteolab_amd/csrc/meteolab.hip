@@ -465,6 +465,17 @@ void freePool(Pool *p) {
   for (void *a : all) if (a) hipFree(a);
   delete p;
 }
+// The pool's arrays that hold a part per slot, each [cap][bytes]: what pool creation allocates and what a pool's growth
+// reallocates with its slots' parts kept.
+struct SlotArray { void **ptr; size_t bytes; };
+std::vector<SlotArray> slotArrays(Pool *p, const Sizes &s) {
+  return {{&p->fields, s.fields}, {&p->stx, s.stage}, {&p->sty, s.stage}, {&p->mass, s.mass}, {&p->phi, s.node8}, {&p->cells, s.node8},
+          {&p->press, s.node8}, {&p->spec, s.spec}, {&p->ehat, s.spec}, {&p->dspec, s.spec}, {&p->mean0, s.mean0}, {&p->emean0, s.mean0},
+          {&p->km, s.node}, {&p->kh, s.node}, {&p->ephi, s.node8}, {&p->cols, s.cols}, {&p->lev, s.lev}, {&p->par, s.par},
+          {&p->scal, s.scal}, {(void **)&p->diag, size_t(DIAG_ROW) * 8}, {(void **)&p->dlist, sizeof(int)}, {&p->lrho, s.node},
+          {&p->lmark, s.lmark}, {&p->lnode, s.lnode}, {&p->chan, s.chan}, {&p->cdep, s.cdep}, {&p->flog, s.flog}, {&p->fillp, s.fillp},
+          {&p->gpart, s.gpart}, {&p->zflag, s.zflag}, {&p->landed, s.landed}};
+}
 }  // namespace
 
 extern "C" {
@@ -525,25 +536,14 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
   if (stageLevelLdsBytes(g, p->rb) > c->prop.sharedMemPerBlock ||
       specLdsBytes(g, 8) + 4096 > c->prop.sharedMemPerBlock) { delete p; return fail(c, ML_E_UNSUPPORTED, "grid too large for a level in LDS"); }
   Sizes s = sizesOf(g, p->rb);
-  size_t cap = capacity;
-  struct { void **ptr; size_t bytes; } allocs[] = {
-    {&p->fields, s.fields * cap}, {&p->stx, s.stage * cap}, {&p->sty, s.stage * cap}, {&p->mass, s.mass * cap}, {&p->phi, s.node8 * cap},
-    {&p->cells, s.node8 * cap}, {&p->press, s.node8 * cap}, {&p->spec, s.spec * cap}, {&p->ehat, s.spec * cap}, {&p->dspec, s.spec * cap}, {&p->mean0, s.mean0 * cap}, {&p->emean0, s.mean0 * cap}, {&p->km, s.node * cap},
-    {&p->kh, s.node * cap}, {&p->ephi, s.node8 * cap}, {&p->cols, s.cols * cap}, {&p->lev, s.lev * cap}, {&p->par, s.par * cap},
-    {&p->scal, s.scal * cap}, {(void **)&p->diag, cap * DIAG_ROW * 8}, {(void **)&p->dlist, cap * sizeof(int)}, {&p->lrho, s.node * cap}, {&p->lmark, s.lmark * cap},
-    {&p->lnode, s.lnode * cap}, {&p->chan, s.chan * cap}, {&p->cdep, s.cdep * cap}, {&p->flog, s.flog * cap},
-    {&p->fillp, s.fillp * cap}, {&p->gpart, s.gpart * cap}, {&p->pmap, size_t(g.layer) * 4}, {&p->zflag, s.zflag * cap},
-    {&p->landed, s.landed * cap}};
+  const size_t cap = capacity;
   p->ftiles = 1; p->gmax = groundBlocks(g);
-  for (auto &a : allocs) {
-    hipError_t e = hipMalloc(a.ptr, a.bytes);
-    if (e == hipSuccess) e = hipMemset(*a.ptr, 0, a.bytes);
-    if (e != hipSuccess) {
-      for (auto &f : allocs) if (*f.ptr) { hipFree(*f.ptr); *f.ptr = nullptr; }
-      delete p;
-      return fail(c, ML_E_HIP, std::string("pool allocation: ") + hipGetErrorString(e));
-    }
+  for (const SlotArray &a : slotArrays(p, s)) {
+    hipError_t e = hipMalloc(a.ptr, a.bytes * cap);
+    if (e == hipSuccess) e = hipMemset(*a.ptr, 0, a.bytes * cap);
+    if (e != hipSuccess) { freePool(p); return fail(c, ML_E_HIP, std::string("pool allocation: ") + hipGetErrorString(e)); }
   }
+  POOLCHK(hipMalloc(&p->pmap, size_t(g.layer) * 4));
   levels(*grid, p->zsH, p->dzsH, p->hzH);
   std::vector<char> sym, twx, twy;
   fillTables(*p, sym, twx, twy);
@@ -629,15 +629,12 @@ int ml_pool_free(ml_ctx *c, int pool) {
   return int(std::count(p->used.begin(), p->used.end(), 0));
 }
 
-int ml_run_create(ml_ctx *c, int pool, const ml_profile *pr, const ml_run_config *rc, int *run) {
-  Pool *p = c ? poolOf(c, pool) : nullptr;
-  if (!p || !pr || !rc || !run) return fail(c, ML_E_ARG, "run: bad pool or arguments");
-  int slot = -1;
-  for (int k = 0; k < p->cap; k++) if (!p->used[k]) { slot = k; break; }
-  if (slot < 0) return fail(c, ML_E_FULL, "pool full");
-  HIPCHK(c, hipSetDevice(c->device));
-  const Geo &g = p->g;
-  double P[N_PARAMS] = {0};
+}  // extern "C"
+
+namespace {
+// The parameters of a run's slot from its sounding and settings (the values of SimConfig the step reads).
+void runParams(const Pool *p, const ml_profile *pr, const ml_run_config *rc, double *P) {
+  for (int k = 0; k < N_PARAMS; k++) P[k] = 0;
   P[P_FRAME_U] = rc->frame_u; P[P_FRAME_V] = rc->frame_v;
   P[P_CORIOLIS] = 2 * 7.2921e-5 * std::sin(rc->latitude * M_PI / 180);
   P[P_CS] = .18 * rc->turbulence / .55;
@@ -648,33 +645,125 @@ int ml_run_create(ml_ctx *c, int pool, const ml_profile *pr, const ml_run_config
   P[P_QVMAX] = pr->qv_max; P[P_SHEAR] = pr->shear; P[P_SHEAR_DEPTH] = pr->shear_depth; P[P_THETA0] = pr->theta0; P[P_DTHETA] = pr->dtheta;
   P[P_ZTR] = pr->ztr; P[P_TTR] = pr->ttr; P[P_RH_DROP] = pr->rh_drop; P[P_RH_TOP] = pr->rh_top; P[P_ELECTRIFIED] = p->electrified; P[P_TIME] = 0;
   for (int k = 0; k < 4; k++) { P[P_WIND_U + k] = pr->wind_u[k]; P[P_WIND_V + k] = pr->wind_v[k]; }
-  HIPCHK(c, hipMemcpyAsync((double *)p->par + size_t(slot) * N_PARAMS, P, sizeof(P), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemsetAsync((double *)p->scal + size_t(slot) * N_SCALARS, 0, N_SCALARS * 8, c->stream));
+}
+
+// `count` runs in free slots of pool `pool` (profiles[k], configs[k]): their parameters, cleared state, then one launch
+// of the base-state integration (a block per run) and one of the initial state for all of them; one wait at the end.
+int createRuns(ml_ctx *c, int pool, int count, const ml_profile *prs, const ml_run_config *rcs, int *runs) {
+  Pool *p = c ? poolOf(c, pool) : nullptr;
+  if (!p || !prs || !rcs || !runs || count < 1) return fail(c, ML_E_ARG, "run: bad pool or arguments");
+  std::vector<int> slots;
+  for (int k = 0; k < p->cap && int(slots.size()) < count; k++) if (!p->used[k]) slots.push_back(k);
+  if (int(slots.size()) < count) return fail(c, ML_E_FULL, "pool full");
+  HIPCHK(c, hipSetDevice(c->device));
+  const Geo &g = p->g;
   Sizes s = sizesOf(g, p->rb);
-  HIPCHK(c, hipMemsetAsync((char *)p->stx + s.stage * slot, 0, s.stage, c->stream));
-  HIPCHK(c, hipMemsetAsync((char *)p->sty + s.stage * slot, 0, s.stage, c->stream));
-  HIPCHK(c, hipMemsetAsync((char *)p->ephi + s.node8 * slot, 0, s.node8, c->stream));
-  HIPCHK(c, hipMemsetAsync((char *)p->press + s.node8 * slot, 0, s.node8, c->stream));
-  HIPCHK(c, hipMemsetAsync((char *)p->lmark + s.lmark * slot, 0, s.lmark, c->stream));
-  HIPCHK(c, hipMemsetAsync((char *)p->flog + s.flog * slot, 0, s.flog, c->stream));
-  double firstFlash = -1;
-  HIPCHK(c, hipMemcpyAsync((double *)p->scal + size_t(slot) * N_SCALARS + S_FIRST_FLASH, &firstFlash, 8, hipMemcpyHostToDevice, c->stream));
-  const int *one = p->single + slot;
+  std::vector<double> P(size_t(count) * N_PARAMS);
+  const double firstFlash = -1;
+  for (int k = 0; k < count; k++) {
+    const int slot = slots[k];
+    runParams(p, prs + k, rcs + k, P.data() + size_t(k) * N_PARAMS);
+    HIPCHK(c, hipMemcpyAsync((double *)p->par + size_t(slot) * N_PARAMS, P.data() + size_t(k) * N_PARAMS, N_PARAMS * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync((double *)p->scal + size_t(slot) * N_SCALARS, 0, N_SCALARS * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync((char *)p->stx + s.stage * slot, 0, s.stage, c->stream));
+    HIPCHK(c, hipMemsetAsync((char *)p->sty + s.stage * slot, 0, s.stage, c->stream));
+    HIPCHK(c, hipMemsetAsync((char *)p->ephi + s.node8 * slot, 0, s.node8, c->stream));
+    HIPCHK(c, hipMemsetAsync((char *)p->press + s.node8 * slot, 0, s.node8, c->stream));
+    HIPCHK(c, hipMemsetAsync((char *)p->lmark + s.lmark * slot, 0, s.lmark, c->stream));
+    HIPCHK(c, hipMemsetAsync((char *)p->flog + s.flog * slot, 0, s.flog, c->stream));
+    HIPCHK(c, hipMemcpyAsync((double *)p->scal + size_t(slot) * N_SCALARS + S_FIRST_FLASH, &firstFlash, 8, hipMemcpyHostToDevice, c->stream));
+  }
+  // The new slots' list: the pool's single-slot list where there is one run, else the diagnostics' run list (stream
+  // ordered with every other use of it).
+  const int *list = p->single + slots[0];
+  if (count > 1) {
+    HIPCHK(c, hipMemcpyAsync(p->dlist, slots.data(), count * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    list = p->dlist;
+  }
   size_t lnp = (size_t(std::ceil((g.height + 2000) / 10)) + 2) * sizeof(double);
   if (p->rb == 8) {
-    Bufs<double> b = bufs<double>(*p, one);
-    hipLaunchKernelGGL(k_base_state<double>, dim3(1), dim3(64), lnp, c->stream, g, b);
-    hipLaunchKernelGGL(k_init<double>, dim3(blocks(g.n, 256), 1), dim3(256), 0, c->stream, g, b);
+    Bufs<double> b = bufs<double>(*p, list);
+    hipLaunchKernelGGL(k_base_state<double>, dim3(count), dim3(64), lnp, c->stream, g, b);
+    hipLaunchKernelGGL(k_init<double>, dim3(blocks(g.n, 256), count), dim3(256), 0, c->stream, g, b);
   } else {
-    Bufs<float> b = bufs<float>(*p, one);
-    hipLaunchKernelGGL(k_base_state<float>, dim3(1), dim3(64), lnp, c->stream, g, b);
-    hipLaunchKernelGGL(k_init<float>, dim3(blocks(g.n, 256), 1), dim3(256), 0, c->stream, g, b);
+    Bufs<float> b = bufs<float>(*p, list);
+    hipLaunchKernelGGL(k_base_state<float>, dim3(count), dim3(64), lnp, c->stream, g, b);
+    hipLaunchKernelGGL(k_init<float>, dim3(blocks(g.n, 256), count), dim3(256), 0, c->stream, g, b);
   }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  p->used[slot] = 1;
-  *run = (pool << SLOT_BITS) | slot;
+  for (int k = 0; k < count; k++) { p->used[slots[k]] = 1; runs[k] = (pool << SLOT_BITS) | slots[k]; }
   return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int ml_run_create(ml_ctx *c, int pool, const ml_profile *pr, const ml_run_config *rc, int *run) {
+  return createRuns(c, pool, 1, pr, rc, run);
+}
+
+int ml_runs_create(ml_ctx *c, int pool, int count, const ml_profile *profiles, const ml_run_config *configs, int *runs) {
+  return createRuns(c, pool, count, profiles, configs, runs);
+}
+
+int ml_pool_grow(ml_ctx *c, int pool, int capacity) {
+  Pool *p = c ? poolOf(c, pool) : nullptr;
+  if (!p) return fail(c, ML_E_ARG, "pool grow: bad pool");
+  if (capacity < p->cap || capacity >= (1 << SLOT_BITS)) return fail(c, ML_E_ARG, "pool grow: a capacity from the pool's own up to 65535");
+  if (capacity == p->cap) return 0;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->side));
+  const Sizes s = sizesOf(p->g, p->rb);
+  const std::vector<SlotArray> arrays = slotArrays(p, s);
+  const size_t old = p->cap, cap = capacity;
+  std::vector<void *> fresh(arrays.size(), nullptr);
+  int *single = nullptr;
+  std::vector<int> list(cap);
+  for (size_t k = 0; k < cap; k++) list[k] = int(k);
+  // The new arrays: the slots' parts copied, the new slots cleared. On an error the pool is left as it was.
+  hipError_t e = hipSuccess;
+  for (size_t k = 0; k < arrays.size() && e == hipSuccess; k++) {
+    e = hipMalloc(&fresh[k], arrays[k].bytes * cap);
+    if (e == hipSuccess) e = hipMemcpyAsync(fresh[k], *arrays[k].ptr, arrays[k].bytes * old, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync((char *)fresh[k] + arrays[k].bytes * old, 0, arrays[k].bytes * (cap - old), c->stream);
+  }
+  if (e == hipSuccess) e = hipMalloc((void **)&single, cap * sizeof(int));
+  if (e == hipSuccess) e = hipMemcpyAsync(single, list.data(), cap * sizeof(int), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    (void)hipStreamSynchronize(c->stream);
+    for (void *f : fresh) if (f) (void)hipFree(f);
+    if (single) (void)hipFree(single);
+    return fail(c, ML_E_HIP, std::string("pool grow: ") + hipGetErrorString(e));
+  }
+  for (size_t k = 0; k < arrays.size(); k++) { (void)hipFree(*arrays[k].ptr); *arrays[k].ptr = fresh[k]; }
+  (void)hipFree(p->single);
+  p->single = single;
+  p->cap = capacity;
+  p->used.resize(cap, 0);
+  // The captured steps hold the old arrays' addresses: each batch is captured again at its next step (its tuning kept).
+  for (auto &kv : p->batches) kv.second->dt = 0;
+  return 0;
+}
+
+int ml_mem_info(ml_ctx *c, int64_t *free_bytes, int64_t *total_bytes) {
+  if (!c || !free_bytes || !total_bytes) return fail(c, ML_E_ARG, "mem info: null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  size_t f = 0, t = 0;
+  HIPCHK(c, hipMemGetInfo(&f, &t));
+  *free_bytes = int64_t(f); *total_bytes = int64_t(t);
+  return 0;
+}
+
+int ml_ctx_query(ml_ctx *c) {
+  if (!c) return ML_E_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const hipError_t e = hipStreamQuery(c->stream);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return fail(c, ML_E_HIP, std::string("query: ") + hipGetErrorString(e));
 }
 
 int ml_run_release(ml_ctx *c, int run) {

@@ -79,11 +79,13 @@ def test_mixed_class_serve_on_the_real_grids(ctx):
         g = C.GRIDS[size]
         reqs.append(S.Request(k, size, 1 + k % 2, 0.0, C.regime_cell(k % 8, (3 * k) % 8, k),
                               C.run(g, seed=k + 1, amp=1.8 + 0.1 * k, offset=(400.0 * k - 2000, -300.0 * k + 1500))))
-    print("case: serve 8 standard + 4 large requests into 5 + 2 slots", flush=True)
+    print("case: serve 8 standard + 4 large requests with memory for 5 + 2 runs", flush=True)
     finals = {}
-    server = S.Server(ctx, {"standard": 5, "large": 2},
-                      on_finish=lambda r: finals.setdefault(r.id, {name: r.handle.field(name) for name in M.CARRIED}))
-    done, clock = server.serve(reqs)
+    memory = {size: n * ctx.slot_bytes(C.GRIDS[size]) for size, n in (("standard", 5), ("large", 2))}
+    with S.Server(memory=memory, chunk=2,
+                  on_finish=lambda r: finals.setdefault(r.id, {name: r.handle.field(name) for name in M.CARRIED})) as server:
+        done, clock = server.serve(reqs)
+        assert server.pools["standard"].capacity == 5 and server.pools["large"].capacity == 2
     assert sorted(r.id for r in done) == list(range(12))
     for r in reqs:
         g = C.GRIDS[r.size]

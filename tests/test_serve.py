@@ -1,5 +1,5 @@
-"""The continuous-batching serving loop (meteolab_amd/serve.py): admission into free slots, rounds of one model
-minute, release; every request ends where it would alone."""
+"""The continuous-batching serving loop (meteolab_amd/serve.py): admission by device memory into pools that grow,
+rounds of one model minute, release; every request ends where it would alone."""
 import numpy as np
 import pytest
 
@@ -29,11 +29,14 @@ def test_requests_served_together_end_as_alone():
             for k in range(6)]
     with M.Context(0, 32) as ctx:
         finals = {}
-        server = S.Server(ctx, {"small": 3}, grids=grids,
-                          on_finish=lambda r: finals.setdefault(r.id, {name: r.handle.field(name) for name in M.CARRIED}))
-        done, clock = server.serve(reqs)
+        # Memory for three runs, a slot at a time: the pool grows to three, and the other requests wait.
+        with S.Server(grids=grids, memory={"small": 3 * ctx.slot_bytes(g)}, chunk=1,
+                      on_finish=lambda r: finals.setdefault(r.id, {name: r.handle.field(name) for name in M.CARRIED})) as server:
+            done, clock = server.serve(reqs)
+            assert server.pools["small"].capacity == 3
         assert sorted(r.id for r in done) == list(range(6))
         assert all(len(r.minute_ms) == r.minutes for r in done)
+        assert max(r.admitted for r in done) > 0  # some waited for a slot
         out = S.summary(done, clock, grids)
         assert out["latency_per_model_minute_ms"]["p50"] > 0
         for r in reqs:
@@ -50,10 +53,9 @@ def test_an_evicted_request_readmitted_from_its_state_file_ends_as_alone():
     g = C.small_grid()
     grids = {"small": g}
     a, b = [S.Request(k, "small", 3, 0.0, C.profile(qv_max=0.013), C.run(g, seed=k + 3, amp=2.5)) for k in range(2)]
-    with M.Context(0, 32) as ctx:
+    with M.Context(0, 32) as ctx, S.Server(grids=grids, memory={"small": 2 * ctx.slot_bytes(g)}, chunk=2,
+                                           on_finish=lambda r: finals.setdefault(r.id, {name: r.handle.field(name) for name in M.CARRIED})) as server:
         finals = {}
-        server = S.Server(ctx, {"small": 2}, grids=grids,
-                          on_finish=lambda r: finals.setdefault(r.id, {name: r.handle.field(name) for name in M.CARRIED}))
         server.admit(a)
         server.admit(b)
         server.round()
