@@ -43,6 +43,12 @@ __global__ void k_electric_factors(Geo g, const double *sym, const double *dzs, 
 // buffers and the twiddles.
 __host__ __device__ inline int solveStride(const Geo &g) { return g.layer / 2 > g.nm ? g.layer / 2 : g.nm; }
 __host__ __device__ inline size_t solveLdsBytes(const Geo &g, int C) { return (size_t(2) * C * solveStride(g) + g.nx + g.ny) * 16; }
+// Between solves the same LDS holds the front of a flash: potential and scale (float64), node (int), link (int16, a
+// channel index below MAX_CHANNEL) and flags, 23 bytes an entry. Entries beyond the capacity live in the run's global
+// front arrays.
+constexpr int FRONT_ENTRY_BYTES = 23;
+__host__ __device__ inline int frontCapacity(const Geo &g, int C) { return int(solveLdsBytes(g, C) / FRONT_ENTRY_BYTES); }
+static_assert(MAX_CHANNEL <= 32767, "a front entry's link is an int16");
 
 // The potential of the carriers as they now stand (Lightning.solve, lightning.ts:228-232) by one block: the charge
 // density of every node into lrho, the interior levels' transforms C levels at a time (the transforms of
@@ -257,7 +263,7 @@ struct FlashState {
   int size, count, ground, touched, left, brk, event, id, pickN, best, prune;
   int hat;  // the right-hand side's spectra of the charge before the last deposit are in ehat (the next solve updates them)
   uint32_t rng;
-  double sum, potential, before, moment, pick, ratio;
+  double sum, potential, before, moment, ratio;
   FrontEntry picked, last;  // the picked entry and the front's last one (which takes the picked one's place)
 };
 
@@ -271,7 +277,15 @@ __device__ inline uint32_t mulberryNext(uint32_t &s, double &out) {
 
 template <class R> struct Discharge {
   const Geo &g; const Bufs<R> &b; const Plan &pl; int slot, chunk; char *smem; double *red; FlashState &st;
+  // The front: entries f < qcap in LDS (qphi .. qflag, carved from smem while no solve runs), the rest in the global
+  // arrays fnode .. fscale (which also keep the LDS entries' node, link and scale while a solve runs).
+  int qcap;
+  double *qphi, *qscale;
+  int *qnode;
+  short *qlink;
+  unsigned char *qflag;
   unsigned char *cflag;  // LDS: channel node c is clear air (as of the last solve)
+  const double *zsL, *eiL;  // LDS: the level heights and E_init of the levels
   int *prunes;           // LDS: the front places a prune takes off
   float *car[4];
   const float *cloudy;
@@ -281,8 +295,11 @@ template <class R> struct Discharge {
   int *mark, *failed, *seen, *pos, *fnode, *flink, *fflag, *nodes, *parent;
   double *fscale, *fphi, *fweight, *nscale, *dep;
 
-  __device__ Discharge(const Geo &g_, const Bufs<R> &b_, const Plan &p_, int s_, int chunk_, char *m_, double *r_, FlashState &st_, unsigned char *cf_, int *pr_)
-      : g(g_), b(b_), pl(p_), slot(s_), chunk(chunk_), smem(m_), red(r_), st(st_), cflag(cf_), prunes(pr_) {
+  __device__ Discharge(const Geo &g_, const Bufs<R> &b_, const Plan &p_, int s_, int chunk_, char *m_, double *r_, FlashState &st_, unsigned char *cf_, int *pr_, int cap_,
+                        const double *zs_, const double *ei_)
+      : g(g_), b(b_), pl(p_), slot(s_), chunk(chunk_), smem(m_), red(r_), st(st_), qcap(cap_), cflag(cf_), zsL(zs_), eiL(ei_), prunes(pr_) {
+    qphi = reinterpret_cast<double *>(smem); qscale = qphi + qcap; qnode = reinterpret_cast<int *>(qscale + qcap);
+    qlink = reinterpret_cast<short *>(qnode + qcap); qflag = reinterpret_cast<unsigned char *>(qlink + qcap);
     car[0] = b.f(slot, F_CHARGE_ICE, g); car[1] = b.f(slot, F_CHARGE_GRAUPEL, g); car[2] = b.f(slot, F_CHARGE_RAIN, g); car[3] = b.f(slot, F_CHARGE_AIR, g);
     cloudy = b.f(slot, F_CLOUDY, g);
     air = b.lv(slot, L_RHO, g); einit = b.lv(slot, L_EINIT, g); cap = b.lv(slot, L_CAPACITANCE, g);
@@ -292,6 +309,20 @@ template <class R> struct Discharge {
     fscale = b.ln(slot, LN_FSCALE, g); fphi = b.ln(slot, LN_FPHI, g); fweight = b.ln(slot, LN_FWEIGHT, g); nscale = b.ln(slot, LN_NSCALE, g);
     nodes = b.chan + size_t(slot) * 2 * MAX_CHANNEL; parent = nodes + MAX_CHANNEL; dep = b.cdep + size_t(slot) * MAX_CHANNEL;
   }
+  // Front entry f, wherever it lives.
+  __device__ int nodeAt(int f) const { return f < qcap ? qnode[f] : fnode[f]; }
+  __device__ int linkAt(int f) const { return f < qcap ? int(qlink[f]) : flink[f]; }
+  __device__ int flagAt(int f) const { return f < qcap ? int(qflag[f]) : fflag[f]; }
+  __device__ double phiAt(int f) const { return f < qcap ? qphi[f] : fphi[f]; }
+  __device__ double scaleAt(int f) const { return f < qcap ? qscale[f] : fscale[f]; }
+  __device__ FrontEntry entryAt(int f) const { return {nodeAt(f), linkAt(f), flagAt(f), phiAt(f), scaleAt(f)}; }
+  __device__ void put(int f, int node, int link, int flag, double p, double s) {
+    if (f < qcap) { qnode[f] = node; qlink[f] = short(link); qflag[f] = (unsigned char)flag; qphi[f] = p; qscale[f] = s; }
+    else { fnode[f] = node; flink[f] = link; fflag[f] = flag; fphi[f] = p; fscale[f] = s; }
+  }
+  __device__ void relink(int f, int link, double s) {
+    if (f < qcap) { qlink[f] = short(link); qscale[f] = s; } else { flink[f] = link; fscale[f] = s; }
+  }
   // Clear air for the channel: no charge to speak of, or no cloud (cloud + ice + snow, lightning.ts:320-322).
   __device__ bool clear(int i) const {
     return fabs(double(rho[i])) < RHO_MIN || cloudy[i] <= float(CLOUDY);
@@ -299,25 +330,50 @@ template <class R> struct Discharge {
   __device__ int flagsOf(int i) const { return (clear(i) ? FF_CLEAR : 0) | (i < g.layer ? FF_GROUND : 0); }
   // The potential of the carriers as they now stand, then the front's and the channel's potentials and flags from it.
   __device__ void solve() {
+    // The LDS entries' node, link and scale wait in the global arrays while the solve takes the LDS; each comes back
+    // with its node's new potential and flags (each thread handles the same entries both times).
+    const int size = st.size, kept = min(size, qcap);
+    for (int f = threadIdx.x; f < kept; f += blockDim.x) { fnode[f] = qnode[f]; flink[f] = qlink[f]; fscale[f] = qscale[f]; }
+    __syncthreads();
     solveBlock(g, b, pl, slot, smem, chunk, nodes, dep, st.count, st.hat != 0);
     if (threadIdx.x == 0) st.hat = 1;
-    for (int f = threadIdx.x; f < st.size; f += blockDim.x) { const int j = fnode[f]; fphi[f] = phi[j]; fflag[f] = flagsOf(j); }
+    for (int f = threadIdx.x; f < size; f += blockDim.x) {
+      const int j = fnode[f];
+      put(f, j, flink[f], flagsOf(j), phi[j], fscale[f]);
+    }
     for (int c = threadIdx.x; c < st.count; c += blockDim.x) cflag[c] = clear(nodes[c]) ? 1 : 0;
     __syncthreads();
   }
   // The interior node with the largest |E|/E_init that has not failed this step (lightning.ts:235-248).
   __device__ void strongest(int stamp) {
+    // SU nodes a thread per round, every load of the round issued before any of them is used: one memory round trip a
+    // round instead of one a node.
+    constexpr int SU = 6;
     unsigned long long key = 0;
-    for (int i = g.layer + threadIdx.x; i < g.n - g.layer; i += blockDim.x) {
-      if (failed[i] == stamp) continue;
-      int z = i / g.layer, y = (i - z * g.layer) / g.nx, x = i - z * g.layer - y * g.nx, row = y * g.nx + z * g.layer;
-      int xp = x + 1 == g.nx ? 0 : x + 1, xm = x == 0 ? g.nx - 1 : x - 1, yp = y + 1 == g.ny ? 0 : y + 1, ym = y == 0 ? g.ny - 1 : y - 1;
-      R ex = R((phi[xp + row] - phi[xm + row]) / (2 * g.dx)), ey = R((phi[x + yp * g.nx + z * g.layer] - phi[x + ym * g.nx + z * g.layer]) / (2 * g.dy));
-      R ez = R((phi[i + g.layer] - phi[i - g.layer]) / (b.zs[z + 1] - b.zs[z - 1]));
-      R lim = R(einit[z]);
-      float r = float((ex * ex + ey * ey + ez * ez) / (lim * lim));
-      unsigned long long k = r > 0 ? (static_cast<unsigned long long>(__float_as_uint(r)) << 32) | static_cast<unsigned int>(0xFFFFFFFFu - unsigned(i)) : 0ull;
-      key = k > key ? k : key;
+    const int lo = g.layer, hi = g.n - g.layer, nx = g.nx, ny = g.ny, layer = g.layer;
+    for (int base = lo + threadIdx.x; base < hi; base += SU * int(blockDim.x)) {
+      double pxp[SU], pxm[SU], pyp[SU], pym[SU], pzp[SU], pzm[SU], dzz[SU], lim2[SU];
+      int fl[SU];
+#pragma unroll
+      for (int u = 0; u < SU; u++) {
+        const int i0 = base + u * int(blockDim.x), i = i0 < hi ? i0 : lo;
+        const int z = i / layer, r = i - z * layer, y = r / nx, x = r - y * nx, row = y * nx + z * layer;
+        const int xp = x + 1 == nx ? 0 : x + 1, xm = x == 0 ? nx - 1 : x - 1, yp = y + 1 == ny ? 0 : y + 1, ym = y == 0 ? ny - 1 : y - 1;
+        pxp[u] = phi[xp + row]; pxm[u] = phi[xm + row]; pyp[u] = phi[x + yp * nx + z * layer]; pym[u] = phi[x + ym * nx + z * layer];
+        pzp[u] = phi[i + layer]; pzm[u] = phi[i - layer]; dzz[u] = b.zs[z + 1] - b.zs[z - 1]; lim2[u] = einit[z];
+        fl[u] = i0 < hi ? failed[i] : stamp;
+      }
+#pragma unroll
+      for (int u = 0; u < SU; u++) {
+        if (fl[u] == stamp) continue;
+        const int i = base + u * int(blockDim.x);
+        R ex = R((pxp[u] - pxm[u]) / (2 * g.dx)), ey = R((pyp[u] - pym[u]) / (2 * g.dy));
+        R ez = R((pzp[u] - pzm[u]) / dzz[u]);
+        R lim = R(lim2[u]);
+        float r = float((ex * ex + ey * ey + ez * ez) / (lim * lim));
+        unsigned long long k = r > 0 ? (static_cast<unsigned long long>(__float_as_uint(r)) << 32) | static_cast<unsigned int>(0xFFFFFFFFu - unsigned(i)) : 0ull;
+        key = k > key ? k : key;
+      }
     }
     key = blockMax(key, reinterpret_cast<unsigned long long *>(red));
     if (threadIdx.x == 0) {
@@ -341,10 +397,12 @@ template <class R> struct Discharge {
       const int mk = mark[j], sn = seen[j], ps = pos[j];
       const double rj = double(rho[j]), sj = nscale[j], phj = phi[j];
       const float cj = cloudy[j];
+      // The link's scale from the level tables in LDS, while the node's loads are in flight.
+      const double hz = zsL[k] - zsL[z], threshold = F_CRIT * .5 * (eiL[z] + eiL[k]);
+      const double sl = sqrt((di * g.dx) * (di * g.dx) + (dj * g.dy) * (dj * g.dy) + hz * hz) * threshold;
       // Node i itself is on the channel (its mark was just set).
       if (!(j == i || mk == id || (k >= z && fabs(rj) < RHO_MIN))) {
-        const double hz = b.zs[k] - b.zs[z], threshold = F_CRIT * .5 * (einit[z] + einit[k]);
-        s = sqrt((di * g.dx) * (di * g.dx) + (dj * g.dy) * (dj * g.dy) + hz * hz) * threshold;
+        s = sl;
         if (sn != id) {
           fresh = true; pj = phj;
           fl = (fabs(rj) < RHO_MIN || cj <= float(CLOUDY) ? FF_CLEAR : 0) | (j < g.layer ? FF_GROUND : 0);
@@ -356,9 +414,9 @@ template <class R> struct Discharge {
     const unsigned long long nb = __ballot(fresh);
     if (fresh) {
       const int f = size + __popcll(nb & ((1ull << lane) - 1));
-      fnode[f] = j; flink[f] = c; fflag[f] = fl; fphi[f] = pj; fscale[f] = s; nscale[j] = s; seen[j] = id; pos[j] = f;
+      put(f, j, c, fl, pj, s); nscale[j] = s; seen[j] = id; pos[j] = f;
     }
-    if (better) { fscale[at] = s; flink[at] = c; nscale[j] = s; }
+    if (better) { relink(at, c, s); nscale[j] = s; }
     if (lane == 0) st.size = size + __popcll(nb);
   }
   // Adds node j (flags fl, potential pj) to the channel from channel node `from` (lightning.ts:293-303): thread 0 the
@@ -380,9 +438,9 @@ template <class R> struct Discharge {
   __device__ void removeAt(int n, int j) {
     const int last = --st.size;
     if (n != last) {
-      const int jl = fnode[last];
-      fnode[n] = jl; flink[n] = flink[last]; fflag[n] = fflag[last]; fphi[n] = fphi[last]; fscale[n] = fscale[last];
-      pos[jl] = n;
+      const FrontEntry e = entryAt(last);
+      put(n, e.node, e.link, e.flag, e.phi, e.scale);
+      pos[e.node] = n;
     }
     pos[j] = -1;
   }
@@ -390,7 +448,7 @@ template <class R> struct Discharge {
   __device__ void removeAt(int n, int j, const FrontEntry &e) {
     const int last = --st.size;
     if (n != last) {
-      fnode[n] = e.node; flink[n] = e.link; fflag[n] = e.flag; fphi[n] = e.phi; fscale[n] = e.scale;
+      put(n, e.node, e.link, e.flag, e.phi, e.scale);
       pos[e.node] = n;
     }
     pos[j] = -1;
@@ -401,7 +459,7 @@ template <class R> struct Discharge {
     const int size = st.size;
     const int per = (size + int(blockDim.x) - 1) / int(blockDim.x), f0 = threadIdx.x * per, f1 = min(size, f0 + per);
     int mine = 0;
-    for (int f = f0; f < f1; f++) mine += flink[f] == from;
+    for (int f = f0; f < f1; f++) mine += linkAt(f) == from;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int inc = mine;
     for (int o = 1; o < 64; o <<= 1) { int v = __shfl_up(inc, o, 64); if (lane >= o) inc += v; }
@@ -411,11 +469,11 @@ template <class R> struct Discharge {
     int at = 0, total = 0;
     for (int w = 0; w < int(blockDim.x) / 64; w++) { if (w < wave) at += waveCnt[w]; total += waveCnt[w]; }
     at += inc - mine;
-    for (int f = f0; f < f1; f++) if (flink[f] == from) { if (at < PRUNE_MAX) prunes[at] = f; at++; }
+    for (int f = f0; f < f1; f++) if (linkAt(f) == from) { if (at < PRUNE_MAX) prunes[at] = f; at++; }
     __syncthreads();
     if (threadIdx.x == 0) {
       if (total > PRUNE_MAX) atomicOr(reinterpret_cast<unsigned long long *>(b.sc(slot) + S_ERROR), (unsigned long long)ERR_PRUNE);
-      for (int k = min(total, PRUNE_MAX) - 1; k >= 0; k--) removeAt(prunes[k], fnode[prunes[k]]);
+      for (int k = min(total, PRUNE_MAX) - 1; k >= 0; k--) removeAt(prunes[k], nodeAt(prunes[k]));
     }
     __syncthreads();
   }
@@ -468,9 +526,9 @@ template <class R> struct Discharge {
   }
   // The weight of front entry f (lightning.ts:317-321).
   __device__ double weightAt(int f, int ground, double potential, double before) const {
-    const double pj = fphi[f];
-    const int fl = fflag[f];
-    const double w = (ground >= 0 && ((fl & FF_GROUND) || pj * before <= 0)) || ((fl & FF_CLEAR) && (pj - potential) * potential >= 0) ? 0 : fabs(potential - pj) / fscale[f] - 1;
+    const double pj = phiAt(f);
+    const int fl = flagAt(f);
+    const double w = (ground >= 0 && ((fl & FF_GROUND) || pj * before <= 0)) || ((fl & FF_CLEAR) && (pj - potential) * potential >= 0) ? 0 : fabs(potential - pj) / scaleAt(f) - 1;
     return w > 0 ? w : 0;
   }
   // Chooses the next link of the front (lightning.ts:307-324): the weights of each thread's run of entries (the
@@ -481,19 +539,10 @@ template <class R> struct Discharge {
     const double potential = st.potential, before = st.before;
     const int per = (size + int(blockDim.x) - 1) / int(blockDim.x), f0 = threadIdx.x * per, f1 = min(size, f0 + per);
     double w[PICK_REGS], local = 0;
-    FrontEntry e[PICK_REGS];
 #pragma unroll
     for (int q = 0; q < PICK_REGS; q++) {
       const int f = f0 + q;
-      if (f < f1) {
-        e[q].node = fnode[f]; e[q].link = flink[f]; e[q].flag = fflag[f]; e[q].phi = fphi[f]; e[q].scale = fscale[f];
-        const double pj = e[q].phi;
-        const double v = (ground >= 0 && ((e[q].flag & FF_GROUND) || pj * before <= 0)) || ((e[q].flag & FF_CLEAR) && (pj - potential) * potential >= 0) ? 0
-                                                                                                                                  : fabs(potential - pj) / e[q].scale - 1;
-        w[q] = v > 0 ? v : 0;
-      } else {
-        w[q] = 0;
-      }
+      w[q] = f < f1 ? weightAt(f, ground, potential, before) : 0.0;
       local += w[q];
     }
     for (int f = f0 + PICK_REGS; f < f1; f++) { const double v = weightAt(f, ground, potential, before); fweight[f] = v; local += v; }
@@ -503,30 +552,30 @@ template <class R> struct Discharge {
     for (int o = 1; o < 64; o <<= 1) { double v = __shfl_up(inc, o, 64); if (lane >= o) inc += v; }
     __shared__ double waveTot[DISCHARGE_THREADS / 64];
     if (lane == 63) waveTot[wave] = inc;
+    if (threadIdx.x == 0) st.pickN = size - 1;
     __syncthreads();
     double base = 0, total = 0;
     for (int v = 0; v < int(blockDim.x) / 64; v++) { if (v < wave) base += waveTot[v]; total += waveTot[v]; }
-    if (threadIdx.x == 0) { st.pickN = size - 1; if (total > 0) { double r; mulberryNext(st.rng, r); st.pick = r * total; } }
-    __syncthreads();
     if (total == 0) return false;
+    // The draw, by thread 0 alone (the run's stream is touched by thread 0 only), handed to the block through LDS.
+    __shared__ double pickS;
+    if (threadIdx.x == 0) { double r; mulberryNext(st.rng, r); pickS = r * total; }
+    __syncthreads();
+    const double pick = pickS;
     double run = base + inc - local;
-    const double pick = st.pick;
-    bool found = false;
+    int mine = -1;
 #pragma unroll
     for (int q = 0; q < PICK_REGS; q++) {
-      if (!found && f0 + q < f1) { run += w[q]; if (run >= pick) { atomicMin(&st.pickN, f0 + q); found = true; } }
+      if (mine < 0 && f0 + q < f1) { run += w[q]; if (run >= pick) mine = f0 + q; }
     }
-    for (int f = f0 + PICK_REGS; !found && f < f1; f++) { run += fweight[f]; if (run >= pick) { atomicMin(&st.pickN, f); found = true; } }
+    for (int f = f0 + PICK_REGS; mine < 0 && f < f1; f++) { run += fweight[f]; if (run >= pick) mine = f; }
+    // The first entry whose prefix reaches the draw: a thread's entries come before the next thread's, so a wave's is
+    // its lowest lane's that found one, and the block's the least of the waves'.
+    const unsigned long long any = __ballot(mine >= 0);
+    if (any && lane == __ffsll((long long)any) - 1) atomicMin(&st.pickN, mine);
     __syncthreads();
-    const int n = st.pickN, last = size - 1;
-    auto handOn = [&](int f, FrontEntry &out) {
-      if (f - f0 >= PICK_REGS) { out = {fnode[f], flink[f], fflag[f], fphi[f], fscale[f]}; return; }
-#pragma unroll
-      for (int q = 0; q < PICK_REGS; q++) if (f == f0 + q) out = e[q];
-    };
-    if (n >= f0 && n < f1) handOn(n, st.picked);
-    if (last >= f0 && last < f1) handOn(last, st.last);
-    __syncthreads();
+    // Thread 0 hands on the picked entry and the front's last one (which takes the picked one's place).
+    if (threadIdx.x == 0) { st.picked = entryAt(st.pickN); st.last = entryAt(size - 1); }
     return true;
   }
   // One flash from node `start` (lightning.ts:287-328). Returns its charge (0: none) and records its kind.
@@ -591,7 +640,7 @@ template <class R> struct Discharge {
 
 // The discharge of each run of the launch (lightning.ts:261-276): one block per run, `chunk` levels a solve pass.
 // advanced: the run's clock has already moved on by dt (the discharge of a step run beside the next step's dynamics).
-template <class R> __global__ __launch_bounds__(DISCHARGE_THREADS) void k_discharge(Geo g, Bufs<R> b, Plan pl, double dt, int limit, int chunk, int advanced) {
+template <class R> __global__ __launch_bounds__(DISCHARGE_THREADS) void k_discharge(Geo g, Bufs<R> b, Plan pl, double dt, int limit, int chunk, int advanced, int cap) {
   int slot = b.runs[blockIdx.x];
   double *S = b.sc(slot);
   unsigned long long key0 = reinterpret_cast<unsigned long long *>(S)[S_STRONGEST];
@@ -602,7 +651,9 @@ template <class R> __global__ __launch_bounds__(DISCHARGE_THREADS) void k_discha
   __shared__ FlashState st;
   __shared__ unsigned char cflag[MAX_CHANNEL];
   __shared__ int prunes[PRUNE_MAX];
-  Discharge<R> d(g, b, pl, slot, chunk, smem, red, st, cflag, prunes);
+  __shared__ double zsL[64], eiL[64];  // nz <= 64 (pool creation checks it)
+  for (int k = threadIdx.x; k < g.nz; k += blockDim.x) { zsL[k] = b.zs[k]; eiL[k] = b.lv(slot, L_EINIT, g)[k]; }
+  Discharge<R> d(g, b, pl, slot, chunk, smem, red, st, cflag, prunes, cap, zsL, eiL);
   const double *P = b.pr(slot);
   const double time = advanced ? P[P_TIME] : P[P_TIME] + dt;
   if (threadIdx.x == 0) {

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -72,6 +73,7 @@ struct Pool {
        *ehat = nullptr, *emean0 = nullptr, *dspec = nullptr, *fillp = nullptr, *gpart = nullptr, *zflag = nullptr, *landed = nullptr;
   int ftiles = 1, gmax = 1;  // strip tiles of a level; blocks of a run in the launches that carry charge to the ground
   int chunk = 1;          // levels per pass of the discharge's own electric solve (as many as the block's LDS holds)
+  int frontCap = 0;       // entries of a flash's front the discharge keeps in that LDS between its solves
   RotLevels rot{};        // the rotation diagnostics' levels and threshold
   int *single = nullptr;  // single[s] = s: the run list of one slot
   DiagLevels dlev{};      // the levels the model diagnostics look at
@@ -202,7 +204,8 @@ template <class R> void launchDischarge(const Pool &p, const Bufs<R> &b, int nr,
   const unsigned ib = blocks(size_t(g.layer) * (g.nz - 2), 256);
   // flashLimit (lightning.ts:46): 10 flashes per 3 s of model time.
   const int limit = std::max(1, int(std::floor(10 * dt / 3 + .5)));
-  hipLaunchKernelGGL(k_discharge<R>, dim3(nr), dim3(DISCHARGE_THREADS), solveLdsBytes(g, p.chunk), s, g, b, p.plan, dt, limit, p.chunk, advanced);
+  hipLaunchKernelGGL(k_discharge<R>, dim3(nr), dim3(DISCHARGE_THREADS), solveLdsBytes(g, p.chunk), s, g, b, p.plan, dt, limit, p.chunk, advanced,
+                     p.frontCap);
   if (t) t->mark("k_discharge");
   hipLaunchKernelGGL(k_conduct<R>, dim3(ib, nr), dim3(256), 0, s, g, b, dt);
   if (t) t->mark("k_conduct");
@@ -569,6 +572,9 @@ int ml_pool_create(ml_ctx *c, const ml_grid *grid, int capacity, int electrified
   while (chunk < L && solveLdsBytes(g, chunk + 1) + dischargeStatic <= ldsMax) chunk++;
   if (solveLdsBytes(g, chunk) + dischargeStatic > ldsMax) { freePool(p); return fail(c, ML_E_UNSUPPORTED, "grid too large for the discharge's level in LDS"); }
   p->chunk = chunk;
+  // METEOLAB_FRONT_LDS_ENTRIES lowers the front's LDS capacity (a check of the global overflow; results are the same).
+  p->frontCap = frontCapacity(g, chunk);
+  if (const char *env = std::getenv("METEOLAB_FRONT_LDS_ENTRIES")) p->frontCap = std::max(0, std::min(p->frontCap, std::atoi(env)));
   const int dyn = int(solveLdsBytes(g, chunk));
   if (p->rb == 8) POOLCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_discharge<double>), hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
   else POOLCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_discharge<float>), hipFuncAttributeMaxDynamicSharedMemorySize, dyn));

@@ -225,12 +225,18 @@ def oracle_restart(o, g, seed=1):
     return ref
 
 
-def test_lightning_matches_the_oracle_flash_for_flash(ctx, mature_small):
+@pytest.mark.parametrize("front_lds", [None, 16], ids=["front-in-lds", "front-spilled"])
+def test_lightning_matches_the_oracle_flash_for_flash(ctx, mature_small, front_lds, monkeypatch):
     # From 32.5 min into the small-grid storm the first flash comes at step 656 and 24 more follow by step 700 (the
     # oracle's counts): the same flashes, step by step, from the same seeded streams, and the charge fields after them.
+    # With front_lds the pool keeps only that many entries of a flash's front in LDS and the rest in its global arrays
+    # (METEOLAB_FRONT_LDS_ENTRIES, read at pool creation): the same flashes.
     g = C.small_grid()
     ref = oracle_restart(mature_small, g)
+    if front_lds is not None:
+        monkeypatch.setenv("METEOLAB_FRONT_LDS_ENTRIES", str(front_lds))
     r = restart(ctx, mature_small, g)
+    monkeypatch.delenv("METEOLAB_FRONT_LDS_ENTRIES", raising=False)
     counts = []
     for k in range(50):
         ref.step(1); r.step(1)
