@@ -47,6 +47,9 @@ __host__ __device__ inline size_t solveLdsBytes(const Geo &g, int C) { return (s
 // channel index below MAX_CHANNEL) and flags, 23 bytes an entry. Entries beyond the capacity live in the run's global
 // front arrays.
 constexpr int FRONT_ENTRY_BYTES = 23;
+// LDS pointers by address space, so that an LDS access is a DS instruction of its own (never a FLAT access that a select
+// between an LDS and a global place would otherwise make of it).
+template <class T> using Lds = __attribute__((address_space(3))) T;
 __host__ __device__ inline int frontCapacity(const Geo &g, int C) { return int(solveLdsBytes(g, C) / FRONT_ENTRY_BYTES); }
 static_assert(MAX_CHANNEL <= 32767, "a front entry's link is an int16");
 
@@ -256,6 +259,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
 }
 
 // ---- the flashes -----------------------------------------------------------------------------------------------
+
 // A front entry as a pick hands it on.
 struct FrontEntry { int node, link, flag; double phi, scale; };
 
@@ -280,12 +284,13 @@ template <class R> struct Discharge {
   // The front: entries f < qcap in LDS (qphi .. qflag, carved from smem while no solve runs), the rest in the global
   // arrays fnode .. fscale (which also keep the LDS entries' node, link and scale while a solve runs).
   int qcap;
-  double *qphi, *qscale;
-  int *qnode;
-  short *qlink;
-  unsigned char *qflag;
+  Lds<double> *qphi, *qscale;
+  Lds<int> *qnode;
+  Lds<short> *qlink;
+  Lds<unsigned char> *qflag;
   unsigned char *cflag;  // LDS: channel node c is clear air (as of the last solve)
   const double *zsL, *eiL;  // LDS: the level heights and E_init of the levels
+  Div dLay, dRow;           // node index -> level, level place -> row
   int *prunes;           // LDS: the front places a prune takes off
   float *car[4];
   const float *cloudy;
@@ -297,9 +302,9 @@ template <class R> struct Discharge {
 
   __device__ Discharge(const Geo &g_, const Bufs<R> &b_, const Plan &p_, int s_, int chunk_, char *m_, double *r_, FlashState &st_, unsigned char *cf_, int *pr_, int cap_,
                         const double *zs_, const double *ei_)
-      : g(g_), b(b_), pl(p_), slot(s_), chunk(chunk_), smem(m_), red(r_), st(st_), qcap(cap_), cflag(cf_), zsL(zs_), eiL(ei_), prunes(pr_) {
-    qphi = reinterpret_cast<double *>(smem); qscale = qphi + qcap; qnode = reinterpret_cast<int *>(qscale + qcap);
-    qlink = reinterpret_cast<short *>(qnode + qcap); qflag = reinterpret_cast<unsigned char *>(qlink + qcap);
+      : g(g_), b(b_), pl(p_), slot(s_), chunk(chunk_), smem(m_), red(r_), st(st_), qcap(cap_), cflag(cf_), zsL(zs_), eiL(ei_), dLay(g_.layer), dRow(g_.nx), prunes(pr_) {
+    qphi = (Lds<double> *)(smem); qscale = qphi + qcap; qnode = (Lds<int> *)(qscale + qcap);
+    qlink = (Lds<short> *)(qnode + qcap); qflag = (Lds<unsigned char> *)(qlink + qcap);
     car[0] = b.f(slot, F_CHARGE_ICE, g); car[1] = b.f(slot, F_CHARGE_GRAUPEL, g); car[2] = b.f(slot, F_CHARGE_RAIN, g); car[3] = b.f(slot, F_CHARGE_AIR, g);
     cloudy = b.f(slot, F_CLOUDY, g);
     air = b.lv(slot, L_RHO, g); einit = b.lv(slot, L_EINIT, g); cap = b.lv(slot, L_CAPACITANCE, g);
@@ -310,11 +315,11 @@ template <class R> struct Discharge {
     nodes = b.chan + size_t(slot) * 2 * MAX_CHANNEL; parent = nodes + MAX_CHANNEL; dep = b.cdep + size_t(slot) * MAX_CHANNEL;
   }
   // Front entry f, wherever it lives.
-  __device__ int nodeAt(int f) const { return f < qcap ? qnode[f] : fnode[f]; }
-  __device__ int linkAt(int f) const { return f < qcap ? int(qlink[f]) : flink[f]; }
-  __device__ int flagAt(int f) const { return f < qcap ? int(qflag[f]) : fflag[f]; }
-  __device__ double phiAt(int f) const { return f < qcap ? qphi[f] : fphi[f]; }
-  __device__ double scaleAt(int f) const { return f < qcap ? qscale[f] : fscale[f]; }
+  __device__ int nodeAt(int f) const { if (f < qcap) return qnode[f]; return fnode[f]; }
+  __device__ int linkAt(int f) const { if (f < qcap) return int(qlink[f]); return flink[f]; }
+  __device__ int flagAt(int f) const { if (f < qcap) return int(qflag[f]); return fflag[f]; }
+  __device__ double phiAt(int f) const { if (f < qcap) return qphi[f]; return fphi[f]; }
+  __device__ double scaleAt(int f) const { if (f < qcap) return qscale[f]; return fscale[f]; }
   __device__ FrontEntry entryAt(int f) const { return {nodeAt(f), linkAt(f), flagAt(f), phiAt(f), scaleAt(f)}; }
   __device__ void put(int f, int node, int link, int flag, double p, double s) {
     if (f < qcap) { qnode[f] = node; qlink[f] = short(link); qflag[f] = (unsigned char)flag; qphi[f] = p; qscale[f] = s; }
@@ -357,7 +362,7 @@ template <class R> struct Discharge {
 #pragma unroll
       for (int u = 0; u < SU; u++) {
         const int i0 = base + u * int(blockDim.x), i = i0 < hi ? i0 : lo;
-        const int z = i / layer, r = i - z * layer, y = r / nx, x = r - y * nx, row = y * nx + z * layer;
+        const int z = dLay.q(i), r = i - z * layer, y = dRow.q(r), x = r - y * nx, row = y * nx + z * layer;
         const int xp = x + 1 == nx ? 0 : x + 1, xm = x == 0 ? nx - 1 : x - 1, yp = y + 1 == ny ? 0 : y + 1, ym = y == 0 ? ny - 1 : y - 1;
         pxp[u] = phi[xp + row]; pxm[u] = phi[xm + row]; pyp[u] = phi[x + yp * nx + z * layer]; pym[u] = phi[x + ym * nx + z * layer];
         pzp[u] = phi[i + layer]; pzm[u] = phi[i - layer]; dzz[u] = b.zs[z + 1] - b.zs[z - 1]; lim2[u] = einit[z];
@@ -386,7 +391,7 @@ template <class R> struct Discharge {
   // the l-th neighbour of the reference's loop (level, row, column). `size` is the front's size before.
   __device__ void extend(int i, int c, int id, int size) {
     const int lane = threadIdx.x;
-    const int z = i / g.layer, y = (i - z * g.layer) / g.nx, x = i - z * g.layer - y * g.nx;
+    const int z = dLay.q(i), r = i - z * g.layer, y = dRow.q(r), x = r - y * g.nx;
     const int k = z - 1 + lane / 9, dj = (lane / 3) % 3 - 1, di = lane % 3 - 1;
     bool fresh = false, better = false;
     int j = 0, at = 0, fl = 0;

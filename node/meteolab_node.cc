@@ -140,6 +140,36 @@ napi_value SlotBytes(napi_env env, napi_callback_info info) {
   napi_value r; napi_create_double(env, double(bytes), &r); return r;
 }
 
+// poolGrow(ctx, pool, capacity): ml_pool_grow, the pool's runs kept.
+napi_value PoolGrow(napi_env env, napi_callback_info info) {
+  napi_value a[3]; if (!args(env, info, 3, a)) return nullptr;
+  ml_ctx *ctx = ctxOf(env, a[0]); if (!ctx) return nullptr;
+  int32_t pool, cap; if (!intOf(env, a[1], pool) || !intOf(env, a[2], cap)) return nullptr;
+  return status(env, ctx, ml_pool_grow(ctx, pool, cap));
+}
+
+// memInfo(ctx) -> [free, total] device bytes (ml_mem_info).
+napi_value MemInfo(napi_env env, napi_callback_info info) {
+  napi_value a[1]; if (!args(env, info, 1, a)) return nullptr;
+  ml_ctx *ctx = ctxOf(env, a[0]); if (!ctx) return nullptr;
+  int64_t fr = 0, tot = 0;
+  if (!status(env, ctx, ml_mem_info(ctx, &fr, &tot))) return nullptr;
+  napi_value r, v0, v1;
+  napi_create_array_with_length(env, 2, &r);
+  napi_create_double(env, double(fr), &v0); napi_create_double(env, double(tot), &v1);
+  napi_set_element(env, r, 0, v0); napi_set_element(env, r, 1, v1);
+  return r;
+}
+
+// ctxQuery(ctx) -> true when the context's work has finished (ml_ctx_query), without waiting.
+napi_value CtxQuery(napi_env env, napi_callback_info info) {
+  napi_value a[1]; if (!args(env, info, 1, a)) return nullptr;
+  ml_ctx *ctx = ctxOf(env, a[0]); if (!ctx) return nullptr;
+  const int r = ml_ctx_query(ctx);
+  if (r < 0) return status(env, ctx, r);
+  napi_value v; napi_get_boolean(env, r == 1, &v); return v;
+}
+
 // runCreate(ctx, pool, profile, config) -> run handle. `profile` and `config` are plain objects with the fields of
 // ml_profile / ml_run_config.
 napi_value RunCreate(napi_env env, napi_callback_info info) {
@@ -318,7 +348,8 @@ napi_value Init(napi_env env, napi_value exports) {
     {"physics", Part<ml_physics>}, {"mix", Part<ml_mix>}, {"afterStep", Part<ml_after_step>}, {"fallParts", FallParts},
     {"setTotals", SetTotals}, {"rotation", Doubles<ml_run_rotation>}, {"refreshRotation", RunOp<ml_run_refresh_rotation>},
     {"modelDiagnostics", ModelDiagnostics}, {"setSwaths", SetSwaths}, {"probabilityMap", ProbabilityMap},
-    {"intervene", Intervene}, {"added", Doubles<ml_run_added>}, {"setAdded", SetAdded},
+    {"intervene", Intervene}, {"added", Doubles<ml_run_added>}, {"setAdded", SetAdded}, {"poolGrow", PoolGrow},
+    {"memInfo", MemInfo}, {"ctxQuery", CtxQuery},
   };
   for (auto &f : fns) {
     napi_value v;

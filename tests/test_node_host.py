@@ -19,7 +19,8 @@ needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reaso
 ABI_FUNCTIONS = ["ctxCreate", "ctxDestroy", "deviceName", "poolCreate", "poolDestroy", "slotBytes", "runCreate", "runRelease",
                  "upload", "download", "uploadColumn", "downloadColumn", "step", "sync", "diag", "levels", "pressure", "flashes", "setTime",
                  "dynamics", "physics", "mix", "afterStep", "fallParts", "setTotals",
-                 "rotation", "refreshRotation", "modelDiagnostics", "setSwaths", "probabilityMap", "intervene", "added", "setAdded"]
+                 "rotation", "refreshRotation", "modelDiagnostics", "setSwaths", "probabilityMap", "intervene", "added", "setAdded",
+                 "poolGrow", "memInfo", "ctxQuery"]
 
 
 def node(code):
