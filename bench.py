@@ -107,8 +107,9 @@ def measured_traffic(kernel, runs, grid, tuning, steps, warmup):
 
 
 def members(pool, grid, count):
+    """The ensemble's first `count` members, made in one batch (ml_runs_create)."""
     prof = C.profile()
-    return [pool.run(prof, C.ensemble_member(grid, k)) for k in range(count)]
+    return pool.runs([(prof, C.ensemble_member(grid, k)) for k in range(count)])
 
 
 def snapshot(run):

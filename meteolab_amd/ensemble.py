@@ -254,7 +254,7 @@ def ensemble(grid, members, minutes, ctx, profile=None, data=False):
     starts = [(prof, C.ensemble_member(grid, k)) for k in range(members)]
     if data:
         starts = [s if k == 0 else perturb_run(grid, *s) for k, s in enumerate(starts)]
-    runs = [pool.run(p, spec) for p, spec in starts]
+    runs = pool.runs(starts)
     results = run_members(runs, minutes)
     peaks = [member_peaks(r) for r in results]
     maps = {kind: probability_map(runs, kind) for kind in MAPS}

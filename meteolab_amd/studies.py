@@ -61,10 +61,8 @@ def regime(steps=600, seeds=2):
         for size, members in classes.items():
             g = C.GRIDS[size]
             pool = ctx.pool(g, len(members))
-            runs = []
             t0 = time.perf_counter()
-            for (i, j, s) in members:
-                runs.append(pool.run(C.regime_cell(i, j, s), C.run(g, seed=1 + s, amp=2.0)))
+            runs = pool.runs([(C.regime_cell(i, j, s), C.run(g, seed=1 + s, amp=2.0)) for (i, j, s) in members])
             ctx.sync()
             create_s = time.perf_counter() - t0
             ms = M.step_timed(runs, steps, 3.0)

@@ -71,3 +71,32 @@ def test_an_evicted_request_readmitted_from_its_state_file_ends_as_alone():
             alone.step(r.minutes * S.MINUTE_STEPS)
             for name in M.CARRIED:
                 assert np.array_equal(finals[r.id][name], alone.field(name)), (r.id, name)
+
+
+@pytest.mark.gpu
+def test_a_grown_pool_keeps_its_runs_and_batched_creation_equals_one_by_one():
+    # ml_pool_grow: runs stepped before the growth go on as they would have; ml_runs_create: runs made in one batch
+    # start (and step) as the same runs made one at a time.
+    g = C.small_grid()
+    starts = [(C.profile(qv_max=0.012 + 0.0005 * k), C.run(g, seed=k + 1, amp=2.0 + 0.2 * k)) for k in range(5)]
+    with M.Context(0, 32) as ctx:
+        pool = ctx.pool(g, 2)
+        first = pool.runs(starts[:2])
+        M.step(first, 5)
+        ctx.sync()
+        pool.grow(5)
+        assert pool.capacity == 5 and pool.free() == 3
+        later = pool.runs(starts[2:])
+        for r, (p, spec) in zip(later, starts[2:]):
+            one = ctx.pool(g, 1).run(p, spec)
+            for name in M.CARRIED:
+                assert np.array_equal(r.field(name), one.field(name)), name
+            one.pool.destroy()
+        M.step(first + later, 5)
+        ctx.sync()
+        for k, r in enumerate(first + later):
+            alone = ctx.pool(g, 1).run(*starts[k])
+            alone.step(10 if k < 2 else 5)
+            for name in M.CARRIED:
+                assert np.array_equal(r.field(name), alone.field(name)), (k, name)
+            alone.pool.destroy()
