@@ -371,10 +371,11 @@ void fillTables(Pool &p, std::vector<char> &sym, std::vector<char> &twx, std::ve
 }
 
 struct Sizes {
-  size_t fields, stage, mass, node, node8, spec, mean0, cols, lev, par, scal, lmark, lnode, chan, cdep, flog, fillp, gpart, zflag, landed;
+  size_t fields, stage, mass, node, node8, spec, mean0, cols, lev, par, scal, diag, dlist, lmark, lnode, chan, cdep, flog, fillp, gpart, zflag,
+      landed;
   size_t slot() const {
-    return fields + 2 * stage + mass + 3 * node + 4 * node8 + 3 * spec + 2 * mean0 + cols + lev + par + scal + lmark + lnode + chan + cdep + flog +
-           fillp + gpart + zflag + landed;
+    return fields + 2 * stage + mass + 3 * node + 4 * node8 + 3 * spec + 2 * mean0 + cols + lev + par + scal + diag + dlist + lmark + lnode + chan +
+           cdep + flog + fillp + gpart + zflag + landed;
   }
 };
 // Blocks of a run in the launches that carry charge to the ground: the charge falls and the conduction.
@@ -383,7 +384,7 @@ Sizes sizesOf(const Geo &g, size_t rb) {
   Sizes s;
   s.fields = size_t(N_FIELDS) * g.n * 4; s.stage = size_t(N_CARRIED) * g.n * 4; s.mass = 3 * size_t(g.n) * 8; s.node = size_t(g.n) * rb; s.node8 = size_t(g.n) * 8;
   s.spec = size_t(g.nz) * g.nm * 2 * 8; s.mean0 = size_t(g.nz) * 8; s.cols = size_t(N_COLUMNS) * g.layer * 4;
-  s.lev = size_t(N_LEVELS) * g.nz * 8; s.par = N_PARAMS * 8; s.scal = N_SCALARS * 8;
+  s.lev = size_t(N_LEVELS) * g.nz * 8; s.par = N_PARAMS * 8; s.scal = N_SCALARS * 8; s.diag = size_t(DIAG_ROW) * 8; s.dlist = sizeof(int);
   s.lmark = size_t(N_LMARK) * g.n * 4; s.lnode = size_t(N_LNODE) * g.n * 8; s.chan = 2 * size_t(MAX_CHANNEL) * 4; s.cdep = size_t(MAX_CHANNEL) * 8;
   s.flog = size_t(FLASH_LOG) * FLASH_FIELDS * 8;
   s.fillp = size_t(N_CONSERVED) * g.nz * 3 * 8;
@@ -475,7 +476,7 @@ std::vector<SlotArray> slotArrays(Pool *p, const Sizes &s) {
   return {{&p->fields, s.fields}, {&p->stx, s.stage}, {&p->sty, s.stage}, {&p->mass, s.mass}, {&p->phi, s.node8}, {&p->cells, s.node8},
           {&p->press, s.node8}, {&p->spec, s.spec}, {&p->ehat, s.spec}, {&p->dspec, s.spec}, {&p->mean0, s.mean0}, {&p->emean0, s.mean0},
           {&p->km, s.node}, {&p->kh, s.node}, {&p->ephi, s.node8}, {&p->cols, s.cols}, {&p->lev, s.lev}, {&p->par, s.par},
-          {&p->scal, s.scal}, {(void **)&p->diag, size_t(DIAG_ROW) * 8}, {(void **)&p->dlist, sizeof(int)}, {&p->lrho, s.node},
+          {&p->scal, s.scal}, {(void **)&p->diag, s.diag}, {(void **)&p->dlist, s.dlist}, {&p->lrho, s.node},
           {&p->lmark, s.lmark}, {&p->lnode, s.lnode}, {&p->chan, s.chan}, {&p->cdep, s.cdep}, {&p->flog, s.flog}, {&p->fillp, s.fillp},
           {&p->gpart, s.gpart}, {&p->zflag, s.zflag}, {&p->landed, s.landed}};
 }

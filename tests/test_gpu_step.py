@@ -64,8 +64,8 @@ def test_base_state_and_initial_state_match(ctx):
 @pytest.mark.parametrize("grid", ["small", "standard", "large"])
 def test_parity_after_one_and_ten_steps(ctx, grid):
     # The three sizes of the workloads: the small test grid, the standard (40x32x50) domain the 1e-5 bar is set on, and
-    # the large (80x64x50) one. The float32 rounding of the transport adds ~1e-6 a step to w's relative difference
-    # while w is still ~0.2 m/s; on the large domain it reaches 1.25e-5 after ten steps (8e-6 on the standard).
+    # the large (80x64x50) one, each held to 1e-5 per field. With the fluxes taken relative to their node, w's relative
+    # difference after ten steps is 1.0e-6 on the small grid, 2.8e-6 on the standard and 5.9e-6 on the large.
     g = C.small_grid() if grid == "small" else C.STANDARD if grid == "standard" else C.LARGE
     prof, run = C.profile(), C.run(g)
     o = OracleRun(g, prof, run)
@@ -82,7 +82,7 @@ def test_parity_after_one_and_ten_steps(ctx, grid):
 
 def test_large_domain_parity_of_the_float64_kernels():
     # The same steps of the large domain with the float64 kernels (Context precision 64): w stays within 1e-5 after ten
-    # steps too, since the float32 transport's rounding is what moves it on the large domain.
+    # steps too (3.5e-6), the float32 rounding of the production kernels aside.
     g = C.LARGE
     prof, run = C.profile(), C.run(g)
     o = OracleRun(g, prof, run)
