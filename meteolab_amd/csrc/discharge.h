@@ -19,6 +19,17 @@ constexpr int FF_CLEAR = 1, FF_GROUND = 2;  // flags of a front entry: clear air
 constexpr int PICK_REGS = 8;                // front entries a thread keeps in registers between the two passes of a pick
 constexpr int PRUNE_MAX = 32;               // links from one channel node (at most its 26 neighbours)
 
+// The block's barrier in this file. The waves of a block hand data to each other through global memory here (the
+// front's entries past its LDS part, the potential, the spectra, charges, marks), and __syncthreads() alone does not
+// order those accesses: its workgroup-scope fences emit no wait for the wave's outstanding vector memory operations on
+// gfx950 (the block's waves share one CU), and measured runs showed a wave's global store issued before the barrier
+// unseen by another wave's load after it now and then (flash counts and potentials that changed with the code's
+// layout and timing, identical runs in one launch agreeing). So each wave first waits for its own loads and stores.
+__device__ inline void blockSync() {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+}
+
 // ---- the block's electric solve (electric.ts:36-60) ---------------------------------------------------------------
 // The modes' tridiagonal systems (electric.ts:42-52) depend on the grid alone, so the pool computes their elimination
 // once, as modeSolve does it: per interior level k and mode, the pivot, the upper factor and the lower coefficient,
@@ -96,12 +107,12 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
       }
       const unsigned long long bal = __ballot(keep);
       if (lane == 0) waveCount[threadIdx.x >> 6] = __popcll(bal);
-      __syncthreads();
+      blockSync();
       int at = total, all = 0;
       for (int v = 0; v < waves; v++) { if (v < int(threadIdx.x >> 6)) at += waveCount[v]; all += waveCount[v]; }
       if (keep) { at += __popcll(bal & ((1ull << lane) - 1)); keys[at] = key; ds[at] = d; }
       total += all;
-      __syncthreads();
+      blockSync();
     }
     // Each thread its modes: the changes of a level summed in registers, then added to the level's spectrum.
     for (int mode = threadIdx.x; mode < nm; mode += blockDim.x) {
@@ -129,7 +140,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
         if (mode == 0) m0[cur] += s0;
       }
     }
-    __syncthreads();
+    blockSync();
   } else {
   // The bottom and top levels' charge density (the interior levels' comes with their transforms).
   for (int t = threadIdx.x; t < 2 * layer; t += blockDim.x) {
@@ -148,7 +159,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
       A[c * ls + r] = {-r0 / EPSILON0, -r1 / EPSILON0};
       B[c * ls + r] = {r0, r1};
     }
-    __syncthreads();
+    blockSync();
     for (int t = threadIdx.x; t < cn * hl; t += blockDim.x) {
       const int c = dHl.q(t), r = t - c * hl, p = dNx.q(r), x = r - p * nx, z = l0 + c + 1;
       const int i0 = z * layer + 2 * p * nx + x;
@@ -161,7 +172,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
       s = waveSum(s);
       if (lane == 0) m0[l0 + c] = s;
     }
-    __syncthreads();
+    blockSync();
     Z *X = fftStages(A, B, nx, cn * half, nx, 1, tx, pl.rx, pl.nrx, -1, half, ls);
     Z *Y = X == A ? B : A;
     for (int t = threadIdx.x; t < cn * half * mw; t += blockDim.x) {
@@ -170,14 +181,14 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
       Y[c * ls + 2 * p * mw + m] = {(z.re + w.re) / 2, (z.im - w.im) / 2};
       Y[c * ls + (2 * p + 1) * mw + m] = {(z.im + w.im) / 2, (w.re - z.re) / 2};
     }
-    __syncthreads();
+    blockSync();
     Z *S = fftStages(Y, X, ny, cn * mw, 1, mw, ty, pl.ry, pl.nry, -1, mw, ls);
     for (int t = threadIdx.x; t < cn * nm; t += blockDim.x) {
       const int c = dNm.q(t), m = t - c * nm;
       double *o = hat + (size_t(l0 + c) * nm + m) * 2;
       o[0] = S[c * ls + m].re; o[1] = S[c * ls + m].im;
     }
-    __syncthreads();
+    blockSync();
   }
   }
   // The modes' columns: forward elimination from the right-hand side's spectra into the potential's, then back
@@ -222,7 +233,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
       }
     }
   }
-  __syncthreads();
+  blockSync();
   // Inverse transforms into the potential's interior levels, normalised by nx ny.
   const double norm = 1.0 / double(layer);
   const int mh = nx / 2;
@@ -234,7 +245,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
       const double *s = spec + (size_t(l0 + c) * nm + m) * 2;
       A[c * ls + m] = {s[0], s[1]};
     }
-    __syncthreads();
+    blockSync();
     Z *Yv = fftStages(A, B, ny, cn * mw, 1, mw, ty, pl.ry, pl.nry, +1, mw, ls);
     Z *Zr = Yv == A ? B : A;
     for (int t = threadIdx.x; t < cn * half * nx; t += blockDim.x) {
@@ -246,7 +257,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
       const double ar = a.re, ai = real ? 0.0 : sg * a.im, br = bb.re, bi = real ? 0.0 : sg * bb.im;
       Zr[c * ls + r] = {ar - bi, ai + br};
     }
-    __syncthreads();
+    blockSync();
     Z *Xv = fftStages(Zr, Yv, nx, cn * half, nx, 1, tx, pl.rx, pl.nrx, +1, half, ls);
     for (int t = threadIdx.x; t < cn * hl; t += blockDim.x) {
       const int c = dHl.q(t), r = t - c * hl, p = dNx.q(r), x = r - p * nx;
@@ -254,7 +265,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
       dst[2 * p * nx + x] = Xv[c * ls + r].re * norm;
       dst[(2 * p + 1) * nx + x] = Xv[c * ls + r].im * norm;
     }
-    __syncthreads();
+    blockSync();
   }
 }
 
@@ -339,7 +350,7 @@ template <class R> struct Discharge {
     // with its node's new potential and flags (each thread handles the same entries both times).
     const int size = st.size, kept = min(size, qcap);
     for (int f = threadIdx.x; f < kept; f += blockDim.x) { fnode[f] = qnode[f]; flink[f] = qlink[f]; fscale[f] = qscale[f]; }
-    __syncthreads();
+    blockSync();
     solveBlock(g, b, pl, slot, smem, chunk, nodes, dep, st.count, st.hat != 0);
     if (threadIdx.x == 0) st.hat = 1;
     for (int f = threadIdx.x; f < size; f += blockDim.x) {
@@ -347,7 +358,7 @@ template <class R> struct Discharge {
       put(f, j, flink[f], flagsOf(j), phi[j], fscale[f]);
     }
     for (int c = threadIdx.x; c < st.count; c += blockDim.x) cflag[c] = clear(nodes[c]) ? 1 : 0;
-    __syncthreads();
+    blockSync();
   }
   // The interior node with the largest |E|/E_init that has not failed this step (lightning.ts:235-248).
   __device__ void strongest(int stamp) {
@@ -385,7 +396,7 @@ template <class R> struct Discharge {
       st.best = key ? int(0xFFFFFFFFu - unsigned(key & 0xFFFFFFFFull)) : -1;
       st.ratio = key ? sqrt(double(__uint_as_float(unsigned(key >> 32)))) : 0;
     }
-    __syncthreads();
+    blockSync();
   }
   // The links of new channel node i (channel index c) to the nodes beside it (lightning.ts:388-404); wave 0, lane l
   // the l-th neighbour of the reference's loop (level, row, column). `size` is the front's size before.
@@ -437,7 +448,7 @@ template <class R> struct Discharge {
       }
       if (!grounded) extend(j, c, id, size);
     }
-    __syncthreads();
+    blockSync();
   }
   // Takes entry n (node j) off the front: the last entry moves into its place (lightning.ts:329). Thread 0.
   __device__ void removeAt(int n, int j) {
@@ -470,17 +481,17 @@ template <class R> struct Discharge {
     for (int o = 1; o < 64; o <<= 1) { int v = __shfl_up(inc, o, 64); if (lane >= o) inc += v; }
     __shared__ int waveCnt[DISCHARGE_THREADS / 64];
     if (lane == 63) waveCnt[wave] = inc;
-    __syncthreads();
+    blockSync();
     int at = 0, total = 0;
     for (int w = 0; w < int(blockDim.x) / 64; w++) { if (w < wave) at += waveCnt[w]; total += waveCnt[w]; }
     at += inc - mine;
     for (int f = f0; f < f1; f++) if (linkAt(f) == from) { if (at < PRUNE_MAX) prunes[at] = f; at++; }
-    __syncthreads();
+    blockSync();
     if (threadIdx.x == 0) {
       if (total > PRUNE_MAX) atomicOr(reinterpret_cast<unsigned long long *>(b.sc(slot) + S_ERROR), (unsigned long long)ERR_PRUNE);
       for (int k = min(total, PRUNE_MAX) - 1; k >= 0; k--) removeAt(prunes[k], nodeAt(prunes[k]));
     }
-    __syncthreads();
+    blockSync();
   }
   // The charge of the channel at potential V onto its nodes (lightning.ts:330-368); returns the neutralized (IC) or
   // lowered (CG) charge. Whole block.
@@ -526,7 +537,7 @@ template <class R> struct Discharge {
       rho[i] = R(after);
       dep[c] = z >= 1 && z <= g.nz - 2 ? -(after - before) / EPSILON0 : 0.0;
     }
-    __syncthreads();
+    blockSync();
     return charge;
   }
   // The weight of front entry f (lightning.ts:317-321).
@@ -558,14 +569,14 @@ template <class R> struct Discharge {
     __shared__ double waveTot[DISCHARGE_THREADS / 64];
     if (lane == 63) waveTot[wave] = inc;
     if (threadIdx.x == 0) st.pickN = size - 1;
-    __syncthreads();
+    blockSync();
     double base = 0, total = 0;
     for (int v = 0; v < int(blockDim.x) / 64; v++) { if (v < wave) base += waveTot[v]; total += waveTot[v]; }
     if (total == 0) return false;
     // The draw, by thread 0 alone (the run's stream is touched by thread 0 only), handed to the block through LDS.
     __shared__ double pickS;
     if (threadIdx.x == 0) { double r; mulberryNext(st.rng, r); pickS = r * total; }
-    __syncthreads();
+    blockSync();
     const double pick = pickS;
     double run = base + inc - local;
     int mine = -1;
@@ -578,7 +589,7 @@ template <class R> struct Discharge {
     // its lowest lane's that found one, and the block's the least of the waves'.
     const unsigned long long any = __ballot(mine >= 0);
     if (any && lane == __ffsll((long long)any) - 1) atomicMin(&st.pickN, mine);
-    __syncthreads();
+    blockSync();
     // Thread 0 hands on the picked entry and the front's last one (which takes the picked one's place).
     if (threadIdx.x == 0) { st.picked = entryAt(st.pickN); st.last = entryAt(size - 1); }
     return true;
@@ -593,12 +604,12 @@ template <class R> struct Discharge {
       st.id = int(id); st.size = 0; st.count = 0; st.sum = 0; st.potential = 0; st.before = 0; st.ground = -1; st.touched = 0; st.left = 0; st.moment = 0;
       st.picked = {start, -1, flagsOf(start), phi[start], 0.0};
     }
-    __syncthreads();
+    blockSync();
     add(st.picked.node, -1, st.picked.flag, st.picked.phi);
     for (;;) {
       const int grown = st.count;
       if (threadIdx.x == 0) st.event = 0;
-      __syncthreads();
+      blockSync();
       while (st.count < MAX_CHANNEL) {
         if (!pickLink()) break;
         if (threadIdx.x == 0) {
@@ -611,18 +622,18 @@ template <class R> struct Discharge {
             if (cflag[p.link]) st.prune = p.link;
           }
         }
-        __syncthreads();
+        blockSync();
         if (st.brk) break;
         if (st.prune >= 0) prune(st.prune);
         add(st.picked.node, st.picked.link, st.picked.flag, st.picked.phi);
         if (st.touched) {
-          __syncthreads();
+          blockSync();
           if (threadIdx.x == 0) { st.touched = 0; st.event = 1; }
-          __syncthreads();
+          blockSync();
           break;
         }
       }
-      __syncthreads();
+      blockSync();
       const bool grounded = st.ground >= 0;
       double q = deposit(st.potential, grounded);
       if (grounded) lowered += q; else neutralized += q;
@@ -635,7 +646,7 @@ template <class R> struct Discharge {
         for (int c = threadIdx.x; c < st.count; c += blockDim.x) s += double(phi[nodes[c]]);
         s = blockSum(s, red);
         if (threadIdx.x == 0) { st.sum = s; st.potential = s / st.count; }
-        __syncthreads();
+        blockSync();
       }
     }
     cg = st.ground >= 0; count = st.count; potentialOut = cg ? st.before : st.potential; momentOut = cg ? st.moment : 0;
@@ -671,7 +682,7 @@ template <class R> __global__ __launch_bounds__(DISCHARGE_THREADS) void k_discha
     st.best = int(0xFFFFFFFFu - unsigned(key0 & 0xFFFFFFFFull));
     st.ratio = sqrt(double(__uint_as_float(unsigned(key0 >> 32))));
   }
-  __syncthreads();
+  blockSync();
   const int stamp = st.id;
   for (int k = 0; k <= limit; k++) {
     if (k > 0) d.solve();
@@ -681,7 +692,7 @@ template <class R> __global__ __launch_bounds__(DISCHARGE_THREADS) void k_discha
     int start = st.best;
     bool cg; int count; double potential, moment;
     double charge = d.flash(start, cg, count, potential, moment);
-    if (charge == 0) { if (threadIdx.x == 0) d.failed[start] = stamp; __syncthreads(); continue; }
+    if (charge == 0) { if (threadIdx.x == 0) d.failed[start] = stamp; blockSync(); continue; }
     if (threadIdx.x == 0) {
       long long c; memcpy(&c, S + (cg ? S_FLASH_CG : S_FLASH_IC), 8); c++; memcpy(S + (cg ? S_FLASH_CG : S_FLASH_IC), &c, 8);
       if (cg) S[S_CHARGE_GROUND] += charge;
@@ -691,7 +702,7 @@ template <class R> __global__ __launch_bounds__(DISCHARGE_THREADS) void k_discha
       e[FL_TIME] = time; e[FL_CG] = cg; e[FL_START] = start; e[FL_NODES] = count; e[FL_CHARGE] = charge; e[FL_POTENTIAL] = potential; e[FL_MOMENT] = moment;
       nlog++; memcpy(S + S_FLASH_LOG, &nlog, 8);
     }
-    __syncthreads();
+    blockSync();
   }
 }
 

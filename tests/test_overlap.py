@@ -65,10 +65,12 @@ def storm():
 
 
 def test_the_overlapped_steps_end_as_the_steps_in_line(storm):
-    # One context: the transport layout is fixed per grid in a context (the layouts round differently in float32).
+    # One context, a pool of the same three runs for each way of stepping them.
     results = []
     with M.Context(0, 32) as ctx:
-        for overlap, calls in ((False, [6]), (True, [6]), (True, [1, 2, 3])):
+        # In line in one call and in one-step calls, overlapped in one call, in calls of 1, 2, 3 and in one-step calls
+        # (each a step, then its discharge after the step's end).
+        for overlap, calls in ((False, [6]), (False, [1] * 6), (True, [6]), (True, [1, 2, 3]), (True, [1] * 6)):
             ctx.set_overlap(overlap)
             runs = setup(ctx, storm)
             print("overlap", overlap, "calls", calls, flush=True)
@@ -78,6 +80,6 @@ def test_the_overlapped_steps_end_as_the_steps_in_line(storm):
             results.append([state(r) for r in runs])
             assert results[-1][0][2]["flashesIC"] + results[-1][0][2]["flashesCG"] > 0
             runs[0].pool.destroy()
-    for k in (1, 2):
+    for k in range(1, len(results)):
         for a, b in zip(results[0], results[k]):
             same(a, b)
