@@ -305,7 +305,7 @@ template <class R> struct Discharge {
   int *prunes;           // LDS: the front places a prune takes off
   float *car[4];
   const float *cloudy;
-  const double *air, *einit, *cap;
+  const double *air, *cap;
   double *phi;
   R *rho;
   int *mark, *failed, *seen, *pos, *fnode, *flink, *fflag, *nodes, *parent;
@@ -318,7 +318,7 @@ template <class R> struct Discharge {
     qlink = (Lds<short> *)(qnode + qcap); qflag = (Lds<unsigned char> *)(qlink + qcap);
     car[0] = b.f(slot, F_CHARGE_ICE, g); car[1] = b.f(slot, F_CHARGE_GRAUPEL, g); car[2] = b.f(slot, F_CHARGE_RAIN, g); car[3] = b.f(slot, F_CHARGE_AIR, g);
     cloudy = b.f(slot, F_CLOUDY, g);
-    air = b.lv(slot, L_RHO, g); einit = b.lv(slot, L_EINIT, g); cap = b.lv(slot, L_CAPACITANCE, g);
+    air = b.lv(slot, L_RHO, g); cap = b.lv(slot, L_CAPACITANCE, g);
     phi = b.arr(b.ephi, slot, g); rho = b.arr(b.lrho, slot, g);
     mark = b.lm(slot, LM_MARK, g); failed = b.lm(slot, LM_FAILED, g); seen = b.lm(slot, LM_SEEN, g); pos = b.lm(slot, LM_POS, g);
     fnode = b.lm(slot, LM_FNODE, g); flink = b.lm(slot, LM_FLINK, g); fflag = b.lm(slot, LM_FFLAG, g);
@@ -376,7 +376,7 @@ template <class R> struct Discharge {
         const int z = dLay.q(i), r = i - z * layer, y = dRow.q(r), x = r - y * nx, row = y * nx + z * layer;
         const int xp = x + 1 == nx ? 0 : x + 1, xm = x == 0 ? nx - 1 : x - 1, yp = y + 1 == ny ? 0 : y + 1, ym = y == 0 ? ny - 1 : y - 1;
         pxp[u] = phi[xp + row]; pxm[u] = phi[xm + row]; pyp[u] = phi[x + yp * nx + z * layer]; pym[u] = phi[x + ym * nx + z * layer];
-        pzp[u] = phi[i + layer]; pzm[u] = phi[i - layer]; dzz[u] = b.zs[z + 1] - b.zs[z - 1]; lim2[u] = einit[z];
+        pzp[u] = phi[i + layer]; pzm[u] = phi[i - layer]; dzz[u] = zsL[z + 1] - zsL[z - 1]; lim2[u] = eiL[z];
         fl[u] = i0 < hi ? failed[i] : stamp;
       }
 #pragma unroll
