@@ -399,8 +399,9 @@ template <class R> struct Discharge {
     blockSync();
   }
   // The links of new channel node i (channel index c) to the nodes beside it (lightning.ts:388-404); wave 0, lane l
-  // the l-th neighbour of the reference's loop (level, row, column). `size` is the front's size before.
-  __device__ void extend(int i, int c, int id, int size) {
+  // the l-th neighbour of the reference's loop (level, row, column). `size` is the front's size before. Returns
+  // whether an entry it wrote or relinked lies past the front's LDS part (in global memory, read next by every wave).
+  __device__ bool extend(int i, int c, int id, int size) {
     const int lane = threadIdx.x;
     const int z = dLay.q(i), r = i - z * g.layer, y = dRow.q(r), x = r - y * g.nx;
     const int k = z - 1 + lane / 9, dj = (lane / 3) % 3 - 1, di = lane % 3 - 1;
@@ -434,9 +435,11 @@ template <class R> struct Discharge {
     }
     if (better) { relink(at, c, s); nscale[j] = s; }
     if (lane == 0) st.size = size + __popcll(nb);
+    return size + __popcll(nb) > qcap || __ballot(better && at >= qcap) != 0;
   }
   // Adds node j (flags fl, potential pj) to the channel from channel node `from` (lightning.ts:293-303): thread 0 the
-  // channel, wave 0 the new links. Ends with a barrier.
+  // channel, wave 0 the new links. Ends with a barrier. Of wave 0's global stores only front entries past the LDS part
+  // are read by other waves before a later barrier that waits for them, so wave 0 waits here only when it wrote one.
   __device__ void add(int j, int from, int fl, double pj) {
     if (threadIdx.x < 64) {
       const int c = st.count, size = st.size, id = st.id;
@@ -446,9 +449,9 @@ template <class R> struct Discharge {
         if (grounded) { st.ground = j; st.before = st.potential; st.potential = 0; st.touched = 1; }
         else if (st.ground < 0) { st.sum += pj; st.potential = st.sum / (c + 1); }
       }
-      if (!grounded) extend(j, c, id, size);
+      if (!grounded && extend(j, c, id, size)) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
-    blockSync();
+    __syncthreads();
   }
   // Takes entry n (node j) off the front: the last entry moves into its place (lightning.ts:329). Thread 0.
   __device__ void removeAt(int n, int j) {
@@ -553,6 +556,8 @@ template <class R> struct Discharge {
   __device__ bool pickLink() {
     const int size = st.size, ground = st.ground;
     const double potential = st.potential, before = st.before;
+    // The run's stream as it stands: every thread reads it before the first barrier, thread 0 advances it after it.
+    const uint32_t rng0 = st.rng;
     const int per = (size + int(blockDim.x) - 1) / int(blockDim.x), f0 = threadIdx.x * per, f1 = min(size, f0 + per);
     double w[PICK_REGS], local = 0;
 #pragma unroll
@@ -573,11 +578,11 @@ template <class R> struct Discharge {
     double base = 0, total = 0;
     for (int v = 0; v < int(blockDim.x) / 64; v++) { if (v < wave) base += waveTot[v]; total += waveTot[v]; }
     if (total == 0) return false;
-    // The draw, by thread 0 alone (the run's stream is touched by thread 0 only), handed to the block through LDS.
-    __shared__ double pickS;
-    if (threadIdx.x == 0) { double r; mulberryNext(st.rng, r); pickS = r * total; }
-    blockSync();
-    const double pick = pickS;
+    // The draw: every thread takes it from the same state and the same total, so all hold the same value.
+    uint32_t rs = rng0;
+    double r;
+    mulberryNext(rs, r);
+    const double pick = r * total;
     double run = base + inc - local;
     int mine = -1;
 #pragma unroll
@@ -590,8 +595,9 @@ template <class R> struct Discharge {
     const unsigned long long any = __ballot(mine >= 0);
     if (any && lane == __ffsll((long long)any) - 1) atomicMin(&st.pickN, mine);
     blockSync();
-    // Thread 0 hands on the picked entry and the front's last one (which takes the picked one's place).
-    if (threadIdx.x == 0) { st.picked = entryAt(st.pickN); st.last = entryAt(size - 1); }
+    // Thread 0 advances the stream and hands on the picked entry and the front's last one (which takes the picked
+    // one's place).
+    if (threadIdx.x == 0) { st.rng = rs; st.picked = entryAt(st.pickN); st.last = entryAt(size - 1); }
     return true;
   }
   // One flash from node `start` (lightning.ts:287-328). Returns its charge (0: none) and records its kind.
@@ -618,11 +624,14 @@ template <class R> struct Discharge {
           // The first step out of the cloud ends the stage; the link stays on the front.
           if (!st.left && st.ground < 0 && (p.flag & FF_CLEAR)) { st.left = 1; st.event = 1; st.brk = 1; }
           else {
+            // The last entry moves into the picked one's place: other waves read it next if that is in global memory.
+            const bool glob = st.pickN >= qcap && st.pickN != st.size - 1;
             removeAt(st.pickN, p.node, st.last);
             if (cflag[p.link]) st.prune = p.link;
+            if (glob) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
           }
         }
-        blockSync();
+        __syncthreads();
         if (st.brk) break;
         if (st.prune >= 0) prune(st.prune);
         add(st.picked.node, st.picked.link, st.picked.flag, st.picked.phi);
