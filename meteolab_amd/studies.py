@@ -53,17 +53,24 @@ def precision(steps=1200, grid="large", every=200):
 
 
 def regime(steps=600, seeds=2):
+    """The regime map: 8 moisture caps x 8 shears x `seeds` seeds, the standard-domain cells (shear index < 6) and the
+    large ones batched by size class. Each class is timed alone (graph-replayed steps, device events), then both
+    classes are stepped again from their initial state at the same time, each on a context of its own, on the wall
+    clock: a class's discharge leaves most CUs idle, which the other class's steps then fill."""
     import meteolab_amd as M
     cells = [(i, j, s) for s in range(seeds) for i in range(8) for j in range(8)]
     classes = {"standard": [c for c in cells if c[1] < 6], "large": [c for c in cells if c[1] >= 6]}
     out = {"steps": steps, "cells": {}, "classes": {}}
-    with M.Context(0, 32) as ctx:
+    starts = {size: [(C.regime_cell(i, j, s), C.run(C.GRIDS[size], seed=1 + s, amp=2.0)) for (i, j, s) in members]
+              for size, members in classes.items()}
+    ctxs = {size: M.Context(0, 32) for size in classes}
+    try:
         for size, members in classes.items():
             g = C.GRIDS[size]
-            pool = ctx.pool(g, len(members))
+            pool = ctxs[size].pool(g, len(members))
             t0 = time.perf_counter()
-            runs = pool.runs([(C.regime_cell(i, j, s), C.run(g, seed=1 + s, amp=2.0)) for (i, j, s) in members])
-            ctx.sync()
+            runs = pool.runs(starts[size])
+            ctxs[size].sync()
             create_s = time.perf_counter() - t0
             ms = M.step_timed(runs, steps, 3.0)
             for (i, j, s), r in zip(members, runs):
@@ -71,9 +78,36 @@ def regime(steps=600, seeds=2):
                                                **_metrics(r.diagnostics())}
             out["classes"][size] = {"runs": len(members), "ms_per_step": float(ms.mean()), "create_s": create_s,
                                     "cell_steps_per_s": len(members) * C.nodes(g) * steps / (ms.sum() / 1e3)}
-    total_ms = sum(v["ms_per_step"] * steps for v in out["classes"].values())
-    total_cells = sum(v["runs"] * C.nodes(C.GRIDS[k]) * steps for k, v in out["classes"].items())
-    out["cell_steps_per_s"] = total_cells / (total_ms / 1e3)
+            pool.destroy()
+        total_cells = sum(v["runs"] * C.nodes(C.GRIDS[k]) * steps for k, v in out["classes"].items())
+        total_ms = sum(v["ms_per_step"] * steps for v in out["classes"].values())
+        out["one_class_at_a_time"] = {"cell_steps_per_s": total_cells / (total_ms / 1e3), "device_s": total_ms / 1e3}
+        # Both classes at once from their initial state: the same runs, so the same metrics.
+        batches = {}
+        for size, members in classes.items():
+            pool = ctxs[size].pool(C.GRIDS[size], len(members))
+            batches[size] = (pool, pool.runs(starts[size]))
+            M.step(batches[size][1], 1, 3.0)   # graphs captured and the transport tuned before the clock starts
+        for size in classes:
+            ctxs[size].sync()
+        t0 = time.perf_counter()
+        for size in classes:
+            M.step(batches[size][1], steps - 1, 3.0)
+        for size in classes:
+            ctxs[size].sync()
+        wall = time.perf_counter() - t0
+        cells_conc = sum(len(m) * C.nodes(C.GRIDS[k]) * (steps - 1) for k, m in classes.items())
+        out["both_classes_at_once"] = {"cell_steps_per_s": cells_conc / wall, "wall_s": wall, "steps": steps - 1}
+        out["cell_steps_per_s"] = out["both_classes_at_once"]["cell_steps_per_s"]
+        out["metrics_equal_one_class_at_a_time"] = all(
+            {k: v for k, v in _metrics(r.diagnostics()).items()} == {k: v for k, v in out["cells"][f"{i},{j},{s}"].items()
+                                                                    if k not in ("size", "qv_max_gkg", "shear", "seed")}
+            for size, members in classes.items() for (i, j, s), r in zip(members, batches[size][1]))
+        for pool, _ in batches.values():
+            pool.destroy()
+    finally:
+        for c in ctxs.values():
+            c.close()
     return out
 
 
