@@ -470,8 +470,20 @@ template <class R> __global__ void k_fill_total(Geo g, Bufs<R> b, int tiles) {
   const int slot = b.runs[blockIdx.x], t = threadIdx.x;
   if (t >= 3 * N_CONSERVED) return;
   double s = 0;
-  for (int z = 0; z < g.nz; z++)
-    for (int k = 0; k < tiles; k++) s += b.fillPart(slot, t / 3, z, k, g)[t % 3];
+  if (tiles == 1) {
+    // The parts in level order, 8 levels' loads in flight at a time (the adds stay in order).
+    constexpr int ZB = 8;
+    for (int z0 = 0; z0 < g.nz; z0 += ZB) {
+      double v[ZB];
+#pragma unroll
+      for (int q = 0; q < ZB; q++) v[q] = z0 + q < g.nz ? b.fillPart(slot, t / 3, z0 + q, 0, g)[t % 3] : 0.0;
+#pragma unroll
+      for (int q = 0; q < ZB; q++) if (z0 + q < g.nz) s += v[q];
+    }
+  } else {
+    for (int z = 0; z < g.nz; z++)
+      for (int k = 0; k < tiles; k++) s += b.fillPart(slot, t / 3, z, k, g)[t % 3];
+  }
   b.sc(slot)[S_FILL + t] = s;
 }
 constexpr double TINY_WATER = 1e-10;
