@@ -277,6 +277,10 @@ struct FrontEntry { int node, link, flag; double phi, scale; };
 struct FlashState {
   int size, count, ground, touched, left, brk, event, id, pickN, best, prune;
   int hat;  // the right-hand side's spectra of the charge before the last deposit are in ehat (the next solve updates them)
+  // The step's failure mark (the flash id at the launch's start). Written once before the first barrier and never
+  // after: every thread reads it from here, not from `id`, which thread 0 moves on at each flash's start with no
+  // barrier between (a wave still to read `id` would then take the next id and miss the marks of failed starts).
+  int stamp;
   uint32_t rng;
   double sum, potential, before, moment, ratio;
   FrontEntry picked, last;  // the picked entry and the front's last one (which takes the picked one's place)
@@ -684,7 +688,7 @@ template <class R> __global__ __launch_bounds__(DISCHARGE_THREADS) void k_discha
   if (threadIdx.x == 0) {
     long long id; memcpy(&id, S + S_FLASH_ID, 8); id++; memcpy(S + S_FLASH_ID, &id, 8);
     // The step's electric solve kept its right-hand side's spectra and level sums: every solve here is an update.
-    st.id = int(id); st.size = 0; st.count = 0; st.hat = 1;
+    st.id = int(id); st.stamp = int(id); st.size = 0; st.count = 0; st.hat = 1;
     // The stream of this step: the seed (SimConfig.seed + 1) xor the model time in tenths of a second times 0x9E3779B1.
     uint32_t seed = uint32_t(int(P[P_SEED]) + 1), tt = uint32_t((long long)floor(time * 10 + .5));
     st.rng = seed ^ (tt * 0x9E3779B1u);
@@ -692,7 +696,7 @@ template <class R> __global__ __launch_bounds__(DISCHARGE_THREADS) void k_discha
     st.ratio = sqrt(double(__uint_as_float(unsigned(key0 >> 32))));
   }
   blockSync();
-  const int stamp = st.id;
+  const int stamp = st.stamp;
   for (int k = 0; k <= limit; k++) {
     if (k > 0) d.solve();
     if (k == limit) break;

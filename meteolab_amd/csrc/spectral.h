@@ -93,7 +93,10 @@ template <int P, class R> __device__ inline void fftPass(const cx<R> *src, cx<R>
 // (s % per) * sStride (several levels' sequences at once: `per` sequences a level, levels `outer` apart);
 // tw[k] = exp(-2 pi i k/N) (sign -1, forward) or its conjugate (sign +1). Stockham stages ping-pong between a and b;
 // returns the buffer that holds the result. Ends with a barrier.
-template <class R> __device__ cx<R> *fftStages(cx<R> *a, cx<R> *b, int N, int count, int sStride, int kStride, const cx<R> *tw, const int *rad, int nrad, int sign,
+// Inlined into every caller, so that the buffers' address space (LDS) is known and the passes use DS instructions: an
+// out-of-line copy takes generic pointers (FLAT loads and stores) and costs the discharge kernel its spill-free
+// register budget.
+template <class R> __device__ __attribute__((always_inline)) cx<R> *fftStages(cx<R> *a, cx<R> *b, int N, int count, int sStride, int kStride, const cx<R> *tw, const int *rad, int nrad, int sign,
                                                int per = 1 << 30, int outer = 0) {
   cx<R> *src = a, *dst = b;
   int Ns = 1;
