@@ -109,3 +109,28 @@ def test_the_charged_cloud_flashes_as_the_oracle():
             ref = o.field(name).astype(np.float64)
             diff = np.linalg.norm(r.field(name).astype(np.float64) - ref) / max(np.linalg.norm(ref), 1e-300)
             assert diff <= 1e-5, (name, diff)
+
+
+def test_a_flashing_step_repeats_exactly():
+    """The two charged clear-air clouds made anew eight times, each time in a new pool, and stepped once (9 flashes
+    each): every repetition ends as the first, bit for bit. The discharge's result must not depend on how its waves
+    are scheduled: its failure marks, front and potential pass between waves through LDS and global memory."""
+    g = C.small_grid()
+    dry = C.profile(qv_max=0.0)
+    spec = C.run(g, seed=7, amp=0.0, noise_amp=0.0)
+    outs = []
+    with M.Context(0, 32) as ctx:
+        for rep in range(8):
+            print("repetition", rep, flush=True)
+            pool = ctx.pool(g, 2)
+            runs = [pool.run(dry, spec), pool.run(dry, dict(spec, seed=11))]
+            for r in runs:
+                charged(g, r, dry, spec)
+            M.step(runs, 1)
+            ctx.sync()
+            outs.append([state(r) for r in runs])
+            pool.destroy()
+    assert outs[0][0][2]["flashesIC"] + outs[0][0][2]["flashesCG"] > 1
+    for k in range(1, len(outs)):
+        for a, b in zip(outs[0], outs[k]):
+            same(a, b)
