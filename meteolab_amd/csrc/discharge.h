@@ -114,31 +114,50 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
       total += all;
       blockSync();
     }
-    // Each thread its modes: the changes of a level summed in registers, then added to the level's spectrum.
+    // Each thread its modes: the changes of a level summed in registers, then added to the level's spectrum. The
+    // changes are taken EU at a time: their keys, the twiddles' LDS loads and products first, then the sums in
+    // change order (the same additions in the same order as one change at a time).
+    constexpr int EU = 4;
     for (int mode = threadIdx.x; mode < nm; mode += blockDim.x) {
       const int n = dMw.q(mode), m = mode - n * mw;
       int cur = -1;
       double ar = 0, ai = 0, s0 = 0;
-      for (int e = 0; e < total; e++) {
-        const int key = keys[e], l = key >> 24, y = (key >> 12) & 0xFFF, x = key & 0xFFF;
-        const double d = ds[e];
-        if (l != cur) {
-          if (cur >= 0) {
-            double *o = hat + (size_t(cur) * nm + mode) * 2;
-            o[0] += ar; o[1] += ai;
-            if (mode == 0) m0[cur] += s0;
-          }
-          cur = l; ar = 0; ai = 0; s0 = 0;
+      auto flush = [&]() {
+        if (cur >= 0) {
+          double *o = hat + (size_t(cur) * nm + mode) * 2;
+          o[0] += ar; o[1] += ai;
+          if (mode == 0) m0[cur] += s0;
         }
+      };
+      auto twiddle = [&](int key) {
+        const int y = (key >> 12) & 0xFFF, x = key & 0xFFF;
         const int mx = m * x - dNx.q(m * x) * nx, nyy = n * y - dNy.q(n * y) * ny;
-        const Z w = cmul(tx[mx], ty[nyy]);
+        return cmul(tx[mx], ty[nyy]);
+      };
+      int e = 0;
+      for (; e + EU <= total; e += EU) {
+        int kq[EU];
+        double dq[EU];
+        Z wq[EU];
+#pragma unroll
+        for (int q = 0; q < EU; q++) { kq[q] = keys[e + q]; dq[q] = ds[e + q]; }
+#pragma unroll
+        for (int q = 0; q < EU; q++) wq[q] = twiddle(kq[q]);
+#pragma unroll
+        for (int q = 0; q < EU; q++) {
+          const int l = kq[q] >> 24;
+          if (l != cur) { flush(); cur = l; ar = 0; ai = 0; s0 = 0; }
+          ar += dq[q] * wq[q].re; ai += dq[q] * wq[q].im; s0 += dq[q];
+        }
+      }
+      for (; e < total; e++) {
+        const int key = keys[e], l = key >> 24;
+        const double d = ds[e];
+        if (l != cur) { flush(); cur = l; ar = 0; ai = 0; s0 = 0; }
+        const Z w = twiddle(key);
         ar += d * w.re; ai += d * w.im; s0 += d;
       }
-      if (cur >= 0) {
-        double *o = hat + (size_t(cur) * nm + mode) * 2;
-        o[0] += ar; o[1] += ai;
-        if (mode == 0) m0[cur] += s0;
-      }
+      flush();
     }
     blockSync();
   } else {
