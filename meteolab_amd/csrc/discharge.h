@@ -117,7 +117,7 @@ template <class R> __device__ void solveBlock(const Geo &g, const Bufs<R> &b, co
     // Each thread its modes: the changes of a level summed in registers, then added to the level's spectrum. The
     // changes are taken EU at a time: their keys, the twiddles' LDS loads and products first, then the sums in
     // change order (the same additions in the same order as one change at a time).
-    constexpr int EU = 4;
+    constexpr int EU = 8;
     for (int mode = threadIdx.x; mode < nm; mode += blockDim.x) {
       const int n = dMw.q(mode), m = mode - n * mw;
       int cur = -1;
